@@ -1,0 +1,178 @@
+"""Benchmark: triangle COUNT on R-MAT scale 22 (edge factor 16) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 22] [--no-cpu]
+
+A step is one triangle-count query over the device-resident trie (the
+reference's QUERY TIME: trie build and loading excluded).  The tuned+learned
+form is timed for `value`; the plain GPU drop-in is timed beside it; the CPU
+oracle (a port of the reference engine) is timed on a bounded sample on the
+host cores.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "R-MAT-22 triangle count edges/s + % HBM peak, tuned+learned vs plain, 1 GPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_tuning(E):
+    """Autotuned knobs and learned selector weights committed with the package."""
+    from emptyheaded_amd import tuning
+    applied = tuning.apply_committed()
+    return applied
+
+
+def time_form(E, ctx, T, plan, form, steps, warmup):
+    for _ in range(warmup):
+        E.join_count(ctx, plan, [T], form)
+    ctx.synchronize()
+    kernel_ms, total = [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        total, st = E.join_count(ctx, plan, [T], form, stats=True)
+        kernel_ms.append(st["main_kernel_ms"])
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    return total, (t1 - t0) * 1e3 / steps, sum(kernel_ms) / len(kernel_ms), st
+
+
+def cpu_baseline(src, dst, m, expect, budget_s=20.0):
+    from emptyheaded_amd import plans
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    OT = O.Trie(src, dst)
+    build_s = time.perf_counter() - t
+    # stratified sample: every stride-th chunk of 128 level-0 values
+    stride = 64
+    while True:
+        t = time.perf_counter()
+        O.join_count_sampled(plans.TRIANGLE, [OT], cores, stride)
+        dt = time.perf_counter() - t
+        if dt * 4 > budget_s or stride == 1:
+            break
+        stride = max(1, stride // 4)
+    full = None
+    if stride == 1:
+        full = O.join_count(plans.TRIANGLE, [OT], cores)
+        assert full == expect, (full, expect)
+    est_full_s = dt * stride
+    return {"value": m / est_full_s, "unit": "edges/s", "cores": cores, "kind": "port",
+            "sample": ("full workload" if stride == 1 else
+                       "every %d-th chunk of 128 level-0 vertices (1/%d of heads), time scaled by %d" %
+                       (stride, stride, stride)),
+            "sample_seconds": dt, "trie_build_seconds": build_s}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the join's multi-GPU form (replicated trie, first attribute "
+              "sharded, one 8-byte all-reduce) is a design, not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import emptyheaded_amd as E
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import plans
+
+    t = time.perf_counter()
+    src, dst, nv = G.rmat_graph(args.scale, args.edge_factor)
+    m = len(src)
+    log("graph: scale %d, %d vertices, %d oriented edges (%.1fs)" % (args.scale, nv, m, time.perf_counter() - t))
+    balg = G.algorithmic_bytes(src, dst, nv)
+    tuning = load_tuning(E)
+    log("tuning:", json.dumps(tuning))
+
+    ctx = E.Context(0)
+    T_rule = E.Trie(ctx, src, dst, E.POLICY_RULE)
+    T_learn = E.Trie(ctx, src, dst, E.POLICY_LEARNED if tuning.get("selector") else E.POLICY_RULE)
+    info = T_learn.info()
+    log("tries built: rule %.2f ms, learned %.2f ms; bitsets %d (learned) vs %d (rule)" %
+        (T_rule.info()["build_ms"], info["build_ms"], info["num_bitsets"], T_rule.info()["num_bitsets"]))
+
+    res = {}
+    count, ms, kms, st = time_form(E, ctx, T_learn, plans.TRIANGLE, E.FORM_LEARNED, args.steps, args.warmup)
+    res["learned"] = (count, ms, kms, st)
+    log("tuned+learned: %d triangles, %.3f ms/step, main kernel %.3f ms" % (count, ms, kms))
+    if not args.no_plain:
+        pc, pms, pkms, pst = time_form(E, ctx, T_rule, plans.TRIANGLE, E.FORM_PLAIN, max(2, args.steps // 2), 1)
+        res["plain"] = (pc, pms, pkms, pst)
+        log("plain: %d triangles, %.3f ms/step, main kernel %.3f ms" % (pc, pms, pkms))
+        assert pc == count, "plain and tuned+learned disagree"
+    tc, tms, tkms, tst = time_form(E, ctx, T_rule, plans.TRIANGLE, E.FORM_TUNED, args.steps, args.warmup)
+    res["tuned"] = (tc, tms, tkms, tst)
+    log("tuned (rule layouts): %d triangles, %.3f ms/step, main kernel %.3f ms" % (tc, tms, tkms))
+    assert tc == count
+
+    achieved = balg / (kms * 1e-3)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_k_tuned_rmat22.json")
+    if os.path.exists(tf) and args.scale == 22:
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": m / (ms * 1e-3),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "TriangleAgg COUNT(*) on R-MAT scale %d, edge factor %d, (a,b,c,d)=(0.57,0.19,0.19,0.05), "
+                        "symmetrized, ascending-degree relabel, pruned src<dst" % (args.scale, args.edge_factor),
+            "scale": args.scale, "edge_factor": args.edge_factor, "vertices": nv, "oriented_edges": m,
+            "triangles": count, "form": "tuned+learned",
+        },
+        "results_per_s": count / (ms * 1e-3),
+        "algorithmic_bytes": balg,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "k_tuned",
+                     "kernel_ms": kms},
+        "tuned_rule_ms_per_step": tms,
+        "tuning": tuning,
+    }
+    if "plain" in res:
+        out["plain_ms_per_step"] = res["plain"][1]
+        out["plain_edges_per_s"] = m / (res["plain"][1] * 1e-3)
+        out["speedup_vs_plain"] = res["plain"][1] / ms
+        out["plain_pairs_by_alg"] = res["plain"][3]["pairs_by_alg"]
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(src, dst, m, count)
+    else:
+        out["cpu_baseline"] = None
+    ctx.synchronize()
+    T_rule.close()
+    T_learn.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

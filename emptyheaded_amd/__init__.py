@@ -1,0 +1,284 @@
+"""emptyheaded_amd -- MI355X-native generic-join engine (libehx) for EmptyHeaded's
+storage-engine hot path.
+
+This module is a thin ctypes binding of the C ABI in include/ehx.h and
+include/ehx_ext.h.  The library must be built (python -m emptyheaded_amd.build);
+importing the module without it raises, and nothing here falls back to a CPU
+implementation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libehx.so")
+
+OK = 0
+RANGE_BITSET, UINTEGER = 0, 1
+POLICY_RULE, POLICY_ALL_UINT, POLICY_ALL_BITSET, POLICY_LEARNED = 0, 1, 2, 3
+FORM_PLAIN, FORM_TUNED, FORM_LEARNED = 0, 1, 2
+MAX_ATTRS, MAX_OPS = 8, 8
+ALGORITHMS = ("merge", "gallop", "probe", "and", "empty")
+
+
+class EhxError(RuntimeError):
+    pass
+
+
+class Op(C.Structure):
+    _fields_ = [("rel", C.c_int32), ("parent", C.c_int32)]
+
+
+class Plan(C.Structure):
+    _fields_ = [("num_attrs", C.c_int32), ("num_ops", C.c_int32 * MAX_ATTRS),
+                ("ops", (Op * MAX_OPS) * MAX_ATTRS)]
+
+
+class TrieMeta(C.Structure):
+    _fields_ = [("num_rows", C.c_uint64), ("num_heads", C.c_uint64), ("num_bitsets", C.c_uint64),
+                ("bitset_words", C.c_uint64), ("bitset_elems", C.c_uint64), ("max_card", C.c_uint32),
+                ("head_layout", C.c_uint32), ("dense_nav", C.c_uint32), ("max_value", C.c_uint32),
+                ("build_ms", C.c_double)]
+
+
+class JoinStats(C.Structure):
+    _fields_ = [("level_ms", C.c_double * MAX_ATTRS), ("frontier", C.c_uint64 * MAX_ATTRS),
+                ("pairs_by_alg", C.c_uint64 * 5), ("total_ms", C.c_double), ("main_kernel_ms", C.c_double),
+                ("form_used", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {"level_ms": list(self.level_ms), "frontier": list(self.frontier),
+                "pairs_by_alg": dict(zip(ALGORITHMS, list(self.pairs_by_alg))),
+                "total_ms": self.total_ms, "main_kernel_ms": self.main_kernel_ms, "form_used": self.form_used}
+
+
+class TuneParams(C.Structure):
+    _fields_ = [("lds_slots", C.c_int32), ("big_deg", C.c_int32), ("waves_per_cu", C.c_int32),
+                ("grab", C.c_int32), ("bitmap_all", C.c_int32), ("use_graph", C.c_int32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+# Every symbol the headers declare, with ctypes signatures.
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+SIGNATURES = {
+    "ehx_version": ([], C.c_int),
+    "ehx_last_error": ([], C.c_char_p),
+    "ehx_device_count": ([C.POINTER(C.c_int)], C.c_int),
+    "ehx_ctx_create": ([C.c_int, C.POINTER(_vp)], C.c_int),
+    "ehx_ctx_destroy": ([_vp], C.c_int),
+    "ehx_ctx_synchronize": ([_vp], C.c_int),
+    "ehx_trie_build": ([_vp, _vp, _vp, C.c_uint64, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
+    "ehx_trie_destroy": ([_vp], C.c_int),
+    "ehx_trie_info": ([_vp, C.POINTER(TrieMeta)], C.c_int),
+    "ehx_trie_export": ([_vp, _vp, _u32p, _u32p], C.c_int),
+    "ehx_trie_node_info": ([_vp, _vp, C.POINTER(C.c_uint8), _u32p], C.c_int),
+    "ehx_trie_lookup": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64)], C.c_int),
+    "ehx_trie_device_arrays": ([_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)], C.c_int),
+    "ehx_intersect_count_batch": ([_vp, _vp, _u32p, _vp, _u32p, C.c_uint64, _u64p], C.c_int),
+    "ehx_intersect_batch": ([_vp, _vp, _u32p, _vp, _u32p, C.c_uint64, _u64p, _u32p, C.c_uint64, _u64p],
+                            C.c_int),
+    "ehx_join_count": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, C.POINTER(C.c_int64),
+                        C.POINTER(JoinStats)], C.c_int),
+    "ehx_join_count_range": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, C.c_uint64, C.c_uint64,
+                              C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
+    "ehx_gen_er": ([C.c_uint64, C.c_uint64, C.c_uint64, _u32p, _u32p], C.c_int),
+    "ehx_gen_rmat": ([C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, _u32p, _u32p],
+                     C.c_int),
+    "ehx_graph_prepare": ([_u32p, _u32p, C.c_uint64, C.c_int, C.c_int, C.POINTER(_u32p), C.POINTER(_u32p),
+                           _u64p, _u32p], C.c_int),
+    "ehx_free": ([_vp], None),
+    "ehx_tune_get": ([C.POINTER(TuneParams)], C.c_int),
+    "ehx_tune_set": ([C.POINTER(TuneParams)], C.c_int),
+    "ehx_selector_num_params": ([], C.c_int),
+    "ehx_selector_load": ([C.c_int, C.POINTER(C.c_uint16), C.c_uint64, C.POINTER(C.c_float), C.c_float],
+                          C.c_int),
+    "ehx_selector_clear": ([], C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libehx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EhxError("libehx.so is not built: run `python -m emptyheaded_amd.build` (%s)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.argtypes = args
+                f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().ehx_last_error()
+        raise EhxError("ehx error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def ptr_u32(a):
+    return a.ctypes.data_as(_u32p)
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().ehx_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    """One GPU: a HIP stream and grow-only scratch (ehx_ctx)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        check(lib().ehx_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().ehx_ctx_destroy(self.h))
+            self.h = None
+
+    def synchronize(self):
+        check(lib().ehx_ctx_synchronize(self.h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Trie:
+    """Device trie over a binary relation (src, dst) -- Trie<A,M> of the reference."""
+
+    def __init__(self, ctx, src, dst, policy=POLICY_RULE):
+        s, d = u32(src), u32(dst)
+        if s.shape != d.shape:
+            raise ValueError("columns differ in length")
+        h = _vp()
+        check(lib().ehx_trie_build(ctx.h, s.ctypes.data, d.ctypes.data, len(s), policy, 0, C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.policy = policy
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().ehx_trie_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        m = TrieMeta()
+        check(lib().ehx_trie_info(self.h, C.byref(m)))
+        return {f: getattr(m, f) for f, _ in TrieMeta._fields_}
+
+    @property
+    def num_rows(self):
+        return self.info()["num_rows"]
+
+    def tuples(self):
+        n = self.num_rows
+        a = np.zeros(n, dtype=np.uint32)
+        b = np.zeros(n, dtype=np.uint32)
+        check(lib().ehx_trie_export(self.ctx.h, self.h, ptr_u32(a), ptr_u32(b)))
+        return a, b
+
+    def node_info(self):
+        nh = self.info()["num_heads"]
+        t = np.zeros(nh, dtype=np.uint8)
+        c = np.zeros(nh, dtype=np.uint32)
+        check(lib().ehx_trie_node_info(self.ctx.h, self.h, t.ctypes.data_as(C.POINTER(C.c_uint8)), ptr_u32(c)))
+        return t, c
+
+    def lookup(self, keys):
+        k = u32(keys)
+        out = np.zeros(len(k), dtype=np.int64)
+        check(lib().ehx_trie_lookup(self.ctx.h, self.h, ptr_u32(k), len(k),
+                                    out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+
+def intersect_count(ctx, ta, keys_a, tb, keys_b):
+    ka, kb = u32(keys_a), u32(keys_b)
+    out = np.zeros(len(ka), dtype=np.uint64)
+    check(lib().ehx_intersect_count_batch(ctx.h, ta.h, ptr_u32(ka), tb.h, ptr_u32(kb), len(ka),
+                                          out.ctypes.data_as(_u64p)))
+    return out
+
+
+def intersect(ctx, ta, keys_a, tb, keys_b):
+    """Materialized intersections: (offsets[n+1], values)."""
+    ka, kb = u32(keys_a), u32(keys_b)
+    n = len(ka)
+    offs = np.zeros(n + 1, dtype=np.uint64)
+    total = C.c_uint64(0)
+    cap = int(intersect_count(ctx, ta, ka, tb, kb).sum()) if n else 0
+    vals = np.zeros(max(cap, 1), dtype=np.uint32)
+    check(lib().ehx_intersect_batch(ctx.h, ta.h, ptr_u32(ka), tb.h, ptr_u32(kb), n, offs.ctypes.data_as(_u64p),
+                                    ptr_u32(vals), cap, C.byref(total)))
+    return offs, vals[: total.value]
+
+
+def make_plan(levels):
+    """levels: one list per attribute of (rel, parent) operands; parent -1 = head set."""
+    if not 1 <= len(levels) <= MAX_ATTRS:
+        raise ValueError("1..%d attributes" % MAX_ATTRS)
+    p = Plan()
+    p.num_attrs = len(levels)
+    for k, ops in enumerate(levels):
+        p.num_ops[k] = len(ops)
+        for i, (rel, parent) in enumerate(ops):
+            p.ops[k][i].rel = rel
+            p.ops[k][i].parent = parent
+    return p
+
+
+def join_count(ctx, levels, tries, form=FORM_PLAIN, begin=None, end=None, stats=False):
+    p = make_plan(levels) if not isinstance(levels, Plan) else levels
+    arr = (_vp * len(tries))(*[t.h.value for t in tries])
+    res = C.c_int64(0)
+    st = JoinStats()
+    if begin is None and end is None:
+        check(lib().ehx_join_count(ctx.h, C.byref(p), arr, form, C.byref(res), C.byref(st)))
+    else:
+        check(lib().ehx_join_count_range(ctx.h, C.byref(p), arr, form, begin or 0,
+                                         2**64 - 1 if end is None else end, C.byref(res), C.byref(st)))
+    return (res.value, st.as_dict()) if stats else res.value
+
+
+def tune_get():
+    t = TuneParams()
+    check(lib().ehx_tune_get(C.byref(t)))
+    return t.as_dict()
+
+
+def tune_set(**kw):
+    t = TuneParams()
+    check(lib().ehx_tune_get(C.byref(t)))
+    for k, v in kw.items():
+        setattr(t, k, int(v))
+    check(lib().ehx_tune_set(C.byref(t)))
+    return t.as_dict()
+
+
+from . import graphs, plans  # noqa: E402,F401

@@ -1,0 +1,210 @@
+// ehx_api.cpp -- extern "C" entry points of libehx (declared in include/ehx.h
+// and include/ehx_ext.h), error reporting, contexts.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/ehx.h"
+#include "../../include/ehx_ext.h"
+#include "ehx_internal.h"
+#include "ehx_tuned.h"
+
+namespace ehx {
+
+static thread_local std::string t_err;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_err = buf;
+}
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+
+int trie_build(ehx_ctx *, const uint32_t *, const uint32_t *, uint64_t, int, int, ehx_trie **);
+int trie_destroy(ehx_trie *);
+int trie_export(ehx_ctx *, const ehx_trie *, uint32_t *, uint32_t *);
+int trie_node_info(ehx_ctx *, const ehx_trie *, uint8_t *, uint32_t *);
+int trie_lookup(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *);
+int join_count(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint64_t, uint64_t, int64_t *, ehx_join_stats *);
+int intersect_pairs(ehx_ctx *, const ehx_trie *, const uint32_t *, const ehx_trie *, const uint32_t *, uint64_t,
+                    uint64_t *, uint64_t *, uint32_t *, uint64_t, uint64_t *);
+void tuned_forget_graphs();
+int selector_load(int device, const uint16_t *params, const float *norm, float max_words);
+int selector_clear();
+
+}  // namespace ehx
+
+using namespace ehx;
+
+extern "C" {
+
+int ehx_version(void) { return 1; }
+
+const char *ehx_last_error(void) { return t_err.c_str(); }
+
+int ehx_device_count(int *count) {
+  if (!count) return fail(EHX_ERR_INVALID, "null count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return EHX_OK;
+}
+
+int ehx_ctx_create(int device, ehx_ctx **out) {
+  if (!out) return fail(EHX_ERR_INVALID, "null out");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(EHX_ERR_DEVICE, "no GPU visible to HIP");
+  if (device < 0 || device >= n) return fail(EHX_ERR_INVALID, "device %d of %d", device, n);
+  EHX_HIP(hipSetDevice(device));
+  ehx_ctx *c = new ehx_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete c; return fail(EHX_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e)); }
+  for (auto &ev : c->ev) {
+    e = hipEventCreate(&ev);
+    if (e != hipSuccess) { delete c; return fail(EHX_ERR_DEVICE, "hipEventCreate: %s", hipGetErrorString(e)); }
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return EHX_OK;
+}
+
+int ehx_ctx_destroy(ehx_ctx *ctx) {
+  if (!ctx) return EHX_OK;
+  EHX_HIP(hipSetDevice(ctx->device));
+  EHX_HIP(hipStreamSynchronize(ctx->stream));
+  tuned_forget_graphs();
+  for (auto &b : ctx->scratch) b.release();
+  for (auto &ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return EHX_OK;
+}
+
+int ehx_ctx_synchronize(ehx_ctx *ctx) {
+  if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
+  EHX_HIP(hipSetDevice(ctx->device));
+  EHX_HIP(hipStreamSynchronize(ctx->stream));
+  return EHX_OK;
+}
+
+int ehx_trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t rows, int policy, int on_device,
+                   ehx_trie **out) {
+  if (!c0 || !c1) return fail(EHX_ERR_INVALID, "ehx_trie_build: null column");
+  return trie_build(ctx, c0, c1, rows, policy, on_device, out);
+}
+
+int ehx_trie_destroy(ehx_trie *t) { return trie_destroy(t); }
+
+int ehx_trie_info(const ehx_trie *t, ehx_trie_meta *out) {
+  if (!t || !out) return fail(EHX_ERR_INVALID, "null argument");
+  *out = t->info;
+  return EHX_OK;
+}
+
+int ehx_trie_export(ehx_ctx *ctx, const ehx_trie *t, uint32_t *c0, uint32_t *c1) {
+  if (!ctx || !t || !c0 || !c1) return fail(EHX_ERR_INVALID, "null argument");
+  return trie_export(ctx, t, c0, c1);
+}
+
+int ehx_trie_node_info(ehx_ctx *ctx, const ehx_trie *t, uint8_t *types, uint32_t *cards) {
+  if (!ctx || !t) return fail(EHX_ERR_INVALID, "null argument");
+  return trie_node_info(ctx, t, types, cards);
+}
+
+int ehx_trie_lookup(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *keys, uint64_t n, int64_t *nodes) {
+  if (!ctx || !t || (n && (!keys || !nodes))) return fail(EHX_ERR_INVALID, "null argument");
+  return trie_lookup(ctx, t, keys, n, nodes);
+}
+
+int ehx_trie_device_arrays(const ehx_trie *t, const uint32_t **head_vals, const uint32_t **offsets,
+                           const uint32_t **values) {
+  if (!t) return fail(EHX_ERR_INVALID, "null trie");
+  if (head_vals) *head_vals = t->head;
+  if (offsets) *offsets = t->off;
+  if (values) *values = t->vals;
+  return EHX_OK;
+}
+
+int ehx_intersect_count_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, const ehx_trie *tb,
+                              const uint32_t *keys_b, uint64_t n, uint64_t *counts) {
+  if (!counts && n) return fail(EHX_ERR_INVALID, "null counts");
+  return intersect_pairs(ctx, ta, keys_a, tb, keys_b, n, counts, nullptr, nullptr, 0, nullptr);
+}
+
+int ehx_intersect_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, const ehx_trie *tb,
+                        const uint32_t *keys_b, uint64_t n, uint64_t *offsets, uint32_t *values, uint64_t cap,
+                        uint64_t *total) {
+  if (!offsets || (!values && cap)) return fail(EHX_ERR_INVALID, "null output");
+  return intersect_pairs(ctx, ta, keys_a, tb, keys_b, n, nullptr, offsets, values, cap, total);
+}
+
+int ehx_join_count(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, int64_t *result,
+                   ehx_join_stats *stats) {
+  if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
+  return join_count(ctx, plan, tries, form, 0, UINT64_MAX, result, stats);
+}
+
+int ehx_join_count_range(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, uint64_t begin,
+                         uint64_t end, int64_t *result, ehx_join_stats *stats) {
+  if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
+  return join_count(ctx, plan, tries, form, begin, end, result, stats);
+}
+
+int ehx_tune_get(ehx_tune_params *out) {
+  if (!out) return fail(EHX_ERR_INVALID, "null out");
+  const TuneParams &p = tune_params();
+  out->lds_slots = p.lds_slots;
+  out->big_deg = p.big_deg;
+  out->waves_per_cu = p.waves_per_cu;
+  out->grab = p.grab;
+  out->bitmap_all = p.bitmap_all;
+  out->use_graph = p.use_graph;
+  return EHX_OK;
+}
+
+int ehx_tune_set(const ehx_tune_params *in) {
+  if (!in) return fail(EHX_ERR_INVALID, "null params");
+  if (in->lds_slots != 2048 && in->lds_slots != 4096 && in->lds_slots != 8192)
+    return fail(EHX_ERR_INVALID, "lds_slots must be 2048, 4096 or 8192");
+  if (in->big_deg < 1 || in->waves_per_cu < 1 || in->waves_per_cu > 64 || in->grab < 1 || in->grab > 64)
+    return fail(EHX_ERR_INVALID, "tune parameter out of range");
+  TuneParams &p = tune_params();
+  p.lds_slots = in->lds_slots;
+  p.big_deg = in->big_deg;
+  p.waves_per_cu = in->waves_per_cu;
+  p.grab = in->grab;
+  p.bitmap_all = in->bitmap_all ? 1 : 0;
+  p.use_graph = in->use_graph ? 1 : 0;
+  return EHX_OK;
+}
+
+int ehx_selector_load(int device, const uint16_t *params, uint64_t nparams, const float *norm, float max_words) {
+  if (!params || !norm) return fail(EHX_ERR_INVALID, "null weights");
+  if (nparams != (uint64_t)ehx_selector_num_params()) return fail(EHX_ERR_INVALID, "selector expects %d params", ehx_selector_num_params());
+  return selector_load(device, params, norm, max_words);
+}
+
+int ehx_selector_clear(void) { return selector_clear(); }
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+// ehx_internal.h -- shared declarations of libehx (HIP for gfx950).
+//
+// Device data layout of a two-level trie (one binary relation), struct of
+// arrays in HBM:
+//   head[nh]          sorted distinct first-column values
+//   off[nh+1]         child range of head node h: vals[off[h] .. off[h+1])
+//   vals[m]           second-column values, sorted within each child set
+//   layout[nh+1]      per-set layout (EHX_UINTEGER / EHX_RANGE_BITSET); [nh] is the head set
+//   bs_off[nh+2]      word range of set h in the bitset pool (empty for uint sets)
+//   bs_start[nh+1]    first 64-bit word index (value >> 6) covered by set h
+//   words[W], rank[W] range_bitset words and per-word rank (elements before the word)
+//   dense[dense_len]  value -> head node (or -1): the NextLevel[data] array of a dense head
+// Every set keeps its sorted values (for enumeration); the layout decides which
+// representation intersections read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/ehx.h"
+
+#define EHX_MAX_RELS 8
+
+namespace ehx {
+
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define EHX_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return ::ehx::fail(EHX_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);                                                \
+  } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);
+  void release();
+  template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+}  // namespace ehx
+
+// Events bracketing the last level's main launch.
+#define EHX_EV_K0 (2 * EHX_MAX_ATTRS + 2)
+#define EHX_EV_K1 (2 * EHX_MAX_ATTRS + 3)
+
+struct ehx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2 * EHX_MAX_ATTRS + 4] = {};
+  ehx::DevBuf scratch[24];
+  int num_cus = 256;
+};
+
+struct ehx_trie {
+  int device = 0;
+  uint64_t m = 0, nh = 0;
+  uint32_t *head = nullptr, *off = nullptr, *vals = nullptr;
+  uint8_t *layout = nullptr;
+  uint32_t *bs_off = nullptr, *bs_start = nullptr, *rank = nullptr;
+  uint64_t *words = nullptr;
+  int32_t *dense = nullptr;
+  uint32_t dense_len = 0;
+  ehx_trie_meta info{};
+  void *block = nullptr;  // single allocation holding most arrays
+};
+
+namespace ehx {
+
+// Scratch slots of a context.
+enum Slot {
+  S_SCAN0 = 0, S_SCAN1, S_SCAN2, S_SCAN3,  // scan block sums (recursion levels)
+  S_SORT_A, S_SORT_B, S_HIST, S_FLAGS, S_POS, S_MISC,
+  S_F0, S_F1, S_F2, S_F3, S_F4, S_F5, S_F6, S_F7,  // frontier columns
+  S_COUNTS, S_OFFS, S_QUEUE, S_ACC, S_IN0, S_IN1
+};
+
+// ---- device view of a trie --------------------------------------------------
+struct DTrie {
+  const uint32_t *head, *off, *vals;
+  const uint8_t *layout;
+  const uint32_t *bs_off, *bs_start, *rank;
+  const uint64_t *words;
+  const int32_t *dense;
+  uint32_t dense_len, nh;
+};
+
+inline DTrie view(const ehx_trie *t) {
+  DTrie d;
+  d.head = t->head; d.off = t->off; d.vals = t->vals; d.layout = t->layout;
+  d.bs_off = t->bs_off; d.bs_start = t->bs_start; d.rank = t->rank; d.words = t->words;
+  d.dense = t->dense; d.dense_len = t->dense_len; d.nh = (uint32_t)t->nh;
+  return d;
+}
+
+// A set on the device: sorted values plus, for bitsets, the word window.
+struct DSet {
+  const uint32_t *v;
+  const uint64_t *w;
+  uint32_t n, W, start, lay;
+};
+
+__device__ __forceinline__ int32_t node_of(const DTrie &t, uint32_t x) {
+  if (t.dense) return (x < t.dense_len) ? t.dense[x] : -1;
+  uint32_t lo = 0, hi = t.nh;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (t.head[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return (lo < t.nh && t.head[lo] == x) ? (int32_t)lo : -1;
+}
+
+__device__ __forceinline__ DSet set_of(const DTrie &t, uint32_t h) {
+  DSet s;
+  if (h == t.nh) { s.v = t.head; s.n = t.nh; }
+  else { uint32_t a = t.off[h]; s.v = t.vals + a; s.n = t.off[h + 1] - a; }
+  s.lay = t.layout[h];
+  uint32_t b = t.bs_off[h];
+  s.W = t.bs_off[h + 1] - b;
+  s.w = t.words + b;
+  s.start = t.bs_start[h];
+  return s;
+}
+
+__device__ __forceinline__ bool bit_test(const DSet &s, uint32_t x) {
+  uint32_t wi = (x >> 6) - s.start;
+  if ((x >> 6) < s.start || wi >= s.W) return false;
+  return (s.w[wi] >> (x & 63)) & 1ull;
+}
+
+// Lower bound in a sorted array (first index with v[i] >= x).
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t *v, uint32_t lo, uint32_t hi, uint32_t x) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (v[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ bool set_contains(const DSet &s, uint32_t x) {
+  if (s.lay == EHX_RANGE_BITSET) return bit_test(s, x);
+  uint32_t i = lower_bound(s.v, 0, s.n, x);
+  return i < s.n && s.v[i] == x;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanes_below() {
+  uint32_t l = __lane_id();
+  return l ? (~0ull >> (64 - l)) : 0ull;
+}
+
+template <class T> __device__ __forceinline__ T wave_incl_scan(T v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T t = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v += t;
+  }
+  return v;
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// ---- host-side primitives (ehx_prims.hip) ------------------------------------
+// Exclusive scan of n counts into out[0..n]; out[n] = total.  `level` picks the
+// scratch slots used for recursion.
+int scan_u32_to_u64(ehx_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n);
+int scan_u64(ehx_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n);
+int scan_u32(ehx_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
+// Stable LSD radix sort of n 64-bit keys on their low `bits` bits; keys[] holds
+// the result, tmp[] has n entries.
+int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits);
+int reduce_max_u32(ehx_ctx *ctx, const uint32_t *in, uint64_t n, uint32_t *host_out);
+int fill_u32(ehx_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
+int fill_u64(ehx_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
+int copy_d2h(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int copy_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+}  // namespace ehx

@@ -1,0 +1,338 @@
+// ehx_join.hip -- level-synchronous generic join (COUNT) on the device.
+//
+// The generated code runs the join depth-first per thread, consuming each
+// intersection inside host lambdas (EHGenerator.scala:273-520).  Here each
+// level runs as device launches over a materialized frontier of prefix tuples:
+//   level 0     ParTrieBuilder::build_aggregated_set over the head sets
+//               (TrieBuilder.cpp:478-578): flag + scan + compact;
+//   level k     TrieBuilder::build_aggregated_set per tuple (TrieBuilder.cpp:
+//               242-348): count pass, exclusive scan, write pass (the frontier
+//               stays in lexicographic order, the order foreach_aggregate visits);
+//   last level  count only, folded into one int64 (par::reducer, parallel.hpp:8-34).
+// The plain form gives every tuple one wavefront; the tuned and learned forms
+// replace the last level with the degree-binned persistent kernel in
+// ehx_tuned.hip.
+#include <stdio.h>
+#include <string.h>
+
+#include "ehx_join_common.h"
+#include "ehx_tuned.h"
+
+namespace ehx {
+
+struct Frontier {
+  const uint32_t *col[EHX_MAX_ATTRS];
+};
+struct OutFrontier {
+  uint32_t *col[EHX_MAX_ATTRS];
+};
+
+enum { MODE_COUNT = 0, MODE_WRITE = 1, MODE_SUM = 2 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Frontier in, int k, uint64_t n,
+                                                     uint32_t *__restrict__ counts, const uint64_t *__restrict__ offs,
+                                                     OutFrontier outf, unsigned long long *__restrict__ acc) {
+  __shared__ unsigned long long s_acc[6];
+  if (MODE == MODE_SUM) {
+    if (threadIdx.x < 6) s_acc[threadIdx.x] = 0;
+    __syncthreads();
+  }
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long sum = 0, algc[5] = {0, 0, 0, 0, 0};
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nwaves) {
+    uint32_t tuple[EHX_MAX_ATTRS];
+    for (int j = 0; j < k; j++) tuple[j] = in.col[j][t];
+    DSet S[EHX_MAX_OPS];
+    uint32_t c = 0;
+    int alg = ALG_EMPTY;
+    if (resolve(T, L, tuple, S)) {
+      if (MODE == MODE_WRITE) {
+        uint64_t o = offs[t];
+        c = wave_intersect<true>(S, L.nops, outf.col[k] + o, &alg);
+        for (int j = 0; j < k; j++)
+          for (uint32_t i = __lane_id(); i < c; i += 64) outf.col[j][o + i] = tuple[j];
+      } else {
+        c = wave_intersect<false>(S, L.nops, nullptr, &alg);
+      }
+    }
+    if (MODE == MODE_COUNT && __lane_id() == 0) counts[t] = c;
+    if (MODE == MODE_SUM) {
+      sum += c;
+      algc[alg]++;
+    }
+  }
+  if (MODE == MODE_SUM) {
+    if (__lane_id() == 0) {
+      atomicAdd(&s_acc[0], sum);
+      for (int a = 0; a < 5; a++)
+        if (algc[a]) atomicAdd(&s_acc[1 + a], algc[a]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 6 && s_acc[threadIdx.x]) atomicAdd(&acc[threadIdx.x], s_acc[threadIdx.x]);
+  }
+}
+
+// Level 0: flags over the driver head set, membership in the other head sets.
+__global__ void k_level0_flags(Tries T, LevelDesc L, int driver, uint32_t *__restrict__ flags) {
+  const DTrie &td = T.t[L.rel[driver]];
+  DSet D = set_of(td, td.nh);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D.n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t x = D.v[i];
+    bool hit = true;
+    for (int o = 0; o < L.nops && hit; o++) {
+      if (o == driver) continue;
+      const DTrie &t = T.t[L.rel[o]];
+      hit = node_of(t, x) >= 0;  // membership in a head set = a child exists
+    }
+    flags[i] = hit ? 1u : 0u;
+  }
+}
+
+__global__ void k_compact_u32(const uint32_t *__restrict__ v, const uint32_t *__restrict__ flags,
+                              const uint64_t *__restrict__ pos, uint64_t n, uint32_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (flags[i]) out[pos[i]] = v[i];
+}
+
+static int check_plan(const ehx_plan *p, ehx_trie *const *tries, int *nrels) {
+  if (!p || !tries) return fail(EHX_ERR_INVALID, "null plan or tries");
+  if (p->num_attrs < 1 || p->num_attrs > EHX_MAX_ATTRS) return fail(EHX_ERR_INVALID, "num_attrs %d out of range", p->num_attrs);
+  int mr = -1;
+  for (int k = 0; k < p->num_attrs; k++) {
+    if (p->num_ops[k] < 1 || p->num_ops[k] > EHX_MAX_OPS)
+      return fail(EHX_ERR_INVALID, "level %d: num_ops %d out of range", k, p->num_ops[k]);
+    for (int i = 0; i < p->num_ops[k]; i++) {
+      const ehx_op &o = p->ops[k][i];
+      if (o.rel < 0 || o.rel >= EHX_MAX_RELS) return fail(EHX_ERR_INVALID, "level %d op %d: rel %d", k, i, o.rel);
+      if (o.parent >= k) return fail(EHX_ERR_INVALID, "level %d op %d: parent %d not before level", k, i, o.parent);
+      if (o.rel > mr) mr = o.rel;
+    }
+  }
+  for (int r = 0; r <= mr; r++)
+    if (!tries[r]) return fail(EHX_ERR_INVALID, "relation %d has no trie", r);
+  *nrels = mr + 1;
+  return EHX_OK;
+}
+
+static LevelDesc level_desc(const ehx_plan *p, int k) {
+  LevelDesc L;
+  memset(&L, 0, sizeof(L));
+  L.nops = p->num_ops[k];
+  for (int i = 0; i < L.nops; i++) {
+    L.rel[i] = p->ops[k][i].rel;
+    L.parent[i] = p->ops[k][i].parent;
+  }
+  return L;
+}
+
+static const uint64_t FRONTIER_BUDGET = 96ull << 30;  // bytes per materialized level
+
+int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form, uint64_t begin, uint64_t end,
+               int64_t *result, ehx_join_stats *stats) {
+  int nrels = 0, rc;
+  if ((rc = check_plan(p, tries, &nrels))) return rc;
+  if (!result) return fail(EHX_ERR_INVALID, "null result");
+  if (form < EHX_FORM_PLAIN || form > EHX_FORM_LEARNED) return fail(EHX_ERR_INVALID, "bad form %d", form);
+  EHX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  ehx_join_stats S;
+  memset(&S, 0, sizeof(S));
+  Tries T;
+  memset(&T, 0, sizeof(T));
+  for (int r = 0; r < nrels; r++) {
+    if (tries[r]->device != ctx->device) return fail(EHX_ERR_INVALID, "trie %d lives on another device", r);
+    T.t[r] = view(tries[r]);
+  }
+  const int K = p->num_attrs;
+  EHX_HIP(hipEventRecord(ctx->ev[0], st));
+
+  // ---- level 0 -------------------------------------------------------------
+  LevelDesc L0 = level_desc(p, 0);
+  int driver = 0;
+  for (int i = 1; i < L0.nops; i++)
+    if (tries[L0.rel[i]]->nh <= tries[L0.rel[driver]]->nh) driver = i;
+  bool single = true;
+  for (int i = 0; i < L0.nops; i++)
+    if (L0.rel[i] != L0.rel[driver]) single = false;
+  const ehx_trie *td = tries[L0.rel[driver]];
+  const uint32_t *f0;
+  uint64_t n0;
+  if (single) {
+    f0 = td->head;  // head ∩ head of one trie is the head itself
+    n0 = td->nh;
+  } else {
+    DevBuf &fl = ctx->scratch[S_FLAGS], &ps = ctx->scratch[S_POS], &fb = ctx->scratch[S_F0];
+    if ((rc = fl.ensure(td->nh * 4 + 4)) || (rc = ps.ensure(td->nh * 8 + 8)) || (rc = fb.ensure(td->nh * 4 + 4))) return rc;
+    k_level0_flags<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(T, L0, driver, fl.as<uint32_t>());
+    EHX_HIP(hipGetLastError());
+    if ((rc = scan_u32_to_u64(ctx, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh))) return rc;
+    k_compact_u32<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(td->head, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh,
+                                                                  fb.as<uint32_t>());
+    EHX_HIP(hipGetLastError());
+    if ((rc = copy_d2h(ctx, &n0, ps.as<uint64_t>() + td->nh, 8))) return rc;
+    f0 = fb.as<uint32_t>();
+  }
+  if (begin > n0) begin = n0;
+  if (end > n0) end = n0;
+  if (end < begin) end = begin;
+  f0 += begin;
+  n0 = end - begin;
+  S.frontier[0] = n0;
+  EHX_HIP(hipEventRecord(ctx->ev[1], st));
+
+  DevBuf &ac = ctx->scratch[S_ACC];
+  if ((rc = ac.ensure(8 * 8))) return rc;
+  if ((rc = fill_u64(ctx, ac.as<uint64_t>(), 0, 8))) return rc;
+
+  Frontier in;
+  memset(&in, 0, sizeof(in));
+  in.col[0] = f0;
+  uint64_t n = n0;
+  int64_t total = 0;
+  int form_used = EHX_FORM_PLAIN;
+  if (K == 1) {
+    total = (int64_t)n0;
+  } else {
+    // ---- middle levels (materialized) ------------------------------------------
+    int last_mat = K - 2;  // materialize levels 1..K-2
+    bool tuned = false;
+    TunedPlan tp;
+    if (form != EHX_FORM_PLAIN && tuned_applicable(p, &tp)) {
+      tuned = true;
+      last_mat = K - 3;  // the tuned kernel enumerates level K-2 itself
+    }
+    for (int k = 1; k <= last_mat; k++) {
+      LevelDesc L = level_desc(p, k);
+      DevBuf &cb = ctx->scratch[S_COUNTS], &ob = ctx->scratch[S_OFFS];
+      if ((rc = cb.ensure(n * 4 + 4)) || (rc = ob.ensure(n * 8 + 8))) return rc;
+      unsigned G = grid_for(n * 64, 256, 65536);
+      OutFrontier none;
+      memset(&none, 0, sizeof(none));
+      if (n) {
+        k_level_plain<MODE_COUNT><<<G, 256, 0, st>>>(T, L, in, k, n, cb.as<uint32_t>(), nullptr, none, nullptr);
+        EHX_HIP(hipGetLastError());
+      }
+      if ((rc = scan_u32_to_u64(ctx, cb.as<uint32_t>(), ob.as<uint64_t>(), n))) return rc;
+      uint64_t nn = 0;
+      if ((rc = copy_d2h(ctx, &nn, ob.as<uint64_t>() + n, 8))) return rc;
+      if (nn * 4ull * (k + 1) > FRONTIER_BUDGET)
+        return fail(EHX_ERR_UNSUPPORTED, "level %d frontier of %llu tuples exceeds the %llu-byte budget", k,
+                    (unsigned long long)nn, (unsigned long long)FRONTIER_BUDGET);
+      DevBuf &fb = ctx->scratch[S_F0 + k];
+      if ((rc = fb.ensure(nn * 4ull * (k + 1) + 4))) return rc;
+      OutFrontier out;
+      memset(&out, 0, sizeof(out));
+      for (int j = 0; j <= k; j++) out.col[j] = fb.as<uint32_t>() + (uint64_t)j * nn;
+      if (n && nn) {
+        k_level_plain<MODE_WRITE><<<G, 256, 0, st>>>(T, L, in, k, n, nullptr, ob.as<uint64_t>(), out, nullptr);
+        EHX_HIP(hipGetLastError());
+      }
+      for (int j = 0; j <= k; j++) in.col[j] = out.col[j];
+      n = nn;
+      S.frontier[k] = nn;
+      EHX_HIP(hipEventRecord(ctx->ev[1 + k], st));
+    }
+    // ---- last level -------------------------------------------------------------
+    // events: ev[1 + k] closes materialized level k; ev[K] closes the last level
+    if (tuned) {
+      TunedArgs ta;
+      ta.T = T;
+      for (int j = 0; j < EHX_MAX_ATTRS; j++) ta.prefix[j] = in.col[j];
+      ta.nprefix = n;
+      if ((rc = tuned_run(ctx, tp, ta, form, ac.as<unsigned long long>()))) return rc;
+      form_used = form;
+    } else {
+      LevelDesc L = level_desc(p, K - 1);
+      EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
+      if (n) {
+        unsigned G = grid_for(n * 64, 256, 65536);
+        OutFrontier none;
+        memset(&none, 0, sizeof(none));
+        k_level_plain<MODE_SUM><<<G, 256, 0, st>>>(T, L, in, K - 1, n, nullptr, nullptr, none,
+                                                   ac.as<unsigned long long>());
+        EHX_HIP(hipGetLastError());
+      }
+      EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));
+    }
+    EHX_HIP(hipEventRecord(ctx->ev[K], st));
+    unsigned long long accv[8];
+    if ((rc = copy_d2h(ctx, accv, ac.p, 64))) return rc;
+    total = (int64_t)accv[0];
+    for (int a = 0; a < 5; a++) S.pairs_by_alg[a] = accv[1 + a];
+    float ms = 0;
+    for (int k = 1; k <= last_mat; k++)
+      if (hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]) == hipSuccess) S.level_ms[k] = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev[last_mat + 1], ctx->ev[K]) == hipSuccess) S.level_ms[K - 1] = ms;
+    if (n && hipEventElapsedTime(&ms, ctx->ev[EHX_EV_K0], ctx->ev[EHX_EV_K1]) == hipSuccess) S.main_kernel_ms = ms;
+  }
+  EHX_HIP(hipEventRecord(ctx->ev[K + 1], st));
+  EHX_HIP(hipEventSynchronize(ctx->ev[K + 1]));
+  float ms = 0;
+  EHX_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  S.level_ms[0] = ms;
+  EHX_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[K + 1]));
+  S.total_ms = ms;
+  S.form_used = form_used;
+  *result = total;
+  if (stats) *stats = S;
+  return EHX_OK;
+}
+
+// ---- batched pair intersections ---------------------------------------------------
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_pairs(DTrie A, DTrie B, const uint32_t *__restrict__ ka,
+                                               const uint32_t *__restrict__ kb, uint64_t n,
+                                               unsigned long long *__restrict__ counts,
+                                               const unsigned long long *__restrict__ offs, uint32_t *__restrict__ out) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nwaves) {
+    int32_t ha = node_of(A, ka[t]), hb = node_of(B, kb[t]);
+    uint32_t c = 0;
+    int alg;
+    if (ha >= 0 && hb >= 0) {
+      DSet S[2] = {set_of(A, (uint32_t)ha), set_of(B, (uint32_t)hb)};
+      c = wave_intersect<EMIT>(S, 2, EMIT ? out + offs[t] : nullptr, &alg);
+    }
+    if (!EMIT && __lane_id() == 0) counts[t] = c;
+  }
+}
+
+int intersect_pairs(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, const ehx_trie *tb,
+                    const uint32_t *keys_b, uint64_t n, uint64_t *counts, uint64_t *offsets, uint32_t *values,
+                    uint64_t cap, uint64_t *total) {
+  if (!ctx || !ta || !tb || (n && (!keys_a || !keys_b))) return fail(EHX_ERR_INVALID, "null argument");
+  EHX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (n == 0) {
+    if (offsets) offsets[0] = 0;
+    if (total) *total = 0;
+    return EHX_OK;
+  }
+  DevBuf &a = ctx->scratch[S_IN0], &b = ctx->scratch[S_IN1], &c = ctx->scratch[S_COUNTS], &o = ctx->scratch[S_OFFS];
+  int rc;
+  if ((rc = a.ensure(n * 4)) || (rc = b.ensure(n * 4)) || (rc = c.ensure(n * 8 + 8)) || (rc = o.ensure(n * 8 + 8)))
+    return rc;
+  EHX_HIP(hipMemcpyAsync(a.p, keys_a, n * 4, hipMemcpyHostToDevice, st));
+  EHX_HIP(hipMemcpyAsync(b.p, keys_b, n * 4, hipMemcpyHostToDevice, st));
+  unsigned G = grid_for(n * 64, 256, 65536);
+  k_pairs<false><<<G, 256, 0, st>>>(view(ta), view(tb), a.as<uint32_t>(), b.as<uint32_t>(), n,
+                                    c.as<unsigned long long>(), nullptr, nullptr);
+  EHX_HIP(hipGetLastError());
+  if (counts) return copy_d2h(ctx, counts, c.p, n * 8);
+  if ((rc = scan_u64(ctx, c.as<uint64_t>(), o.as<uint64_t>(), n))) return rc;
+  uint64_t tot = 0;
+  if ((rc = copy_d2h(ctx, &tot, o.as<uint64_t>() + n, 8))) return rc;
+  if (total) *total = tot;
+  if (tot > cap) return fail(EHX_ERR_NOMEM, "intersection needs %llu values, capacity %llu", (unsigned long long)tot,
+                             (unsigned long long)cap);
+  DevBuf &v = ctx->scratch[S_MISC];
+  if ((rc = v.ensure(tot * 4 + 4))) return rc;
+  k_pairs<true><<<G, 256, 0, st>>>(view(ta), view(tb), a.as<uint32_t>(), b.as<uint32_t>(), n, nullptr,
+                                   o.as<unsigned long long>(), v.as<uint32_t>());
+  EHX_HIP(hipGetLastError());
+  if ((rc = copy_d2h(ctx, offsets, o.p, (n + 1) * 8))) return rc;
+  return copy_d2h(ctx, values, v.p, tot * 4);
+}
+
+}  // namespace ehx

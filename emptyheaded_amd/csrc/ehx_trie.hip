@@ -1,0 +1,366 @@
+// ehx_trie.hip -- device trie construction for one binary relation.
+//
+// Restates Trie<A,M>::Trie(path, max_set_sizes, attr_in, annotation)
+// (src/Trie.cpp:380-510) for HBM:
+//   1. pack each row as (c0 << b1) | c1 and radix sort (SortColumns order,
+//      Trie.cpp:173-186 / tbb::parallel_sort :402-404);
+//   2. drop repeated rows (encode_tail, :291-302) with flag + exclusive scan;
+//   3. split into head values and child offsets (produce_ranges, :192-224) with
+//      a second flag + scan;
+//   4. per-set layout (hybrid::get_type, hybrid.hpp:103-111, or a forced or
+//      learned policy), word counts, scan -> bitset pool offsets;
+//   5. fill range_bitset words and per-word ranks (range_bitset::build,
+//      range_bitset.hpp:123-157);
+//   6. dense value -> node array when the head is dense (TrieBlock::nextSize /
+//      get_next_block, TrieBlock.hpp:26-35,86-111).
+#include <stdio.h>
+#include <string.h>
+
+#include "ehx_internal.h"
+#include "ehx_selector.h"
+
+namespace ehx {
+
+__global__ void k_pack(const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1, uint64_t n, int b1,
+                       uint64_t *__restrict__ keys) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    keys[i] = ((uint64_t)c0[i] << b1) | (uint64_t)c1[i];
+}
+
+// flag[i] = 1 when keys[i] starts a new run of (keys >> shift).
+__global__ void k_run_flags(const uint64_t *__restrict__ keys, uint64_t n, int shift, uint32_t *__restrict__ flags) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    flags[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1u : 0u;
+}
+
+__global__ void k_compact_keys(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ flags,
+                               const uint64_t *__restrict__ pos, uint64_t n, uint64_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    if (flags[i]) out[pos[i]] = keys[i];
+}
+
+__global__ void k_split(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ hflags,
+                        const uint64_t *__restrict__ hpos, uint64_t m, int b1, uint32_t *__restrict__ head,
+                        uint32_t *__restrict__ off, uint32_t *__restrict__ vals, uint64_t nh) {
+  const uint64_t mask = (b1 >= 64) ? ~0ull : ((1ull << b1) - 1);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t k = keys[i];
+    vals[i] = (uint32_t)(k & mask);
+    if (hflags[i]) {
+      uint64_t h = hpos[i];
+      head[h] = (uint32_t)(k >> b1);
+      off[h] = (uint32_t)i;
+    }
+    if (i == 0) off[nh] = (uint32_t)m;
+  }
+}
+
+// Layout and bitset word count per set; set nh is the head set.
+__global__ void k_layouts(DTrie t, int policy, SelectorWeights sel, uint8_t *__restrict__ layout,
+                          uint32_t *__restrict__ bs_start, uint32_t *__restrict__ wcount) {
+  const uint32_t nh = t.nh;
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h <= nh; h += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t *v;
+    uint32_t n;
+    if (h == nh) { v = t.head; n = nh; }
+    else { v = t.vals + t.off[h]; n = t.off[h + 1] - t.off[h]; }
+    uint32_t first = n ? v[0] : 0, last = n ? v[n - 1] : 0;
+    uint64_t span = (n > 1) ? (uint64_t)(last - first) : 0;
+    int lay;
+    if (n == 0) lay = EHX_UINTEGER;
+    else if (policy == EHX_POLICY_ALL_UINT) lay = EHX_UINTEGER;
+    else if (policy == EHX_POLICY_ALL_BITSET) lay = EHX_RANGE_BITSET;
+    else {
+      // common::is_sparse (common.hpp:56-67): sparse iff len <= 2 or span/len > 128;
+      // for 32-bit values the double test equals span > 128*len.
+      lay = (n <= 2 || span > 128ull * n) ? EHX_UINTEGER : EHX_RANGE_BITSET;
+      if (policy == EHX_POLICY_LEARNED && sel.valid) lay = selector_layout(sel, n, span, first, last, lay);
+    }
+    layout[h] = (uint8_t)lay;
+    bs_start[h] = first >> 6;
+    wcount[h] = (lay == EHX_RANGE_BITSET && n) ? ((last >> 6) - (first >> 6) + 1) : 0;
+  }
+}
+
+// One wave per set: OR the bits of a bitset set into its words.
+__global__ __launch_bounds__(256) void k_fill_bits(DTrie t, uint64_t *__restrict__ words) {
+  const uint32_t nh = t.nh;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t h = wave; h <= nh; h += nwaves) {
+    if (t.layout[h] != EHX_RANGE_BITSET) continue;
+    DSet s = set_of(t, (uint32_t)h);
+    uint64_t *w = words + t.bs_off[h];
+    for (uint32_t i = __lane_id(); i < s.n; i += 64) {
+      uint32_t x = s.v[i];
+      atomicOr((unsigned long long *)&w[(x >> 6) - s.start], (unsigned long long)(1ull << (x & 63)));
+    }
+  }
+}
+
+// One wave per bitset set: rank[w] = elements in the set's words before w.
+__global__ __launch_bounds__(256) void k_fill_rank(DTrie t, const uint64_t *__restrict__ words,
+                                                   uint32_t *__restrict__ rank) {
+  const uint32_t nh = t.nh;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t h = wave; h <= nh; h += nwaves) {
+    if (t.layout[h] != EHX_RANGE_BITSET) continue;
+    uint32_t b = t.bs_off[h], W = t.bs_off[h + 1] - b;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < W; base += 64) {
+      uint32_t i = base + __lane_id();
+      uint32_t c = (i < W) ? (uint32_t)__popcll(words[b + i]) : 0u;
+      uint32_t incl = wave_incl_scan(c);
+      if (i < W) rank[b + i] = carry + incl - c;
+      carry += __shfl(incl, 63, 64);
+    }
+  }
+}
+
+__global__ void k_dense(const uint32_t *__restrict__ head, uint64_t nh, int32_t *__restrict__ dense) {
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += (uint64_t)gridDim.x * blockDim.x)
+    dense[head[h]] = (int32_t)h;
+}
+
+// Per-set statistics for ehx_trie_info.
+__global__ void k_stats(DTrie t, unsigned long long *__restrict__ acc) {
+  unsigned long long nb = 0, el = 0;
+  uint32_t mx = 0;
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < t.nh; h += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t n = t.off[h + 1] - t.off[h];
+    mx = max(mx, n);
+    if (t.layout[h] == EHX_RANGE_BITSET) { nb++; el += n; }
+  }
+  nb = wave_sum(nb);
+  el = wave_sum(el);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+  if (__lane_id() == 0) {
+    atomicAdd(&acc[0], nb);
+    atomicAdd(&acc[1], el);
+    atomicMax(&acc[2], (unsigned long long)mx);
+  }
+}
+
+static int bits_of(uint32_t v) {
+  int b = 0;
+  while (b < 32 && (v >> b)) b++;
+  return b;
+}
+
+static void free_trie_arrays(ehx_trie *t) {
+  if (t->block) (void)hipFree(t->block);
+  t->block = nullptr;
+}
+
+int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t rows, int policy, int on_device,
+               ehx_trie **out) {
+  if (!ctx || !out) return fail(EHX_ERR_INVALID, "ehx_trie_build: null argument");
+  if (rows == 0) return fail(EHX_ERR_INVALID, "ehx_trie_build: empty relation (Trie.cpp:397 asserts num_rows != 0)");
+  if (rows >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "ehx_trie_build: at most 2^32-1 rows");
+  if (policy < EHX_POLICY_RULE || policy > EHX_POLICY_LEARNED) return fail(EHX_ERR_INVALID, "bad layout policy %d", policy);
+  EHX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  EHX_HIP(hipEventRecord(ctx->ev[0], st));
+  const uint32_t *d0 = c0, *d1 = c1;
+  int rc;
+  if (!on_device) {
+    DevBuf &i0 = ctx->scratch[S_IN0], &i1 = ctx->scratch[S_IN1];
+    if ((rc = i0.ensure(rows * 4)) || (rc = i1.ensure(rows * 4))) return rc;
+    EHX_HIP(hipMemcpyAsync(i0.p, c0, rows * 4, hipMemcpyHostToDevice, st));
+    EHX_HIP(hipMemcpyAsync(i1.p, c1, rows * 4, hipMemcpyHostToDevice, st));
+    d0 = i0.as<uint32_t>();
+    d1 = i1.as<uint32_t>();
+  }
+  uint32_t max0 = 0, max1 = 0;
+  if ((rc = reduce_max_u32(ctx, d0, rows, &max0)) || (rc = reduce_max_u32(ctx, d1, rows, &max1))) return rc;
+  const int b0 = bits_of(max0), b1 = bits_of(max1);
+  DevBuf &ka = ctx->scratch[S_SORT_A], &kb = ctx->scratch[S_SORT_B];
+  DevBuf &fl = ctx->scratch[S_FLAGS], &ps = ctx->scratch[S_POS];
+  if ((rc = ka.ensure(rows * 8)) || (rc = kb.ensure(rows * 8)) || (rc = fl.ensure(rows * 4)) ||
+      (rc = ps.ensure((rows + 1) * 8)))
+    return rc;
+  uint64_t *keys = ka.as<uint64_t>(), *tmp = kb.as<uint64_t>();
+  uint32_t *flags = fl.as<uint32_t>();
+  uint64_t *pos = ps.as<uint64_t>();
+  const unsigned G = grid_for(rows, 256, 16384);
+  k_pack<<<G, 256, 0, st>>>(d0, d1, rows, b1, keys);
+  EHX_HIP(hipGetLastError());
+  if ((rc = radix_sort_u64(ctx, keys, tmp, rows, b0 + b1))) return rc;
+  // unique rows
+  k_run_flags<<<G, 256, 0, st>>>(keys, rows, 0, flags);
+  EHX_HIP(hipGetLastError());
+  if ((rc = scan_u32_to_u64(ctx, flags, pos, rows))) return rc;
+  uint64_t m = 0;
+  if ((rc = copy_d2h(ctx, &m, pos + rows, 8))) return rc;
+  k_compact_keys<<<G, 256, 0, st>>>(keys, flags, pos, rows, tmp);
+  EHX_HIP(hipGetLastError());
+  // head runs
+  k_run_flags<<<G, 256, 0, st>>>(tmp, m, b1, flags);
+  EHX_HIP(hipGetLastError());
+  if ((rc = scan_u32_to_u64(ctx, flags, pos, m))) return rc;
+  uint64_t nh = 0;
+  if ((rc = copy_d2h(ctx, &nh, pos + m, 8))) return rc;
+
+  ehx_trie *t = new ehx_trie();
+  t->device = ctx->device;
+  t->m = m;
+  t->nh = nh;
+  // dense navigation array when it costs at most ~64 entries per head plus 4 MiB
+  uint32_t maxhead = 0;
+  // the last head value is the largest (sorted); fetched after the split
+  // sizes: head nh, off nh+1, vals m, layout nh+1, bs_off nh+2, bs_start nh+1
+  size_t o_head = 0;
+  size_t o_off = o_head + ((nh * 4 + 255) & ~255ull);
+  size_t o_vals = o_off + (((nh + 1) * 4 + 255) & ~255ull);
+  size_t o_lay = o_vals + ((m * 4 + 255) & ~255ull);
+  size_t o_bsoff = o_lay + (((nh + 1) + 255) & ~255ull);
+  size_t o_bsst = o_bsoff + (((nh + 2) * 4 + 255) & ~255ull);
+  size_t total = o_bsst + (((nh + 1) * 4 + 255) & ~255ull);
+  hipError_t e = hipMalloc(&t->block, total);
+  if (e != hipSuccess) {
+    delete t;
+    return fail(EHX_ERR_NOMEM, "trie arrays (%zu bytes): %s", total, hipGetErrorString(e));
+  }
+  uint8_t *B = (uint8_t *)t->block;
+  t->head = (uint32_t *)(B + o_head);
+  t->off = (uint32_t *)(B + o_off);
+  t->vals = (uint32_t *)(B + o_vals);
+  t->layout = B + o_lay;
+  t->bs_off = (uint32_t *)(B + o_bsoff);
+  t->bs_start = (uint32_t *)(B + o_bsst);
+  k_split<<<grid_for(m, 256, 16384), 256, 0, st>>>(tmp, flags, pos, m, b1, t->head, t->off, t->vals, nh);
+  e = hipGetLastError();
+  if (e != hipSuccess) { free_trie_arrays(t); delete t; return fail(EHX_ERR_DEVICE, "split: %s", hipGetErrorString(e)); }
+  if ((rc = copy_d2h(ctx, &maxhead, t->head + (nh - 1), 4))) { free_trie_arrays(t); delete t; return rc; }
+
+  // layouts, bitset pool
+  DevBuf &wc = ctx->scratch[S_COUNTS];
+  if ((rc = wc.ensure((nh + 2) * 4))) { free_trie_arrays(t); delete t; return rc; }
+  SelectorWeights sel = selector_current();
+  DTrie dv = view(t);
+  dv.layout = t->layout;
+  k_layouts<<<grid_for(nh + 1, 256, 16384), 256, 0, st>>>(dv, policy, sel, t->layout, t->bs_start,
+                                                          wc.as<uint32_t>());
+  if ((rc = scan_u32(ctx, wc.as<uint32_t>(), t->bs_off, nh + 1))) { free_trie_arrays(t); delete t; return rc; }
+  uint32_t W = 0;
+  if ((rc = copy_d2h(ctx, &W, t->bs_off + (nh + 1), 4))) { free_trie_arrays(t); delete t; return rc; }
+  uint64_t dense_len = (uint64_t)maxhead + 1;
+  bool dense = dense_len <= 64ull * nh + (1ull << 20);
+  size_t wbytes = ((uint64_t)W * 8 + 255) & ~255ull, rbytes = ((uint64_t)W * 4 + 255) & ~255ull;
+  size_t dbytes = dense ? ((dense_len * 4 + 255) & ~255ull) : 0;
+  void *blk2 = nullptr;
+  e = hipMalloc(&blk2, wbytes + rbytes + dbytes + 256);
+  if (e != hipSuccess) { free_trie_arrays(t); delete t; return fail(EHX_ERR_NOMEM, "bitset pool: %s", hipGetErrorString(e)); }
+  // Keep the second block alive by chaining it: store its pointer in words.
+  t->words = (uint64_t *)blk2;
+  t->rank = (uint32_t *)((uint8_t *)blk2 + wbytes);
+  t->dense = dense ? (int32_t *)((uint8_t *)blk2 + wbytes + rbytes) : nullptr;
+  t->dense_len = dense ? (uint32_t)dense_len : 0;
+  if ((rc = fill_u64(ctx, t->words, 0, W))) return rc;
+  dv = view(t);
+  k_fill_bits<<<grid_for((nh + 1) * 64, 256, 16384), 256, 0, st>>>(dv, t->words);
+  EHX_HIP(hipGetLastError());
+  k_fill_rank<<<grid_for((nh + 1) * 64, 256, 16384), 256, 0, st>>>(dv, t->words, t->rank);
+  EHX_HIP(hipGetLastError());
+  if (dense) {
+    if ((rc = fill_u32(ctx, (uint32_t *)t->dense, 0xFFFFFFFFu, dense_len))) return rc;
+    k_dense<<<grid_for(nh, 256, 16384), 256, 0, st>>>(t->head, nh, t->dense);
+    EHX_HIP(hipGetLastError());
+  }
+  // stats
+  DevBuf &ac = ctx->scratch[S_ACC];
+  if ((rc = ac.ensure(64))) return rc;
+  if ((rc = fill_u64(ctx, ac.as<uint64_t>(), 0, 4))) return rc;
+  k_stats<<<grid_for(nh, 256, 4096), 256, 0, st>>>(dv, ac.as<unsigned long long>());
+  EHX_HIP(hipGetLastError());
+  EHX_HIP(hipEventRecord(ctx->ev[1], st));
+  unsigned long long acc[4] = {0, 0, 0, 0};
+  if ((rc = copy_d2h(ctx, acc, ac.p, 32))) return rc;
+  uint8_t hl = 0;
+  if ((rc = copy_d2h(ctx, &hl, t->layout + nh, 1))) return rc;
+  float ms = 0;
+  EHX_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  ehx_trie_meta &I = t->info;
+  I.num_rows = m;
+  I.num_heads = nh;
+  I.num_bitsets = acc[0];
+  I.bitset_elems = acc[1];
+  I.max_card = (uint32_t)acc[2];
+  I.bitset_words = W;
+  I.head_layout = hl;
+  I.dense_nav = dense ? 1u : 0u;
+  I.max_value = max0 > max1 ? max0 : max1;
+  I.build_ms = ms;
+  *out = t;
+  return EHX_OK;
+}
+
+int trie_destroy(ehx_trie *t) {
+  if (!t) return EHX_OK;
+  EHX_HIP(hipSetDevice(t->device));
+  if (t->block) EHX_HIP(hipFree(t->block));
+  if (t->words) EHX_HIP(hipFree(t->words));
+  delete t;
+  return EHX_OK;
+}
+
+__global__ void k_export(DTrie t, uint32_t *__restrict__ c0) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < t.nh; h += nwaves) {
+    uint32_t a = t.off[h], b = t.off[h + 1], x = t.head[h];
+    for (uint32_t i = a + __lane_id(); i < b; i += 64) c0[i] = x;
+  }
+}
+
+int trie_export(ehx_ctx *ctx, const ehx_trie *t, uint32_t *c0, uint32_t *c1) {
+  EHX_HIP(hipSetDevice(ctx->device));
+  DevBuf &b = ctx->scratch[S_IN0];
+  int rc = b.ensure(t->m * 4 + 4);
+  if (rc) return rc;
+  k_export<<<grid_for(t->nh * 64, 256, 16384), 256, 0, ctx->stream>>>(view(t), b.as<uint32_t>());
+  EHX_HIP(hipGetLastError());
+  if ((rc = copy_d2h(ctx, c0, b.p, t->m * 4))) return rc;
+  return copy_d2h(ctx, c1, t->vals, t->m * 4);
+}
+
+__global__ void k_node_info(DTrie t, uint8_t *__restrict__ types, uint32_t *__restrict__ cards) {
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < t.nh; h += (uint64_t)gridDim.x * blockDim.x) {
+    types[h] = t.layout[h];
+    cards[h] = t.off[h + 1] - t.off[h];
+  }
+}
+
+int trie_node_info(ehx_ctx *ctx, const ehx_trie *t, uint8_t *types, uint32_t *cards) {
+  EHX_HIP(hipSetDevice(ctx->device));
+  DevBuf &b = ctx->scratch[S_IN0];
+  int rc = b.ensure(t->nh * 5 + 64);
+  if (rc) return rc;
+  uint32_t *dc = b.as<uint32_t>();
+  uint8_t *dt = (uint8_t *)(dc + t->nh);
+  k_node_info<<<grid_for(t->nh, 256, 16384), 256, 0, ctx->stream>>>(view(t), dt, dc);
+  EHX_HIP(hipGetLastError());
+  if (types && (rc = copy_d2h(ctx, types, dt, t->nh))) return rc;
+  if (cards && (rc = copy_d2h(ctx, cards, dc, t->nh * 4))) return rc;
+  return EHX_OK;
+}
+
+__global__ void k_lookup(DTrie t, const uint32_t *__restrict__ keys, uint64_t n, int64_t *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = node_of(t, keys[i]);
+}
+
+int trie_lookup(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *keys, uint64_t n, int64_t *nodes) {
+  EHX_HIP(hipSetDevice(ctx->device));
+  if (!n) return EHX_OK;
+  DevBuf &a = ctx->scratch[S_IN0], &b = ctx->scratch[S_IN1];
+  int rc;
+  if ((rc = a.ensure(n * 4)) || (rc = b.ensure(n * 8))) return rc;
+  if ((rc = copy_h2d(ctx, a.p, keys, n * 4))) return rc;
+  k_lookup<<<grid_for(n, 256, 16384), 256, 0, ctx->stream>>>(view(t), a.as<uint32_t>(), n, b.as<int64_t>());
+  EHX_HIP(hipGetLastError());
+  return copy_d2h(ctx, nodes, b.p, n * 8);
+}
+
+}  // namespace ehx

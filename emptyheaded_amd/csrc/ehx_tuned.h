@@ -1,0 +1,48 @@
+// ehx_tuned.h -- the batched last level of a clique-shaped plan.
+//
+// Applies when the last level's operands are one child set of x_{K-2} (in
+// relation x_rel) plus child sets S_1..S_s of earlier attributes, and level K-2
+// intersects exactly those S_i plus head-set filters.  Then for each prefix p
+// (a tuple of levels 0..K-3)
+//     count(p) = sum over x in S(p) ∩ heads of |child(x_rel, x) ∩ S(p)|,
+// with S(p) = S_1 ∩ .. ∩ S_s.  Triangle (S = N(a)) and 4-clique
+// (S = N(a) ∩ N(b)) have this shape.  One persistent kernel drains a queue of
+// prefixes binned by estimated work (largest first); each wavefront stages S(p)
+// in LDS once -- as a bitmap when S(p) is a bitset that fits, as an open-
+// addressing hash table otherwise -- and streams every child(x) through it.
+#pragma once
+
+#include "ehx_join_common.h"
+
+namespace ehx {
+
+struct TunedPlan {
+  int32_t K;
+  int32_t ns;
+  int32_t s_rel[EHX_MAX_OPS], s_parent[EHX_MAX_OPS];
+  int32_t nhf;
+  int32_t h_rel[EHX_MAX_OPS];
+  int32_t x_rel;
+};
+
+struct TunedArgs {
+  Tries T;
+  const uint32_t *prefix[EHX_MAX_ATTRS];
+  uint64_t nprefix;
+};
+
+// Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.
+struct TuneParams {
+  int32_t lds_slots;      // 32-bit LDS slots per wavefront (2048, 4096 or 8192)
+  int32_t big_deg;        // child sets at least this long are streamed by the whole wave
+  int32_t waves_per_cu;   // persistent grid size
+  int32_t grab;           // queue items taken per atomic
+  int32_t bitmap_all;     // 1: stage any set whose word window fits as a bitmap
+  int32_t use_graph;      // capture the level in a hipGraph and replay it
+};
+
+bool tuned_applicable(const ehx_plan *p, TunedPlan *tp);
+int tuned_run(ehx_ctx *ctx, const TunedPlan &tp, const TunedArgs &a, int form, unsigned long long *acc);
+TuneParams &tune_params();
+
+}  // namespace ehx

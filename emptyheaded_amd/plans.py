@@ -1,0 +1,53 @@
+"""Join plans as the EmptyHeaded code generator emits them.
+
+A plan lists, per attribute in attribute order, the operand sets intersected
+at that level: (rel, parent) with parent = -1 for the head set of relation
+`rel` and parent = j for the child set of attribute j's value.  The operand
+order follows the accessor order of the generated code
+(query_compiler/src/main/scala/codegen/query/EHGenerator.scala:273-838); the
+attribute order is the sorted attribute names (parsers/Datalog.scala:150-188).
+"""
+
+# TriangleAgg(;z) :- Edge(a,b),Edge(b,c),Edge(a,c),z:long<-[COUNT(*)].
+# Iterators Edge_a_b, Edge_b_c, Edge_a_c all read trie Edge_0_1 (relation 0):
+#   a: head(Edge_a_b) ∩ head(Edge_a_c)
+#   b: child(Edge_a_b, a) ∩ head(Edge_b_c)
+#   c: child(Edge_b_c, b) ∩ child(Edge_a_c, a)
+TRIANGLE = [[(0, -1), (0, -1)],
+            [(0, 0), (0, -1)],
+            [(0, 1), (0, 0)]]
+
+# FliqueAgg(;z) :- Edge(a,b),Edge(b,c),Edge(a,c),Edge(a,d),Edge(b,d),Edge(c,d),COUNT(*).
+#   a: heads of Edge_a_b, Edge_a_c, Edge_a_d
+#   b: child(Edge_a_b, a) ∩ head(Edge_b_c) ∩ head(Edge_b_d)
+#   c: child(Edge_b_c, b) ∩ child(Edge_a_c, a) ∩ head(Edge_c_d)
+#   d: child(Edge_a_d, a) ∩ child(Edge_b_d, b) ∩ child(Edge_c_d, c)
+FOUR_CLIQUE = [[(0, -1), (0, -1), (0, -1)],
+               [(0, 0), (0, -1), (0, -1)],
+               [(0, 1), (0, 0), (0, -1)],
+               [(0, 0), (0, 1), (0, 2)]]
+
+# Closed 4-walks over a symmetric relation S (both edge directions):
+#   Q(;z) :- S(a,b),S(b,c),S(c,d),S(a,d),COUNT(*)
+# counts homomorphic images of the 4-cycle; the number of 4-cycles of the
+# undirected graph is (Q - 2*sum(deg^2) + 2*m) / 8 (four_cycles below).
+FOUR_CYCLE_WALKS = [[(0, -1), (0, -1)],
+                    [(0, 0), (0, -1)],
+                    [(0, 1), (0, -1)],
+                    [(0, 2), (0, 0)]]
+
+# 2-hop path through two relations R(x,y), T(y,z):  Q(;z) :- R(x,y),T(y,z),COUNT(*)
+# with relation 0 = R keyed by x and relation 1 = T keyed by y.
+TWO_HOP = [[(0, -1)],
+           [(0, 0), (1, -1)],
+           [(1, 1)]]
+
+
+def four_cycles(closed_walks, degrees):
+    """Number of undirected 4-cycles from the closed 4-walk count."""
+    deg = [int(d) for d in degrees]
+    m2 = sum(deg)  # 2m
+    s2 = sum(d * d for d in deg)
+    v = closed_walks - 2 * s2 + m2
+    assert v % 8 == 0, "closed-walk count inconsistent with degrees"
+    return v // 8

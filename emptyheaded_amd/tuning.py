@@ -1,0 +1,24 @@
+"""Committed tuning state: the autotuner's chosen knobs (tuning.json) and the
+learned layout selector's weights (selector.json), both produced on an MI355X
+by emptyheaded_amd.autotune and emptyheaded_amd.selector."""
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_JSON = os.path.join(HERE, "tuning.json")
+SELECTOR_JSON = os.path.join(HERE, "selector.json")
+
+
+def apply_committed(device=0):
+    import emptyheaded_amd as E
+    applied = {"params": None, "selector": None}
+    if os.path.exists(TUNING_JSON):
+        with open(TUNING_JSON) as f:
+            t = json.load(f)
+        applied["params"] = E.tune_set(**t["best"])
+    else:
+        applied["params"] = E.tune_get()
+    if os.path.exists(SELECTOR_JSON):
+        from . import selector
+        applied["selector"] = selector.load_committed(device)
+    return applied

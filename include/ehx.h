@@ -1,0 +1,182 @@
+/*
+ * ehx.h -- C ABI of the MI355X generic-join engine (libehx.so).
+ *
+ * This is the drop-in boundary beneath EmptyHeaded's templated storage-engine
+ * API: plain pointers and sizes, integer status codes, no C++ or torch types.
+ * The C++ mirror of the reference API (emptyheaded_amd/engine/include) and the
+ * Python binding (emptyheaded_amd/__init__.py) both sit on top of it.
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference tree).
+ *
+ * Conventions: every function returns 0 (EHX_OK) on success or a negative
+ * EHX_ERR_* code; ehx_last_error() returns a message for the calling thread.
+ * Device memory is owned by ehx objects and released by the matching
+ * ehx_*_destroy call.  Results are exact integers.
+ */
+#ifndef EHX_H
+#define EHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EHX_OK 0
+#define EHX_ERR_INVALID (-1)     /* bad argument or plan */
+#define EHX_ERR_DEVICE (-2)      /* HIP runtime error or no GPU */
+#define EHX_ERR_NOMEM (-3)       /* allocation failed */
+#define EHX_ERR_UNSUPPORTED (-4) /* shape outside documented limits */
+
+/* Set layouts: type::layout values, storage_engine/include/utils/common.hpp:77-85. */
+#define EHX_RANGE_BITSET 0
+#define EHX_UINTEGER 1
+
+/* Layout policies for trie build.  RULE is the reference density rule
+ * (common.hpp:56-67, hybrid.hpp:103-111); ALL_UINT / ALL_BITSET force a layout
+ * (layout never changes results); LEARNED uses the selector loaded with
+ * ehx_selector_load and falls back to RULE when none is loaded. */
+#define EHX_POLICY_RULE 0
+#define EHX_POLICY_ALL_UINT 1
+#define EHX_POLICY_ALL_BITSET 2
+#define EHX_POLICY_LEARNED 3
+
+/* Executor forms. */
+#define EHX_FORM_PLAIN 0   /* one wavefront per prefix tuple and level */
+#define EHX_FORM_TUNED 1   /* degree-binned persistent last level, LDS-staged */
+#define EHX_FORM_LEARNED 2 /* tuned, with per-prefix staging chosen by the selector */
+
+#define EHX_MAX_ATTRS 8
+#define EHX_MAX_OPS 8
+
+typedef struct ehx_ctx ehx_ctx;
+typedef struct ehx_trie ehx_trie;
+
+/* One operand of a join level: parent < 0 is the head set of relation `rel`
+ * (ParTrieIterator::head / TrieIterator::get_block(0)); parent = j is the child
+ * set of attribute x_j in `rel` (TrieIterator::get_next_block(0, x_j) then
+ * get_block(1), TrieIterator.cpp:23-42). */
+typedef struct {
+  int32_t rel;
+  int32_t parent;
+} ehx_op;
+
+/* A single-bag COUNT(*) plan over binary relations, in attribute order: the
+ * information the code generator emits as nested build_aggregated_set calls
+ * (query_compiler/.../EHGenerator.scala:273-838). */
+typedef struct {
+  int32_t num_attrs;
+  int32_t num_ops[EHX_MAX_ATTRS];
+  ehx_op ops[EHX_MAX_ATTRS][EHX_MAX_OPS];
+} ehx_plan;
+
+typedef struct {
+  uint64_t num_rows;      /* distinct tuples (Trie::num_rows) */
+  uint64_t num_heads;     /* distinct first-column values */
+  uint64_t num_bitsets;   /* child sets stored as range_bitset */
+  uint64_t bitset_words;  /* 64-bit words in the bitset pool */
+  uint64_t bitset_elems;  /* elements held by bitset child sets */
+  uint32_t max_card;      /* largest child set */
+  uint32_t head_layout;   /* EHX_UINTEGER or EHX_RANGE_BITSET */
+  uint32_t dense_nav;     /* 1 if child lookup is a dense array (NextLevel[data]) */
+  uint32_t max_value;     /* largest value in either column */
+  double build_ms;        /* device time of the build */
+} ehx_trie_meta;
+
+typedef struct {
+  double level_ms[EHX_MAX_ATTRS];        /* device time per level */
+  uint64_t frontier[EHX_MAX_ATTRS];      /* tuples materialized per level */
+  uint64_t pairs_by_alg[5];              /* last level: merge, gallop, probe, and, empty */
+  double total_ms;                       /* device time of the whole query */
+  double main_kernel_ms;                 /* device time of the last level's main launch */
+  int32_t form_used;                     /* form that ran the last level */
+  int32_t pad;
+} ehx_join_stats;
+
+/* ---- library / context -------------------------------------------------- */
+int ehx_version(void);
+const char *ehx_last_error(void);
+/* Number of visible GPUs (0 on a machine without one). */
+int ehx_device_count(int *count);
+/* A context owns a HIP stream and grow-only scratch on one GPU.  Replaces the
+ * reference's thread_pool::initializeThreadPool (src/utils/thread_pool.cpp:116-130)
+ * as the per-process execution resource. */
+int ehx_ctx_create(int device, ehx_ctx **out);
+int ehx_ctx_destroy(ehx_ctx *ctx);
+int ehx_ctx_synchronize(ehx_ctx *ctx);
+
+/* ---- tries --------------------------------------------------------------- */
+/* Build a two-level trie from a binary relation (columns c0, c1; rows may
+ * repeat, duplicates are removed).  Replaces Trie<A,M>::Trie(path, max_set_sizes,
+ * attr_in, annotation) (src/Trie.cpp:380-510): device radix sort of the rows,
+ * unique, segmented scans into per-level value/offset arrays, per-set layout by
+ * `policy`, range_bitset pool.  c0/c1 are host pointers, or device pointers when
+ * on_device != 0. */
+int ehx_trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t rows,
+                   int policy, int on_device, ehx_trie **out);
+int ehx_trie_destroy(ehx_trie *t);
+int ehx_trie_info(const ehx_trie *t, ehx_trie_meta *out);
+/* Trie<A,M>::foreach (src/Trie.cpp:130-160): the distinct tuples in
+ * lexicographic order, copied to host arrays of num_rows entries. */
+int ehx_trie_export(ehx_ctx *ctx, const ehx_trie *t, uint32_t *c0, uint32_t *c1);
+/* Per head node: layout of its child set and its cardinality (Set::type and
+ * Set::cardinality, storage_engine/include/trie/set/Set.hpp:19-24). */
+int ehx_trie_node_info(ehx_ctx *ctx, const ehx_trie *t, uint8_t *types, uint32_t *cards);
+/* Child lookup for a batch of keys (TrieBlock::get_next_block(data),
+ * storage_engine/include/trie/TrieBlock.hpp:86-111): node index or -1. */
+int ehx_trie_lookup(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *keys, uint64_t n,
+                    int64_t *nodes);
+/* Raw device views (for C++ callers that launch their own work). */
+int ehx_trie_device_arrays(const ehx_trie *t, const uint32_t **head_vals, const uint32_t **offsets,
+                           const uint32_t **values);
+
+/* ---- set intersections ---------------------------------------------------- */
+/* |child(ta, keys_a[i]) ∩ child(tb, keys_b[i])| for i < n: the count-only
+ * ops::set_intersect (storage_engine/include/trie/set/ops/intersection/set.hpp:
+ * 131-167) with the reference's per-pair algorithm (merge, gallop when
+ * max/min > 32, uint x bitset probe, bitset AND+popcount).  Host arrays. */
+int ehx_intersect_count_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a,
+                              const ehx_trie *tb, const uint32_t *keys_b, uint64_t n,
+                              uint64_t *counts);
+/* Materializing batch: the sorted intersections, concatenated into `values`
+ * (capacity `cap`), with offsets[n+1]; ops::set_intersect(C,A,B) (set.hpp:53-91).
+ * *total receives the number of values even when it exceeds cap (then nothing
+ * is written and EHX_ERR_NOMEM is returned). */
+int ehx_intersect_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a,
+                        const ehx_trie *tb, const uint32_t *keys_b, uint64_t n,
+                        uint64_t *offsets, uint32_t *values, uint64_t cap, uint64_t *total);
+
+/* ---- generic join --------------------------------------------------------- */
+/* COUNT(*) of a single-bag plan: what run_<id> computes for an aggregated bag
+ * (EHGenerator.scala:134-153 with ParTrieBuilder/TrieBuilder::build_aggregated_set,
+ * src/TrieBuilder.cpp:242-348,478-578, folded by par::reducer,
+ * storage_engine/include/utils/parallel.hpp:8-34).  tries[r] backs relation r.
+ * stats may be NULL. */
+int ehx_join_count(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form,
+                   int64_t *result, ehx_join_stats *stats);
+/* Same, restricted to level-0 values with index in [begin, end). */
+int ehx_join_count_range(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form,
+                         uint64_t begin, uint64_t end, int64_t *result, ehx_join_stats *stats);
+
+/* ---- synthetic inputs (host) ---------------------------------------------- */
+/* Counter-based SplitMix64 generators: identical output for identical
+ * arguments on any machine. */
+int ehx_gen_er(uint64_t num_vertices, uint64_t samples, uint64_t seed, uint32_t *src,
+               uint32_t *dst);
+int ehx_gen_rmat(int scale, uint64_t samples, double a, double b, double c, uint64_t seed,
+                 uint32_t *src, uint32_t *dst);
+/* Undirected clean-up: drop self loops, symmetrize, de-duplicate; relabel
+ * (0 keep ids, 1 ascending-degree rank with ties by id and isolated vertices
+ * dropped, 2 rank among used ids as dictionary encoding does, src/Encoding.cpp:
+ * 21-48); then orient (0 both directions, 1 src < dst, 2 src > dst).  Output
+ * arrays are allocated by the library (free with ehx_free). */
+int ehx_graph_prepare(const uint32_t *src, const uint32_t *dst, uint64_t n, int relabel,
+                      int orient, uint32_t **out_src, uint32_t **out_dst, uint64_t *out_m,
+                      uint32_t *out_num_vertices);
+void ehx_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+/*
+ * ehx_oracle.h -- CPU restatement of EmptyHeaded's storage-engine hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine in
+ * emptyheaded_amd/: only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it.  The product path never links or calls it.
+ *
+ * It restates, in plain C, the reference algorithms cited next to each entry
+ * point (paths relative to the reference tree's storage_engine/):
+ *   - layout rule            include/utils/common.hpp:56-67, layouts/hybrid.hpp:103-138
+ *   - Set header             include/trie/set/Set.hpp:19-24,164-172
+ *   - uinteger layout        layouts/uinteger.hpp:100-110,272-281; utils/utils.hpp:22-43
+ *   - range_bitset layout    layouts/range_bitset.hpp:95-157,443-470; ops/sse_masks.hpp:12-21
+ *   - set_intersect dispatch ops/intersection/set.hpp:53-167
+ *   - uint x uint            ops/intersection/uinteger.hpp:1209-1261 (merge vs gallop rule)
+ *   - uint x bitset          ops/intersection/hetero.hpp:29-82
+ *   - bitset x bitset        ops/intersection/bitset.hpp:526-567,619-666
+ *   - trie build             src/Trie.cpp:173-302,380-510
+ *   - child lookup           include/trie/TrieBlock.hpp:26-35,86-111
+ *   - N-way builder          src/TrieBuilder.cpp:242-348,478-578
+ *   - dynamic scheduler      src/utils/parallel.cpp:42-79 (chunks of 128)
+ *   - COUNT fold             query_compiler/.../EHGenerator.scala:399-520,616-641
+ *
+ * Every result on this path is an exact integer.
+ */
+#ifndef EHX_ORACLE_H
+#define EHX_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* type::layout values (include/utils/common.hpp:77-85). */
+enum { ORC_RANGE_BITSET = 0, ORC_UINTEGER = 1 };
+
+/* Layout policies for trie build: the density rule, or forced layouts used by
+ * the layout-independence checks. */
+enum { ORC_POLICY_RULE = 0, ORC_POLICY_ALL_UINT = 1, ORC_POLICY_ALL_BITSET = 2 };
+
+typedef struct {
+  uint32_t cardinality;
+  uint32_t range;            /* Set.range: max element (Set.hpp:166) */
+  uint64_t number_of_bytes;  /* bytes of layout data */
+  uint8_t type;              /* ORC_UINTEGER or ORC_RANGE_BITSET */
+  const uint8_t *data;       /* layout bytes in the reference's format */
+} orc_set;
+
+int orc_is_sparse(uint64_t length, uint64_t range);
+int orc_layout_of(const uint32_t *v, uint64_t n);
+uint64_t orc_bytes_needed(uint64_t length, const uint32_t *v, int layout);
+uint64_t orc_set_build(uint8_t *out, const uint32_t *v, uint64_t n, int layout, orc_set *s);
+int64_t orc_set_find(const orc_set *s, uint32_t key);
+uint64_t orc_set_decode(const orc_set *s, uint32_t *out);
+
+/* Intersections.  orc_intersect materializes into cbuf (caller sized to the
+ * larger operand's number_of_bytes + 8); orc_intersect_count only counts. */
+uint64_t orc_intersect(orc_set *c, uint8_t *cbuf, const orc_set *a, const orc_set *b);
+uint64_t orc_intersect_count(const orc_set *a, const orc_set *b);
+/* Which algorithm the reference picks for a layout pair:
+ * 0 = merge, 1 = gallop, 2 = uint x bitset probe, 3 = bitset AND, 4 = empty. */
+int orc_algorithm_of(const orc_set *a, const orc_set *b);
+
+/* Individual algorithms (sorted uint32 inputs). */
+uint64_t orc_uu_merge(uint32_t *out, const uint32_t *a, uint64_t na, const uint32_t *b, uint64_t nb);
+uint64_t orc_uu_gallop(uint32_t *out, const uint32_t *rare, uint64_t nr, const uint32_t *freq, uint64_t nf);
+
+/* Two-level trie over a binary relation (columns c0, c1). */
+typedef struct orc_trie orc_trie;
+orc_trie *orc_trie_build(const uint32_t *c0, const uint32_t *c1, uint64_t rows, int policy);
+void orc_trie_free(orc_trie *t);
+uint64_t orc_trie_num_rows(const orc_trie *t);   /* distinct tuples */
+void orc_trie_head(const orc_trie *t, orc_set *out);
+int orc_trie_child(const orc_trie *t, uint32_t key, orc_set *out); /* 0 = absent */
+uint64_t orc_trie_tuples(const orc_trie *t, uint32_t *c0, uint32_t *c1);
+/* Per head node: layout type of its child set, its cardinality. */
+void orc_trie_node_info(const orc_trie *t, uint8_t *types, uint32_t *cards);
+
+/* Single-bag generic-join COUNT plan over binary relations.  Level k's
+ * candidate set is the intersection of its operands: parent < 0 names the head
+ * set of relation `rel`, parent = j < k the child set of x_j in `rel`. */
+#define ORC_MAX_ATTRS 8
+#define ORC_MAX_OPS 8
+typedef struct { int32_t rel; int32_t parent; } orc_op;
+typedef struct {
+  int32_t num_attrs;
+  int32_t num_ops[ORC_MAX_ATTRS];
+  orc_op ops[ORC_MAX_ATTRS][ORC_MAX_OPS];
+} orc_plan;
+
+/* COUNT(*) of the plan.  Level-0 values in [begin, end) of the level-0 set are
+ * processed (end = UINT64_MAX for all), over nthreads pthreads with dynamic
+ * chunks of 128.  Returns -1 on a malformed plan. */
+int64_t orc_join_count(const orc_plan *p, orc_trie *const *tries, int nthreads,
+                       uint64_t begin, uint64_t end);
+/* Same over every stride-th chunk of 128 level-0 values (a bounded,
+ * stratified sample for CPU timing). */
+int64_t orc_join_count_sampled(const orc_plan *p, orc_trie *const *tries, int nthreads, uint64_t stride);
+/* Size of the level-0 candidate set of a plan. */
+uint64_t orc_join_level0_size(const orc_plan *p, orc_trie *const *tries);
+/* Per level-0 value counts (out has orc_join_level0_size entries). */
+int orc_join_count_per_head(const orc_plan *p, orc_trie *const *tries, int nthreads,
+                            int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""ctypes binding of the CPU oracle (oracle/libehx_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libehx_oracle.so")
+
+RANGE_BITSET, UINTEGER = 0, 1
+POLICY_RULE, POLICY_ALL_UINT, POLICY_ALL_BITSET = 0, 1, 2
+MAX_ATTRS, MAX_OPS = 8, 8
+
+
+class OrcSet(C.Structure):
+    _fields_ = [("cardinality", C.c_uint32), ("range", C.c_uint32),
+                ("number_of_bytes", C.c_uint64), ("type", C.c_uint8),
+                ("data", C.c_void_p)]
+
+
+class OrcOp(C.Structure):
+    _fields_ = [("rel", C.c_int32), ("parent", C.c_int32)]
+
+
+class OrcPlan(C.Structure):
+    _fields_ = [("num_attrs", C.c_int32), ("num_ops", C.c_int32 * MAX_ATTRS),
+                ("ops", (OrcOp * MAX_OPS) * MAX_ATTRS)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        u32p, u64 = C.POINTER(C.c_uint32), C.c_uint64
+        L.orc_is_sparse.argtypes = [u64, u64]
+        L.orc_is_sparse.restype = C.c_int
+        L.orc_layout_of.argtypes = [u32p, u64]
+        L.orc_layout_of.restype = C.c_int
+        L.orc_bytes_needed.argtypes = [u64, u32p, C.c_int]
+        L.orc_bytes_needed.restype = u64
+        L.orc_set_build.argtypes = [C.c_void_p, u32p, u64, C.c_int, C.POINTER(OrcSet)]
+        L.orc_set_build.restype = u64
+        L.orc_set_find.argtypes = [C.POINTER(OrcSet), C.c_uint32]
+        L.orc_set_find.restype = C.c_int64
+        L.orc_set_decode.argtypes = [C.POINTER(OrcSet), u32p]
+        L.orc_set_decode.restype = u64
+        L.orc_intersect.argtypes = [C.POINTER(OrcSet), C.c_void_p, C.POINTER(OrcSet), C.POINTER(OrcSet)]
+        L.orc_intersect.restype = u64
+        L.orc_intersect_count.argtypes = [C.POINTER(OrcSet), C.POINTER(OrcSet)]
+        L.orc_intersect_count.restype = u64
+        L.orc_algorithm_of.argtypes = [C.POINTER(OrcSet), C.POINTER(OrcSet)]
+        L.orc_algorithm_of.restype = C.c_int
+        L.orc_uu_merge.argtypes = [u32p, u32p, u64, u32p, u64]
+        L.orc_uu_merge.restype = u64
+        L.orc_uu_gallop.argtypes = [u32p, u32p, u64, u32p, u64]
+        L.orc_uu_gallop.restype = u64
+        L.orc_trie_build.argtypes = [u32p, u32p, u64, C.c_int]
+        L.orc_trie_build.restype = C.c_void_p
+        L.orc_trie_free.argtypes = [C.c_void_p]
+        L.orc_trie_num_rows.argtypes = [C.c_void_p]
+        L.orc_trie_num_rows.restype = u64
+        L.orc_trie_head.argtypes = [C.c_void_p, C.POINTER(OrcSet)]
+        L.orc_trie_child.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(OrcSet)]
+        L.orc_trie_child.restype = C.c_int
+        L.orc_trie_tuples.argtypes = [C.c_void_p, u32p, u32p]
+        L.orc_trie_tuples.restype = u64
+        L.orc_trie_node_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), u32p]
+        L.orc_join_count.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int, u64, u64]
+        L.orc_join_count.restype = C.c_int64
+        L.orc_join_count_sampled.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int, u64]
+        L.orc_join_count_sampled.restype = C.c_int64
+        L.orc_join_level0_size.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p)]
+        L.orc_join_level0_size.restype = u64
+        L.orc_join_count_per_head.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int,
+                                              C.POINTER(C.c_int64)]
+        L.orc_join_count_per_head.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class Set:
+    """A set in the reference's byte layout (uinteger or range_bitset)."""
+
+    def __init__(self, values, layout=None):
+        v, p = _u32(np.asarray(values, dtype=np.uint32))
+        L = lib()
+        if layout is None:
+            layout = L.orc_layout_of(p, len(v))
+        self.buf = np.zeros(int(L.orc_bytes_needed(len(v), p, layout)) + 16, dtype=np.uint8)
+        self.s = OrcSet()
+        L.orc_set_build(self.buf.ctypes.data, p, len(v), layout, C.byref(self.s))
+        self.values = v
+
+    @property
+    def layout(self):
+        return self.s.type
+
+    def find(self, key):
+        return lib().orc_set_find(C.byref(self.s), key)
+
+    def decode(self):
+        out = np.zeros(max(1, self.s.cardinality), dtype=np.uint32)
+        n = lib().orc_set_decode(C.byref(self.s), out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out[:n]
+
+    def raw(self):
+        return bytes(self.buf[: self.s.number_of_bytes])
+
+
+def intersect(a, b):
+    """Materialized intersection; returns (values, result_layout, cardinality, range)."""
+    L = lib()
+    cap = max(a.s.number_of_bytes, b.s.number_of_bytes) + 64
+    cbuf = np.zeros(int(cap), dtype=np.uint8)
+    c = OrcSet()
+    L.orc_intersect(C.byref(c), cbuf.ctypes.data, C.byref(a.s), C.byref(b.s))
+    out = np.zeros(max(1, c.cardinality), dtype=np.uint32)
+    n = L.orc_set_decode(C.byref(c), out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out[:n], c.type, c.cardinality, c.range
+
+
+def intersect_count(a, b):
+    return lib().orc_intersect_count(C.byref(a.s), C.byref(b.s))
+
+
+def algorithm_of(a, b):
+    return lib().orc_algorithm_of(C.byref(a.s), C.byref(b.s))
+
+
+class Trie:
+    """Two-level trie over a binary relation (src, dst)."""
+
+    def __init__(self, src, dst, policy=POLICY_RULE):
+        s, ps = _u32(src)
+        d, pd = _u32(dst)
+        assert len(s) == len(d)
+        self.h = C.c_void_p(lib().orc_trie_build(ps, pd, len(s), policy))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_trie_free(self.h)
+            self.h = None
+
+    @property
+    def num_rows(self):
+        return lib().orc_trie_num_rows(self.h)
+
+    def tuples(self):
+        n = self.num_rows
+        a = np.zeros(max(1, n), dtype=np.uint32)
+        b = np.zeros(max(1, n), dtype=np.uint32)
+        k = lib().orc_trie_tuples(self.h, a.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                  b.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return a[:k], b[:k]
+
+    def head(self):
+        s = OrcSet()
+        lib().orc_trie_head(self.h, C.byref(s))
+        return s
+
+    def child(self, key):
+        s = OrcSet()
+        ok = lib().orc_trie_child(self.h, key, C.byref(s))
+        return s if ok else None
+
+    def node_info(self, nheads):
+        t = np.zeros(max(1, nheads), dtype=np.uint8)
+        c = np.zeros(max(1, nheads), dtype=np.uint32)
+        lib().orc_trie_node_info(self.h, t.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 c.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return t[:nheads], c[:nheads]
+
+
+def make_plan(levels):
+    """levels: list (one per attribute) of lists of (rel, parent) pairs."""
+    p = OrcPlan()
+    p.num_attrs = len(levels)
+    for k, ops in enumerate(levels):
+        p.num_ops[k] = len(ops)
+        for i, (rel, parent) in enumerate(ops):
+            p.ops[k][i].rel = rel
+            p.ops[k][i].parent = parent
+    return p
+
+
+def _tries_arg(tries):
+    arr = (C.c_void_p * len(tries))(*[t.h.value for t in tries])
+    return arr
+
+
+def join_count(levels, tries, nthreads=1, begin=0, end=2**64 - 1):
+    p = make_plan(levels)
+    return lib().orc_join_count(C.byref(p), _tries_arg(tries), nthreads, begin, end)
+
+
+def join_count_sampled(levels, tries, nthreads, stride):
+    p = make_plan(levels)
+    return lib().orc_join_count_sampled(C.byref(p), _tries_arg(tries), nthreads, stride)
+
+
+def join_level0_size(levels, tries):
+    p = make_plan(levels)
+    return lib().orc_join_level0_size(C.byref(p), _tries_arg(tries))
+
+
+def join_count_per_head(levels, tries, nthreads=1):
+    p = make_plan(levels)
+    n = join_level0_size(levels, tries)
+    out = np.zeros(max(1, n), dtype=np.int64)
+    rc = lib().orc_join_count_per_head(C.byref(p), _tries_arg(tries), nthreads,
+                                       out.ctypes.data_as(C.POINTER(C.c_int64)))
+    assert rc == 0
+    return out[:n]

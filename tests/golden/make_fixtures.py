@@ -1,0 +1,78 @@
+"""Writes the committed golden fixtures for the generic-join parity tests.
+
+Inputs are data files held by the reference's own tests (SNAP ego-Facebook,
+a public dataset): test/graph/data/facebook_pruned.tsv (88,234 edges, every
+row src > dst) used by test/graph/travis.py:197-358.  The expected results are
+the constants those tests assert:
+  TriangleAgg  = 1,612,010   (travis.py:197-213)
+  FliqueAgg    = 30,004,668  (travis.py:238-254)
+  Triangle     = 1,612,010 rows (travis.py:302-318)
+and, for the graph with both edge directions (facebook_duplicated.tsv),
+9,672,060 triangle tuples (docs/notebooks/Getting Started.ipynb, cell 22).
+
+Run once in a container that holds the reference tree:
+    python tests/golden/make_fixtures.py /root/reference
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load_tsv(path):
+    raw = open(path, "rb").read()
+    arr = np.loadtxt(path, dtype=np.int64, delimiter="\t", ndmin=2)
+    return raw, arr
+
+
+def main(ref_root):
+    src = os.path.join(ref_root, "test", "graph", "data")
+    raw, fb = load_tsv(os.path.join(src, "facebook_pruned.tsv"))
+    assert fb.shape == (88234, 2) and fb.min() >= 0 and fb.max() < 2**32
+    np.savez_compressed(os.path.join(HERE, "facebook_pruned.npz"),
+                        src=fb[:, 0].astype(np.uint32), dst=fb[:, 1].astype(np.uint32))
+    rawd, dup = load_tsv(os.path.join(src, "facebook_duplicated.tsv"))
+    rawt, simple = load_tsv(os.path.join(src, "simple.tsv"))
+    np.savez_compressed(os.path.join(HERE, "simple.npz"),
+                        src=simple[:, 0].astype(np.uint32), dst=simple[:, 1].astype(np.uint32))
+    # facebook_duplicated holds both directions of every edge, under the
+    # original SNAP vertex ids (facebook_pruned is a relabelled copy).
+    np.savez_compressed(os.path.join(HERE, "facebook_duplicated.npz"),
+                        src=dup[:, 0].astype(np.uint32), dst=dup[:, 1].astype(np.uint32))
+    pairs = {tuple(r) for r in dup.tolist()}
+    dup_is_sym = all((y, x) in pairs for (x, y) in pairs)
+    meta = {
+        "facebook_pruned": {
+            "source": "test/graph/data/facebook_pruned.tsv",
+            "sha256": hashlib.sha256(raw).hexdigest(),
+            "rows": int(fb.shape[0]),
+            "TriangleAgg": 1612010,
+            "FliqueAgg": 30004668,
+            "Triangle_rows": 1612010,
+            "cite": "test/graph/travis.py:197-213,238-254,302-318",
+        },
+        "facebook_duplicated": {
+            "source": "test/graph/data/facebook_duplicated.tsv",
+            "sha256": hashlib.sha256(rawd).hexdigest(),
+            "rows": int(dup.shape[0]),
+            "symmetric": bool(dup_is_sym),
+            "Triangle_rows": 9672060,
+            "cite": "docs/notebooks/Getting Started.ipynb cell 22",
+        },
+        "simple": {
+            "source": "test/graph/data/simple.tsv",
+            "sha256": hashlib.sha256(rawt).hexdigest(),
+            "rows": int(simple.shape[0]),
+        },
+    }
+    with open(os.path.join(HERE, "fixtures.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+    print(json.dumps(meta, indent=1))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

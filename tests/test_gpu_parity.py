@@ -1,0 +1,213 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle and the
+reference's golden values.  Results are exact integers, so every comparison is
+bit-exact."""
+import sys
+
+import numpy as np
+import pytest
+
+import emptyheaded_amd as E
+from emptyheaded_amd import graphs as G
+from emptyheaded_amd import plans
+
+pytestmark = pytest.mark.gpu
+
+FORMS = [E.FORM_PLAIN, E.FORM_TUNED, E.FORM_LEARNED]
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def random_relation(rng, n_heads, rows, hi):
+    src = rng.integers(0, n_heads, size=rows).astype(np.uint32)
+    dst = rng.integers(0, hi, size=rows).astype(np.uint32)
+    return src, dst
+
+
+@pytest.mark.parametrize("policy", [E.POLICY_RULE, E.POLICY_ALL_UINT, E.POLICY_ALL_BITSET])
+def test_trie_build_matches_oracle(ctx, oracle, policy):
+    rng = np.random.default_rng(policy)
+    src, dst = random_relation(rng, 500, 20000, 30000)
+    dst[:6000] = rng.integers(0, 800, size=6000)          # dense sets -> bitsets
+    src = np.concatenate([src, src[:500], [501, 501, 502]]).astype(np.uint32)  # repeated rows, extreme values
+    dst = np.concatenate([dst, dst[:500], [2**32 - 1, 2**32 - 2, 0]]).astype(np.uint32)
+    say("trie build policy", policy)
+    T = E.Trie(ctx, src, dst, policy)
+    O = oracle.Trie(src, dst, policy)
+    a, b = T.tuples()
+    oa, ob = O.tuples()
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    info = T.info()
+    assert info["num_rows"] == O.num_rows
+    types, cards = T.node_info()
+    otypes, ocards = O.node_info(len(types))
+    assert np.array_equal(cards, ocards)
+    assert np.array_equal(types, otypes)
+    heads = np.unique(src)
+    nodes = T.lookup(np.concatenate([heads, [10**6, 2**32 - 1]]))
+    assert np.array_equal(nodes[: len(heads)], np.arange(len(heads))) and (nodes[len(heads):] == -1).all()
+
+
+def test_trie_build_large_keys(ctx, oracle):
+    rng = np.random.default_rng(3)
+    src = rng.integers(0, 2**32, size=30000, dtype=np.uint64).astype(np.uint32)
+    dst = rng.integers(0, 2**32, size=30000, dtype=np.uint64).astype(np.uint32)
+    src[:10000] = src[0]
+    T = E.Trie(ctx, src, dst)
+    a, b = T.tuples()
+    oa, ob = oracle.Trie(src, dst).tuples()
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    assert T.info()["dense_nav"] == 0
+
+
+def set_collection(rng, sizes, hi, dense_frac=0.0):
+    src, dst = [], []
+    for i, n in enumerate(sizes):
+        if rng.random() < dense_frac:
+            base = int(rng.integers(0, max(1, hi - 64 * n)))
+            v = np.unique(base + rng.integers(0, 64 * n + 1, size=n))
+        else:
+            v = np.unique(rng.integers(0, hi, size=n, dtype=np.int64))
+        v = v[v < 2**32]
+        src.append(np.full(len(v), i, np.uint32))
+        dst.append(v.astype(np.uint32))
+    return np.concatenate(src), np.concatenate(dst)
+
+
+@pytest.mark.parametrize("pa,pb", [(E.POLICY_ALL_UINT, E.POLICY_ALL_UINT), (E.POLICY_ALL_UINT, E.POLICY_ALL_BITSET),
+                                   (E.POLICY_ALL_BITSET, E.POLICY_ALL_UINT),
+                                   (E.POLICY_ALL_BITSET, E.POLICY_ALL_BITSET), (E.POLICY_RULE, E.POLICY_RULE)])
+def test_intersections_all_layout_pairs(ctx, oracle, pa, pb):
+    rng = np.random.default_rng(10 * pa + pb)
+    sizes = np.concatenate([rng.integers(1, 40, 150), rng.integers(40, 3000, 60), [1, 2, 3, 64, 65, 2000, 5000]])
+    src, dst = set_collection(rng, sizes, 200000, dense_frac=0.5)
+    say("intersections", pa, pb)
+    TA, TB = E.Trie(ctx, src, dst, pa), E.Trie(ctx, src, dst, pb)
+    nsets = int(src.max()) + 1
+    ka = rng.integers(0, nsets, 3000).astype(np.uint32)
+    kb = rng.integers(0, nsets, 3000).astype(np.uint32)
+    ka[:5] = [nsets + 5, 0, 1, 2, 3]  # an absent key -> empty
+    sets = {i: dst[src == i] for i in range(nsets)}
+    exp = [np.intersect1d(sets.get(int(x), np.zeros(0, np.uint32)), sets.get(int(y), np.zeros(0, np.uint32)))
+           for x, y in zip(ka, kb)]
+    cnt = E.intersect_count(ctx, TA, ka, TB, kb)
+    assert np.array_equal(cnt, np.array([len(e) for e in exp], np.uint64))
+    offs, vals = E.intersect(ctx, TA, ka, TB, kb)
+    for i in range(len(ka)):
+        assert np.array_equal(vals[offs[i]:offs[i + 1]], exp[i]), i
+
+
+def test_gallop_and_merge_boundaries(ctx):
+    rng = np.random.default_rng(77)
+    rare = np.unique(rng.integers(0, 10**6, 10)).astype(np.uint32)
+    freq = [np.unique(rng.integers(0, 10**6, k)).astype(np.uint32) for k in (200, 320, 330, 5000)]
+    for f in freq:
+        f = np.union1d(f, rare[::2])
+        src = np.concatenate([np.zeros(len(rare), np.uint32), np.ones(len(f), np.uint32)])
+        dst = np.concatenate([rare, f]).astype(np.uint32)
+        T = E.Trie(ctx, src, dst, E.POLICY_ALL_UINT)
+        assert E.intersect_count(ctx, T, [0], T, [1])[0] == len(np.intersect1d(rare, f))
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("policy", [E.POLICY_RULE, E.POLICY_ALL_UINT, E.POLICY_ALL_BITSET])
+def test_facebook_golden(ctx, facebook, fixtures_meta, form, policy):
+    say("facebook form", form, "policy", policy)
+    T = E.Trie(ctx, *facebook, policy=policy)
+    assert E.join_count(ctx, plans.TRIANGLE, [T], form) == fixtures_meta["facebook_pruned"]["TriangleAgg"]
+    assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], form) == fixtures_meta["facebook_pruned"]["FliqueAgg"]
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_facebook_duplicated_golden(ctx, facebook_dup, fixtures_meta, form):
+    T = E.Trie(ctx, *facebook_dup)
+    assert E.join_count(ctx, plans.TRIANGLE, [T], form) == fixtures_meta["facebook_duplicated"]["Triangle_rows"]
+
+
+@pytest.mark.parametrize("scale", [10, 14, 16])
+def test_rmat_matches_oracle(ctx, oracle, scale):
+    s, d, n = G.rmat_graph(scale)
+    say("rmat", scale, len(s))
+    O = oracle.Trie(s, d)
+    T = E.Trie(ctx, s, d)
+    tri = oracle.join_count(plans.TRIANGLE, [O], 8)
+    fq = oracle.join_count(plans.FOUR_CLIQUE, [O], 8)
+    for form in FORMS:
+        assert E.join_count(ctx, plans.TRIANGLE, [T], form) == tri
+        assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], form) == fq
+
+
+def test_er_config_matches_oracle(ctx, oracle):
+    s, d, n = G.er_graph()
+    O = oracle.Trie(s, d)
+    exp = oracle.join_count(plans.TRIANGLE, [O], 8)
+    for policy in (E.POLICY_RULE, E.POLICY_ALL_UINT):
+        T = E.Trie(ctx, s, d, policy)
+        for form in FORMS:
+            assert E.join_count(ctx, plans.TRIANGLE, [T], form) == exp
+
+
+def test_four_cycle_walks_and_two_hop(ctx, oracle):
+    s, d, n = G.rmat_graph(11, orient=G.BOTH)
+    O, T = oracle.Trie(s, d), E.Trie(ctx, s, d)
+    exp = oracle.join_count(plans.FOUR_CYCLE_WALKS, [O], 8)
+    for form in FORMS:
+        assert E.join_count(ctx, plans.FOUR_CYCLE_WALKS, [T], form) == exp
+    rng = np.random.default_rng(2)
+    r = rng.integers(0, 3000, size=(20000, 2)).astype(np.uint32)
+    t = rng.integers(0, 3000, size=(15000, 2)).astype(np.uint32)
+    OR, OT = oracle.Trie(r[:, 0], r[:, 1]), oracle.Trie(t[:, 0], t[:, 1])
+    TR, TT = E.Trie(ctx, r[:, 0], r[:, 1]), E.Trie(ctx, t[:, 0], t[:, 1])
+    assert E.join_count(ctx, plans.TWO_HOP, [TR, TT]) == oracle.join_count(plans.TWO_HOP, [OR, OT], 4)
+
+
+def test_range_restriction(ctx, oracle, facebook):
+    T = E.Trie(ctx, *facebook)
+    O = oracle.Trie(*facebook)
+    n0 = oracle.join_level0_size(plans.TRIANGLE, [O])
+    for form in FORMS:
+        parts = [E.join_count(ctx, plans.TRIANGLE, [T], form, b, min(b + 700, n0)) for b in range(0, n0, 700)]
+        assert sum(parts) == 1612010
+        assert E.join_count(ctx, plans.TRIANGLE, [T], form, 100, 900) == oracle.join_count(plans.TRIANGLE, [O], 2, 100, 900)
+
+
+def test_graph_replay_matches_eager(ctx):
+    s, d, n = G.rmat_graph(15)
+    T = E.Trie(ctx, s, d)
+    eager = E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_PLAIN)
+    saved = E.tune_get()
+    try:
+        E.tune_set(use_graph=1)
+        for i in range(4):
+            assert E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_TUNED) == eager
+            assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_PLAIN) >= 0  # eager work between replays
+            ctx.synchronize()
+        E.tune_set(use_graph=0)
+        assert E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_TUNED) == eager
+    finally:
+        E.tune_set(**saved)
+
+
+@pytest.mark.parametrize("lds,big,bitmap_all", [(2048, 16, 0), (8192, 256, 1), (4096, 64, 1)])
+def test_tune_params_never_change_results(ctx, oracle, lds, big, bitmap_all):
+    s, d, n = G.rmat_graph(13)
+    exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
+    T = E.Trie(ctx, s, d)
+    saved = E.tune_get()
+    try:
+        E.tune_set(lds_slots=lds, big_deg=big, bitmap_all=bitmap_all, grab=3, waves_per_cu=4)
+        assert E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_TUNED) == exp
+        assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_TUNED) == E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_PLAIN)
+    finally:
+        E.tune_set(**saved)
+
+
+def test_rmat20_config_full_size(ctx, oracle):
+    """R-MAT scale 20, edge factor 16: plain GPU drop-in vs the CPU oracle."""
+    s, d, n = G.rmat_graph(20)
+    say("rmat20 edges", len(s))
+    exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 16)
+    T = E.Trie(ctx, s, d)
+    for form in FORMS:
+        assert E.join_count(ctx, plans.TRIANGLE, [T], form) == exp
