@@ -55,7 +55,7 @@ class JoinStats(C.Structure):
 
 class TuneParams(C.Structure):
     _fields_ = [("lds_slots", C.c_int32), ("big_deg", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("grab", C.c_int32), ("bitmap_all", C.c_int32), ("use_graph", C.c_int32)]
+                ("grab", C.c_int32), ("bitmap_all", C.c_int32), ("use_graph", C.c_int32), ("big_work", C.c_int32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -82,6 +82,10 @@ SIGNATURES = {
     "ehx_intersect_count_batch": ([_vp, _vp, _u32p, _vp, _u32p, C.c_uint64, _u64p], C.c_int),
     "ehx_intersect_batch": ([_vp, _vp, _u32p, _vp, _u32p, C.c_uint64, _u64p, _u32p, C.c_uint64, _u64p],
                             C.c_int),
+    "ehx_intersect_nodes_count": ([_vp, C.c_int, C.POINTER(_vp), C.POINTER(_u32p), C.c_uint64, _u64p], C.c_int),
+    "ehx_intersect_nodes": ([_vp, C.c_int, C.POINTER(_vp), C.POINTER(_u32p), C.c_uint64, _u64p, _u32p, C.c_uint64,
+                             _u64p], C.c_int),
+    "ehx_trie_host_arrays": ([_vp, _vp, _u32p, _u32p], C.c_int),
     "ehx_join_count": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, C.POINTER(C.c_int64),
                         C.POINTER(JoinStats)], C.c_int),
     "ehx_join_count_range": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, C.c_uint64, C.c_uint64,
@@ -98,6 +102,16 @@ SIGNATURES = {
     "ehx_selector_load": ([C.c_int, C.POINTER(C.c_uint16), C.c_uint64, C.POINTER(C.c_float), C.c_float],
                           C.c_int),
     "ehx_selector_clear": ([], C.c_int),
+    "ehx_trie_set_features": ([_vp, _vp, C.POINTER(C.c_float)], C.c_int),
+    "ehx_join_record": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, _u64p, C.c_uint64, _u64p,
+                         C.POINTER(C.c_int64)], C.c_int),
+    "ehx_mlp_create": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint64,
+                        C.POINTER(_vp)], C.c_int),
+    "ehx_mlp_destroy": ([_vp], C.c_int),
+    "ehx_mlp_grad": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                      C.POINTER(C.c_float)], C.c_int),
+    "ehx_mlp_predict": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint64,
+                         C.POINTER(C.c_float)], C.c_int),
 }
 
 _lib = None

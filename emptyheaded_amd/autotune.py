@@ -1,0 +1,113 @@
+"""Autotuner for the tuned form's knobs (LDS staging size, wave/workgroup bin
+cut, long-child-set cutoff, grid size, queue grab, bitmap rule, hipGraph).
+
+Coordinate descent over a small grid on training workloads (R-MAT graphs
+smaller than the evaluation graph), timing the tuned last level with HIP
+events; every configuration's count is checked against the first one (knobs
+never change results).  Writes a timing DB:
+
+    python -m emptyheaded_amd.autotune --out gpurun_out/tuning.json
+"""
+import argparse
+import json
+import sys
+import time
+
+GRID = {
+    "lds_slots": [2048, 4096, 8192],
+    "big_deg": [16, 32, 64, 128],
+    "waves_per_cu": [8, 16, 24, 32, 48],
+    "grab": [1, 2, 4, 8],
+    "big_work": [1024, 4096, 16384, 65536],
+    "bitmap_all": [0, 1],
+    "use_graph": [0, 1],
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measure(E, ctx, T, plan, form, reps):
+    E.join_count(ctx, plan, [T], form)  # warm (and capture, in graph mode)
+    ctx.synchronize()
+    best, total = None, None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        total, st = E.join_count(ctx, plan, [T], form, stats=True)
+        dt = (time.perf_counter() - t0) * 1e3
+        best = dt if best is None else min(best, dt)
+    return total, best
+
+
+def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID):
+    """workloads: list of (name, trie, plan).  Objective: sum of best-of-reps
+    wall ms per query over the workloads."""
+    cur = E.tune_get()
+    ref = {}
+    history = []
+
+    def evaluate(params):
+        E.tune_set(**params)
+        tot = 0.0
+        for name, T, plan in workloads:
+            cnt, ms = measure(E, ctx, T, plan, E.FORM_TUNED, reps)
+            if name in ref and ref[name] != cnt:
+                raise AssertionError("knobs changed the result of %s: %d vs %d" % (name, cnt, ref[name]))
+            ref.setdefault(name, cnt)
+            tot += ms
+        history.append({"params": dict(params), "ms": tot})
+        log("  %.3f ms  %s" % (tot, json.dumps(params)))
+        return tot
+
+    best_ms = evaluate(cur)
+    for _ in range(passes):
+        improved = False
+        for knob, values in grid.items():
+            for v in values:
+                if cur[knob] == v:
+                    continue
+                trial = dict(cur)
+                trial[knob] = v
+                ms = evaluate(trial)
+                if ms < best_ms * 0.99:
+                    best_ms, cur, improved = ms, trial, True
+        if not improved:
+            break
+    E.tune_set(**cur)
+    return cur, best_ms, history, ref
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default="gpurun_out/tuning.json")
+    ap.add_argument("--scales", default="20")
+    ap.add_argument("--clique-scale", type=int, default=17)
+    ap.add_argument("--passes", type=int, default=2)
+    args = ap.parse_args()
+    import emptyheaded_amd as E
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import plans
+    ctx = E.Context(0)
+    workloads, keep = [], []
+    for s in [int(x) for x in args.scales.split(",")]:
+        src, dst, nv = G.rmat_graph(s)
+        T = E.Trie(ctx, src, dst)
+        keep.append(T)
+        workloads.append(("triangle_rmat%d" % s, T, plans.TRIANGLE))
+    if args.clique_scale:
+        src, dst, nv = G.rmat_graph(args.clique_scale)
+        T = E.Trie(ctx, src, dst)
+        keep.append(T)
+        workloads.append(("clique4_rmat%d" % args.clique_scale, T, plans.FOUR_CLIQUE))
+    best, ms, hist, ref = tune(E, ctx, workloads, passes=args.passes)
+    out = {"best": best, "objective_ms": ms, "workloads": [w[0] for w in workloads], "counts": ref,
+           "history": hist}
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    log("best", json.dumps(best), "%.3f ms" % ms)
+    print(json.dumps({"best": best, "objective_ms": ms}))
+
+
+if __name__ == "__main__":
+    main()

@@ -61,7 +61,29 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_dropin()
     return LIB
+
+
+ENGINE = os.path.join(HERE, "engine")
+DROPIN_LIB = os.path.join(ENGINE, "libehx_dropin.so")
+
+
+def build_dropin():
+    """The C++ drop-in API runtime plus the restated generated queries, linked
+    against libehx.so (found next to the package through the rpath)."""
+    srcs = [os.path.join(ENGINE, "src", "dropin.cpp")] + sorted(
+        os.path.join(ENGINE, "queries", f) for f in os.listdir(os.path.join(ENGINE, "queries")) if f.endswith(".cpp"))
+    cxx = shutil.which("g++") or "g++"
+    tmp = DROPIN_LIB + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-variable",
+           "-I", os.path.join(ENGINE, "include"), "-o", tmp] + srcs + [
+        "-L", HERE, "-lehx", "-Wl,-rpath,$ORIGIN/..", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, DROPIN_LIB)
+    return DROPIN_LIB
 
 
 if __name__ == "__main__":

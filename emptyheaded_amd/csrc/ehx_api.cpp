@@ -41,9 +41,12 @@ int trie_destroy(ehx_trie *);
 int trie_export(ehx_ctx *, const ehx_trie *, uint32_t *, uint32_t *);
 int trie_node_info(ehx_ctx *, const ehx_trie *, uint8_t *, uint32_t *);
 int trie_lookup(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *);
-int join_count(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint64_t, uint64_t, int64_t *, ehx_join_stats *);
+int join_count(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint64_t, uint64_t, int64_t *, ehx_join_stats *,
+               const RecordReq *);
 int intersect_pairs(ehx_ctx *, const ehx_trie *, const uint32_t *, const ehx_trie *, const uint32_t *, uint64_t,
                     uint64_t *, uint64_t *, uint32_t *, uint64_t, uint64_t *);
+int intersect_nodes(ehx_ctx *, int, ehx_trie *const *, const uint32_t *const *, uint64_t, uint64_t *, uint64_t *,
+                    uint32_t *, uint64_t, uint64_t *);
 void tuned_forget_graphs();
 int selector_load(int device, const uint16_t *params, const float *norm, float max_words);
 int selector_clear();
@@ -159,16 +162,45 @@ int ehx_intersect_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a
   return intersect_pairs(ctx, ta, keys_a, tb, keys_b, n, nullptr, offsets, values, cap, total);
 }
 
+int ehx_intersect_nodes_count(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint32_t *const *nodes,
+                              uint64_t n, uint64_t *counts) {
+  if (!counts && n) return fail(EHX_ERR_INVALID, "null counts");
+  return intersect_nodes(ctx, nsets, tries, nodes, n, counts, nullptr, nullptr, 0, nullptr);
+}
+
+int ehx_intersect_nodes(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint32_t *const *nodes, uint64_t n,
+                        uint64_t *offsets, uint32_t *values, uint64_t cap, uint64_t *total) {
+  if (!offsets || (!values && cap)) return fail(EHX_ERR_INVALID, "null output");
+  return intersect_nodes(ctx, nsets, tries, nodes, n, nullptr, offsets, values, cap, total);
+}
+
+int ehx_trie_host_arrays(ehx_ctx *ctx, const ehx_trie *t, uint32_t *head_vals, uint32_t *offsets) {
+  if (!ctx || !t) return fail(EHX_ERR_INVALID, "null argument");
+  EHX_HIP(hipSetDevice(ctx->device));
+  int rc;
+  if (head_vals && (rc = copy_d2h(ctx, head_vals, t->head, t->nh * 4))) return rc;
+  if (offsets && (rc = copy_d2h(ctx, offsets, t->off, (t->nh + 1) * 4))) return rc;
+  return EHX_OK;
+}
+
 int ehx_join_count(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, int64_t *result,
                    ehx_join_stats *stats) {
   if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
-  return join_count(ctx, plan, tries, form, 0, UINT64_MAX, result, stats);
+  return join_count(ctx, plan, tries, form, 0, UINT64_MAX, result, stats, nullptr);
 }
 
 int ehx_join_count_range(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, uint64_t begin,
                          uint64_t end, int64_t *result, ehx_join_stats *stats) {
   if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
-  return join_count(ctx, plan, tries, form, begin, end, result, stats);
+  return join_count(ctx, plan, tries, form, begin, end, result, stats, nullptr);
+}
+
+int ehx_join_record(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int force_mode, uint64_t *cycles,
+                    uint64_t cap, uint64_t *nprefix, int64_t *result) {
+  if (!ctx || !cycles || !nprefix) return fail(EHX_ERR_INVALID, "null argument");
+  if (force_mode < -1 || force_mode > 1) return fail(EHX_ERR_INVALID, "force_mode must be -1, 0 or 1");
+  RecordReq r{cycles, cap, nprefix, force_mode};
+  return join_count(ctx, plan, tries, EHX_FORM_TUNED, 0, UINT64_MAX, result, nullptr, &r);
 }
 
 int ehx_tune_get(ehx_tune_params *out) {
@@ -180,6 +212,7 @@ int ehx_tune_get(ehx_tune_params *out) {
   out->grab = p.grab;
   out->bitmap_all = p.bitmap_all;
   out->use_graph = p.use_graph;
+  out->big_work = p.big_work;
   return EHX_OK;
 }
 
@@ -187,7 +220,8 @@ int ehx_tune_set(const ehx_tune_params *in) {
   if (!in) return fail(EHX_ERR_INVALID, "null params");
   if (in->lds_slots != 2048 && in->lds_slots != 4096 && in->lds_slots != 8192)
     return fail(EHX_ERR_INVALID, "lds_slots must be 2048, 4096 or 8192");
-  if (in->big_deg < 1 || in->waves_per_cu < 1 || in->waves_per_cu > 64 || in->grab < 1 || in->grab > 64)
+  if (in->big_deg < 1 || in->waves_per_cu < 1 || in->waves_per_cu > 64 || in->grab < 1 || in->grab > 64 ||
+      in->big_work < 1)
     return fail(EHX_ERR_INVALID, "tune parameter out of range");
   TuneParams &p = tune_params();
   p.lds_slots = in->lds_slots;
@@ -196,6 +230,7 @@ int ehx_tune_set(const ehx_tune_params *in) {
   p.grab = in->grab;
   p.bitmap_all = in->bitmap_all ? 1 : 0;
   p.use_graph = in->use_graph ? 1 : 0;
+  p.big_work = in->big_work;
   return EHX_OK;
 }
 

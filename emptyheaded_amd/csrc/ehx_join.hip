@@ -129,7 +129,7 @@ static LevelDesc level_desc(const ehx_plan *p, int k) {
 static const uint64_t FRONTIER_BUDGET = 96ull << 30;  // bytes per materialized level
 
 int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form, uint64_t begin, uint64_t end,
-               int64_t *result, ehx_join_stats *stats) {
+               int64_t *result, ehx_join_stats *stats, const RecordReq *rec) {
   int nrels = 0, rc;
   if ((rc = check_plan(p, tries, &nrels))) return rc;
   if (!result) return fail(EHX_ERR_INVALID, "null result");
@@ -235,12 +235,25 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
     }
     // ---- last level -------------------------------------------------------------
     // events: ev[1 + k] closes materialized level k; ev[K] closes the last level
+    if (rec && !tuned) return fail(EHX_ERR_UNSUPPORTED, "recording needs a plan the tuned form runs");
     if (tuned) {
       TunedArgs ta;
+      memset(&ta, 0, sizeof(ta));
       ta.T = T;
       for (int j = 0; j < EHX_MAX_ATTRS; j++) ta.prefix[j] = in.col[j];
       ta.nprefix = n;
+      ta.force_mode = -1;
+      DevBuf &rb = ctx->scratch[S_IN1];
+      if (rec) {
+        if ((rc = rb.ensure(n * 8 + 8)) || (rc = fill_u64(ctx, rb.as<uint64_t>(), 0, n))) return rc;
+        ta.rec = rb.as<unsigned long long>();
+        ta.force_mode = rec->force_mode;
+      }
       if ((rc = tuned_run(ctx, tp, ta, form, ac.as<unsigned long long>()))) return rc;
+      if (rec) {
+        if (rec->n_out) *rec->n_out = n;
+        if ((rc = copy_d2h(ctx, rec->host_out, rb.p, 8 * (n < rec->cap ? n : rec->cap)))) return rc;
+      }
       form_used = form;
     } else {
       LevelDesc L = level_desc(p, K - 1);
@@ -330,6 +343,70 @@ int intersect_pairs(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, co
   if ((rc = v.ensure(tot * 4 + 4))) return rc;
   k_pairs<true><<<G, 256, 0, st>>>(view(ta), view(tb), a.as<uint32_t>(), b.as<uint32_t>(), n, nullptr,
                                    o.as<unsigned long long>(), v.as<uint32_t>());
+  EHX_HIP(hipGetLastError());
+  if ((rc = copy_d2h(ctx, offsets, o.p, (n + 1) * 8))) return rc;
+  return copy_d2h(ctx, values, v.p, tot * 4);
+}
+
+// ---- batched N-way intersections of trie nodes ------------------------------------
+struct NodeOps {
+  DTrie t[EHX_MAX_OPS];
+  const uint32_t *nodes[EHX_MAX_OPS];
+  int32_t nsets;
+};
+
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_nodes(NodeOps O, uint64_t n, unsigned long long *__restrict__ counts,
+                                               const unsigned long long *__restrict__ offs, uint32_t *__restrict__ out) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nwaves) {
+    DSet S[EHX_MAX_OPS];
+    for (int s = 0; s < O.nsets; s++) S[s] = set_of(O.t[s], O.nodes[s][t]);
+    int alg;
+    uint32_t c = wave_intersect<EMIT>(S, O.nsets, EMIT ? out + offs[t] : nullptr, &alg);
+    if (!EMIT && __lane_id() == 0) counts[t] = c;
+  }
+}
+
+int intersect_nodes(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint32_t *const *nodes, uint64_t n,
+                    uint64_t *counts, uint64_t *offsets, uint32_t *values, uint64_t cap, uint64_t *total) {
+  if (!ctx || !tries || !nodes || nsets < 1 || nsets > EHX_MAX_OPS) return fail(EHX_ERR_INVALID, "bad node batch");
+  EHX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (n == 0) {
+    if (offsets) offsets[0] = 0;
+    if (total) *total = 0;
+    return EHX_OK;
+  }
+  NodeOps O;
+  memset(&O, 0, sizeof(O));
+  O.nsets = nsets;
+  DevBuf &nb = ctx->scratch[S_IN0], &c = ctx->scratch[S_COUNTS], &o = ctx->scratch[S_OFFS];
+  int rc;
+  if ((rc = nb.ensure((uint64_t)nsets * n * 4)) || (rc = c.ensure(n * 8 + 8)) || (rc = o.ensure(n * 8 + 8))) return rc;
+  for (int s = 0; s < nsets; s++) {
+    if (!tries[s] || !nodes[s]) return fail(EHX_ERR_INVALID, "operand %d: null trie or nodes", s);
+    if (tries[s]->device != ctx->device) return fail(EHX_ERR_INVALID, "operand %d lives on another device", s);
+    for (uint64_t i = 0; i < n; i++)
+      if (nodes[s][i] > tries[s]->nh) return fail(EHX_ERR_INVALID, "operand %d call %llu: node out of range", s,
+                                                  (unsigned long long)i);
+    O.t[s] = view(tries[s]);
+    O.nodes[s] = nb.as<uint32_t>() + (uint64_t)s * n;
+    EHX_HIP(hipMemcpyAsync((void *)O.nodes[s], nodes[s], n * 4, hipMemcpyHostToDevice, st));
+  }
+  unsigned G = grid_for(n * 64, 256, 65536);
+  k_nodes<false><<<G, 256, 0, st>>>(O, n, c.as<unsigned long long>(), nullptr, nullptr);
+  EHX_HIP(hipGetLastError());
+  if (counts) return copy_d2h(ctx, counts, c.p, n * 8);
+  if ((rc = scan_u64(ctx, c.as<uint64_t>(), o.as<uint64_t>(), n))) return rc;
+  uint64_t tot = 0;
+  if ((rc = copy_d2h(ctx, &tot, o.as<uint64_t>() + n, 8))) return rc;
+  if (total) *total = tot;
+  if (tot > cap) return fail(EHX_ERR_NOMEM, "intersection needs %llu values, capacity %llu", (unsigned long long)tot,
+                             (unsigned long long)cap);
+  DevBuf &v = ctx->scratch[S_MISC];
+  if ((rc = v.ensure(tot * 4 + 4))) return rc;
+  k_nodes<true><<<G, 256, 0, st>>>(O, n, nullptr, o.as<unsigned long long>(), v.as<uint32_t>());
   EHX_HIP(hipGetLastError());
   if ((rc = copy_d2h(ctx, offsets, o.p, (n + 1) * 8))) return rc;
   return copy_d2h(ctx, values, v.p, tot * 4);

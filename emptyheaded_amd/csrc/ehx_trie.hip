@@ -74,7 +74,8 @@ __global__ void k_layouts(DTrie t, int policy, SelectorWeights sel, uint8_t *__r
       // common::is_sparse (common.hpp:56-67): sparse iff len <= 2 or span/len > 128;
       // for 32-bit values the double test equals span > 128*len.
       lay = (n <= 2 || span > 128ull * n) ? EHX_UINTEGER : EHX_RANGE_BITSET;
-      if (policy == EHX_POLICY_LEARNED && sel.valid) lay = selector_layout(sel, n, span, first, last, lay);
+      if (policy == EHX_POLICY_LEARNED && sel.valid && h < nh)
+        lay = selector_layout(sel, n, span, first, last, set_work(t, v, n), lay);
     }
     layout[h] = (uint8_t)lay;
     bs_start[h] = first >> 6;
@@ -82,38 +83,32 @@ __global__ void k_layouts(DTrie t, int policy, SelectorWeights sel, uint8_t *__r
   }
 }
 
-// One wave per set: OR the bits of a bitset set into its words.
-__global__ __launch_bounds__(256) void k_fill_bits(DTrie t, uint64_t *__restrict__ words) {
+// One wave per bitset set, one lane per 64-bit word: the word's elements are
+// v[lo, hi) with lo = lower_bound(64 * word); rank[word] = lo.  No atomics.
+__global__ __launch_bounds__(256) void k_fill_words(DTrie t, uint64_t *__restrict__ words,
+                                                    uint32_t *__restrict__ rank) {
   const uint32_t nh = t.nh;
   const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t h = wave; h <= nh; h += nwaves) {
     if (t.layout[h] != EHX_RANGE_BITSET) continue;
-    DSet s = set_of(t, (uint32_t)h);
-    uint64_t *w = words + t.bs_off[h];
-    for (uint32_t i = __lane_id(); i < s.n; i += 64) {
-      uint32_t x = s.v[i];
-      atomicOr((unsigned long long *)&w[(x >> 6) - s.start], (unsigned long long)(1ull << (x & 63)));
-    }
-  }
-}
-
-// One wave per bitset set: rank[w] = elements in the set's words before w.
-__global__ __launch_bounds__(256) void k_fill_rank(DTrie t, const uint64_t *__restrict__ words,
-                                                   uint32_t *__restrict__ rank) {
-  const uint32_t nh = t.nh;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t h = wave; h <= nh; h += nwaves) {
-    if (t.layout[h] != EHX_RANGE_BITSET) continue;
-    uint32_t b = t.bs_off[h], W = t.bs_off[h + 1] - b;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < W; base += 64) {
-      uint32_t i = base + __lane_id();
-      uint32_t c = (i < W) ? (uint32_t)__popcll(words[b + i]) : 0u;
-      uint32_t incl = wave_incl_scan(c);
-      if (i < W) rank[b + i] = carry + incl - c;
-      carry += __shfl(incl, 63, 64);
+    const DSet s = set_of(t, (uint32_t)h);
+    const uint32_t b = t.bs_off[h];
+    for (uint32_t i = __lane_id(); i < s.W; i += 64) {
+      const uint64_t lo_val = 64ull * (s.start + i);
+      uint32_t lo = 0, hi = s.n;
+      while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if ((uint64_t)s.v[mid] < lo_val) lo = mid + 1; else hi = mid;
+      }
+      uint64_t wv = 0;
+      uint32_t k = lo;
+      while (k < s.n && (uint64_t)s.v[k] < lo_val + 64) {
+        wv |= 1ull << (s.v[k] & 63);
+        k++;
+      }
+      words[b + i] = wv;
+      rank[b + i] = lo;
     }
   }
 }
@@ -258,11 +253,8 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   t->rank = (uint32_t *)((uint8_t *)blk2 + wbytes);
   t->dense = dense ? (int32_t *)((uint8_t *)blk2 + wbytes + rbytes) : nullptr;
   t->dense_len = dense ? (uint32_t)dense_len : 0;
-  if ((rc = fill_u64(ctx, t->words, 0, W))) return rc;
   dv = view(t);
-  k_fill_bits<<<grid_for((nh + 1) * 64, 256, 16384), 256, 0, st>>>(dv, t->words);
-  EHX_HIP(hipGetLastError());
-  k_fill_rank<<<grid_for((nh + 1) * 64, 256, 16384), 256, 0, st>>>(dv, t->words, t->rank);
+  k_fill_words<<<grid_for((nh + 1) * 64, 256, 16384), 256, 0, st>>>(dv, t->words, t->rank);
   EHX_HIP(hipGetLastError());
   if (dense) {
     if ((rc = fill_u32(ctx, (uint32_t *)t->dense, 0xFFFFFFFFu, dense_len))) return rc;

@@ -29,6 +29,9 @@ struct TunedArgs {
   Tries T;
   const uint32_t *prefix[EHX_MAX_ATTRS];
   uint64_t nprefix;
+  unsigned long long *rec;  // per-prefix shader-clock cycles when recording, else null
+  int32_t force_mode;       // -1: staging by layout; 0: bitmap where it fits; 1: hash where it fits
+  int32_t pad;
 };
 
 // Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.
@@ -39,6 +42,17 @@ struct TuneParams {
   int32_t grab;           // queue items taken per atomic
   int32_t bitmap_all;     // 1: stage any set whose word window fits as a bitmap
   int32_t use_graph;      // capture the level in a hipGraph and replay it
+  int32_t big_work;       // prefixes with at least this much estimated work go to the workgroup bin
+};
+
+constexpr int TUNED_MAX_S = 4;  // operand sets of S(p) the tuned kernel accepts
+
+// A request to time every prefix of the tuned level under a forced staging.
+struct RecordReq {
+  uint64_t *host_out;  // cycles per prefix (host)
+  uint64_t cap;
+  uint64_t *n_out;     // number of prefixes
+  int force_mode;      // -1 layout-driven, 0 bitmap, 1 hash
 };
 
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp);

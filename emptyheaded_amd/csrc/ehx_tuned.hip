@@ -2,11 +2,21 @@
 //
 // Pipeline (all device-side, so the whole level can be captured in a hipGraph):
 //   k_classify     estimate the work of each prefix (sampled sum of child-set
-//                  sizes over S(p)) and bin it by log2(work), largest first;
-//   k_bucket_scan  bin offsets and queue length; resets the queue cursor;
-//   k_enqueue      scatter prefix ids into the binned queue;
-//   k_tuned        persistent wavefronts pull `grab` prefixes per atomic, stage
-//                  S(p) in LDS (bitmap or hash) and stream each child(x) through it.
+//                  sizes over S(p)) and bin it: a "block" bin for prefixes whose
+//                  work or staged set is large, a "wave" bin for the rest, each
+//                  ordered by log2(work), largest first;
+//   k_bucket_scan  bin offsets, queue length, block-bin size; resets cursors;
+//   k_enqueue      scatter prefix ids into the binned queue (one global atomic
+//                  per bin per workgroup);
+//   k_tuned        persistent 4-wave workgroups.  Phase 1: the workgroup takes
+//                  block-bin prefixes one at a time, stages S(p) once in LDS and
+//                  its four waves split the child sets.  Phase 2: each wave takes
+//                  wave-bin prefixes on its own with a quarter of the LDS.
+// Staging: S(p) as an LDS bitmap when it is a bitset whose word window fits,
+// else as an open-addressing hash table behind an LDS Bloom-filter bitmap (most
+// probes miss and stop at the filter), else probed in HBM.  Child sets shorter
+// than big_deg are expanded into an LDS owner window so every lane streams one
+// element per step; longer ones are streamed by the whole wave.
 // Results are exact: the count is the same intersection the plain form
 // computes, summed in uint64 (order-independent).
 #include <stdio.h>
@@ -16,7 +26,7 @@
 
 namespace ehx {
 
-static TuneParams g_params = {4096, 64, 9, 2, 0, 0};
+static TuneParams g_params = {4096, 32, 24, 2, 0, 0, 4096};
 TuneParams &tune_params() { return g_params; }
 
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
@@ -31,12 +41,15 @@ bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
     else if (o.parent >= 0) { tp->s_rel[tp->ns] = o.rel; tp->s_parent[tp->ns] = o.parent; tp->ns++; }
     else return false;
   }
-  if (xcount != 1 || tp->ns < 1) return false;
+  if (xcount != 1 || tp->ns < 1 || tp->ns > TUNED_MAX_S) return false;
   bool used[EHX_MAX_OPS] = {false};
   int matched = 0;
   for (int i = 0; i < p->num_ops[K - 2]; i++) {
     const ehx_op &o = p->ops[K - 2][i];
-    if (o.parent < 0) { tp->h_rel[tp->nhf++] = o.rel; continue; }
+    if (o.parent < 0) {
+      if (o.rel != tp->x_rel) tp->h_rel[tp->nhf++] = o.rel;  // x's own head is implied by its child lookup
+      continue;
+    }
     bool found = false;
     for (int s = 0; s < tp->ns && !found; s++)
       if (!used[s] && tp->s_rel[s] == o.rel && tp->s_parent[s] == o.parent) { used[s] = true; found = true; }
@@ -47,14 +60,21 @@ bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
 }
 
 static constexpr uint32_t EMPTY = 0xFFFFFFFFu;
-static constexpr uint32_t NBUCKETS = 64, BUCKET_NONE = 63;
+static constexpr uint32_t NBUCKETS = 64, BUCKET_NONE = 63, WAVE_BIN = 32;
+static constexpr int OWN = 256;  // owner window per wave
 
-__device__ __forceinline__ uint32_t hash32(uint32_t x, uint32_t shift) { return (x * 0x9E3779B1u) >> shift; }
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
+template <int MAXS>
 __device__ __forceinline__ bool resolve_s(const TunedPlan &P, const TunedArgs &A, uint64_t p, DSet *S) {
   uint32_t pv[EHX_MAX_ATTRS];
   for (int j = 0; j <= P.K - 3; j++) pv[j] = A.prefix[j][p];
-  for (int i = 0; i < P.ns; i++) {
+#pragma unroll
+  for (int i = 0; i < MAXS; i++) {
+    if (i >= P.ns) break;
     const DTrie &t = A.T.t[P.s_rel[i]];
     int32_t h = node_of(t, pv[P.s_parent[i]]);
     if (h < 0) return false;
@@ -70,22 +90,25 @@ __device__ __forceinline__ bool head_ok(const TunedPlan &P, const TunedArgs &A, 
   return true;
 }
 
-__global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ bucket, unsigned *__restrict__ hist) {
+// Wave capacity of the hash table for a given LDS slot budget.
+__host__ __device__ constexpr uint32_t wave_hash_cap(int slots) { return (uint32_t)slots / 8; }
+
+__global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ bucket, unsigned *__restrict__ hist,
+                           int slots, int big_work) {
   __shared__ unsigned h[NBUCKETS];
   if (threadIdx.x < NBUCKETS) h[threadIdx.x] = 0;
   __syncthreads();
   const DTrie &tx = A.T.t[P.x_rel];
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.nprefix;
        p += (uint64_t)gridDim.x * blockDim.x) {
-    DSet S[EHX_MAX_OPS];
+    DSet S[TUNED_MAX_S];
     uint32_t b = BUCKET_NONE;
-    if (resolve_s(P, A, p, S)) {
+    if (resolve_s<TUNED_MAX_S>(P, A, p, S)) {
       int mi = 0;
       for (int i = 1; i < P.ns; i++)
         if (S[i].n < S[mi].n) mi = i;
       const DSet D = S[mi];
-      // sample up to 32 elements of the smallest S set
-      const uint32_t ns = D.n < 32 ? D.n : 32;
+      const uint32_t ns = D.n < 32 ? D.n : 32;  // sample up to 32 elements of the smallest S set
       uint64_t w = 0;
 #pragma unroll 8
       for (uint32_t k = 0; k < ns; k++) {
@@ -95,8 +118,13 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
       }
       w = w * D.n / (ns ? ns : 1) + D.n;
       if (P.ns > 1) w = w / 2 + 1;
+      // the wave bin needs S(p) to fit a quarter of the LDS (hash) unless it is a fitting bitset
+      bool fits_wave = D.n <= wave_hash_cap(slots);
+      if (!fits_wave && P.ns == 1 && D.lay == EHX_RANGE_BITSET && D.n)
+        fits_wave = ((D.v[D.n - 1] >> 5) - (D.v[0] >> 5) + 1) <= (uint32_t)slots / 4;
+      const bool big = w >= (uint64_t)big_work || !fits_wave;
       uint32_t lg = 63 - __clzll(w | 1);
-      b = 31 - (lg > 31 ? 31 : lg);
+      b = (big ? 0u : WAVE_BIN) + (31 - (lg > 31 ? 31 : lg));
     }
     bucket[p] = b;
     atomicAdd(&h[b], 1u);
@@ -105,120 +133,192 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
   if (threadIdx.x < NBUCKETS && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
-// ctl layout: [0..63] hist, [64..127] bucket base / cursor, [128] queue length, [129] pull cursor
+// ctl: [0..63] hist, [64..127] bin base / cursor, [128] queue length,
+//      [129] block-bin cursor, [130] wave-bin cursor, [131] block-bin size
 __global__ void k_bucket_scan(unsigned *__restrict__ ctl) {
   const uint32_t lane = threadIdx.x;
   unsigned c = (lane < BUCKET_NONE) ? ctl[lane] : 0u;
   unsigned incl = wave_incl_scan(c);
   ctl[64 + lane] = incl - c;
-  if (lane == 63) { ctl[128] = incl; ctl[129] = 0; }
+  if (lane == WAVE_BIN - 1) ctl[131] = incl;
+  if (lane == 63) { ctl[128] = incl; ctl[129] = 0; ctl[130] = 0; }
 }
 
-__global__ void k_enqueue(const uint32_t *__restrict__ bucket, uint64_t n, unsigned *__restrict__ ctl,
-                          uint32_t *__restrict__ queue) {
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t b = bucket[p];
-    if (b != BUCKET_NONE) queue[atomicAdd(&ctl[64 + b], 1u)] = (uint32_t)p;
+__global__ __launch_bounds__(256) void k_enqueue(const uint32_t *__restrict__ bucket, uint64_t n,
+                                                 unsigned *__restrict__ ctl, uint32_t *__restrict__ queue) {
+  __shared__ unsigned cnt[NBUCKETS], base[NBUCKETS];
+  for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x; t0 < n; t0 += (uint64_t)gridDim.x * blockDim.x) {
+    if (threadIdx.x < NBUCKETS) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t p = t0 + threadIdx.x;
+    const uint32_t b = (p < n) ? bucket[p] : BUCKET_NONE;
+    unsigned local = 0;
+    if (b != BUCKET_NONE) local = atomicAdd(&cnt[b], 1u);
+    __syncthreads();
+    if (threadIdx.x < NBUCKETS && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&ctl[64 + threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (b != BUCKET_NONE) queue[base[b] + local] = (uint32_t)p;
+    __syncthreads();
   }
 }
 
-template <int SLOTS, int VCAP>
-__device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, uint32_t *tab, uint32_t *sv,
-                                 uint32_t *cst, uint32_t *cof, int big_deg, int bitmap_all) {
+// Staging of one S(p) in LDS.  mode 0 bitmap, 1 hash + filter, 2 probe in HBM.
+struct Stage {
+  int mode;
+  uint32_t bm_base, bm_words;
+  uint32_t hshift, hmask, fshift;
+  const uint32_t *tab, *flt;
+};
+
+__device__ __forceinline__ uint32_t hash_t(uint32_t c, uint32_t shift) { return (c * 0x9E3779B1u) >> shift; }
+__device__ __forceinline__ uint32_t hash_f(uint32_t c, uint32_t shift) { return (c * 0x85EBCA77u) >> shift; }
+
+template <int MAXS>
+__device__ __forceinline__ uint32_t probe(const Stage &s, const DSet *S, int ns, uint32_t c) {
+  if (s.mode == 0) {
+    const uint32_t w = (c >> 5) - s.bm_base;
+    return (w < s.bm_words) ? ((s.tab[w] >> (c & 31)) & 1u) : 0u;
+  }
+  if (s.mode == 1) {
+    const uint32_t f = hash_f(c, s.fshift);
+    if (!((s.flt[f >> 5] >> (f & 31)) & 1u)) return 0u;
+    uint32_t h = hash_t(c, s.hshift);
+    for (;;) {
+      const uint32_t v = s.tab[h];
+      if (v == c) return 1u;
+      if (v == EMPTY) return 0u;
+      h = (h + 1) & s.hmask;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MAXS; i++)
+    if (i < ns && !set_contains(S[i], c)) return 0u;
+  return 1u;
+}
+
+// One prefix, processed by a group: the whole 4-wave workgroup (BLOCK) or one
+// wave.  tab/flt/sv are the group's LDS; own is the calling wave's window.
+template <bool BLOCK, int MAXS>
+__device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, uint32_t *tab, uint32_t tab_slots,
+                                 uint32_t *flt, uint32_t flt_words, uint32_t *sv, uint32_t sv_cap, uint32_t *own,
+                                 int big_deg, int bitmap_all, int &mode_used) {
   const uint32_t lane = __lane_id();
-  DSet S[EHX_MAX_OPS];
-  if (!resolve_s(P, A, p, S)) return 0;
+  const uint32_t gtid = BLOCK ? threadIdx.x : lane;
+  const uint32_t gsize = BLOCK ? blockDim.x : 64u;
+  const uint32_t wave = BLOCK ? (threadIdx.x >> 6) : 0u;
+  const uint32_t nwaves = BLOCK ? (blockDim.x >> 6) : 1u;
+  auto gsync = [&]() {
+    if (BLOCK) __syncthreads();
+    else wave_sync();
+  };
+  DSet S[MAXS];
+  if (!resolve_s<MAXS>(P, A, p, S)) return 0;
   int mi = 0;
-  for (int i = 1; i < P.ns; i++)
-    if (S[i].n <= S[mi].n) mi = i;
+#pragma unroll
+  for (int i = 1; i < MAXS; i++)
+    if (i < P.ns && S[i].n <= S[mi].n) mi = i;
   const uint32_t *src = nullptr;
   uint32_t sn = 0;
   if (P.ns == 1) {
     src = S[0].v;
     sn = S[0].n;
-  } else if (VCAP > 0 && S[mi].n <= (uint32_t)VCAP) {
-    int alg;
-    sn = wave_intersect<true>(S, P.ns, sv, &alg);
-    __syncthreads();
-    if (sn == 0) return 0;
-    src = sv;
+  } else if (MAXS > 1) {
+    uint32_t mn = S[0].n;
+#pragma unroll
+    for (int i = 1; i < MAXS; i++)
+      if (i < P.ns && S[i].n < mn) mn = S[i].n;
+    if (mn <= sv_cap) {
+      // materialize S(p) = S_1 ∩ .. ∩ S_s (in a workgroup, by its first wave)
+      int alg;
+      if (BLOCK) {
+        __shared__ uint32_t s_sn;
+        if (wave == 0) {
+          uint32_t c = wave_intersect<true>(S, P.ns, sv, &alg);
+          if (lane == 0) s_sn = c;
+        }
+        __syncthreads();
+        sn = s_sn;
+        __syncthreads();
+      } else {
+        sn = wave_intersect<true>(S, P.ns, sv, &alg);
+        wave_sync();
+      }
+      if (sn == 0) return 0;
+      src = sv;
+    }
   }
-  // staging decision (wave-uniform)
-  int mode = 2;  // 0 bitmap, 1 hash, 2 probe the S sets in HBM
-  uint32_t bm_base = 0, bm_words = 0, hshift = 0, hmask = 0;
+  // staging decision (group-uniform)
+  Stage st;
+  st.mode = 2;
+  mode_used = 2;
+  st.tab = tab;
+  st.flt = flt;
+  st.bm_base = st.bm_words = st.hshift = st.hmask = st.fshift = 0;
   if (src) {
     const uint32_t first = src[0], last = src[sn - 1];
     const uint32_t w32 = (last >> 5) - (first >> 5) + 1;
     const bool is_bitset = (P.ns == 1) ? (S[0].lay == EHX_RANGE_BITSET)
                                        : (sn > 2 && (uint64_t)(last - first) <= 128ull * sn);
-    if (w32 <= (uint32_t)SLOTS && (is_bitset || bitmap_all)) mode = 0;
-    else if (sn <= (uint32_t)SLOTS / 2 && last != EMPTY) mode = 1;
-    if (mode == 0) {
-      bm_base = first >> 5;
-      bm_words = w32;
-      for (uint32_t i = lane; i < w32; i += 64) tab[i] = 0;
-      __syncthreads();
+    const bool bm_fits = w32 <= tab_slots, hash_fits = sn <= tab_slots / 2 && last != EMPTY;
+    if (A.force_mode == 0) st.mode = bm_fits ? 0 : (hash_fits ? 1 : 2);
+    else if (A.force_mode == 1) st.mode = hash_fits ? 1 : (bm_fits ? 0 : 2);
+    else if (bm_fits && (is_bitset || bitmap_all)) st.mode = 0;
+    else if (hash_fits) st.mode = 1;
+    if (st.mode == 0) {
+      st.bm_base = first >> 5;
+      st.bm_words = w32;
+      for (uint32_t i = gtid; i < w32; i += gsize) tab[i] = 0;
+      gsync();
       if (P.ns == 1 && S[0].lay == EHX_RANGE_BITSET) {
-        // copy the range_bitset words; 32-bit word i of the bitmap covers
-        // values [(bm_base + i) * 32, +32)
-        for (uint32_t k = lane; k < S[0].W; k += 64) {
-          uint64_t w = S[0].w[k];
-          uint32_t i0 = (S[0].start + k) * 2u - bm_base;  // wraps to 0xFFFFFFFF when the low half precedes first
-          if (i0 < w32) tab[i0] = (uint32_t)w;
-          if (i0 + 1u < w32) tab[i0 + 1u] = (uint32_t)(w >> 32);
+        // copy range_bitset words; 32-bit word i of the bitmap covers values [(bm_base + i) * 32, +32)
+        for (uint32_t k = gtid; k < S[0].W; k += gsize) {
+          const uint64_t wv = S[0].w[k];
+          const uint32_t i0 = (S[0].start + k) * 2u - st.bm_base;  // wraps when the low half precedes `first`
+          if (i0 < w32) tab[i0] = (uint32_t)wv;
+          if (i0 + 1u < w32) tab[i0 + 1u] = (uint32_t)(wv >> 32);
         }
       } else {
-        for (uint32_t i = lane; i < sn; i += 64) {
-          uint32_t x = src[i];
-          atomicOr(&tab[(x >> 5) - bm_base], 1u << (x & 31));
+        for (uint32_t i = gtid; i < sn; i += gsize) {
+          const uint32_t x = src[i];
+          atomicOr(&tab[(x >> 5) - st.bm_base], 1u << (x & 31));
         }
       }
-    } else if (mode == 1) {
+    } else if (st.mode == 1) {
       uint32_t H = 64, lg = 6;
       while (H < 2 * sn) { H <<= 1; lg++; }
-      hshift = 32 - lg;
-      hmask = H - 1;
-      for (uint32_t i = lane; i < H; i += 64) tab[i] = EMPTY;
-      __syncthreads();
-      for (uint32_t i = lane; i < sn; i += 64) {
-        uint32_t x = src[i];
-        uint32_t h = hash32(x, hshift);
-        while (atomicCAS(&tab[h], EMPTY, x) != EMPTY) h = (h + 1) & hmask;
+      st.hshift = 32 - lg;
+      st.hmask = H - 1;
+      uint32_t lgf = 5;
+      while ((1u << lgf) < flt_words * 32) lgf++;
+      st.fshift = 32 - lgf;
+      for (uint32_t i = gtid; i < H; i += gsize) tab[i] = EMPTY;
+      for (uint32_t i = gtid; i < flt_words; i += gsize) flt[i] = 0;
+      gsync();
+      for (uint32_t i = gtid; i < sn; i += gsize) {
+        const uint32_t x = src[i];
+        uint32_t h = hash_t(x, st.hshift);
+        while (atomicCAS(&tab[h], EMPTY, x) != EMPTY) h = (h + 1) & st.hmask;
+        const uint32_t f = hash_f(x, st.fshift);
+        atomicOr(&flt[f >> 5], 1u << (f & 31));
       }
     }
   }
-  __syncthreads();
-
-  auto probe = [&](uint32_t c) -> uint32_t {
-    if (mode == 0) {
-      uint32_t w = (c >> 5) - bm_base;
-      return (w < bm_words) ? ((tab[w] >> (c & 31)) & 1u) : 0u;
-    }
-    if (mode == 1) {
-      uint32_t h = hash32(c, hshift);
-      for (;;) {
-        uint32_t v = tab[h];
-        if (v == c) return 1u;
-        if (v == EMPTY) return 0u;
-        h = (h + 1) & hmask;
-      }
-    }
-    for (int i = 0; i < P.ns; i++)
-      if (!set_contains(S[i], c)) return 0u;
-    return 1u;
-  };
+  gsync();
+  mode_used = st.mode;
 
   const DTrie &tx = A.T.t[P.x_rel];
   const uint32_t *xsrc = src ? src : S[mi].v;
   const uint32_t xn = src ? sn : S[mi].n;
   uint64_t hits = 0;
-  for (uint32_t base = 0; base < xn; base += 64) {
+  for (uint32_t base = wave * 64; base < xn; base += nwaves * 64) {
     const uint32_t idx = base + lane;
     bool ok = idx < xn;
     const uint32_t x = ok ? xsrc[idx] : 0u;
-    if (ok && !src)
-      for (int i = 0; i < P.ns && ok; i++)
-        if (i != mi) ok = set_contains(S[i], x);
+    if (ok && !src) {
+#pragma unroll
+      for (int i = 0; i < MAXS; i++)
+        if (i < P.ns && i != mi && ok) ok = set_contains(S[i], x);
+    }
     if (ok) ok = head_ok(P, A, x);
     const int32_t nx = ok ? node_of(tx, x) : -1;
     uint32_t o = 0, d = 0;
@@ -226,71 +326,96 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
       o = tx.off[nx];
       d = tx.off[nx + 1] - o;
     }
+    // long child sets: the whole wave streams each one
     uint64_t big = __ballot(d >= (uint32_t)big_deg);
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
       const uint32_t bo = __shfl(o, l, 64), bd = __shfl(d, l, 64);
-      for (uint32_t e = lane; e < bd; e += 64) hits += probe(tx.vals[bo + e]);
+      for (uint32_t e = lane; e < bd; e += 64) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + e]);
     }
+    // short ones: expand owners into the LDS window, then one element per lane per step
     const uint32_t ds = (d < (uint32_t)big_deg) ? d : 0u;
     const uint32_t incl = wave_incl_scan(ds);
     const uint32_t T = __shfl(incl, 63, 64);
-    if (T) {
-      cst[lane] = incl - ds;
-      cof[lane] = o;
-      __syncthreads();
-      for (uint32_t e = lane; e < T; e += 64) {
-        uint32_t lo = 0, hi = 64;
-        while (hi - lo > 1) {
-          uint32_t mid = (lo + hi) >> 1;
-          if (cst[mid] <= e) lo = mid; else hi = mid;
-        }
-        hits += probe(tx.vals[cof[lo] + (e - cst[lo])]);
-      }
-      __syncthreads();
+    const uint32_t s0 = incl - ds;
+    const uint32_t rebase = o - s0;  // element e of the window lives at vals[rebase + e] (mod 2^32)
+    for (uint32_t r0 = 0; r0 < T; r0 += OWN) {
+      const uint32_t lo = s0 > r0 ? s0 : r0;
+      const uint32_t hi = (s0 + ds < r0 + OWN) ? s0 + ds : r0 + OWN;
+      for (uint32_t i = lo; i < hi; i++) own[i - r0] = rebase;
+      wave_sync();
+      const uint32_t lim = (T - r0 < (uint32_t)OWN) ? T - r0 : (uint32_t)OWN;
+      for (uint32_t e = lane; e < lim; e += 64) hits += probe<MAXS>(st, S, P.ns, tx.vals[own[e] + r0 + e]);
+      wave_sync();
     }
   }
-  __syncthreads();
+  gsync();
   return hits;
 }
 
-template <int SLOTS, int VCAP>
-__global__ __launch_bounds__(64) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
-                                              unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
-                                              int big_deg, int grab, int bitmap_all) {
+template <int SLOTS, int MAXS>
+__global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
+                                               unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
+                                               int big_deg, int grab, int bitmap_all) {
+  constexpr uint32_t WS = SLOTS / 4;                 // per-wave table slots
+  constexpr uint32_t FWB = SLOTS / 4, FWW = FWB / 4;  // filter words per block / wave
+  constexpr uint32_t VC = (MAXS > 1) ? WS : 1u;      // materialized S capacity per wave
   __shared__ uint32_t tab[SLOTS];
-  __shared__ uint32_t sv[VCAP > 0 ? VCAP : 1];
-  __shared__ uint32_t cst[64], cof[64];
-  const uint32_t qlen = ctl[128];
-  unsigned long long cnt = 0;
+  __shared__ uint32_t flt[FWB];
+  __shared__ uint32_t own[4][OWN];
+  __shared__ uint32_t sv[4 * VC];
+  __shared__ uint32_t s_item;
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t qlen = ctl[128], nbig = ctl[131];
+  uint64_t hits = 0;
+  // phase 1: block bin, one prefix per workgroup
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&ctl[129], 1u);
+    __syncthreads();
+    const uint32_t qi = s_item;
+    __syncthreads();
+    if (qi >= nbig) break;
+    const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
+    int mode = -1;
+    hits += tuned_prefix<true, MAXS>(P, A, queue[qi], tab, SLOTS, flt, FWB, sv, 4 * VC, own[w], big_deg, bitmap_all,
+                                     mode);
+    if (A.rec && threadIdx.x == 0 && mode >= 0)
+      A.rec[queue[qi]] = (__builtin_amdgcn_s_memtime() - t0) | ((unsigned long long)mode << 62);
+  }
+  // phase 2: wave bin, each wave on its own
   for (;;) {
     uint32_t q0 = 0;
-    if (__lane_id() == 0) q0 = atomicAdd(&ctl[129], (unsigned)grab);
-    q0 = __shfl(q0, 0, 64);
+    if (__lane_id() == 0) q0 = atomicAdd(&ctl[130], (unsigned)grab);
+    q0 = __shfl(q0, 0, 64) + nbig;
     if (q0 >= qlen) break;
     const uint32_t q1 = min(q0 + (uint32_t)grab, qlen);
-    for (uint32_t qi = q0; qi < q1; qi++)
-      cnt += tuned_prefix<SLOTS, VCAP>(P, A, queue[qi], tab, sv, cst, cof, big_deg, bitmap_all);
+    for (uint32_t qi = q0; qi < q1; qi++) {
+      const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
+      int mode = -1;
+      hits += tuned_prefix<false, MAXS>(P, A, queue[qi], tab + w * WS, WS, flt + w * FWW, FWW, sv + w * VC, VC,
+                                        own[w], big_deg, bitmap_all, mode);
+      if (A.rec && __lane_id() == 0 && mode >= 0)
+        A.rec[queue[qi]] = (__builtin_amdgcn_s_memtime() - t0) | ((unsigned long long)mode << 62);
+    }
   }
-  cnt = wave_sum(cnt);
-  if (__lane_id() == 0) atomicAdd(acc, cnt);
+  hits = wave_sum(hits);
+  if (__lane_id() == 0) atomicAdd(acc, (unsigned long long)hits);
 }
 
 template <int SLOTS>
 static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, const TunedArgs &A, const uint32_t *queue,
                          unsigned *ctl, unsigned long long *acc, const TuneParams &tp) {
   if (P.ns == 1)
-    k_tuned<SLOTS, 0><<<grid, 64, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all);
+    k_tuned<SLOTS, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all);
   else
-    k_tuned<SLOTS, 2048><<<grid, 64, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all);
+    k_tuned<SLOTS, TUNED_MAX_S><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all);
 }
 
-// One cached graph per context: the key is every value baked into the launches.
+// One cached graph per process: the key is every value baked into the launches.
 struct GraphCache {
   bool valid = false;
   int device = -1;
-  hipStream_t stream = nullptr;
   unsigned char key[sizeof(TunedPlan) + sizeof(TunedArgs) + sizeof(TuneParams) + 64];
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -301,14 +426,15 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
                          uint32_t *bucket, unsigned *ctl, uint32_t *queue, const TuneParams &tp) {
   hipStream_t st = ctx->stream;
   int rc;
-  if ((rc = fill_u32(ctx, ctl, 0u, 130))) return rc;
-  k_classify<<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(P, A, bucket, ctl);
+  if ((rc = fill_u32(ctx, ctl, 0u, 132))) return rc;
+  k_classify<<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(P, A, bucket, ctl, tp.lds_slots, tp.big_work);
   EHX_HIP(hipGetLastError());
   k_bucket_scan<<<1, 64, 0, st>>>(ctl);
   EHX_HIP(hipGetLastError());
-  k_enqueue<<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(bucket, A.nprefix, ctl, queue);
+  k_enqueue<<<grid_for(A.nprefix, 256, 8192), 256, 0, st>>>(bucket, A.nprefix, ctl, queue);
   EHX_HIP(hipGetLastError());
-  unsigned grid = (unsigned)ctx->num_cus * (unsigned)(tp.waves_per_cu > 0 ? tp.waves_per_cu : 1);
+  const unsigned per_cu = (unsigned)(tp.waves_per_cu > 4 ? tp.waves_per_cu / 4 : 1);
+  const unsigned grid = (unsigned)ctx->num_cus * per_cu;
   EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
   switch (tp.lds_slots) {
     case 2048: launch_tuned<2048>(st, grid, P, A, queue, ctl, acc, tp); break;
@@ -349,7 +475,7 @@ int tuned_run(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, int form, un
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc || e != hipSuccess) {
-      // capture refused (e.g. an event record): run this level eagerly from now on
+      // capture refused: run this level eagerly from now on
       if (g) (void)hipGraphDestroy(g);
       (void)hipGetLastError();
       g_params.use_graph = 0;

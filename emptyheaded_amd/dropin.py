@@ -1,0 +1,53 @@
+"""Runs the restated generated queries (emptyheaded_amd/engine/queries) through
+the C++ drop-in API, from a database folder laid out as the generated code
+expects: <db>/relations/<R>/<R>_0_1/ehx_rows.bin."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DROPIN_LIB = os.path.join(HERE, "engine", "libehx_dropin.so")
+QUERIES = ("TriangleAgg", "FliqueAgg")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(DROPIN_LIB):
+            raise RuntimeError("libehx_dropin.so is not built: run `python -m emptyheaded_amd.build`")
+        from . import lib as ehx_lib
+        ehx_lib()  # libehx.so first
+        L = C.CDLL(DROPIN_LIB)
+        for q in QUERIES:
+            f = getattr(L, "c_run_" + q)
+            f.argtypes = [C.c_char_p, C.POINTER(C.c_ulong)]
+            f.restype = C.c_long
+        _lib = L
+    return _lib
+
+
+def write_relation(db, name, src, dst, ordering="0_1"):
+    """Stores a binary relation for Trie<A,M>::load (rows, then both columns)."""
+    d = os.path.join(db, "relations", name, "%s_%s" % (name, ordering))
+    os.makedirs(d, exist_ok=True)
+    s = np.ascontiguousarray(src, np.uint32)
+    t = np.ascontiguousarray(dst, np.uint32)
+    with open(os.path.join(d, "ehx_rows.bin"), "wb") as f:
+        f.write(np.uint64(len(s)).tobytes())
+        f.write(s.tobytes())
+        f.write(t.tobytes())
+    os.makedirs(os.path.join(db, "encodings", "uint32_t"), exist_ok=True)
+    n = int(max(s.max(), t.max())) + 1 if len(s) else 0
+    with open(os.path.join(db, "encodings", "uint32_t", "ehx_encoding.bin"), "wb") as f:
+        f.write(np.uint64(n).tobytes())
+    return d
+
+
+def run(query, db):
+    """Returns (annotation, num_rows) of the query's output bag."""
+    rows = C.c_ulong(0)
+    v = getattr(lib(), "c_run_" + query)(db.encode(), C.byref(rows))
+    return v, rows.value

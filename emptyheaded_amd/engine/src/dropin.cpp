@@ -1,0 +1,371 @@
+// dropin.cpp -- runtime behind the drop-in storage-engine API.
+//
+// Tries are device tries of libehx with a host mirror of the head set (values,
+// child offsets, dense value -> node array) so iterators can navigate on the
+// host.  Builders record their build_aggregated_set calls per level and chunk
+// of level-0 values, and each recorded level is computed in one batched GPU
+// launch (ehx_intersect_nodes[_count]); results are replayed to the generated
+// code's lambdas in their original call order.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <atomic>
+#include <fstream>
+#include <mutex>
+#include <thread>
+
+#include "../../../include/ehx.h"
+#include "../include/emptyheaded.hpp"
+
+typedef TrieBlock<hybrid, ParMemoryBuffer> Block;
+
+namespace ehx_dropin {
+
+struct HostTrie {
+  ehx_trie *dev = nullptr;
+  uint64_t nh = 0, m = 0;
+  std::vector<uint32_t> head, off;
+  std::vector<int32_t> dense;
+  std::vector<Block> blocks;  // nh + 1 (the last is the head block)
+};
+
+static std::mutex g_mu;
+static ehx_ctx *g_ctx = nullptr;
+
+[[noreturn]] void not_implemented(const char *what) {
+  std::cerr << "ERROR: NOT IMPLEMENTED: " << what << std::endl;
+  abort();
+}
+
+static void check(int rc, const char *what) {
+  if (rc != EHX_OK) {
+    std::cerr << "ehx error " << rc << " in " << what << ": " << ehx_last_error() << std::endl;
+    abort();
+  }
+}
+
+static ehx_ctx *ctx() {
+  std::lock_guard<std::mutex> g(g_mu);
+  if (!g_ctx) check(ehx_ctx_create(0, &g_ctx), "ehx_ctx_create");
+  return g_ctx;
+}
+
+HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
+  HostTrie *t = new HostTrie();
+  check(ehx_trie_build(ctx(), c0, c1, rows, EHX_POLICY_RULE, 0, &t->dev), "ehx_trie_build");
+  ehx_trie_meta meta;
+  check(ehx_trie_info(t->dev, &meta), "ehx_trie_info");
+  t->nh = meta.num_heads;
+  t->m = meta.num_rows;
+  t->head.resize(t->nh);
+  t->off.resize(t->nh + 1);
+  check(ehx_trie_host_arrays(ctx(), t->dev, t->head.data(), t->off.data()), "ehx_trie_host_arrays");
+  std::vector<uint8_t> types(t->nh);
+  std::vector<uint32_t> cards(t->nh);
+  check(ehx_trie_node_info(ctx(), t->dev, types.data(), cards.data()), "ehx_trie_node_info");
+  // dense navigation when the head is dense by (cardinality, max): TrieBlock.hpp:26-35
+  const uint32_t maxh = t->nh ? t->head.back() : 0;
+  if (t->nh && !common::is_sparse(t->nh, maxh)) {
+    t->dense.assign((size_t)maxh + 1, -1);
+    for (uint64_t h = 0; h < t->nh; h++) t->dense[t->head[h]] = (int32_t)h;
+  }
+  t->blocks.resize(t->nh + 1);
+  for (uint64_t h = 0; h <= t->nh; h++) {
+    Block &b = t->blocks[h];
+    b.trie = t;
+    b.node = (uint32_t)h;
+    if (h < t->nh) {
+      b.set.cardinality = t->off[h + 1] - t->off[h];
+      b.set.type = types[h] == EHX_RANGE_BITSET ? type::RANGE_BITSET : type::UINTEGER;
+      b.set.number_of_bytes = 4ull * b.set.cardinality;
+    } else {
+      b.set.cardinality = (uint32_t)t->nh;
+      b.set.range = maxh;
+      b.set.values = t->head.data();
+      b.set.type = meta.head_layout == EHX_RANGE_BITSET ? type::RANGE_BITSET : type::UINTEGER;
+      b.set.number_of_bytes = 4ull * t->nh;
+    }
+  }
+  return t;
+}
+
+static std::string rows_file(const std::string &path) { return path + "/ehx_rows.bin"; }
+
+HostTrie *load_host_trie(const std::string &path) {
+  std::ifstream in(rows_file(path), std::ios::binary);
+  if (!in) {
+    std::cerr << "ERROR: cannot open " << rows_file(path) << std::endl;
+    abort();
+  }
+  uint64_t rows = 0;
+  in.read((char *)&rows, 8);
+  std::vector<uint32_t> c0(rows), c1(rows);
+  in.read((char *)c0.data(), 4 * rows);
+  in.read((char *)c1.data(), 4 * rows);
+  if (!in) {
+    std::cerr << "ERROR: truncated " << rows_file(path) << std::endl;
+    abort();
+  }
+  return build_host_trie(c0.data(), c1.data(), rows);
+}
+
+void host_trie_tuples(const HostTrie *t, std::vector<uint32_t> &c0, std::vector<uint32_t> &c1) {
+  c0.resize(t->m);
+  c1.resize(t->m);
+  check(ehx_trie_export(ctx(), t->dev, c0.data(), c1.data()), "ehx_trie_export");
+}
+
+void save_host_trie(const HostTrie *t, const std::string &path) {
+  std::vector<uint32_t> c0, c1;
+  host_trie_tuples(t, c0, c1);
+  std::ofstream out(rows_file(path), std::ios::binary);
+  uint64_t rows = c0.size();
+  out.write((const char *)&rows, 8);
+  out.write((const char *)c0.data(), 4 * rows);
+  out.write((const char *)c1.data(), 4 * rows);
+}
+
+void free_host_trie(HostTrie *t) {
+  if (t->dev) ehx_trie_destroy(t->dev);
+  delete t;
+}
+
+size_t host_trie_rows(const HostTrie *t) { return t->m; }
+uint32_t num_heads(const HostTrie *t) { return (uint32_t)t->nh; }
+void *block_of(const HostTrie *t, uint32_t node) { return (void *)&t->blocks[node]; }
+
+long head_node(const HostTrie *t, uint32_t data) {
+  if (!t->dense.empty()) return data < t->dense.size() ? t->dense[data] : -1;
+  auto it = std::lower_bound(t->head.begin(), t->head.end(), data);
+  return (it != t->head.end() && *it == data) ? (long)(it - t->head.begin()) : -1;
+}
+
+// ---- batched execution -----------------------------------------------------------
+
+static void run_batch(int nops, const HostTrie *const *tries, std::vector<std::vector<uint32_t>> &nodes, bool count_only,
+                      std::vector<uint64_t> &counts, std::vector<uint64_t> &offsets, std::vector<uint32_t> &values) {
+  const uint64_t n = nodes.empty() ? 0 : nodes[0].size();
+  ehx_trie *dt[8];
+  const uint32_t *np[8];
+  for (int s = 0; s < nops; s++) {
+    dt[s] = tries[s]->dev;
+    np[s] = nodes[s].data();
+  }
+  counts.assign(n, 0);
+  if (n) check(ehx_intersect_nodes_count(ctx(), nops, dt, np, n, counts.data()), "ehx_intersect_nodes_count");
+  if (count_only) return;
+  uint64_t total = 0;
+  for (uint64_t c : counts) total += c;
+  offsets.assign(n + 1, 0);
+  values.assign(total ? total : 1, 0);
+  uint64_t got = 0;
+  if (n) check(ehx_intersect_nodes(ctx(), nops, dt, np, n, offsets.data(), values.data(), total, &got),
+               "ehx_intersect_nodes");
+}
+
+static inline void set_empty(Set<hybrid> *out) {
+  *out = Set<hybrid>(0, 0, 0, type::UINTEGER);
+}
+
+static inline const Block *blk(const Operands &ops, int i) { return (const Block *)ops.blocks[i]; }
+
+static type::layout result_type(const Operands &ops) {
+  // U x U -> U, U x R -> U, R x R -> R (ops/intersection/set.hpp:53-91)
+  for (int i = 0; i < ops.n; i++)
+    if (blk(ops, i)->set.type != type::RANGE_BITSET) return type::UINTEGER;
+  return type::RANGE_BITSET;
+}
+
+// Level-0 results live as long as the builder that asked for them.
+static std::mutex g_head_mu;
+static std::vector<std::vector<uint32_t>> g_head_results;
+
+size_t aggregate_head(const Operands &ops, Set<hybrid> *out) {
+  for (int i = 0; i < ops.n; i++)
+    if (!blk(ops, i)) { set_empty(out); return 0; }
+  bool same = true;
+  for (int i = 1; i < ops.n; i++) same = same && ops.blocks[i] == ops.blocks[0];
+  const Block *b0 = blk(ops, 0);
+  if (same && b0->set.values) {  // head ∩ head of one trie is the head itself
+    *out = b0->set;
+    return out->cardinality;
+  }
+  std::vector<std::vector<uint32_t>> nodes(ops.n, std::vector<uint32_t>(1));
+  const HostTrie *tries[8];
+  for (int i = 0; i < ops.n; i++) {
+    tries[i] = blk(ops, i)->trie;
+    nodes[i][0] = blk(ops, i)->node;
+  }
+  std::vector<uint64_t> counts, offsets;
+  std::vector<uint32_t> values;
+  run_batch(ops.n, tries, nodes, false, counts, offsets, values);
+  std::lock_guard<std::mutex> g(g_head_mu);
+  g_head_results.push_back(std::move(values));
+  *out = Set<hybrid>((uint32_t)counts[0], counts[0] ? g_head_results.back()[counts[0] - 1] : 0, 4 * counts[0],
+                     result_type(ops));
+  out->values = g_head_results.back().data();
+  return counts[0];
+}
+
+struct LevelLog {  // calls one chunk of level-0 values made at one builder level
+  std::vector<uint32_t> nodes;  // call-major, nops per call
+  uint64_t ncalls = 0, base = 0, cursor = 0;
+};
+
+struct LevelBatch {
+  int nops = 0;
+  const HostTrie *tries[8] = {};
+  bool materialized = false;
+  std::vector<uint64_t> counts, offsets;
+  std::vector<uint32_t> values;
+};
+
+struct Session {
+  size_t num_attrs = 0;
+  int record_level = -1;
+  std::vector<std::vector<LevelLog>> logs;  // [chunk][level]
+  std::vector<LevelBatch> batches;          // [level]
+  std::vector<std::once_flag> sig_once;
+};
+
+static Session *g_sess = nullptr;
+static thread_local uint64_t t_chunk = 0;
+static const uint64_t CHUNK = 128;  // par::for_range dynamic chunk (parallel.cpp:65-79)
+
+bool recording() { return g_sess && g_sess->record_level >= 0; }
+
+size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybrid> *out) {
+  Session *S = g_sess;
+  if (!S) not_implemented("TrieBuilder::build_aggregated_set outside par_foreach_aggregate");
+  if (ops.n < 1 || ops.n > 8) not_implemented("build_aggregated_set arity");
+  for (int i = 0; i < ops.n; i++)
+    if (!blk(ops, i)) { set_empty(out); return 0; }  // a NULL TrieBlock is empty (TrieBuilder.cpp:249-254)
+  const bool innermost = level + 2 == num_attrs;
+  if (ops.n == 1 && (innermost || blk(ops, 0)->set.values)) {
+    *out = blk(ops, 0)->set;  // the set itself (TrieBuilder.cpp:223-240)
+    return out->cardinality;
+  }
+  if (level >= S->logs[t_chunk].size()) not_implemented("builder level beyond num_attributes");
+  LevelLog &L = S->logs[t_chunk][level];
+  LevelBatch &B = S->batches[level];
+  if (S->record_level == (int)level) {
+    std::call_once(S->sig_once[level], [&]() {
+      B.nops = ops.n;
+      for (int i = 0; i < ops.n; i++) B.tries[i] = blk(ops, i)->trie;
+      B.materialized = !innermost;
+    });
+    if (B.nops != ops.n) not_implemented("two call sites with different arity at one builder level");
+    for (int i = 0; i < ops.n; i++) {
+      if (B.tries[i] != blk(ops, i)->trie) not_implemented("two call sites on different tries at one builder level");
+      L.nodes.push_back(blk(ops, i)->node);
+    }
+    L.ncalls++;
+    set_empty(out);
+    return 0;
+  }
+  if (S->record_level >= 0 && (int)level > S->record_level) not_implemented("call below the level being recorded");
+  const uint64_t idx = L.base + L.cursor++;
+  if (B.materialized) {
+    const uint64_t a = B.offsets[idx], b = B.offsets[idx + 1];
+    *out = Set<hybrid>((uint32_t)(b - a), b > a ? B.values[b - 1] : 0, 4 * (b - a), result_type(ops));
+    out->values = B.values.data() + a;
+  } else {
+    *out = Set<hybrid>((uint32_t)B.counts[idx], 0, 4 * B.counts[idx], result_type(ops));
+  }
+  return out->cardinality;
+}
+
+static void run_chunks(const Set<hybrid> *level0, size_t nthreads, const std::function<void(size_t, uint32_t)> &body) {
+  const uint64_t n0 = level0->cardinality;
+  const uint64_t nchunks = (n0 + CHUNK - 1) / CHUNK;
+  std::atomic<uint64_t> next(0);
+  auto worker = [&](size_t tid) {
+    for (;;) {
+      const uint64_t c = next.fetch_add(1);
+      if (c >= nchunks) break;
+      t_chunk = c;
+      const uint64_t e = std::min(n0, (c + 1) * CHUNK);
+      for (uint64_t i = c * CHUNK; i < e; i++) body(tid, level0->values[i]);
+    }
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nthreads; t++) th.emplace_back(worker, t);
+  worker(0);
+  for (auto &x : th) x.join();
+}
+
+void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
+                const std::function<void(size_t, uint32_t)> &body) {
+  if (g_sess) not_implemented("nested par_foreach_aggregate");
+  if (level0->cardinality && !level0->values) not_implemented("level-0 set without host values");
+  const uint64_t nchunks = (level0->cardinality + CHUNK - 1) / CHUNK;
+  const size_t levels = num_attrs >= 2 ? num_attrs - 1 : 0;  // builder levels 0..num_attrs-2
+  Session S;
+  S.num_attrs = num_attrs;
+  S.logs.assign(nchunks, std::vector<LevelLog>(levels));
+  S.batches.resize(levels);
+  S.sig_once = std::vector<std::once_flag>(levels);
+  g_sess = &S;
+  for (size_t p = 0; p < levels; p++) {
+    S.record_level = (int)p;
+    for (auto &cl : S.logs)
+      for (auto &l : cl) l.cursor = 0;
+    run_chunks(level0, nthreads, body);
+    // one batch for the level, calls in chunk order
+    LevelBatch &B = S.batches[p];
+    uint64_t total = 0;
+    for (auto &cl : S.logs) {
+      cl[p].base = total;
+      total += cl[p].ncalls;
+    }
+    if (total == 0) continue;
+    std::vector<std::vector<uint32_t>> nodes(B.nops, std::vector<uint32_t>(total));
+    for (auto &cl : S.logs) {
+      LevelLog &L = cl[p];
+      for (uint64_t i = 0; i < L.ncalls; i++)
+        for (int s = 0; s < B.nops; s++) nodes[s][L.base + i] = L.nodes[i * B.nops + s];
+      std::vector<uint32_t>().swap(L.nodes);
+    }
+    run_batch(B.nops, B.tries, nodes, !B.materialized, B.counts, B.offsets, B.values);
+  }
+  S.record_level = -1;
+  for (auto &cl : S.logs)
+    for (auto &l : cl) l.cursor = 0;
+  run_chunks(level0, nthreads, body);
+  g_sess = nullptr;
+}
+
+}  // namespace ehx_dropin
+
+// ---- utils ---------------------------------------------------------------------------
+
+namespace thread_pool {
+void initializeThreadPool() { (void)ehx_dropin::ctx(); }
+void deleteThreadPool() {}
+}  // namespace thread_pool
+
+namespace par {
+size_t for_range(const size_t from, const size_t to, const size_t block_size,
+                 std::function<void(size_t tid, size_t i)> body) {
+  const size_t len = to > from ? to - from : 0;
+  const size_t bs = (block_size == 0 || block_size > len) ? 1 : block_size;
+  std::atomic<size_t> next(0);
+  auto worker = [&](size_t tid) {
+    for (;;) {
+      const size_t s = next.fetch_add(bs);
+      if (s >= len) break;
+      const size_t e = std::min(len, s + bs);
+      for (size_t j = s; j < e; j++) body(tid, from + j);
+    }
+  };
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < NUM_THREADS; t++) th.emplace_back(worker, t);
+  worker(0);
+  for (auto &x : th) x.join();
+  return 1;
+}
+size_t for_range(const size_t from, const size_t to, std::function<void(size_t tid, size_t i)> body) {
+  const size_t len = to > from ? to - from : 0;
+  return for_range(from, to, (len + NUM_THREADS - 1) / (NUM_THREADS ? NUM_THREADS : 1), body);
+}
+}  // namespace par

@@ -1,0 +1,267 @@
+"""Learned layout selector: training data, training, evaluation, loading.
+
+Data: the engine records, per prefix of the tuned triangle level, the shader
+cycles spent with the prefix's set staged as an LDS bitmap and as an LDS hash
+table (ehx_join_record, force_mode 0 and 1).  For the triangle query the
+prefix is a head vertex a and its staged set is N(a), so each sample is one
+child set of the trie: features from ehx_trie_set_features (the same device
+code the trie build evaluates), targets log2(cycles) per staging, masked where
+the forced staging did not fit.
+
+Training runs on the GPU: ehx_mlp_grad computes the loss and gradient with
+the project's own forward/backward kernels; the Adam update of the FP32
+master weights runs on the host (1,410 parameters).  The density rule stays
+the fallback (no weights loaded, or a set too wide to stage as a bitmap) and
+the judge of regret: per prefix, the oracle-best staging is the faster of the
+two recorded runs, and regret is the extra cycles a chooser spends over it.
+
+    python -m emptyheaded_amd.selector --out gpurun_out/selector.json
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import check, lib, make_plan, plans
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SELECTOR_JSON = os.path.join(HERE, "selector.json")
+IN, H, OUT = 8, 32, 2
+NPARAM = H * IN + H + H * H + H + OUT * H + OUT
+MODE_SHIFT = 62
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _sigs():
+    return lib()
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def features(ctx, T):
+    L = _sigs()
+    nh = T.info()["num_heads"]
+    out = np.zeros((nh, IN), np.float32)
+    check(L.ehx_trie_set_features(ctx.h, T.h, _fp(out)))
+    return out
+
+
+def record(ctx, T, force_mode, plan=plans.TRIANGLE):
+    """Cycles per prefix and the staging actually used (-1: no work)."""
+    L = _sigs()
+    p = make_plan(plan)
+    nh = T.info()["num_heads"]
+    cyc = np.zeros(nh, np.uint64)
+    n = C.c_uint64(0)
+    res = C.c_int64(0)
+    arr = (C.c_void_p * 1)(T.h.value)
+    check(L.ehx_join_record(ctx.h, C.byref(p), arr, force_mode, cyc.ctypes.data_as(C.POINTER(C.c_uint64)), nh,
+                            C.byref(n), C.byref(res)))
+    assert n.value == nh, (n.value, nh)
+    mode = np.where(cyc == 0, -1, (cyc >> np.uint64(MODE_SHIFT)).astype(np.int64))
+    cycles = (cyc & np.uint64((1 << MODE_SHIFT) - 1)).astype(np.float64)
+    return cycles, mode, res.value
+
+
+def dataset(ctx, T, reps=2):
+    """X, Y (log2 cycles: bitmap, hash), M (masks), plus raw cycles."""
+    X = features(ctx, T)
+    c0s, c1s = [], []
+    for _ in range(reps):
+        c0, m0, r0 = record(ctx, T, 0)
+        c1, m1, r1 = record(ctx, T, 1)
+        assert r0 == r1, "staging changed the count"
+        c0s.append(c0)
+        c1s.append(c1)
+    c0 = np.min(c0s, axis=0)
+    c1 = np.min(c1s, axis=0)
+    v0 = (m0 == 0) & (c0 > 0)
+    v1 = (m1 == 1) & (c1 > 0)
+    Y = np.stack([np.log2(c0 + 1), np.log2(c1 + 1)], 1).astype(np.float32)
+    M = np.stack([v0, v1], 1).astype(np.float32)
+    return X, Y, M, c0, c1, r0
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAM, np.float32)
+    o = 0
+    for fan_in, fan_out in ((IN, H), (H, H), (H, OUT)):
+        p[o:o + fan_in * fan_out] = rng.normal(0, np.sqrt(2.0 / fan_in), fan_in * fan_out)
+        o += fan_in * fan_out + fan_out
+    return p
+
+
+class GpuMlp:
+    def __init__(self, ctx, X, Y, M):
+        L = _sigs()
+        self.L = L
+        self.X, self.Y, self.M = (np.ascontiguousarray(a, np.float32) for a in (X, Y, M))
+        h = C.c_void_p()
+        check(L.ehx_mlp_create(ctx.h, _fp(self.X), _fp(self.Y), _fp(self.M), len(self.X), C.byref(h)))
+        self.h = h
+
+    def grad(self, params, norm):
+        g = np.zeros(NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        p = np.ascontiguousarray(params, np.float32)
+        n = np.ascontiguousarray(norm, np.float32)
+        check(self.L.ehx_mlp_grad(self.h, _fp(p), _fp(n), _fp(g), _fp(loss)))
+        return float(loss[0]), g
+
+    def close(self):
+        if self.h:
+            check(self.L.ehx_mlp_destroy(self.h))
+            self.h = None
+
+
+def predict(ctx, params, norm, X):
+    L = _sigs()
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros((len(X), OUT), np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    n = np.ascontiguousarray(norm, np.float32)
+    check(L.ehx_mlp_predict(ctx.h, _fp(p), _fp(n), _fp(X), len(X), _fp(out)))
+    return out
+
+
+def make_norm(X):
+    mu = X.mean(0)
+    sd = X.std(0)
+    sd[sd < 1e-6] = 1.0
+    return np.concatenate([mu, 1.0 / sd]).astype(np.float32)
+
+
+def train(ctx, X, Y, M, steps=600, lr=3e-3, seed=0, log_every=100):
+    norm = make_norm(X)
+    p = init_params(seed)
+    # the output bias starts at the mean target so training begins near the data
+    for k in range(OUT):
+        sel = M[:, k] > 0
+        if sel.any():
+            p[NPARAM - OUT + k] = float(Y[sel, k].mean())
+    mlp = GpuMlp(ctx, X, Y, M)
+    m1 = np.zeros(NPARAM, np.float64)
+    m2 = np.zeros(NPARAM, np.float64)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    hist = []
+    master = p.astype(np.float64)
+    try:
+        for t in range(1, steps + 1):
+            loss, g = mlp.grad(master.astype(np.float32), norm)
+            g = g.astype(np.float64)
+            m1 = b1 * m1 + (1 - b1) * g
+            m2 = b2 * m2 + (1 - b2) * g * g
+            step = lr * (m1 / (1 - b1 ** t)) / (np.sqrt(m2 / (1 - b2 ** t)) + eps)
+            master -= step
+            if t == 1 or t % log_every == 0 or t == steps:
+                hist.append((t, loss))
+                log("  step %4d  loss %.4f" % (t, loss))
+    finally:
+        mlp.close()
+    return master.astype(np.float32), norm, hist
+
+
+def regret(c0, c1, M, choose_bitmap):
+    """Extra cycles over the per-prefix best staging, on prefixes where both
+    stagings were recorded."""
+    both = (M[:, 0] > 0) & (M[:, 1] > 0)
+    best = np.minimum(c0, c1)[both]
+    chosen = np.where(choose_bitmap[both], c0[both], c1[both])
+    return {"prefixes": int(both.sum()), "best_cycles": float(best.sum()), "chosen_cycles": float(chosen.sum()),
+            "regret": float((chosen.sum() - best.sum()) / max(best.sum(), 1.0))}
+
+
+def rule_choice(ctx, T):
+    types, cards = T.node_info()
+    return types == 0  # bitset layout -> staged as a bitmap
+
+
+def learned_choice(ctx, params, norm, X, max_words):
+    pred = predict(ctx, params, norm, X)
+    words64 = np.exp2(X[:, 4])
+    n = np.exp2(X[:, 0]) - 1
+    ok = (words64 <= max_words + 0.5) & (n > 2.5)
+    return ok & (pred[:, 0] < pred[:, 1]), pred
+
+
+def params_fp16_bits(params):
+    return np.asarray(params, np.float32).astype(np.float16).view(np.uint16)
+
+
+def load(device, params, norm, max_words):
+    L = lib()
+    bits = np.ascontiguousarray(params_fp16_bits(params))
+    n = np.ascontiguousarray(norm, np.float32)
+    check(L.ehx_selector_load(device, bits.ctypes.data_as(C.POINTER(C.c_uint16)), len(bits), _fp(n),
+                              float(max_words)))
+
+
+def load_committed(device=0, path=SELECTOR_JSON):
+    with open(path) as f:
+        d = json.load(f)
+    load(device, np.array(d["params"], np.float32), np.array(d["norm"], np.float32), d["max_words"])
+    return {"trained_on": d.get("trained_on"), "regret": d.get("regret")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default="gpurun_out/selector.json")
+    ap.add_argument("--train-scales", default="16,17,18,19,20")
+    ap.add_argument("--eval-scale", type=int, default=22)
+    ap.add_argument("--steps", type=int, default=600)
+    args = ap.parse_args()
+    import emptyheaded_amd as E
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import tuning
+    tuning.apply_committed()
+    tp = E.tune_get()
+    max_words = tp["lds_slots"] // 2  # bitmap capacity of a workgroup, in 64-bit words
+    ctx = E.Context(0)
+    Xs, Ys, Ms = [], [], []
+    t0 = time.perf_counter()
+    for s in [int(x) for x in args.train_scales.split(",")]:
+        src, dst, nv = G.rmat_graph(s)
+        T = E.Trie(ctx, src, dst)
+        X, Y, M, c0, c1, cnt = dataset(ctx, T)
+        log("scale %d: %d sets, %d with both stagings" % (s, len(X), int(((M[:, 0] > 0) & (M[:, 1] > 0)).sum())))
+        keep = M.sum(1) > 0
+        Xs.append(X[keep])
+        Ys.append(Y[keep])
+        Ms.append(M[keep])
+        T.close()
+    X, Y, M = np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
+    log("training on %d samples (%.1fs of recording)" % (len(X), time.perf_counter() - t0))
+    params, norm, hist = train(ctx, X, Y, M, steps=args.steps)
+    # evaluation graph
+    src, dst, nv = G.rmat_graph(args.eval_scale)
+    T = E.Trie(ctx, src, dst)
+    Xe, Ye, Me, c0, c1, cnt = dataset(ctx, T)
+    rule = rule_choice(ctx, T)
+    learned, pred = learned_choice(ctx, params, norm, Xe, max_words)
+    out = {
+        "params": params.tolist(), "norm": norm.tolist(), "max_words": max_words,
+        "trained_on": {"scales": args.train_scales, "samples": int(len(X)), "steps": args.steps,
+                       "final_loss": hist[-1][1]},
+        "loss_history": hist,
+        "regret": {"eval_scale": args.eval_scale, "rule": regret(c0, c1, Me, rule),
+                   "learned": regret(c0, c1, Me, learned),
+                   "bitmap_sets": {"rule": int(rule.sum()), "learned": int(learned.sum())}},
+    }
+    log(json.dumps(out["regret"], indent=1))
+    with open(args.out, "w") as f:
+        json.dump(out, f)
+    print(json.dumps(out["regret"]))
+
+
+if __name__ == "__main__":
+    main()

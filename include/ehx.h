@@ -147,6 +147,20 @@ int ehx_intersect_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a
                         const ehx_trie *tb, const uint32_t *keys_b, uint64_t n,
                         uint64_t *offsets, uint32_t *values, uint64_t cap, uint64_t *total);
 
+/* N-way intersections of trie node sets for a batch of n calls:
+ * TrieBuilder::build_aggregated_set over 1, 2 or N TrieBlocks
+ * (src/TrieBuilder.cpp:223-348).  Operand s of call i is node nodes[s][i] of
+ * tries[s]; node == num_heads names the trie's head set.  Host arrays.  The
+ * count-only form writes counts[n]; the materializing form writes the sorted
+ * results concatenated into values (capacity cap) with offsets[n+1]. */
+int ehx_intersect_nodes_count(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint32_t *const *nodes,
+                              uint64_t n, uint64_t *counts);
+int ehx_intersect_nodes(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint32_t *const *nodes, uint64_t n,
+                        uint64_t *offsets, uint32_t *values, uint64_t cap, uint64_t *total);
+/* Host copies of a trie's navigation arrays: head values (num_heads entries)
+ * and child offsets (num_heads + 1 entries), for host-side iterators. */
+int ehx_trie_host_arrays(ehx_ctx *ctx, const ehx_trie *t, uint32_t *head_vals, uint32_t *offsets);
+
 /* ---- generic join --------------------------------------------------------- */
 /* COUNT(*) of a single-bag plan: what run_<id> computes for an aggregated bag
  * (EHGenerator.scala:134-153 with ParTrieBuilder/TrieBuilder::build_aggregated_set,

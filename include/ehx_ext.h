@@ -18,10 +18,11 @@ extern "C" {
 typedef struct {
   int32_t lds_slots;    /* 32-bit LDS slots per wavefront for the staged set: 2048, 4096, 8192 */
   int32_t big_deg;      /* child sets at least this long are streamed by the whole wavefront */
-  int32_t waves_per_cu; /* persistent grid = CUs x waves_per_cu one-wave workgroups */
+  int32_t waves_per_cu; /* persistent grid = CUs x waves_per_cu / 4 four-wave workgroups */
   int32_t grab;         /* queue entries taken per atomic */
   int32_t bitmap_all;   /* stage every set whose word window fits as a bitmap */
   int32_t use_graph;    /* capture the level in a hipGraph and replay it */
+  int32_t big_work;     /* prefixes with at least this much estimated work are staged once per workgroup */
 } ehx_tune_params;
 
 int ehx_tune_get(ehx_tune_params *out);
@@ -35,6 +36,27 @@ int ehx_tune_set(const ehx_tune_params *in);
 int ehx_selector_num_params(void);
 int ehx_selector_load(int device, const uint16_t *params, uint64_t nparams, const float *norm, float max_words);
 int ehx_selector_clear(void);
+
+/* Per-set features of the selector for every child set of a trie (num_heads
+ * rows of 8 floats), computed by the device code the trie build evaluates. */
+int ehx_trie_set_features(ehx_ctx *ctx, const ehx_trie *t, float *out);
+
+/* Engine-recorded timings: runs the tuned form of `plan` once and writes the
+ * shader-clock cycles spent on each prefix (cap entries at most) with its
+ * staged set forced to a bitmap (force_mode 0), a hash table (1), or chosen
+ * by layout (-1); *nprefix receives the number of prefixes. */
+int ehx_join_record(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int force_mode, uint64_t *cycles,
+                    uint64_t cap, uint64_t *nprefix, int64_t *result);
+
+/* Training the selector on the GPU: data (n samples of 8 features, 2 targets,
+ * 2 mask weights) lives on the device; ehx_mlp_grad returns the masked mean
+ * squared error and its gradient with respect to the FP16-rounded weights
+ * (params are FP32 masters, norm as for ehx_selector_load). */
+typedef struct ehx_mlp ehx_mlp;
+int ehx_mlp_create(ehx_ctx *ctx, const float *x, const float *y, const float *m, uint64_t n, ehx_mlp **out);
+int ehx_mlp_destroy(ehx_mlp *h);
+int ehx_mlp_grad(ehx_mlp *h, const float *params, const float *norm, float *grad, float *loss);
+int ehx_mlp_predict(ehx_ctx *ctx, const float *params, const float *norm, const float *x, uint64_t n, float *out);
 
 #ifdef __cplusplus
 }
