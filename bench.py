@@ -25,11 +25,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def load_tuning(E):
+def load_tuning(E, selector_path=None):
     """Autotuned knobs and learned selector weights committed with the package."""
     from emptyheaded_amd import tuning
-    applied = tuning.apply_committed()
-    return applied
+    return tuning.apply_committed(selector_path=selector_path)
 
 
 def time_form(E, ctx, T, plan, form, steps, warmup):
@@ -83,6 +82,9 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--selector", default=None, help="selector weights (default: the committed ones)")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="also run one copy kernel of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the join's multi-GPU form (replicated trie, first attribute "
@@ -100,7 +102,7 @@ def main():
     m = len(src)
     log("graph: scale %d, %d vertices, %d oriented edges (%.1fs)" % (args.scale, nv, m, time.perf_counter() - t))
     balg = G.algorithmic_bytes(src, dst, nv)
-    tuning = load_tuning(E)
+    tuning = load_tuning(E, args.selector)
     log("tuning:", json.dumps(tuning))
 
     ctx = E.Context(0)
@@ -109,6 +111,13 @@ def main():
     info = T_learn.info()
     log("tries built: rule %.2f ms, learned %.2f ms; bitsets %d (learned) vs %d (rule)" %
         (T_rule.info()["build_ms"], info["build_ms"], info["num_bitsets"], T_rule.info()["num_bitsets"]))
+
+    if args.copy_calib:
+        import ctypes as C
+        cms = C.c_double(0)
+        E.check(E.lib().ehx_copy_calibrate(ctx.h, args.copy_calib, C.byref(cms)))
+        log("copy kernel: %d bytes in %.3f ms (%.1f GB/s read+write)" %
+            (args.copy_calib, cms.value, 2 * args.copy_calib / cms.value / 1e6))
 
     res = {}
     count, ms, kms, st = time_form(E, ctx, T_learn, plans.TRIANGLE, E.FORM_LEARNED, args.steps, args.warmup)

@@ -103,6 +103,7 @@ SIGNATURES = {
                           C.c_int),
     "ehx_selector_clear": ([], C.c_int),
     "ehx_trie_set_features": ([_vp, _vp, C.POINTER(C.c_float)], C.c_int),
+    "ehx_copy_calibrate": ([_vp, C.c_uint64, C.POINTER(C.c_double)], C.c_int),
     "ehx_join_record": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, _u64p, C.c_uint64, _u64p,
                          C.POINTER(C.c_int64)], C.c_int),
     "ehx_mlp_create": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint64,

@@ -50,6 +50,7 @@ int intersect_nodes(ehx_ctx *, int, ehx_trie *const *, const uint32_t *const *, 
 void tuned_forget_graphs();
 int selector_load(int device, const uint16_t *params, const float *norm, float max_words);
 int selector_clear();
+int copy_calibrate(ehx_ctx *ctx, uint64_t bytes, double *ms);
 
 }  // namespace ehx
 
@@ -241,5 +242,11 @@ int ehx_selector_load(int device, const uint16_t *params, uint64_t nparams, cons
 }
 
 int ehx_selector_clear(void) { return selector_clear(); }
+
+int ehx_copy_calibrate(ehx_ctx *ctx, uint64_t bytes, double *ms) {
+  if (!ctx || bytes < 16) return fail(EHX_ERR_INVALID, "bad copy size");
+  EHX_HIP(hipSetDevice(ctx->device));
+  return copy_calibrate(ctx, bytes, ms);
+}
 
 }  // extern "C"

@@ -237,6 +237,30 @@ int fill_u64(ehx_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n) {
   return EHX_OK;
 }
 
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// A plain copy of known size: reads and writes `bytes` bytes once each (the
+// reference point for scaling HBM traffic counters).
+int copy_calibrate(ehx_ctx *ctx, uint64_t bytes, double *ms) {
+  bytes &= ~(uint64_t)15;
+  DevBuf &a = ctx->scratch[S_SORT_A], &b = ctx->scratch[S_SORT_B];
+  int rc;
+  if ((rc = a.ensure(bytes)) || (rc = b.ensure(bytes))) return rc;
+  if ((rc = fill_u64(ctx, a.as<uint64_t>(), 0x0123456789ABCDEFull, bytes / 8))) return rc;
+  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], ctx->stream));
+  k_copy16<<<grid_for(bytes / 16, 256, 8192), 256, 0, ctx->stream>>>(a.as<uint4>(), b.as<uint4>(), bytes / 16);
+  EHX_HIP(hipGetLastError());
+  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], ctx->stream));
+  EHX_HIP(hipEventSynchronize(ctx->ev[EHX_EV_K1]));
+  float t = 0;
+  EHX_HIP(hipEventElapsedTime(&t, ctx->ev[EHX_EV_K0], ctx->ev[EHX_EV_K1]));
+  if (ms) *ms = t;
+  return EHX_OK;
+}
+
 int copy_d2h(ehx_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (!bytes) return EHX_OK;
   EHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));

@@ -310,9 +310,12 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
   const uint32_t *xsrc = src ? src : S[mi].v;
   const uint32_t xn = src ? sn : S[mi].n;
   uint64_t hits = 0;
-  for (uint32_t base = wave * 64; base < xn; base += nwaves * 64) {
+  // A wave takes XW elements of S(p) per step: 64 on its own; 16 inside a
+  // workgroup, so the four waves share even a small S(p) evenly.
+  constexpr uint32_t XW = BLOCK ? 16u : 64u;
+  for (uint32_t base = wave * XW; base < xn; base += nwaves * XW) {
     const uint32_t idx = base + lane;
-    bool ok = idx < xn;
+    bool ok = lane < XW && idx < xn;
     const uint32_t x = ok ? xsrc[idx] : 0u;
     if (ok && !src) {
 #pragma unroll
@@ -326,13 +329,32 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
       o = tx.off[nx];
       d = tx.off[nx + 1] - o;
     }
-    // long child sets: the whole wave streams each one
+    // long child sets: the whole wave streams each one with 128-bit loads,
+    // two quads per lane in flight per step
     uint64_t big = __ballot(d >= (uint32_t)big_deg);
     while (big) {
       const int l = __ffsll((unsigned long long)big) - 1;
       big &= big - 1;
       const uint32_t bo = __shfl(o, l, 64), bd = __shfl(d, l, 64);
-      for (uint32_t e = lane; e < bd; e += 64) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + e]);
+      const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
+      if (lane < head) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + lane]);
+      const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
+      const uint32_t n4 = (bd - head) >> 2;
+      uint32_t i = lane;
+      for (; i + 64 < n4; i += 128) {
+        const uint4 v0 = q[i], v1 = q[i + 64];
+        hits += probe<MAXS>(st, S, P.ns, v0.x) + probe<MAXS>(st, S, P.ns, v0.y) +
+                probe<MAXS>(st, S, P.ns, v0.z) + probe<MAXS>(st, S, P.ns, v0.w) +
+                probe<MAXS>(st, S, P.ns, v1.x) + probe<MAXS>(st, S, P.ns, v1.y) +
+                probe<MAXS>(st, S, P.ns, v1.z) + probe<MAXS>(st, S, P.ns, v1.w);
+      }
+      if (i < n4) {
+        const uint4 v0 = q[i];
+        hits += probe<MAXS>(st, S, P.ns, v0.x) + probe<MAXS>(st, S, P.ns, v0.y) +
+                probe<MAXS>(st, S, P.ns, v0.z) + probe<MAXS>(st, S, P.ns, v0.w);
+      }
+      const uint32_t done = head + 4 * n4;
+      if (lane < bd - done) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + done + lane]);
     }
     // short ones: expand owners into the LDS window, then one element per lane per step
     const uint32_t ds = (d < (uint32_t)big_deg) ? d : 0u;

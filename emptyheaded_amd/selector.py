@@ -91,6 +91,15 @@ def dataset(ctx, T, reps=2):
     return X, Y, M, c0, c1, r0
 
 
+def cost_weights(M, c0, c1):
+    """Per-sample loss weights proportional to the prefix's cost: regret is
+    measured in cycles, so expensive prefixes must be fitted best."""
+    big = np.where(M[:, 0] > 0, c0, 0.0)
+    big = np.where((M[:, 1] > 0) & ((M[:, 0] == 0) | (c1 < big)), c1, big)
+    w = big / max(big[big > 0].mean(), 1.0) if (big > 0).any() else np.ones(len(M))
+    return (M * w[:, None]).astype(np.float32)
+
+
 def init_params(seed=0):
     rng = np.random.default_rng(seed)
     p = np.zeros(NPARAM, np.float32)
@@ -219,6 +228,7 @@ def main():
     ap.add_argument("--train-scales", default="16,17,18,19,20")
     ap.add_argument("--eval-scale", type=int, default=22)
     ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--weighting", choices=("uniform", "cost"), default="cost")
     args = ap.parse_args()
     import emptyheaded_amd as E
     from emptyheaded_amd import graphs as G
@@ -235,6 +245,8 @@ def main():
         X, Y, M, c0, c1, cnt = dataset(ctx, T)
         log("scale %d: %d sets, %d with both stagings" % (s, len(X), int(((M[:, 0] > 0) & (M[:, 1] > 0)).sum())))
         keep = M.sum(1) > 0
+        if args.weighting == "cost":
+            M = cost_weights(M, c0, c1)
         Xs.append(X[keep])
         Ys.append(Y[keep])
         Ms.append(M[keep])
@@ -251,10 +263,12 @@ def main():
     out = {
         "params": params.tolist(), "norm": norm.tolist(), "max_words": max_words,
         "trained_on": {"scales": args.train_scales, "samples": int(len(X)), "steps": args.steps,
+                       "weighting": args.weighting,
                        "final_loss": hist[-1][1]},
         "loss_history": hist,
         "regret": {"eval_scale": args.eval_scale, "rule": regret(c0, c1, Me, rule),
                    "learned": regret(c0, c1, Me, learned),
+                   "bitmap_when_fits": regret(c0, c1, Me, Me[:, 0] > 0),
                    "bitmap_sets": {"rule": int(rule.sum()), "learned": int(learned.sum())}},
     }
     log(json.dumps(out["regret"], indent=1))

@@ -9,7 +9,7 @@ TUNING_JSON = os.path.join(HERE, "tuning.json")
 SELECTOR_JSON = os.path.join(HERE, "selector.json")
 
 
-def apply_committed(device=0):
+def apply_committed(device=0, selector_path=None):
     import emptyheaded_amd as E
     applied = {"params": None, "selector": None}
     if os.path.exists(TUNING_JSON):
@@ -18,7 +18,8 @@ def apply_committed(device=0):
         applied["params"] = E.tune_set(**t["best"])
     else:
         applied["params"] = E.tune_get()
-    if os.path.exists(SELECTOR_JSON):
+    sp = selector_path or SELECTOR_JSON
+    if os.path.exists(sp):
         from . import selector
-        applied["selector"] = selector.load_committed(device)
+        applied["selector"] = selector.load_committed(device, sp)
     return applied

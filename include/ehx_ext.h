@@ -58,6 +58,11 @@ int ehx_mlp_destroy(ehx_mlp *h);
 int ehx_mlp_grad(ehx_mlp *h, const float *params, const float *norm, float *grad, float *loss);
 int ehx_mlp_predict(ehx_ctx *ctx, const float *params, const float *norm, const float *x, uint64_t n, float *out);
 
+/* Measurement aid: one launch of a plain 16-byte-per-thread copy kernel that
+ * reads and writes `bytes` bytes (the reference for scaling HBM traffic
+ * counters); *ms receives its event-timed duration. */
+int ehx_copy_calibrate(ehx_ctx *ctx, uint64_t bytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
