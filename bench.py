@@ -128,6 +128,14 @@ def main():
         res["plain"] = (pc, pms, pkms, pst)
         log("plain: %d triangles, %.3f ms/step, main kernel %.3f ms" % (pc, pms, pkms))
         assert pc == count, "plain and tuned+learned disagree"
+    # the same query with its tuned level replayed from a captured hipGraph
+    saved = E.tune_get()
+    E.tune_set(use_graph=1)
+    gc, gms, _, _ = time_form(E, ctx, T_learn, plans.TRIANGLE, E.FORM_LEARNED, args.steps, args.warmup)
+    graph_used = E.tune_get()["use_graph"] == 1
+    E.tune_set(**saved)
+    assert gc == count, "graph replay disagrees"
+    log("tuned+learned, hipGraph replay: %.3f ms/step (graph %s)" % (gms, "used" if graph_used else "refused"))
     tc, tms, tkms, tst = time_form(E, ctx, T_rule, plans.TRIANGLE, E.FORM_TUNED, args.steps, args.warmup)
     res["tuned"] = (tc, tms, tkms, tst)
     log("tuned (rule layouts): %d triangles, %.3f ms/step, main kernel %.3f ms" % (tc, tms, tkms))
@@ -164,6 +172,7 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "k_tuned",
                      "kernel_ms": kms},
         "tuned_rule_ms_per_step": tms,
+        "graph_replay_ms_per_step": gms if graph_used else None,
         "tuning": tuning,
     }
     if "plain" in res:

@@ -90,7 +90,7 @@ __host__ __device__ inline void selector_forward(const __half *w, const float *n
 // limit, the rule's choice otherwise.
 __device__ inline int selector_layout(const SelectorWeights &s, uint32_t n, uint64_t span, uint32_t first,
                                       uint32_t last, uint32_t work, int rule) {
-  if (n <= 2) return rule;
+  if (n == 0) return rule;
   float words = (float)((last >> 6) - (first >> 6) + 1);
   if (words > s.max_words) return 1;  // EHX_UINTEGER: bitset would not fit the staging area
   float f[SEL_IN], o[SEL_OUT];

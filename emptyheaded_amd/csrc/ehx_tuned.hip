@@ -288,11 +288,15 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
       while (H < 2 * sn) { H <<= 1; lg++; }
       st.hshift = 32 - lg;
       st.hmask = H - 1;
+      // the whole filter area, even for small sets: a false positive sends its
+      // lane into the table loop and stalls all 64 lanes, so the filter is
+      // kept as sparse as the LDS allows
+      const uint32_t fw = flt_words;
       uint32_t lgf = 5;
-      while ((1u << lgf) < flt_words * 32) lgf++;
+      while ((1u << lgf) < fw * 32) lgf++;
       st.fshift = 32 - lgf;
       for (uint32_t i = gtid; i < H; i += gsize) tab[i] = EMPTY;
-      for (uint32_t i = gtid; i < flt_words; i += gsize) flt[i] = 0;
+      for (uint32_t i = gtid; i < fw; i += gsize) flt[i] = 0;
       gsync();
       for (uint32_t i = gtid; i < sn; i += gsize) {
         const uint32_t x = src[i];

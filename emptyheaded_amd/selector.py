@@ -91,6 +91,19 @@ def dataset(ctx, T, reps=2):
     return X, Y, M, c0, c1, r0
 
 
+def difference_targets(Y, M, c0, c1):
+    """Targets for the decision itself: output 0 learns log2(bitmap cost) -
+    log2(hash cost), output 1 learns 0, on prefixes where both stagings were
+    recorded, weighted by |bitmap cost - hash cost| (the regret a wrong choice
+    costs).  The selector's rule 'output 0 < output 1' then picks the cheaper."""
+    both = (M[:, 0] > 0) & (M[:, 1] > 0)
+    Yd = np.stack([Y[:, 0] - Y[:, 1], np.zeros(len(Y), np.float32)], 1).astype(np.float32)
+    gap = np.abs(c0 - c1)
+    w = np.where(both, gap, 0.0)
+    w = w / max(w[w > 0].mean(), 1.0) if (w > 0).any() else w
+    return Yd, np.stack([w, w], 1).astype(np.float32)
+
+
 def cost_weights(M, c0, c1):
     """Per-sample loss weights proportional to the prefix's cost: regret is
     measured in cycles, so expensive prefixes must be fitted best."""
@@ -199,7 +212,7 @@ def learned_choice(ctx, params, norm, X, max_words):
     pred = predict(ctx, params, norm, X)
     words64 = np.exp2(X[:, 4])
     n = np.exp2(X[:, 0]) - 1
-    ok = (words64 <= max_words + 0.5) & (n > 2.5)
+    ok = (words64 <= max_words + 0.5) & (n > 0.5)
     return ok & (pred[:, 0] < pred[:, 1]), pred
 
 
@@ -229,6 +242,7 @@ def main():
     ap.add_argument("--eval-scale", type=int, default=22)
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--weighting", choices=("uniform", "cost"), default="cost")
+    ap.add_argument("--target", choices=("costs", "diff"), default="diff")
     args = ap.parse_args()
     import emptyheaded_amd as E
     from emptyheaded_amd import graphs as G
@@ -244,9 +258,11 @@ def main():
         T = E.Trie(ctx, src, dst)
         X, Y, M, c0, c1, cnt = dataset(ctx, T)
         log("scale %d: %d sets, %d with both stagings" % (s, len(X), int(((M[:, 0] > 0) & (M[:, 1] > 0)).sum())))
-        keep = M.sum(1) > 0
-        if args.weighting == "cost":
+        if args.target == "diff":
+            Y, M = difference_targets(Y, M, c0, c1)
+        elif args.weighting == "cost":
             M = cost_weights(M, c0, c1)
+        keep = M.sum(1) > 0
         Xs.append(X[keep])
         Ys.append(Y[keep])
         Ms.append(M[keep])
@@ -263,7 +279,7 @@ def main():
     out = {
         "params": params.tolist(), "norm": norm.tolist(), "max_words": max_words,
         "trained_on": {"scales": args.train_scales, "samples": int(len(X)), "steps": args.steps,
-                       "weighting": args.weighting,
+                       "weighting": args.weighting, "target": args.target,
                        "final_loss": hist[-1][1]},
         "loss_history": hist,
         "regret": {"eval_scale": args.eval_scale, "rule": regret(c0, c1, Me, rule),
