@@ -45,6 +45,97 @@ def time_form(E, ctx, T, plan, form, steps, warmup):
     return total, (t1 - t0) * 1e3 / steps, sum(kernel_ms) / len(kernel_ms), st
 
 
+def bench_four_clique(E, ctx, steps, scale=18):
+    """4-clique COUNT on R-MAT scale 18: plain vs tuned vs tuned+learned."""
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import plans
+    s, d, n = G.rmat_graph(scale)
+    out = {"workload": "FliqueAgg COUNT(*) on R-MAT scale %d, edge factor 16, pruned src<dst" % scale,
+           "oriented_edges": len(s)}
+    T_rule = E.Trie(ctx, s, d, E.POLICY_RULE)
+    T_learn = E.Trie(ctx, s, d, E.POLICY_LEARNED if _selector_loaded(E) else E.POLICY_RULE)
+    counts = set()
+    for name, T, form in (("plain", T_rule, E.FORM_PLAIN), ("tuned", T_rule, E.FORM_TUNED),
+                          ("learned", T_learn, E.FORM_LEARNED)):
+        c, ms, kms, st = time_form(E, ctx, T, plans.FOUR_CLIQUE, form, steps, 1)
+        counts.add(c)
+        out[name + "_ms"] = ms
+        out[name + "_last_level_ms"] = kms
+        log("4-clique R-MAT-%d %s: %d, %.3f ms" % (scale, name, c, ms))
+    assert len(counts) == 1, "4-clique forms disagree"
+    out["four_cliques"] = counts.pop()
+    out["speedup_learned_vs_plain"] = out["plain_ms"] / out["learned_ms"]
+    T_rule.close()
+    T_learn.close()
+    return out
+
+
+def _selector_loaded(E):
+    from emptyheaded_amd import tuning
+    return tuning.selector_active()
+
+
+def bench_egonets(E, ctx, steps):
+    """4,096 ego-networks from R-MAT scale 22, triangle and 4-cycle count per
+    net: one batched launch per query kind vs one launch per net."""
+    import numpy as np
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import plans
+    t = time.perf_counter()
+    s, d, n = G.rmat_graph(22, orient=G.BOTH)
+    es, ed, base, off = G.egonets(s, d, n)
+    del s, d
+    ss, sd = G.symmetric(es, ed)
+    nets = len(base) - 1
+    log("ego batch: %d nets, %d edges, %d vertices (%.1fs)" % (nets, len(es), int(base[-1]), time.perf_counter() - t))
+    T = E.Trie(ctx, es, ed, E.POLICY_LEARNED if _selector_loaded(E) else E.POLICY_RULE)
+    S = E.Trie(ctx, ss, sd)
+    saved = E.tune_get()
+    # batched: per-segment counts, the tuned triangle level replayed from a hipGraph
+    E.tune_set(use_graph=1)
+    tri, _ = E.join_count_segments(ctx, plans.TRIANGLE, [T], base, E.FORM_LEARNED)
+    cyc, _ = E.cycle4_count(ctx, S, bounds=base)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        E.join_count_segments(ctx, plans.TRIANGLE, [T], base, E.FORM_LEARNED)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        E.cycle4_count(ctx, S, bounds=base)
+    t2 = time.perf_counter()
+    E.tune_set(**saved)
+    batched_tri, batched_cyc = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps
+    # one launch sequence per net (eager: every net has its own range)
+    E.tune_set(use_graph=0)
+    heads = np.unique(es)
+    lo, hi = np.searchsorted(heads, base[:-1]), np.searchsorted(heads, base[1:])
+    pq_tri = np.zeros(nets, dtype=np.int64)
+    pq_cyc = np.zeros(nets, dtype=np.int64)
+    t0 = time.perf_counter()
+    for q in range(nets):
+        pq_tri[q] = E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_LEARNED, int(lo[q]), int(hi[q]))
+    t1 = time.perf_counter()
+    for q in range(nets):
+        pq_cyc[q] = E.cycle4_count(ctx, S, int(base[q]), int(base[q + 1]))
+    t2 = time.perf_counter()
+    E.tune_set(**saved)
+    assert np.array_equal(pq_tri, tri) and np.array_equal(pq_cyc, cyc), "batched and per-net counts disagree"
+    out = {"workload": "4096 ego-networks (radius 1, trimmed to log-uniform 64..65536 edges) cut from R-MAT "
+                       "scale 22; triangle COUNT and 4-cycle COUNT per net",
+           "nets": nets, "edges": int(len(es)), "vertices": int(base[-1]),
+           "triangles": int(tri.sum()), "cycles4": int(cyc.sum()),
+           "batched_triangle_ms": batched_tri, "batched_cycle4_ms": batched_cyc,
+           "batched_ms": batched_tri + batched_cyc,
+           "per_query_triangle_ms": (t1 - t0) * 1e3, "per_query_cycle4_ms": (t2 - t1) * 1e3,
+           "per_query_ms": (t2 - t0) * 1e3}
+    out["speedup_batched_vs_per_query"] = out["per_query_ms"] / out["batched_ms"]
+    log("ego batch: batched %.2f ms (tri %.2f, 4-cycle %.2f); per-net launches %.1f ms" %
+        (out["batched_ms"], batched_tri, batched_cyc, out["per_query_ms"]))
+    T.close()
+    S.close()
+    return out
+
+
 def cpu_baseline(src, dst, m, expect, budget_s=20.0):
     from emptyheaded_amd import plans
     from oracle import oracle as O
@@ -83,6 +174,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--selector", default=None, help="selector weights (default: the committed ones)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 4-clique R-MAT-18 and ego-network batch measurements")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one copy kernel of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
@@ -180,6 +273,9 @@ def main():
         out["plain_edges_per_s"] = m / (res["plain"][1] * 1e-3)
         out["speedup_vs_plain"] = res["plain"][1] / ms
         out["plain_pairs_by_alg"] = res["plain"][3]["pairs_by_alg"]
+    if not args.no_configs:
+        out["configs"] = {"rmat18_four_clique": bench_four_clique(E, ctx, max(2, args.steps // 2)),
+                          "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2))}
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(src, dst, m, count)

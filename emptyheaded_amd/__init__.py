@@ -90,6 +90,13 @@ SIGNATURES = {
                         C.POINTER(JoinStats)], C.c_int),
     "ehx_join_count_range": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, C.c_uint64, C.c_uint64,
                               C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
+    "ehx_join_count_segments": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, _u32p, C.c_uint64,
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
+    "ehx_cycle4_count": ([_vp, _vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
+    "ehx_cycle4_count_segments": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(JoinStats)], C.c_int),
+    "ehx_gen_egonets": ([_u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                         C.POINTER(_u32p), C.POINTER(_u32p), _u64p, _u32p, _u64p], C.c_int),
     "ehx_gen_er": ([C.c_uint64, C.c_uint64, C.c_uint64, _u32p, _u32p], C.c_int),
     "ehx_gen_rmat": ([C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, _u32p, _u32p],
                      C.c_int),
@@ -279,6 +286,37 @@ def join_count(ctx, levels, tries, form=FORM_PLAIN, begin=None, end=None, stats=
         check(lib().ehx_join_count_range(ctx.h, C.byref(p), arr, form, begin or 0,
                                          2**64 - 1 if end is None else end, C.byref(res), C.byref(st)))
     return (res.value, st.as_dict()) if stats else res.value
+
+
+def join_count_segments(ctx, levels, tries, bounds, form=FORM_PLAIN, stats=False):
+    """Per-segment COUNT(*): counts[s] for level-0 values in [bounds[s], bounds[s+1])."""
+    p = make_plan(levels) if not isinstance(levels, Plan) else levels
+    arr = (_vp * len(tries))(*[t.h.value for t in tries])
+    b = np.ascontiguousarray(bounds, dtype=np.uint32)
+    counts = np.zeros(max(len(b) - 1, 1), dtype=np.int64)
+    res = C.c_int64(0)
+    st = JoinStats()
+    check(lib().ehx_join_count_segments(ctx.h, C.byref(p), arr, form, ptr_u32(b), len(b) - 1,
+                                        counts.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(res), C.byref(st)))
+    counts = counts[: len(b) - 1]
+    return (counts, res.value, st.as_dict()) if stats else (counts, res.value)
+
+
+def cycle4_count(ctx, sym, begin=None, end=None, bounds=None, stats=False):
+    """4-cycles of the undirected graph held in both directions by trie `sym`;
+    with bounds, also the per-segment counts (by each cycle's largest vertex)."""
+    res = C.c_int64(0)
+    st = JoinStats()
+    if bounds is None:
+        check(lib().ehx_cycle4_count(ctx.h, sym.h, begin or 0, 2**64 - 1 if end is None else end, C.byref(res),
+                                     C.byref(st)))
+        return (res.value, st.as_dict()) if stats else res.value
+    b = np.ascontiguousarray(bounds, dtype=np.uint32)
+    counts = np.zeros(max(len(b) - 1, 1), dtype=np.int64)
+    check(lib().ehx_cycle4_count_segments(ctx.h, sym.h, ptr_u32(b), len(b) - 1,
+                                          counts.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(res), C.byref(st)))
+    counts = counts[: len(b) - 1]
+    return (counts, res.value, st.as_dict()) if stats else (counts, res.value)
 
 
 def tune_get():

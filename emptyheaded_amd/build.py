@@ -25,7 +25,7 @@ SOURCES = [
     "ehx_tuned.hip",
     "ehx_selector.hip",
     "ehx_mlp.hip",
-    "ehx_record.hip",
+    "ehx_cycles.hip",
 ]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -41,8 +41,14 @@ int trie_destroy(ehx_trie *);
 int trie_export(ehx_ctx *, const ehx_trie *, uint32_t *, uint32_t *);
 int trie_node_info(ehx_ctx *, const ehx_trie *, uint8_t *, uint32_t *);
 int trie_lookup(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *);
+struct SegOut;
 int join_count(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint64_t, uint64_t, int64_t *, ehx_join_stats *,
-               const RecordReq *);
+               const RecordReq *, const SegOut *);
+int join_count_segments(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, const uint32_t *, uint64_t, int64_t *,
+                        int64_t *, ehx_join_stats *);
+int cycle4_count(ehx_ctx *, const ehx_trie *, uint64_t, uint64_t, const SegOut *, int64_t *, ehx_join_stats *);
+int cycle4_count_segments(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *, int64_t *,
+                          ehx_join_stats *);
 int intersect_pairs(ehx_ctx *, const ehx_trie *, const uint32_t *, const ehx_trie *, const uint32_t *, uint64_t,
                     uint64_t *, uint64_t *, uint32_t *, uint64_t, uint64_t *);
 int intersect_nodes(ehx_ctx *, int, ehx_trie *const *, const uint32_t *const *, uint64_t, uint64_t *, uint64_t *,
@@ -187,13 +193,29 @@ int ehx_trie_host_arrays(ehx_ctx *ctx, const ehx_trie *t, uint32_t *head_vals, u
 int ehx_join_count(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, int64_t *result,
                    ehx_join_stats *stats) {
   if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
-  return join_count(ctx, plan, tries, form, 0, UINT64_MAX, result, stats, nullptr);
+  return join_count(ctx, plan, tries, form, 0, UINT64_MAX, result, stats, nullptr, nullptr);
 }
 
 int ehx_join_count_range(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, uint64_t begin,
                          uint64_t end, int64_t *result, ehx_join_stats *stats) {
   if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
-  return join_count(ctx, plan, tries, form, begin, end, result, stats, nullptr);
+  return join_count(ctx, plan, tries, form, begin, end, result, stats, nullptr, nullptr);
+}
+
+int ehx_join_count_segments(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form,
+                            const uint32_t *bounds, uint64_t nseg, int64_t *counts, int64_t *result,
+                            ehx_join_stats *stats) {
+  return join_count_segments(ctx, plan, tries, form, bounds, nseg, counts, result, stats);
+}
+
+int ehx_cycle4_count(ehx_ctx *ctx, const ehx_trie *sym, uint64_t begin, uint64_t end, int64_t *result,
+                     ehx_join_stats *stats) {
+  return cycle4_count(ctx, sym, begin, end, nullptr, result, stats);
+}
+
+int ehx_cycle4_count_segments(ehx_ctx *ctx, const ehx_trie *sym, const uint32_t *bounds, uint64_t nseg,
+                              int64_t *counts, int64_t *result, ehx_join_stats *stats) {
+  return cycle4_count_segments(ctx, sym, bounds, nseg, counts, result, stats);
 }
 
 int ehx_join_record(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int force_mode, uint64_t *cycles,
@@ -201,7 +223,7 @@ int ehx_join_record(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, 
   if (!ctx || !cycles || !nprefix) return fail(EHX_ERR_INVALID, "null argument");
   if (force_mode < -1 || force_mode > 1) return fail(EHX_ERR_INVALID, "force_mode must be -1, 0 or 1");
   RecordReq r{cycles, cap, nprefix, force_mode};
-  return join_count(ctx, plan, tries, EHX_FORM_TUNED, 0, UINT64_MAX, result, nullptr, &r);
+  return join_count(ctx, plan, tries, EHX_FORM_TUNED, 0, UINT64_MAX, result, nullptr, &r, nullptr);
 }
 
 int ehx_tune_get(ehx_tune_params *out) {

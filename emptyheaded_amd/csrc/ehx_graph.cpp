@@ -8,7 +8,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <math.h>
+
 #include <algorithm>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -211,6 +214,149 @@ extern "C" int ehx_graph_prepare(const uint32_t *src, const uint32_t *dst, uint6
   *out_dst = od;
   *out_m = m;
   if (out_nv) *out_nv = nv;
+  return EHX_OK;
+}
+
+// ---- ego-network batch --------------------------------------------------------
+namespace {
+
+struct EgoNet {
+  std::vector<uint64_t> edges;  // local (u << 32 | v), u < v after relabel
+  uint32_t nv = 0;
+};
+
+// One ego-network: center c and its neighbors in hash order join one at a time
+// (a radius-1 BFS from c); every join adds the edges from the new vertex to the
+// vertices already in, in adjacency order, and the net stops at `target` edges.
+// Returns the edges in join order as local (earlier, later) pairs, center = 0.
+uint64_t grow_ego(const uint64_t *off, const uint32_t *adj, uint32_t c, uint64_t target, uint64_t seed,
+                  std::vector<uint32_t> &pos, std::vector<uint32_t> &members, std::vector<uint64_t> &out) {
+  out.clear();
+  members.clear();
+  std::vector<std::pair<uint64_t, uint32_t>> order;
+  order.reserve(off[c + 1] - off[c]);
+  for (uint64_t i = off[c]; i < off[c + 1]; i++) order.emplace_back(mix64(seed ^ adj[i]), adj[i]);
+  std::sort(order.begin(), order.end());
+  pos[c] = 0;
+  members.push_back(c);
+  for (auto &hv : order) {
+    if (out.size() >= target) break;
+    const uint32_t v = hv.second, k = (uint32_t)members.size();
+    pos[v] = k;
+    members.push_back(v);
+    for (uint64_t i = off[v]; i < off[v + 1] && out.size() < target; i++) {
+      const uint32_t u = adj[i];
+      if (pos[u] < k) out.push_back(((uint64_t)pos[u] << 32) | k);
+    }
+  }
+  for (uint32_t v : members) pos[v] = 0xFFFFFFFFu;
+  return out.size();
+}
+
+}  // namespace
+
+extern "C" int ehx_gen_egonets(const uint32_t *src, const uint32_t *dst, uint64_t m, uint32_t num_vertices,
+                               uint32_t num_nets, uint32_t min_edges, uint32_t max_edges, uint64_t seed,
+                               uint32_t **out_src, uint32_t **out_dst, uint64_t *out_m, uint32_t *net_base,
+                               uint64_t *net_off) {
+  if ((!src || !dst) && m) return ehx::fail(EHX_ERR_INVALID, "ehx_gen_egonets: null input");
+  if (!out_src || !out_dst || !out_m || !net_base || !net_off)
+    return ehx::fail(EHX_ERR_INVALID, "ehx_gen_egonets: null output");
+  if (min_edges < 1 || max_edges < min_edges) return ehx::fail(EHX_ERR_INVALID, "ehx_gen_egonets: bad edge range");
+  const uint32_t n = num_vertices;
+  for (uint64_t i = 0; i < m; i++) {
+    if (src[i] >= n || dst[i] >= n) return ehx::fail(EHX_ERR_INVALID, "ehx_gen_egonets: vertex id >= num_vertices");
+    if (i && (src[i] < src[i - 1] || (src[i] == src[i - 1] && dst[i] <= dst[i - 1])))
+      return ehx::fail(EHX_ERR_INVALID, "ehx_gen_egonets: rows must be sorted and unique");
+  }
+  // CSR of the (symmetric) input
+  std::vector<uint64_t> off(n + 1, 0);
+  for (uint64_t i = 0; i < m; i++) off[src[i] + 1]++;
+  for (uint32_t v = 0; v < n; v++) off[v + 1] += off[v];
+  // vertices by ascending degree (ties by id): candidate pools are suffixes
+  std::vector<uint32_t> byd(n);
+  for (uint32_t v = 0; v < n; v++) byd[v] = v;
+  std::stable_sort(byd.begin(), byd.end(),
+                   [&](uint32_t a, uint32_t b) { return off[a + 1] - off[a] < off[b + 1] - off[b]; });
+  auto deg = [&](uint32_t v) { return off[v + 1] - off[v]; };
+  std::vector<EgoNet> nets(num_nets);
+  const double lr = std::log((double)max_edges / (double)min_edges);
+  unsigned T = num_threads();
+  if (T > num_nets) T = num_nets ? num_nets : 1;
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < T; t++) {
+    th.emplace_back([&, t]() {
+      std::vector<uint32_t> pos(n, 0xFFFFFFFFu), members;
+      std::vector<uint64_t> e;
+      for (uint32_t q = t; q < num_nets; q += T) {
+        const uint64_t qs = mix64(seed ^ (0xE90ull << 40) ^ q);
+        uint64_t target = (uint64_t)std::floor((double)min_edges * std::exp(lr * unit(rnd(qs, 0))));
+        target = std::min<uint64_t>(std::max<uint64_t>(target, min_edges), max_edges);
+        // pool: vertices of degree >= target / 2; halve the pool towards the hubs on every miss
+        uint64_t lo = std::lower_bound(byd.begin(), byd.end(), target / 2,
+                                       [&](uint32_t v, uint64_t d) { return deg(v) < d; }) - byd.begin();
+        if (lo >= n) lo = n ? n - 1 : 0;
+        uint64_t best = 0, best_seed = 0;
+        uint32_t best_c = n ? byd[n - 1] : 0;
+        for (uint32_t attempt = 0; n; attempt++) {
+          const uint32_t c = byd[lo + rnd(qs, 1 + attempt) % (n - lo)];
+          const uint64_t hs = rnd(qs, 1000 + attempt);
+          const uint64_t got = grow_ego(off.data(), dst, c, target, hs, pos, members, e);
+          if (got >= target) break;
+          if (got > best) { best = got; best_c = c; best_seed = hs; }
+          if (lo == n - 1 || attempt >= 64) {  // nothing larger to draw: keep the largest net seen
+            grow_ego(off.data(), dst, best_c, target, best_seed, pos, members, e);
+            break;
+          }
+          lo = lo + (n - lo) / 2;
+        }
+        // relabel by ascending degree inside the net (ties by join order), orient u < v
+        EgoNet &N = nets[q];
+        N.nv = (uint32_t)members.size();
+        std::vector<uint32_t> d(N.nv, 0), rank(N.nv);
+        for (uint64_t x : e) { d[x >> 32]++; d[x & 0xFFFFFFFFu]++; }
+        std::vector<uint32_t> ord(N.nv);
+        for (uint32_t i = 0; i < N.nv; i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return d[a] < d[b]; });
+        for (uint32_t r = 0; r < N.nv; r++) rank[ord[r]] = r;
+        N.edges.resize(e.size());
+        for (size_t i = 0; i < e.size(); i++) {
+          uint32_t u = rank[e[i] >> 32], v = rank[e[i] & 0xFFFFFFFFu];
+          if (u > v) std::swap(u, v);
+          N.edges[i] = ((uint64_t)u << 32) | v;
+        }
+        std::sort(N.edges.begin(), N.edges.end());
+      }
+    });
+  }
+  for (auto &x : th) x.join();
+  uint64_t total = 0, vtot = 0;
+  for (uint32_t q = 0; q < num_nets; q++) {
+    net_base[q] = (uint32_t)vtot;
+    net_off[q] = total;
+    vtot += nets[q].nv;
+    total += nets[q].edges.size();
+    if (vtot > 0xFFFFFFFFull) return ehx::fail(EHX_ERR_UNSUPPORTED, "ehx_gen_egonets: more than 2^32 vertices");
+  }
+  net_base[num_nets] = (uint32_t)vtot;
+  net_off[num_nets] = total;
+  uint32_t *os = (uint32_t *)malloc(4 * (total ? total : 1));
+  uint32_t *od = (uint32_t *)malloc(4 * (total ? total : 1));
+  if (!os || !od) {
+    free(os);
+    free(od);
+    return ehx::fail(EHX_ERR_NOMEM, "ehx_gen_egonets: out of host memory");
+  }
+  for (uint32_t q = 0; q < num_nets; q++) {
+    const uint64_t o = net_off[q];
+    for (size_t i = 0; i < nets[q].edges.size(); i++) {
+      os[o + i] = net_base[q] + (uint32_t)(nets[q].edges[i] >> 32);
+      od[o + i] = net_base[q] + (uint32_t)(nets[q].edges[i] & 0xFFFFFFFFu);
+    }
+  }
+  *out_src = os;
+  *out_dst = od;
+  *out_m = total;
   return EHX_OK;
 }
 

@@ -55,7 +55,7 @@ struct ehx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[2 * EHX_MAX_ATTRS + 4] = {};
-  ehx::DevBuf scratch[24];
+  ehx::DevBuf scratch[28];
   int num_cus = 256;
 };
 
@@ -79,7 +79,9 @@ enum Slot {
   S_SCAN0 = 0, S_SCAN1, S_SCAN2, S_SCAN3,  // scan block sums (recursion levels)
   S_SORT_A, S_SORT_B, S_HIST, S_FLAGS, S_POS, S_MISC,
   S_F0, S_F1, S_F2, S_F3, S_F4, S_F5, S_F6, S_F7,  // frontier columns
-  S_COUNTS, S_OFFS, S_QUEUE, S_ACC, S_IN0, S_IN1
+  S_COUNTS, S_OFFS, S_QUEUE, S_ACC, S_IN0, S_IN1,
+  S_SEGB, S_SEGO,  // segment bounds and per-segment counters
+  S_NUM_SLOTS
 };
 
 // ---- device view of a trie --------------------------------------------------

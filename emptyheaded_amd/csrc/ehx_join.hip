@@ -32,7 +32,8 @@ enum { MODE_COUNT = 0, MODE_WRITE = 1, MODE_SUM = 2 };
 template <int MODE>
 __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Frontier in, int k, uint64_t n,
                                                      uint32_t *__restrict__ counts, const uint64_t *__restrict__ offs,
-                                                     OutFrontier outf, unsigned long long *__restrict__ acc) {
+                                                     OutFrontier outf, unsigned long long *__restrict__ acc,
+                                                     SegOut seg) {
   __shared__ unsigned long long s_acc[6];
   if (MODE == MODE_SUM) {
     if (threadIdx.x < 6) s_acc[threadIdx.x] = 0;
@@ -60,6 +61,7 @@ __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Front
     if (MODE == MODE_SUM) {
       sum += c;
       algc[alg]++;
+      if (seg.out && __lane_id() == 0) seg_add(seg, tuple[0], c);
     }
   }
   if (MODE == MODE_SUM) {
@@ -128,8 +130,14 @@ static LevelDesc level_desc(const ehx_plan *p, int k) {
 
 static const uint64_t FRONTIER_BUDGET = 96ull << 30;  // bytes per materialized level
 
+// Single-attribute plans: every level-0 value is one result tuple.
+__global__ void k_seg_level0(const uint32_t *__restrict__ f0, uint64_t n, SegOut seg) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    seg_add(seg, f0[i], 1ull);
+}
+
 int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form, uint64_t begin, uint64_t end,
-               int64_t *result, ehx_join_stats *stats, const RecordReq *rec) {
+               int64_t *result, ehx_join_stats *stats, const RecordReq *rec, const SegOut *seg) {
   int nrels = 0, rc;
   if ((rc = check_plan(p, tries, &nrels))) return rc;
   if (!result) return fail(EHX_ERR_INVALID, "null result");
@@ -193,6 +201,10 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
   int form_used = EHX_FORM_PLAIN;
   if (K == 1) {
     total = (int64_t)n0;
+    if (seg && n0) {
+      k_seg_level0<<<grid_for(n0, 256, 4096), 256, 0, st>>>(f0, n0, *seg);
+      EHX_HIP(hipGetLastError());
+    }
   } else {
     // ---- middle levels (materialized) ------------------------------------------
     int last_mat = K - 2;  // materialize levels 1..K-2
@@ -210,7 +222,8 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       OutFrontier none;
       memset(&none, 0, sizeof(none));
       if (n) {
-        k_level_plain<MODE_COUNT><<<G, 256, 0, st>>>(T, L, in, k, n, cb.as<uint32_t>(), nullptr, none, nullptr);
+        k_level_plain<MODE_COUNT><<<G, 256, 0, st>>>(T, L, in, k, n, cb.as<uint32_t>(), nullptr, none, nullptr,
+                                                     SegOut{});
         EHX_HIP(hipGetLastError());
       }
       if ((rc = scan_u32_to_u64(ctx, cb.as<uint32_t>(), ob.as<uint64_t>(), n))) return rc;
@@ -225,7 +238,8 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       memset(&out, 0, sizeof(out));
       for (int j = 0; j <= k; j++) out.col[j] = fb.as<uint32_t>() + (uint64_t)j * nn;
       if (n && nn) {
-        k_level_plain<MODE_WRITE><<<G, 256, 0, st>>>(T, L, in, k, n, nullptr, ob.as<uint64_t>(), out, nullptr);
+        k_level_plain<MODE_WRITE><<<G, 256, 0, st>>>(T, L, in, k, n, nullptr, ob.as<uint64_t>(), out, nullptr,
+                                                     SegOut{});
         EHX_HIP(hipGetLastError());
       }
       for (int j = 0; j <= k; j++) in.col[j] = out.col[j];
@@ -243,6 +257,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       for (int j = 0; j < EHX_MAX_ATTRS; j++) ta.prefix[j] = in.col[j];
       ta.nprefix = n;
       ta.force_mode = -1;
+      if (seg) ta.seg = *seg;
       DevBuf &rb = ctx->scratch[S_IN1];
       if (rec) {
         if ((rc = rb.ensure(n * 8 + 8)) || (rc = fill_u64(ctx, rb.as<uint64_t>(), 0, n))) return rc;
@@ -263,7 +278,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
         OutFrontier none;
         memset(&none, 0, sizeof(none));
         k_level_plain<MODE_SUM><<<G, 256, 0, st>>>(T, L, in, K - 1, n, nullptr, nullptr, none,
-                                                   ac.as<unsigned long long>());
+                                                   ac.as<unsigned long long>(), seg ? *seg : SegOut{});
         EHX_HIP(hipGetLastError());
       }
       EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));
@@ -290,6 +305,41 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
   *result = total;
   if (stats) *stats = S;
   return EHX_OK;
+}
+
+// ---- per-segment counts -------------------------------------------------------------
+int seg_begin(ehx_ctx *ctx, const uint32_t *bounds, uint64_t nseg, SegOut *so) {
+  if (!bounds || nseg == 0) return fail(EHX_ERR_INVALID, "segments: null bounds or nseg == 0");
+  if (nseg >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "segments: more than 2^32 segments");
+  for (uint64_t s = 0; s < nseg; s++)
+    if (bounds[s] > bounds[s + 1]) return fail(EHX_ERR_INVALID, "segments: bounds not ascending at %llu",
+                                               (unsigned long long)s);
+  DevBuf &b = ctx->scratch[S_SEGB], &o = ctx->scratch[S_SEGO];
+  int rc;
+  if ((rc = b.ensure((nseg + 1) * 4)) || (rc = o.ensure(nseg * 8))) return rc;
+  if ((rc = copy_h2d(ctx, b.p, bounds, (nseg + 1) * 4))) return rc;
+  if ((rc = fill_u64(ctx, o.as<uint64_t>(), 0, nseg))) return rc;
+  so->bounds = b.as<uint32_t>();
+  so->out = o.as<unsigned long long>();
+  so->nseg = (uint32_t)nseg;
+  so->pad = 0;
+  return EHX_OK;
+}
+
+int seg_end(ehx_ctx *ctx, const SegOut &so, int64_t *counts) {
+  if (!counts) return EHX_OK;
+  return copy_d2h(ctx, counts, so.out, (size_t)so.nseg * 8);
+}
+
+int join_count_segments(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form, const uint32_t *bounds,
+                        uint64_t nseg, int64_t *counts, int64_t *result, ehx_join_stats *stats) {
+  if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
+  EHX_HIP(hipSetDevice(ctx->device));
+  SegOut so;
+  int rc;
+  if ((rc = seg_begin(ctx, bounds, nseg, &so))) return rc;
+  if ((rc = join_count(ctx, p, tries, form, 0, UINT64_MAX, result, stats, nullptr, &so))) return rc;
+  return seg_end(ctx, so, counts);
 }
 
 // ---- batched pair intersections ---------------------------------------------------

@@ -166,6 +166,12 @@ __device__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg
   return wave_nway<EMIT>(S, n, out);
 }
 
+// Orders the LDS accesses of one wavefront's lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ---- plan on the device --------------------------------------------------------
 struct LevelDesc {
   int32_t nops;
@@ -190,6 +196,33 @@ __device__ __forceinline__ bool resolve(const Tries &T, const LevelDesc &L, cons
     }
   }
   return true;
+}
+
+// Per-segment COUNT output: a result tuple whose first attribute value v lies in
+// [bounds[s], bounds[s+1]) adds to out[s].  Many independent queries packed into
+// one trie (disjoint value ranges) get their own counts from one launch.
+struct SegOut {
+  const uint32_t *bounds;   // nseg + 1 ascending level-0 values (device)
+  unsigned long long *out;  // nseg counters (device); null when no segments were asked for
+  uint32_t nseg;
+  uint32_t pad;
+};
+
+__device__ __forceinline__ int32_t seg_of(const SegOut &s, uint32_t v) {
+  if (v < s.bounds[0] || v >= s.bounds[s.nseg]) return -1;
+  uint32_t lo = 0, hi = s.nseg;  // last s with bounds[s] <= v
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (s.bounds[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return (int32_t)lo;
+}
+
+__device__ __forceinline__ void seg_add(const SegOut &s, uint32_t v, unsigned long long c) {
+  if (!c) return;
+  const int32_t i = seg_of(s, v);
+  if (i >= 0) atomicAdd(&s.out[i], c);
 }
 
 }  // namespace ehx

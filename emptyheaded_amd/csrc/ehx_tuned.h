@@ -32,11 +32,12 @@ struct TunedArgs {
   unsigned long long *rec;  // per-prefix shader-clock cycles when recording, else null
   int32_t force_mode;       // -1: staging by layout; 0: bitmap where it fits; 1: hash where it fits
   int32_t pad;
+  SegOut seg;               // per-segment counts by first attribute value (seg.out null: off)
 };
 
 // Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.
 struct TuneParams {
-  int32_t lds_slots;      // 32-bit LDS slots per wavefront (2048, 4096 or 8192)
+  int32_t lds_slots;      // 32-bit LDS slots per workgroup (2048, 4096 or 8192), a quarter per wavefront
   int32_t big_deg;        // child sets at least this long are streamed by the whole wave
   int32_t waves_per_cu;   // persistent grid size
   int32_t grab;           // queue items taken per atomic

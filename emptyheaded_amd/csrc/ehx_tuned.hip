@@ -63,11 +63,6 @@ static constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 static constexpr uint32_t NBUCKETS = 64, BUCKET_NONE = 63, WAVE_BIN = 32;
 static constexpr int OWN = 256;  // owner window per wave
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 template <int MAXS>
 __device__ __forceinline__ bool resolve_s(const TunedPlan &P, const TunedArgs &A, uint64_t p, DSet *S) {
   uint32_t pv[EHX_MAX_ATTRS];
@@ -404,8 +399,13 @@ __global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const u
     if (qi >= nbig) break;
     const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
     int mode = -1;
-    hits += tuned_prefix<true, MAXS>(P, A, queue[qi], tab, SLOTS, flt, FWB, sv, 4 * VC, own[w], big_deg, bitmap_all,
-                                     mode);
+    const uint64_t h = tuned_prefix<true, MAXS>(P, A, queue[qi], tab, SLOTS, flt, FWB, sv, 4 * VC, own[w], big_deg,
+                                                bitmap_all, mode);
+    hits += h;
+    if (A.seg.out) {
+      const uint64_t hw = wave_sum(h);
+      if (__lane_id() == 0) seg_add(A.seg, A.prefix[0][queue[qi]], hw);
+    }
     if (A.rec && threadIdx.x == 0 && mode >= 0)
       A.rec[queue[qi]] = (__builtin_amdgcn_s_memtime() - t0) | ((unsigned long long)mode << 62);
   }
@@ -419,8 +419,13 @@ __global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const u
     for (uint32_t qi = q0; qi < q1; qi++) {
       const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
       int mode = -1;
-      hits += tuned_prefix<false, MAXS>(P, A, queue[qi], tab + w * WS, WS, flt + w * FWW, FWW, sv + w * VC, VC,
-                                        own[w], big_deg, bitmap_all, mode);
+      const uint64_t h = tuned_prefix<false, MAXS>(P, A, queue[qi], tab + w * WS, WS, flt + w * FWW, FWW,
+                                                   sv + w * VC, VC, own[w], big_deg, bitmap_all, mode);
+      hits += h;
+      if (A.seg.out) {
+        const uint64_t hw = wave_sum(h);
+        if (__lane_id() == 0) seg_add(A.seg, A.prefix[0][queue[qi]], hw);
+      }
       if (A.rec && __lane_id() == 0 && mode >= 0)
         A.rec[queue[qi]] = (__builtin_amdgcn_s_memtime() - t0) | ((unsigned long long)mode << 62);
     }

@@ -53,7 +53,8 @@ struct TrieBuilder {
     tmp_level--;
   }
 
-  // Materializing output (§ next components): not part of the aggregate path.
+  // Materializing output tries: not part of the aggregate path; the device join
+  // materializes result tuples through ehx_join_materialize instead.
   size_t build_set(const size_t, const TrieBlock<hybrid, M> *) { ehx_dropin::not_implemented("TrieBuilder::build_set"); }
   size_t build_set(const size_t, const TrieBlock<hybrid, M> *, const TrieBlock<hybrid, M> *) {
     ehx_dropin::not_implemented("TrieBuilder::build_set");

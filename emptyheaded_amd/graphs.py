@@ -78,3 +78,37 @@ def algorithmic_bytes(src, dst, num_vertices):
     dout = np.bincount(src, minlength=num_vertices).astype(np.int64)
     din = np.bincount(dst, minlength=num_vertices).astype(np.int64)
     return int(4 * np.sum(dout * (dout + din)))
+
+
+EGO_SEED = SEED_BASE + 0x40  # ego-network batch drawn from R-MAT scale 22
+
+
+def egonets(src_sym, dst_sym, num_vertices, num_nets=4096, min_edges=64, max_edges=65536, seed=EGO_SEED):
+    """Batch of ego-networks cut from an undirected graph held in both
+    directions (see ehx_gen_egonets).  Returns (src, dst, net_base, net_off):
+    oriented src < dst edges of all nets, net q owning vertex ids
+    [net_base[q], net_base[q+1]) and edges [net_off[q], net_off[q+1])."""
+    s, d = np.ascontiguousarray(src_sym, np.uint32), np.ascontiguousarray(dst_sym, np.uint32)
+    os_, od_ = C.POINTER(C.c_uint32)(), C.POINTER(C.c_uint32)()
+    m = C.c_uint64(0)
+    base = np.zeros(num_nets + 1, dtype=np.uint32)
+    off = np.zeros(num_nets + 1, dtype=np.uint64)
+    check(lib().ehx_gen_egonets(ptr_u32(s), ptr_u32(d), len(s), num_vertices, num_nets, min_edges, max_edges, seed,
+                                C.byref(os_), C.byref(od_), C.byref(m), ptr_u32(base),
+                                off.ctypes.data_as(C.POINTER(C.c_uint64))))
+    try:
+        n = m.value
+        a = np.ctypeslib.as_array(os_, shape=(max(n, 1),))[:n].copy()
+        b = np.ctypeslib.as_array(od_, shape=(max(n, 1),))[:n].copy()
+    finally:
+        lib().ehx_free(C.cast(os_, C.c_void_p))
+        lib().ehx_free(C.cast(od_, C.c_void_p))
+    return a, b, base, off
+
+
+def symmetric(src, dst):
+    """Both directions of an oriented edge list, rows sorted (ids kept)."""
+    s = np.concatenate([src, dst]).astype(np.uint32)
+    d = np.concatenate([dst, src]).astype(np.uint32)
+    o = np.lexsort((d, s))
+    return s[o], d[o]

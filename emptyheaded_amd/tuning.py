@@ -9,7 +9,16 @@ TUNING_JSON = os.path.join(HERE, "tuning.json")
 SELECTOR_JSON = os.path.join(HERE, "selector.json")
 
 
+_selector_active = False
+
+
+def selector_active():
+    """True once apply_committed has loaded selector weights."""
+    return _selector_active
+
+
 def apply_committed(device=0, selector_path=None):
+    global _selector_active
     import emptyheaded_amd as E
     applied = {"params": None, "selector": None}
     if os.path.exists(TUNING_JSON):
@@ -22,4 +31,5 @@ def apply_committed(device=0, selector_path=None):
     if os.path.exists(sp):
         from . import selector
         applied["selector"] = selector.load_committed(device, sp)
+        _selector_active = bool(applied["selector"])
     return applied

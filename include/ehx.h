@@ -172,6 +172,31 @@ int ehx_join_count(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, i
 /* Same, restricted to level-0 values with index in [begin, end). */
 int ehx_join_count_range(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form,
                          uint64_t begin, uint64_t end, int64_t *result, ehx_join_stats *stats);
+/* Many independent queries in one launch.  The queries' relations are packed
+ * into one trie with disjoint first-attribute value ranges; counts[s] receives
+ * the COUNT(*) of result tuples whose level-0 value lies in
+ * [bounds[s], bounds[s+1]).  It equals what a separate run of the query over
+ * segment s's tuples returns (one c_run_<id> call per query,
+ * query_compiler/src/main/scala/codegen/query/QueryPlan.scala:25-39, each folded
+ * by par::reducer, storage_engine/include/utils/parallel.hpp:8-34).  bounds:
+ * nseg + 1 ascending host values; counts: nseg host entries; *result: the sum
+ * over all level-0 values. */
+int ehx_join_count_segments(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form,
+                            const uint32_t *bounds, uint64_t nseg, int64_t *counts, int64_t *result,
+                            ehx_join_stats *stats);
+
+/* ---- 4-cycle count -------------------------------------------------------- */
+/* Number of 4-cycles (each cycle once) of the undirected graph whose edges the
+ * trie `sym` holds in both directions, restricted to cycles whose largest
+ * vertex is a head value with index in [begin, end).  The reference engine has
+ * no 4-cycle query (its 4-cycle would be a two-bag plan); the count is the
+ * graph-theoretic one: sum over a of sum over c < a of C(|N(a) ∩ N(c) ∩ [0,a)|, 2). */
+int ehx_cycle4_count(ehx_ctx *ctx, const ehx_trie *sym, uint64_t begin, uint64_t end,
+                     int64_t *result, ehx_join_stats *stats);
+/* Per-segment form: a cycle counts for the segment holding its largest vertex. */
+int ehx_cycle4_count_segments(ehx_ctx *ctx, const ehx_trie *sym, const uint32_t *bounds,
+                              uint64_t nseg, int64_t *counts, int64_t *result,
+                              ehx_join_stats *stats);
 
 /* ---- synthetic inputs (host) ---------------------------------------------- */
 /* Counter-based SplitMix64 generators: identical output for identical
@@ -188,6 +213,23 @@ int ehx_gen_rmat(int scale, uint64_t samples, double a, double b, double c, uint
 int ehx_graph_prepare(const uint32_t *src, const uint32_t *dst, uint64_t n, int relabel,
                       int orient, uint32_t **out_src, uint32_t **out_dst, uint64_t *out_m,
                       uint32_t *out_num_vertices);
+/* Ego-network batch: num_nets subgraphs cut from an undirected graph given in
+ * both directions (rows sorted and unique, as ehx_graph_prepare with orient 0
+ * returns).  Net q has a target of t_q = floor(min_edges * (max_edges /
+ * min_edges)^u) edges, u uniform in [0,1): log-uniform sizes.  Its center is
+ * drawn among vertices of degree >= t_q / 2 (the pool halves towards the
+ * highest degrees after each center whose net stays below t_q, at most 64
+ * draws, then the largest net drawn is kept); the center's neighbors join in
+ * hash order (a radius-1 BFS), each adding its edges to the vertices already
+ * in, until the net holds t_q edges.  Vertices of net q are renumbered
+ * net_base[q] + rank, rank by ascending degree inside the net; edges are
+ * oriented src < dst and sorted.  net_base[num_nets + 1] and
+ * net_off[num_nets + 1] (edge offsets) are caller arrays; the edge arrays are
+ * allocated by the library (free with ehx_free). */
+int ehx_gen_egonets(const uint32_t *src, const uint32_t *dst, uint64_t m, uint32_t num_vertices,
+                    uint32_t num_nets, uint32_t min_edges, uint32_t max_edges, uint64_t seed,
+                    uint32_t **out_src, uint32_t **out_dst, uint64_t *out_m, uint32_t *net_base,
+                    uint64_t *net_off);
 void ehx_free(void *p);
 
 #ifdef __cplusplus

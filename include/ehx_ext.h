@@ -16,7 +16,7 @@ extern "C" {
 
 /* Knobs of the degree-binned persistent last level. */
 typedef struct {
-  int32_t lds_slots;    /* 32-bit LDS slots per wavefront for the staged set: 2048, 4096, 8192 */
+  int32_t lds_slots;    /* 32-bit LDS slots per workgroup for the staged sets (a quarter per wavefront): 2048, 4096, 8192 */
   int32_t big_deg;      /* child sets at least this long are streamed by the whole wavefront */
   int32_t waves_per_cu; /* persistent grid = CUs x waves_per_cu / 4 four-wave workgroups */
   int32_t grab;         /* queue entries taken per atomic */

@@ -614,3 +614,73 @@ int64_t orc_join_count_sampled(const orc_plan *p, orc_trie *const *tries, int nt
 int orc_join_count_per_head(const orc_plan *p, orc_trie *const *tries, int nthreads, int64_t *out) {
   return run_join(p, tries, nthreads, 0, UINT64_MAX, out, 1) < 0 ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* 4-cycles of an undirected graph (no counterpart in the reference engine,  */
+/* whose queries have no 4-cycle; the count is the graph-theoretic one).      */
+/* Symmetric CSR: off[n+1], adj sorted per vertex.  Each cycle is charged to  */
+/* its largest vertex a: for every wedge a-b-c with b < a, c < a, H[c] += 1,  */
+/* then cycles(a) = sum_c H[c](H[c]-1)/2.                                     */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  const uint64_t *off;
+  const uint32_t *adj;
+  uint32_t n;
+  uint64_t next;
+  int64_t *per_vertex;
+} c4_ctx;
+
+typedef struct {
+  c4_ctx *c;
+  int64_t sum;
+} c4_thread;
+
+static void *c4_worker(void *arg) {
+  c4_thread *th = (c4_thread *)arg;
+  c4_ctx *c = th->c;
+  uint32_t *H = (uint32_t *)calloc((size_t)c->n + 1, sizeof(uint32_t));
+  uint32_t *touched = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)c->n + 1));
+  th->sum = 0;
+  for (;;) {
+    uint64_t s = __atomic_fetch_add(&c->next, 64, __ATOMIC_RELAXED);
+    if (s >= c->n) break;
+    uint64_t e = s + 64 < c->n ? s + 64 : c->n;
+    for (uint64_t a = s; a < e; a++) {
+      uint64_t nt = 0;
+      int64_t cyc = 0;
+      for (uint64_t i = c->off[a]; i < c->off[a + 1] && c->adj[i] < a; i++) {
+        const uint32_t b = c->adj[i];
+        for (uint64_t j = c->off[b]; j < c->off[b + 1] && c->adj[j] < a; j++) {
+          const uint32_t x = c->adj[j];
+          if (H[x]++ == 0) touched[nt++] = x;
+        }
+      }
+      for (uint64_t t = 0; t < nt; t++) {
+        const uint64_t h = H[touched[t]];
+        cyc += (int64_t)(h * (h - 1) / 2);
+        H[touched[t]] = 0;
+      }
+      if (c->per_vertex) c->per_vertex[a] = cyc;
+      th->sum += cyc;
+    }
+  }
+  free(H);
+  free(touched);
+  return NULL;
+}
+
+int64_t orc_cycle4_count(const uint64_t *off, const uint32_t *adj, uint32_t n, int nthreads, int64_t *per_vertex) {
+  if (nthreads < 1) nthreads = 1;
+  c4_ctx c = {off, adj, n, 0, per_vertex};
+  c4_thread *th = (c4_thread *)calloc((size_t)nthreads, sizeof(c4_thread));
+  pthread_t *tid = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+  for (int t = 0; t < nthreads; t++) th[t].c = &c;
+  for (int t = 1; t < nthreads; t++) pthread_create(&tid[t], NULL, c4_worker, &th[t]);
+  c4_worker(&th[0]);
+  int64_t total = 0;
+  for (int t = 1; t < nthreads; t++) pthread_join(tid[t], NULL);
+  for (int t = 0; t < nthreads; t++) total += th[t].sum;
+  free(th);
+  free(tid);
+  return total;
+}

@@ -104,6 +104,12 @@ uint64_t orc_join_level0_size(const orc_plan *p, orc_trie *const *tries);
 int orc_join_count_per_head(const orc_plan *p, orc_trie *const *tries, int nthreads,
                             int64_t *out);
 
+/* 4-cycles (each once) of an undirected graph in symmetric CSR form (off[n+1],
+ * adjacency sorted per vertex); per_vertex[a] (may be NULL) receives the cycles
+ * whose largest vertex is a.  No reference counterpart. */
+int64_t orc_cycle4_count(const uint64_t *off, const uint32_t *adj, uint32_t n, int nthreads,
+                         int64_t *per_vertex);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,6 +88,8 @@ def lib():
         L.orc_join_count_per_head.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int,
                                               C.POINTER(C.c_int64)]
         L.orc_join_count_per_head.restype = C.c_int
+        L.orc_cycle4_count.argtypes = [C.POINTER(C.c_uint64), u32p, C.c_uint32, C.c_int, C.POINTER(C.c_int64)]
+        L.orc_cycle4_count.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -230,3 +232,18 @@ def join_count_per_head(levels, tries, nthreads=1):
                                        out.ctypes.data_as(C.POINTER(C.c_int64)))
     assert rc == 0
     return out[:n]
+
+
+def cycle4_count(src, dst, num_vertices, nthreads=1, per_vertex=False):
+    """4-cycles of the undirected graph whose edges (src, dst) hold both
+    directions, rows sorted.  With per_vertex, also the cycles charged to each
+    vertex (the cycle's largest vertex)."""
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    dst = np.ascontiguousarray(dst, dtype=np.uint32)
+    off = np.zeros(num_vertices + 1, dtype=np.uint64)
+    np.cumsum(np.bincount(src, minlength=num_vertices), out=off[1:])
+    pv = np.zeros(max(1, num_vertices), dtype=np.int64)
+    tot = lib().orc_cycle4_count(off.ctypes.data_as(C.POINTER(C.c_uint64)), dst.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 num_vertices, nthreads,
+                                 pv.ctypes.data_as(C.POINTER(C.c_int64)) if per_vertex else None)
+    return (tot, pv[:num_vertices]) if per_vertex else tot

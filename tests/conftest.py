@@ -56,3 +56,10 @@ def fixtures_meta():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "fixtures.json")) as f:
         return json.load(f)
+
+
+@pytest.fixture(scope="session")
+def synthetic_meta():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "synthetic.json")) as f:
+        return json.load(f)

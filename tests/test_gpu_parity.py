@@ -211,3 +211,17 @@ def test_rmat20_config_full_size(ctx, oracle):
     T = E.Trie(ctx, s, d)
     for form in FORMS:
         assert E.join_count(ctx, plans.TRIANGLE, [T], form) == exp
+
+
+def test_rmat18_four_clique_config(ctx, oracle, synthetic_meta):
+    """4-clique on R-MAT scale 18, edge factor 16: every form, against the oracle."""
+    s, d, n = G.rmat_graph(18)
+    meta = synthetic_meta["rmat18"]
+    assert len(s) == meta["edges"]
+    exp = oracle.join_count(plans.FOUR_CLIQUE, [oracle.Trie(s, d)], 16)
+    assert exp == meta["four_cliques"]
+    for policy in (E.POLICY_RULE, E.POLICY_LEARNED):
+        T = E.Trie(ctx, s, d, policy)
+        for form in FORMS:
+            say("rmat18 4-clique policy", policy, "form", form)
+            assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], form) == exp
