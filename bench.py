@@ -31,14 +31,15 @@ def load_tuning(E, selector_path=None):
     return tuning.apply_committed(selector_path=selector_path)
 
 
-def time_form(E, ctx, T, plan, form, steps, warmup):
+def time_form(E, ctx, T, plan, form, steps, warmup, tries=None):
+    tries = tries or [T]
     for _ in range(warmup):
-        E.join_count(ctx, plan, [T], form)
+        E.join_count(ctx, plan, tries, form)
     ctx.synchronize()
     kernel_ms, total = [], None
     t0 = time.perf_counter()
     for _ in range(steps):
-        total, st = E.join_count(ctx, plan, [T], form, stats=True)
+        total, st = E.join_count(ctx, plan, tries, form, stats=True)
         kernel_ms.append(st["main_kernel_ms"])
     ctx.synchronize()
     t1 = time.perf_counter()
@@ -136,6 +137,51 @@ def bench_egonets(E, ctx, steps):
     return out
 
 
+def bench_triples(E, ctx, steps, pool_budget_words=8 << 30):
+    """2-hop star and path joins over the two most frequent predicates of a
+    50M-triple table (64 Zipf(1.1) predicates), under each layout policy; the
+    learned selector's regret is taken against the best policy per query."""
+    from emptyheaded_amd import triples as TR
+    t = time.perf_counter()
+    s, p, o = TR.triple_table()
+    rels = TR.by_predicate(s, p, o, 64)
+    del s, p, o
+    R = [TR.distinct(*rels[0]), TR.distinct(*rels[1])]
+    del rels
+    exp = {"star": TR.star_count(R[0], R[1]), "path": TR.path_count(R[0], R[1])}
+    log("triples: P0 %d rows, P1 %d rows (%.1fs)" % (len(R[0][0]), len(R[1][0]), time.perf_counter() - t))
+    out = {"workload": "COUNT(*) of P0(s,o1),P1(s,o2) (star) and P0(x,y),P1(y,z) (path); P0, P1 = the two most "
+                       "frequent of 64 Zipf(1.1) predicates in 50M R-MAT-22 (s,o) triples",
+           "rows": [len(R[0][0]), len(R[1][0])], "star": exp["star"], "path": exp["path"], "policies": {}}
+    policies = [("rule", E.POLICY_RULE), ("all_uint", E.POLICY_ALL_UINT), ("all_bitset", E.POLICY_ALL_BITSET)]
+    if _selector_loaded(E):
+        policies.append(("learned", E.POLICY_LEARNED))
+    for name, pol in policies:
+        if pol == E.POLICY_ALL_BITSET:
+            words = sum(TR.bitset_pool_words(*r) for r in R)
+            if words > pool_budget_words:
+                out["policies"][name] = {"infeasible": "bitset pool of %.1f GB" % (words * 8 / 1e9)}
+                continue
+        T = [E.Trie(ctx, *r, policy=pol) for r in R]
+        res = {"build_ms": sum(x.info()["build_ms"] for x in T)}
+        for q, plan in (("star", TR.STAR2), ("path", TR.PATH2)):
+            c, ms, _, _ = time_form(E, ctx, T[0], plan, E.FORM_LEARNED if pol == E.POLICY_LEARNED else E.FORM_PLAIN,
+                                    steps, 1, tries=T)
+            assert c == exp[q], (name, q, c, exp[q])
+            res[q + "_ms"] = ms
+        out["policies"][name] = res
+        log("triples %s: star %.3f ms, path %.3f ms" % (name, res["star_ms"], res["path_ms"]))
+        for x in T:
+            x.close()
+    for q in ("star", "path"):
+        times = {k: v[q + "_ms"] for k, v in out["policies"].items() if q + "_ms" in v}
+        best = min(times.values())
+        out[q + "_best_policy"] = min(times, key=times.get)
+        for k, v in times.items():
+            out["policies"][k][q + "_regret"] = v / best - 1.0
+    return out
+
+
 def cpu_baseline(src, dst, m, expect, budget_s=20.0):
     from emptyheaded_amd import plans
     from oracle import oracle as O
@@ -175,7 +221,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--selector", default=None, help="selector weights (default: the committed ones)")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the 4-clique R-MAT-18 and ego-network batch measurements")
+                    help="skip the 4-clique R-MAT-18, ego-network batch and triple-table measurements")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one copy kernel of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
@@ -275,7 +321,8 @@ def main():
         out["plain_pairs_by_alg"] = res["plain"][3]["pairs_by_alg"]
     if not args.no_configs:
         out["configs"] = {"rmat18_four_clique": bench_four_clique(E, ctx, max(2, args.steps // 2)),
-                          "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2))}
+                          "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2)),
+                          "triple_star_path": bench_triples(E, ctx, max(2, args.steps // 2))}
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(src, dst, m, count)

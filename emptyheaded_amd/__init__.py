@@ -98,6 +98,7 @@ SIGNATURES = {
     "ehx_gen_egonets": ([_u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                          C.POINTER(_u32p), C.POINTER(_u32p), _u64p, _u32p, _u64p], C.c_int),
     "ehx_gen_er": ([C.c_uint64, C.c_uint64, C.c_uint64, _u32p, _u32p], C.c_int),
+    "ehx_gen_zipf": ([C.c_uint64, C.c_uint32, C.c_double, C.c_uint64, _u32p], C.c_int),
     "ehx_gen_rmat": ([C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_uint64, _u32p, _u32p],
                      C.c_int),
     "ehx_graph_prepare": ([_u32p, _u32p, C.c_uint64, C.c_int, C.c_int, C.POINTER(_u32p), C.POINTER(_u32p),

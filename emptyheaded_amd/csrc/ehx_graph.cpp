@@ -217,6 +217,25 @@ extern "C" int ehx_graph_prepare(const uint32_t *src, const uint32_t *dst, uint6
   return EHX_OK;
 }
 
+// Zipf(s) over k ranks: P(rank r) proportional to 1 / (r + 1)^s, by inverse CDF.
+extern "C" int ehx_gen_zipf(uint64_t n, uint32_t k, double s, uint64_t seed, uint32_t *out) {
+  if (!out && n) return ehx::fail(EHX_ERR_INVALID, "ehx_gen_zipf: null output");
+  if (k == 0 || !(s >= 0.0)) return ehx::fail(EHX_ERR_INVALID, "ehx_gen_zipf: k >= 1 and s >= 0 required");
+  std::vector<double> cdf(k);
+  double acc = 0.0;
+  for (uint32_t r = 0; r < k; r++) cdf[r] = (acc += std::pow((double)(r + 1), -s));
+  for (uint32_t r = 0; r < k; r++) cdf[r] /= acc;
+  cdf[k - 1] = 1.0;
+  parallel_for(n, [&](uint64_t a, uint64_t b) {
+    for (uint64_t i = a; i < b; i++) {
+      const double u = unit(rnd(seed, i));
+      out[i] = (uint32_t)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+      if (out[i] >= k) out[i] = k - 1;
+    }
+  });
+  return EHX_OK;
+}
+
 // ---- ego-network batch --------------------------------------------------------
 namespace {
 

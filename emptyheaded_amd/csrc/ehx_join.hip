@@ -75,6 +75,37 @@ __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Front
   }
 }
 
+// Last level with one operand: its count is the operand's cardinality
+// (build_aggregated_set over one TrieBlock, TrieBuilder.cpp:223-240), so one
+// thread per tuple suffices.
+__global__ __launch_bounds__(256) void k_level_card(Tries T, LevelDesc L, Frontier in, uint64_t n,
+                                                    unsigned long long *__restrict__ acc, SegOut seg) {
+  const DTrie &t = T.t[L.rel[0]];
+  const int par = L.parent[0];
+  unsigned long long sum = 0, nonempty = 0, empty = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t c;
+    if (par < 0) {
+      c = t.nh;
+    } else {
+      const int32_t h = node_of(t, in.col[par][i]);
+      c = h >= 0 ? t.off[h + 1] - t.off[h] : 0u;
+    }
+    sum += c;
+    if (c) nonempty++;
+    else empty++;
+    if (seg.out && c) seg_add(seg, in.col[0][i], c);
+  }
+  sum = wave_sum(sum);
+  nonempty = wave_sum(nonempty);
+  empty = wave_sum(empty);
+  if (__lane_id() == 0) {
+    if (sum) atomicAdd(&acc[0], sum);
+    if (nonempty) atomicAdd(&acc[1 + ALG_MERGE], nonempty);
+    if (empty) atomicAdd(&acc[1 + ALG_EMPTY], empty);
+  }
+}
+
 // Level 0: flags over the driver head set, membership in the other head sets.
 __global__ void k_level0_flags(Tries T, LevelDesc L, int driver, uint32_t *__restrict__ flags) {
   const DTrie &td = T.t[L.rel[driver]];
@@ -273,7 +304,11 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
     } else {
       LevelDesc L = level_desc(p, K - 1);
       EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
-      if (n) {
+      if (n && L.nops == 1) {
+        k_level_card<<<grid_for(n, 256, 16384), 256, 0, st>>>(T, L, in, n, ac.as<unsigned long long>(),
+                                                              seg ? *seg : SegOut{});
+        EHX_HIP(hipGetLastError());
+      } else if (n) {
         unsigned G = grid_for(n * 64, 256, 65536);
         OutFrontier none;
         memset(&none, 0, sizeof(none));

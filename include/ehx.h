@@ -205,6 +205,9 @@ int ehx_gen_er(uint64_t num_vertices, uint64_t samples, uint64_t seed, uint32_t 
                uint32_t *dst);
 int ehx_gen_rmat(int scale, uint64_t samples, double a, double b, double c, uint64_t seed,
                  uint32_t *src, uint32_t *dst);
+/* Zipf(s) ranks in [0, k): P(r) proportional to (r + 1)^-s (inverse CDF of a
+ * SplitMix64 uniform per index). */
+int ehx_gen_zipf(uint64_t n, uint32_t k, double s, uint64_t seed, uint32_t *out);
 /* Undirected clean-up: drop self loops, symmetrize, de-duplicate; relabel
  * (0 keep ids, 1 ascending-degree rank with ties by id and isolated vertices
  * dropped, 2 rank among used ids as dictionary encoding does, src/Encoding.cpp:
