@@ -165,10 +165,11 @@ def bench_triples(E, ctx, steps, pool_budget_words=8 << 30):
         T = [E.Trie(ctx, *r, policy=pol) for r in R]
         res = {"build_ms": sum(x.info()["build_ms"] for x in T)}
         for q, plan in (("star", TR.STAR2), ("path", TR.PATH2)):
-            c, ms, _, _ = time_form(E, ctx, T[0], plan, E.FORM_LEARNED if pol == E.POLICY_LEARNED else E.FORM_PLAIN,
-                                    steps, 1, tries=T)
+            c, ms, _, st = time_form(E, ctx, T[0], plan, E.FORM_LEARNED if pol == E.POLICY_LEARNED else E.FORM_PLAIN,
+                                     steps, 1, tries=T)
             assert c == exp[q], (name, q, c, exp[q])
             res[q + "_ms"] = ms
+            res[q + "_level_ms"] = [round(x, 4) for x in st["level_ms"][:3]]
         out["policies"][name] = res
         log("triples %s: star %.3f ms, path %.3f ms" % (name, res["star_ms"], res["path_ms"]))
         for x in T:

@@ -23,9 +23,9 @@
 
 namespace ehx {
 
-constexpr uint32_t C4_SLOTS = 2048;            // LDS hash slots per wavefront
+constexpr uint32_t C4_SLOTS = 1024;            // LDS hash slots per wavefront
 constexpr uint32_t C4_WAVE_MAX = C4_SLOTS / 2;  // wedge bound handled by one wavefront
-constexpr uint32_t C4_DENSE = 16384;           // dense LDS counters per workgroup
+constexpr uint32_t C4_DENSE = 8192;            // dense LDS counters per workgroup
 constexpr uint32_t C4_GSLOTS = 1u << 19;       // global table slots per workgroup
 constexpr uint32_t C4_EMPTY = 0xFFFFFFFFu;
 // ctl words
@@ -74,18 +74,26 @@ __device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t
   s_start[lane] = incl - len;
   s_off[lane] = p0;
   wave_sync();
-  for (uint32_t f0 = 0; f0 < tot; f0 += 64) {
-    const uint32_t fi = f0 + lane;
-    if (fi < tot) {
-      uint32_t lo = 0, hi = 63;  // last owner whose list starts at or before fi
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (s_start[mid] <= fi) lo = mid;
-        else hi = mid - 1;
+  // four positions per lane per round: their loads are independent and overlap
+  for (uint32_t f0 = 0; f0 < tot; f0 += 256) {
+    uint32_t c[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t fi = f0 + 64 * u + lane;
+      c[u] = C4_EMPTY;
+      if (fi < tot) {
+        uint32_t lo = 0, hi = 63;  // last owner whose list starts at or before fi
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (s_start[mid] <= fi) lo = mid;
+          else hi = mid - 1;
+        }
+        c[u] = S.vals[s_off[lo] + (fi - s_start[lo])];
       }
-      const uint32_t c = S.vals[s_off[lo] + (fi - s_start[lo])];
-      if (c < a) f(c);
     }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (c[u] < a) f(c[u]);
   }
   wave_sync();
 }
@@ -280,7 +288,7 @@ int cycle4_count(ehx_ctx *ctx, const ehx_trie *t, uint64_t begin, uint64_t end, 
   if (ctl[C4_ERR]) return fail(EHX_ERR_DEVICE, "4-cycle count: endpoint table overflow");
   S.frontier[1] = ctl[C4_BIG_COUNT];
   if (ctl[C4_BIG_COUNT]) {
-    const uint32_t gmax = 2u * (uint32_t)ctx->num_cus;  // two 64 KB-LDS workgroups per CU
+    const uint32_t gmax = 4u * (uint32_t)ctx->num_cus;  // four 32 KB-LDS workgroups per CU
     const unsigned grid = ctl[C4_BIG_COUNT] < gmax ? ctl[C4_BIG_COUNT] : gmax;
     DevBuf &kb = ctx->scratch[S_SORT_A], &nb = ctx->scratch[S_SORT_B];
     if ((rc = kb.ensure((uint64_t)grid * C4_GSLOTS * 4)) || (rc = nb.ensure((uint64_t)grid * C4_GSLOTS * 4))) return rc;

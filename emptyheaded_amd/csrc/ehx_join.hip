@@ -33,7 +33,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Frontier in, int k, uint64_t n,
                                                      uint32_t *__restrict__ counts, const uint64_t *__restrict__ offs,
                                                      OutFrontier outf, unsigned long long *__restrict__ acc,
-                                                     SegOut seg) {
+                                                     SegOut seg, int pick) {
   __shared__ unsigned long long s_acc[6];
   if (MODE == MODE_SUM) {
     if (threadIdx.x < 6) s_acc[threadIdx.x] = 0;
@@ -50,11 +50,11 @@ __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Front
     if (resolve(T, L, tuple, S)) {
       if (MODE == MODE_WRITE) {
         uint64_t o = offs[t];
-        c = wave_intersect<true>(S, L.nops, outf.col[k] + o, &alg);
+        c = wave_intersect<true>(S, L.nops, outf.col[k] + o, &alg, pick != 0);
         for (int j = 0; j < k; j++)
           for (uint32_t i = __lane_id(); i < c; i += 64) outf.col[j][o + i] = tuple[j];
       } else {
-        c = wave_intersect<false>(S, L.nops, nullptr, &alg);
+        c = wave_intersect<false>(S, L.nops, nullptr, &alg, pick != 0);
       }
     }
     if (MODE == MODE_COUNT && __lane_id() == 0) counts[t] = c;
@@ -96,13 +96,23 @@ __global__ __launch_bounds__(256) void k_level_card(Tries T, LevelDesc L, Fronti
     else empty++;
     if (seg.out && c) seg_add(seg, in.col[0][i], c);
   }
+  // one set of atomics per workgroup: same-address atomics serialize in L2
+  __shared__ unsigned long long s_acc[3];
+  if (threadIdx.x < 3) s_acc[threadIdx.x] = 0;
+  __syncthreads();
   sum = wave_sum(sum);
   nonempty = wave_sum(nonempty);
   empty = wave_sum(empty);
   if (__lane_id() == 0) {
-    if (sum) atomicAdd(&acc[0], sum);
-    if (nonempty) atomicAdd(&acc[1 + ALG_MERGE], nonempty);
-    if (empty) atomicAdd(&acc[1 + ALG_EMPTY], empty);
+    atomicAdd(&s_acc[0], sum);
+    atomicAdd(&s_acc[1], nonempty);
+    atomicAdd(&s_acc[2], empty);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_acc[0]) atomicAdd(&acc[0], s_acc[0]);
+    if (s_acc[1]) atomicAdd(&acc[1 + ALG_MERGE], s_acc[1]);
+    if (s_acc[2]) atomicAdd(&acc[1 + ALG_EMPTY], s_acc[2]);
   }
 }
 
@@ -184,6 +194,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
     T.t[r] = view(tries[r]);
   }
   const int K = p->num_attrs;
+  const int pick = form != EHX_FORM_PLAIN;  // per-pair algorithm by cost in the tuned and learned forms
   EHX_HIP(hipEventRecord(ctx->ev[0], st));
 
   // ---- level 0 -------------------------------------------------------------
@@ -254,7 +265,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       memset(&none, 0, sizeof(none));
       if (n) {
         k_level_plain<MODE_COUNT><<<G, 256, 0, st>>>(T, L, in, k, n, cb.as<uint32_t>(), nullptr, none, nullptr,
-                                                     SegOut{});
+                                                     SegOut{}, pick);
         EHX_HIP(hipGetLastError());
       }
       if ((rc = scan_u32_to_u64(ctx, cb.as<uint32_t>(), ob.as<uint64_t>(), n))) return rc;
@@ -270,7 +281,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       for (int j = 0; j <= k; j++) out.col[j] = fb.as<uint32_t>() + (uint64_t)j * nn;
       if (n && nn) {
         k_level_plain<MODE_WRITE><<<G, 256, 0, st>>>(T, L, in, k, n, nullptr, ob.as<uint64_t>(), out, nullptr,
-                                                     SegOut{});
+                                                     SegOut{}, pick);
         EHX_HIP(hipGetLastError());
       }
       for (int j = 0; j <= k; j++) in.col[j] = out.col[j];
@@ -305,7 +316,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       LevelDesc L = level_desc(p, K - 1);
       EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
       if (n && L.nops == 1) {
-        k_level_card<<<grid_for(n, 256, 16384), 256, 0, st>>>(T, L, in, n, ac.as<unsigned long long>(),
+        k_level_card<<<grid_for(n, 256, (unsigned)ctx->num_cus * 8), 256, 0, st>>>(T, L, in, n, ac.as<unsigned long long>(),
                                                               seg ? *seg : SegOut{});
         EHX_HIP(hipGetLastError());
       } else if (n) {
@@ -313,7 +324,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
         OutFrontier none;
         memset(&none, 0, sizeof(none));
         k_level_plain<MODE_SUM><<<G, 256, 0, st>>>(T, L, in, K - 1, n, nullptr, nullptr, none,
-                                                   ac.as<unsigned long long>(), seg ? *seg : SegOut{});
+                                                   ac.as<unsigned long long>(), seg ? *seg : SegOut{}, pick);
         EHX_HIP(hipGetLastError());
       }
       EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));

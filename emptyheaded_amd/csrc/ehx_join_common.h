@@ -140,9 +140,13 @@ __device__ uint32_t wave_nway(const DSet *S, int n, uint32_t *out) {
   return cnt;
 }
 
-// Candidate set of one tuple: the intersection of n resolved sets.
+// Candidate set of one tuple: the intersection of n resolved sets.  With
+// `pick`, a bitset pair whose smaller set holds fewer values than the pair's
+// overlapping words is probed value by value instead of AND-ed word by word
+// (same result, less traffic for sparse windows); without it the algorithms are
+// exactly the reference's per-pair choice.
 template <bool EMIT>
-__device__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg) {
+__device__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg, bool pick = false) {
   for (int i = 0; i < n; i++)
     if (S[i].n == 0) { *alg = ALG_EMPTY; return 0; }
   if (n == 1) {
@@ -153,7 +157,18 @@ __device__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg
   }
   if (n == 2) {
     const DSet &A = S[0], &B = S[1];
-    if (A.lay == EHX_RANGE_BITSET && B.lay == EHX_RANGE_BITSET) { *alg = ALG_AND; return wave_and<EMIT>(A, B, out); }
+    if (A.lay == EHX_RANGE_BITSET && B.lay == EHX_RANGE_BITSET) {
+      if (pick) {
+        const uint32_t s0 = max(A.start, B.start), e0 = min(A.start + A.W, B.start + B.W);
+        const uint32_t ov = e0 > s0 ? e0 - s0 : 0u;
+        if (min(A.n, B.n) < ov) {
+          *alg = ALG_PROBE;
+          return (A.n <= B.n) ? wave_probe<EMIT>(A, B, out) : wave_probe<EMIT>(B, A, out);
+        }
+      }
+      *alg = ALG_AND;
+      return wave_and<EMIT>(A, B, out);
+    }
     if (A.lay == EHX_RANGE_BITSET) { *alg = ALG_PROBE; return wave_probe<EMIT>(B, A, out); }
     if (B.lay == EHX_RANGE_BITSET) { *alg = ALG_PROBE; return wave_probe<EMIT>(A, B, out); }
     const DSet &r = (A.n > B.n) ? B : A;
