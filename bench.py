@@ -2,11 +2,15 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 22] [--no-cpu]
 
-A step is one triangle-count query over the device-resident trie (the
-reference's QUERY TIME: trie build and loading excluded).  The tuned+learned
-form is timed for `value`; the plain GPU drop-in is timed beside it; the CPU
-oracle (a port of the reference engine) is timed on a bounded sample on the
-host cores.  Prints one JSON line.
+A step is one triangle-count query over the device-resident tries (the
+reference's QUERY TIME: trie build and loading excluded; the reference's loader
+builds both orderings of every relation, Edge_0_1 and Edge_1_0).  `value` is
+the tuned+learned form running the triangle with its middle vertex first; the
+tuned+learned form in the reference's attribute order, the tuned form on
+density-rule layouts and the plain GPU drop-in (the generated code's order) are
+timed beside it; the CPU oracle (a port of the reference engine) is timed on a
+bounded sample on the host cores.  Also measured: 4-clique on R-MAT-18, the
+ego-network batch and the triple-table joins.  Prints one JSON line.
 """
 import argparse
 import json
@@ -53,6 +57,8 @@ def bench_four_clique(E, ctx, steps, scale=18):
     s, d, n = G.rmat_graph(scale)
     out = {"workload": "FliqueAgg COUNT(*) on R-MAT scale %d, edge factor 16, pruned src<dst" % scale,
            "oriented_edges": len(s)}
+    from emptyheaded_amd import tuning as TU
+    TU.apply_plan("four_clique")
     T_rule = E.Trie(ctx, s, d, E.POLICY_RULE)
     T_learn = E.Trie(ctx, s, d, E.POLICY_LEARNED if _selector_loaded(E) else E.POLICY_RULE)
     counts = set()
@@ -66,6 +72,7 @@ def bench_four_clique(E, ctx, steps, scale=18):
     assert len(counts) == 1, "4-clique forms disagree"
     out["four_cliques"] = counts.pop()
     out["speedup_learned_vs_plain"] = out["plain_ms"] / out["learned_ms"]
+    TU.apply_plan("triangle")
     T_rule.close()
     T_learn.close()
     return out
@@ -223,6 +230,8 @@ def main():
     ap.add_argument("--selector", default=None, help="selector weights (default: the committed ones)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 4-clique R-MAT-18, ego-network batch and triple-table measurements")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time only the headline form (for profiling its kernel alone)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one copy kernel of this many bytes (traffic-counter reference)")
     args = ap.parse_args()
@@ -245,12 +254,18 @@ def main():
     tuning = load_tuning(E, args.selector)
     log("tuning:", json.dumps(tuning))
 
+    from emptyheaded_amd import tuning as TU
     ctx = E.Context(0)
+    learned_pol = E.POLICY_LEARNED if tuning.get("selector") else E.POLICY_RULE
     T_rule = E.Trie(ctx, src, dst, E.POLICY_RULE)
-    T_learn = E.Trie(ctx, src, dst, E.POLICY_LEARNED if tuning.get("selector") else E.POLICY_RULE)
+    T_learn = E.Trie(ctx, src, dst, learned_pol)
+    # the reverse ordering Edge_1_0, built for every relation as the reference's loader does
+    R_rule = E.Trie(ctx, dst, src, E.POLICY_RULE)
+    R_learn = E.Trie(ctx, dst, src, learned_pol)
     info = T_learn.info()
-    log("tries built: rule %.2f ms, learned %.2f ms; bitsets %d (learned) vs %d (rule)" %
-        (T_rule.info()["build_ms"], info["build_ms"], info["num_bitsets"], T_rule.info()["num_bitsets"]))
+    log("tries built: rule %.2f ms, learned %.2f ms (+ reverse %.2f / %.2f ms); bitsets %d (learned) vs %d (rule)" %
+        (T_rule.info()["build_ms"], info["build_ms"], R_rule.info()["build_ms"], R_learn.info()["build_ms"],
+         info["num_bitsets"], T_rule.info()["num_bitsets"]))
 
     if args.copy_calib:
         import ctypes as C
@@ -259,34 +274,51 @@ def main():
         log("copy kernel: %d bytes in %.3f ms (%.1f GB/s read+write)" %
             (args.copy_calib, cms.value, 2 * args.copy_calib / cms.value / 1e6))
 
+    MID, REF = "triangle_by_middle", "triangle"
+    PLAN = {MID: (plans.TRIANGLE_BY_MIDDLE, [T_learn, R_learn], [T_rule, R_rule]),
+            REF: (plans.TRIANGLE, [T_learn], [T_rule])}
     res = {}
-    count, ms, kms, st = time_form(E, ctx, T_learn, plans.TRIANGLE, E.FORM_LEARNED, args.steps, args.warmup)
-    res["learned"] = (count, ms, kms, st)
-    log("tuned+learned: %d triangles, %.3f ms/step, main kernel %.3f ms" % (count, ms, kms))
-    if not args.no_plain:
-        pc, pms, pkms, pst = time_form(E, ctx, T_rule, plans.TRIANGLE, E.FORM_PLAIN, max(2, args.steps // 2), 1)
-        res["plain"] = (pc, pms, pkms, pst)
-        log("plain: %d triangles, %.3f ms/step, main kernel %.3f ms" % (pc, pms, pkms))
-        assert pc == count, "plain and tuned+learned disagree"
-    # the same query with its tuned level replayed from a captured hipGraph
-    saved = E.tune_get()
-    E.tune_set(use_graph=1)
-    gc, gms, _, _ = time_form(E, ctx, T_learn, plans.TRIANGLE, E.FORM_LEARNED, args.steps, args.warmup)
-    graph_used = E.tune_get()["use_graph"] == 1
-    E.tune_set(**saved)
-    assert gc == count, "graph replay disagrees"
-    log("tuned+learned, hipGraph replay: %.3f ms/step (graph %s)" % (gms, "used" if graph_used else "refused"))
-    tc, tms, tkms, tst = time_form(E, ctx, T_rule, plans.TRIANGLE, E.FORM_TUNED, args.steps, args.warmup)
-    res["tuned"] = (tc, tms, tkms, tst)
-    log("tuned (rule layouts): %d triangles, %.3f ms/step, main kernel %.3f ms" % (tc, tms, tkms))
-    assert tc == count
+
+    def run(key, plan_name, which, form, steps, warmup):
+        TU.apply_plan(plan_name)
+        plan, learned_tries, rule_tries = PLAN[plan_name]
+        tries = learned_tries if which == "learned" else rule_tries
+        r = time_form(E, ctx, tries[0], plan, form, steps, warmup, tries=tries)
+        res[key] = r
+        log("%-28s %d triangles, %.3f ms/step, last level %.3f ms" % (key, r[0], r[1], r[2]))
+        return r
+
+    # headline: tuned+learned, attribute order by middle vertex
+    count, ms, kms, st = run("learned_by_middle", MID, "learned", E.FORM_LEARNED, args.steps, args.warmup)
+    if not args.headline_only:
+        run("learned_reference_order", REF, "learned", E.FORM_LEARNED, args.steps, args.warmup)
+        run("tuned_rule_by_middle", MID, "rule", E.FORM_TUNED, args.steps, args.warmup)
+        run("tuned_rule_reference_order", REF, "rule", E.FORM_TUNED, args.steps, args.warmup)
+    if not args.no_plain and not args.headline_only:
+        run("plain_reference_order", REF, "rule", E.FORM_PLAIN, max(2, args.steps // 2), 1)
+        run("plain_by_middle", MID, "rule", E.FORM_PLAIN, max(2, args.steps // 2), 1)
+    for k, r in res.items():
+        assert r[0] == count, "%s disagrees: %d vs %d" % (k, r[0], count)
+    # the headline query with its tuned level replayed from a captured hipGraph
+    gms, graph_used = None, False
+    if not args.headline_only:
+        TU.apply_plan(MID)
+        E.tune_set(use_graph=1)
+        gc, gms, _, _ = time_form(E, ctx, T_learn, plans.TRIANGLE_BY_MIDDLE, E.FORM_LEARNED, args.steps, args.warmup,
+                                  tries=[T_learn, R_learn])
+        graph_used = E.tune_get()["use_graph"] == 1
+        TU.apply_plan(MID)
+        assert gc == count, "graph replay disagrees"
+        log("learned_by_middle, hipGraph replay: %.3f ms/step (graph %s)" % (gms, "used" if graph_used else "refused"))
 
     achieved = balg / (kms * 1e-3)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_k_tuned_rmat22.json")
     if os.path.exists(tf) and args.scale == 22:
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("plan") == MID:
+            traffic = tj.get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": m / (ms * 1e-3),
@@ -305,22 +337,28 @@ def main():
                         "symmetrized, ascending-degree relabel, pruned src<dst" % (args.scale, args.edge_factor),
             "scale": args.scale, "edge_factor": args.edge_factor, "vertices": nv, "oriented_edges": m,
             "triangles": count, "form": "tuned+learned",
+            "plan": "attribute order (b, a, c) over Edge_0_1 and Edge_1_0 (middle vertex first)",
         },
         "results_per_s": count / (ms * 1e-3),
         "algorithmic_bytes": balg,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "k_tuned",
-                     "kernel_ms": kms},
-        "tuned_rule_ms_per_step": tms,
+                     "kernel_ms": kms,
+                     "traffic_frac": (traffic / (kms * 1e-3) / HBM_PEAK) if traffic else None},
+        "ms_per_step_by_form": {k: r[1] for k, r in res.items()},
+        "last_level_ms_by_form": {k: r[2] for k, r in res.items()},
         "graph_replay_ms_per_step": gms if graph_used else None,
         "tuning": tuning,
     }
-    if "plain" in res:
-        out["plain_ms_per_step"] = res["plain"][1]
-        out["plain_edges_per_s"] = m / (res["plain"][1] * 1e-3)
-        out["speedup_vs_plain"] = res["plain"][1] / ms
-        out["plain_pairs_by_alg"] = res["plain"][3]["pairs_by_alg"]
-    if not args.no_configs:
+    if "plain_reference_order" in res:
+        out["plain_ms_per_step"] = res["plain_reference_order"][1]
+        out["plain_edges_per_s"] = m / (out["plain_ms_per_step"] * 1e-3)
+        out["speedup_vs_plain"] = out["plain_ms_per_step"] / ms
+        out["speedup_vs_plain_same_plan"] = res["plain_by_middle"][1] / ms
+        out["speedup_learned_vs_plain_reference_order"] = (res["plain_reference_order"][1] /
+                                                         res["learned_reference_order"][1])
+        out["plain_pairs_by_alg"] = res["plain_reference_order"][3]["pairs_by_alg"]
+    if not args.no_configs and not args.headline_only:
         out["configs"] = {"rmat18_four_clique": bench_four_clique(E, ctx, max(2, args.steps // 2)),
                           "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2)),
                           "triple_star_path": bench_triples(E, ctx, max(2, args.steps // 2))}
@@ -330,8 +368,8 @@ def main():
     else:
         out["cpu_baseline"] = None
     ctx.synchronize()
-    T_rule.close()
-    T_learn.close()
+    for T in (T_rule, T_learn, R_rule, R_learn):
+        T.close()
     ctx.close()
     print(json.dumps(out))
 

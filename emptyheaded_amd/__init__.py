@@ -55,7 +55,8 @@ class JoinStats(C.Structure):
 
 class TuneParams(C.Structure):
     _fields_ = [("lds_slots", C.c_int32), ("big_deg", C.c_int32), ("waves_per_cu", C.c_int32),
-                ("grab", C.c_int32), ("bitmap_all", C.c_int32), ("use_graph", C.c_int32), ("big_work", C.c_int32)]
+                ("grab", C.c_int32), ("bitmap_all", C.c_int32), ("use_graph", C.c_int32), ("big_work", C.c_int32),
+                ("stream_lanes", C.c_int32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -92,6 +93,8 @@ SIGNATURES = {
                               C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
     "ehx_join_count_segments": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, _u32p, C.c_uint64,
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
+    "ehx_join_materialize": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, C.POINTER(_u32p), C.c_uint64, _u64p,
+                              C.POINTER(JoinStats)], C.c_int),
     "ehx_cycle4_count": ([_vp, _vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
     "ehx_cycle4_count_segments": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(JoinStats)], C.c_int),
@@ -301,6 +304,23 @@ def join_count_segments(ctx, levels, tries, bounds, form=FORM_PLAIN, stats=False
                                         counts.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(res), C.byref(st)))
     counts = counts[: len(b) - 1]
     return (counts, res.value, st.as_dict()) if stats else (counts, res.value)
+
+
+def join_materialize(ctx, levels, tries, form=FORM_PLAIN):
+    """Every result tuple in lexicographic order: an (rows, num_attrs) uint32 array."""
+    p = make_plan(levels) if not isinstance(levels, Plan) else levels
+    arr = (_vp * len(tries))(*[t.h.value for t in tries])
+    rows = C.c_uint64(0)
+    cols = (_u32p * p.num_attrs)()
+    rc = lib().ehx_join_materialize(ctx.h, C.byref(p), arr, form, cols, 0, C.byref(rows), None)
+    if rc not in (0, -3):
+        check(rc)
+    n = rows.value
+    out = np.zeros((p.num_attrs, max(n, 1)), dtype=np.uint32)
+    for j in range(p.num_attrs):
+        cols[j] = ptr_u32(out[j])
+    check(lib().ehx_join_materialize(ctx.h, C.byref(p), arr, form, cols, n, C.byref(rows), None))
+    return out[:, :n].T.copy()
 
 
 def cycle4_count(ctx, sym, begin=None, end=None, bounds=None, stats=False):

@@ -18,7 +18,8 @@ GRID = {
     "big_deg": [16, 32, 64, 128],
     "waves_per_cu": [8, 16, 24, 32, 48],
     "grab": [1, 2, 4, 8],
-    "big_work": [1024, 4096, 16384, 65536],
+    "big_work": [1024, 4096, 16384, 65536, 262144],
+    "stream_lanes": [4, 8, 16, 32, 64],
     "bitmap_all": [0, 1],
     "use_graph": [0, 1],
 }
@@ -28,20 +29,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measure(E, ctx, T, plan, form, reps):
-    E.join_count(ctx, plan, [T], form)  # warm (and capture, in graph mode)
+def measure(E, ctx, tries, plan, form, reps):
+    E.join_count(ctx, plan, tries, form)  # warm (and capture, in graph mode)
     ctx.synchronize()
     best, total = None, None
     for _ in range(reps):
         t0 = time.perf_counter()
-        total, st = E.join_count(ctx, plan, [T], form, stats=True)
+        total, st = E.join_count(ctx, plan, tries, form, stats=True)
         dt = (time.perf_counter() - t0) * 1e3
         best = dt if best is None else min(best, dt)
     return total, best
 
 
 def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID):
-    """workloads: list of (name, trie, plan).  Objective: sum of best-of-reps
+    """workloads: list of (name, tries, plan).  Objective: sum of best-of-reps
     wall ms per query over the workloads."""
     cur = E.tune_get()
     ref = {}
@@ -50,8 +51,8 @@ def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID):
     def evaluate(params):
         E.tune_set(**params)
         tot = 0.0
-        for name, T, plan in workloads:
-            cnt, ms = measure(E, ctx, T, plan, E.FORM_TUNED, reps)
+        for name, tries, plan in workloads:
+            cnt, ms = measure(E, ctx, tries, plan, E.FORM_TUNED, reps)
             if name in ref and ref[name] != cnt:
                 raise AssertionError("knobs changed the result of %s: %d vs %d" % (name, cnt, ref[name]))
             ref.setdefault(name, cnt)
@@ -78,35 +79,47 @@ def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID):
     return cur, best_ms, history, ref
 
 
+PLANS = ("mixed", "triangle", "triangle_by_middle", "four_clique")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default="gpurun_out/tuning.json")
+    ap.add_argument("--plan", default="mixed", choices=PLANS,
+                    help="mixed: triangle and 4-clique together (the global knobs); otherwise one plan's knobs")
     ap.add_argument("--scales", default="20")
     ap.add_argument("--clique-scale", type=int, default=17)
     ap.add_argument("--passes", type=int, default=2)
     args = ap.parse_args()
     import emptyheaded_amd as E
     from emptyheaded_amd import graphs as G
-    from emptyheaded_amd import plans
+    from emptyheaded_amd import plans, tuning
+    tuning.apply_committed()  # start from the committed knobs
     ctx = E.Context(0)
     workloads, keep = [], []
-    for s in [int(x) for x in args.scales.split(",")]:
-        src, dst, nv = G.rmat_graph(s)
-        T = E.Trie(ctx, src, dst)
-        keep.append(T)
-        workloads.append(("triangle_rmat%d" % s, T, plans.TRIANGLE))
-    if args.clique_scale:
+    if args.plan in ("mixed", "triangle", "triangle_by_middle"):
+        for s in [int(x) for x in args.scales.split(",")]:
+            src, dst, nv = G.rmat_graph(s)
+            T = E.Trie(ctx, src, dst)
+            keep.append(T)
+            if args.plan == "triangle_by_middle":
+                T10 = E.Trie(ctx, dst, src)
+                keep.append(T10)
+                workloads.append(("triangle_by_middle_rmat%d" % s, [T, T10], plans.TRIANGLE_BY_MIDDLE))
+            else:
+                workloads.append(("triangle_rmat%d" % s, [T], plans.TRIANGLE))
+    if args.plan in ("mixed", "four_clique") and args.clique_scale:
         src, dst, nv = G.rmat_graph(args.clique_scale)
         T = E.Trie(ctx, src, dst)
         keep.append(T)
-        workloads.append(("clique4_rmat%d" % args.clique_scale, T, plans.FOUR_CLIQUE))
+        workloads.append(("clique4_rmat%d" % args.clique_scale, [T], plans.FOUR_CLIQUE))
     best, ms, hist, ref = tune(E, ctx, workloads, passes=args.passes)
-    out = {"best": best, "objective_ms": ms, "workloads": [w[0] for w in workloads], "counts": ref,
-           "history": hist}
+    out = {"plan": args.plan, "best": best, "objective_ms": ms, "workloads": [w[0] for w in workloads],
+           "counts": ref, "history": hist}
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
     log("best", json.dumps(best), "%.3f ms" % ms)
-    print(json.dumps({"best": best, "objective_ms": ms}))
+    print(json.dumps({"plan": args.plan, "best": best, "objective_ms": ms}))
 
 
 if __name__ == "__main__":

@@ -47,6 +47,8 @@ int join_count(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint64_t, ui
 int join_count_segments(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, const uint32_t *, uint64_t, int64_t *,
                         int64_t *, ehx_join_stats *);
 int cycle4_count(ehx_ctx *, const ehx_trie *, uint64_t, uint64_t, const SegOut *, int64_t *, ehx_join_stats *);
+int join_materialize(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint32_t *const *, uint64_t, uint64_t *,
+                     ehx_join_stats *);
 int cycle4_count_segments(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *, int64_t *,
                           ehx_join_stats *);
 int intersect_pairs(ehx_ctx *, const ehx_trie *, const uint32_t *, const ehx_trie *, const uint32_t *, uint64_t,
@@ -208,6 +210,11 @@ int ehx_join_count_segments(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const 
   return join_count_segments(ctx, plan, tries, form, bounds, nseg, counts, result, stats);
 }
 
+int ehx_join_materialize(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, uint32_t *const *cols,
+                         uint64_t cap, uint64_t *rows, ehx_join_stats *stats) {
+  return join_materialize(ctx, plan, tries, form, cols, cap, rows, stats);
+}
+
 int ehx_cycle4_count(ehx_ctx *ctx, const ehx_trie *sym, uint64_t begin, uint64_t end, int64_t *result,
                      ehx_join_stats *stats) {
   return cycle4_count(ctx, sym, begin, end, nullptr, result, stats);
@@ -236,6 +243,7 @@ int ehx_tune_get(ehx_tune_params *out) {
   out->bitmap_all = p.bitmap_all;
   out->use_graph = p.use_graph;
   out->big_work = p.big_work;
+  out->stream_lanes = p.stream_lanes;
   return EHX_OK;
 }
 
@@ -246,6 +254,9 @@ int ehx_tune_set(const ehx_tune_params *in) {
   if (in->big_deg < 1 || in->waves_per_cu < 1 || in->waves_per_cu > 64 || in->grab < 1 || in->grab > 64 ||
       in->big_work < 1)
     return fail(EHX_ERR_INVALID, "tune parameter out of range");
+  // >= 4: a group reads the up-to-3-element unaligned head and tail of a set one element per lane
+  if (in->stream_lanes < 4 || in->stream_lanes > 64 || (in->stream_lanes & (in->stream_lanes - 1)))
+    return fail(EHX_ERR_INVALID, "stream_lanes must be a power of two in [4, 64]");
   TuneParams &p = tune_params();
   p.lds_slots = in->lds_slots;
   p.big_deg = in->big_deg;
@@ -254,6 +265,7 @@ int ehx_tune_set(const ehx_tune_params *in) {
   p.bitmap_all = in->bitmap_all ? 1 : 0;
   p.use_graph = in->use_graph ? 1 : 0;
   p.big_work = in->big_work;
+  p.stream_lanes = in->stream_lanes;
   return EHX_OK;
 }
 

@@ -177,6 +177,77 @@ __global__ void k_seg_level0(const uint32_t *__restrict__ f0, uint64_t n, SegOut
     seg_add(seg, f0[i], 1ull);
 }
 
+// Level 0: ParTrieBuilder::build_aggregated_set over the head sets
+// (TrieBuilder.cpp:478-578).  One relation: its head set; several: the driver
+// (smallest) head filtered by membership in the others, then compacted.
+static int level0(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, const Tries &T, const uint32_t **f0_out,
+                  uint64_t *n0_out) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  LevelDesc L0 = level_desc(p, 0);
+  int driver = 0;
+  for (int i = 1; i < L0.nops; i++)
+    if (tries[L0.rel[i]]->nh <= tries[L0.rel[driver]]->nh) driver = i;
+  bool single = true;
+  for (int i = 0; i < L0.nops; i++)
+    if (L0.rel[i] != L0.rel[driver]) single = false;
+  const ehx_trie *td = tries[L0.rel[driver]];
+  if (single) {
+    *f0_out = td->head;  // head ∩ head of one trie is the head itself
+    *n0_out = td->nh;
+    return EHX_OK;
+  }
+  DevBuf &fl = ctx->scratch[S_FLAGS], &ps = ctx->scratch[S_POS], &fb = ctx->scratch[S_F0];
+  if ((rc = fl.ensure(td->nh * 4 + 4)) || (rc = ps.ensure(td->nh * 8 + 8)) || (rc = fb.ensure(td->nh * 4 + 4))) return rc;
+  k_level0_flags<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(T, L0, driver, fl.as<uint32_t>());
+  EHX_HIP(hipGetLastError());
+  if ((rc = scan_u32_to_u64(ctx, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh))) return rc;
+  k_compact_u32<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(td->head, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh,
+                                                                fb.as<uint32_t>());
+  EHX_HIP(hipGetLastError());
+  if ((rc = copy_d2h(ctx, n0_out, ps.as<uint64_t>() + td->nh, 8))) return rc;
+  *f0_out = fb.as<uint32_t>();
+  return EHX_OK;
+}
+
+// Level k materialized: count pass, exclusive scan, write pass (the frontier
+// stays lexicographic, the order foreach_aggregate visits).  The k+1 output
+// columns live in scratch slot S_F0 + k.
+static int expand_level(ehx_ctx *ctx, const Tries &T, const LevelDesc &L, Frontier &in, int k, uint64_t n, int pick,
+                        uint64_t *nn_out) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  DevBuf &cb = ctx->scratch[S_COUNTS], &ob = ctx->scratch[S_OFFS];
+  if ((rc = cb.ensure(n * 4 + 4)) || (rc = ob.ensure(n * 8 + 8))) return rc;
+  unsigned G = grid_for(n * 64, 256, 65536);
+  OutFrontier none;
+  memset(&none, 0, sizeof(none));
+  if (n) {
+    k_level_plain<MODE_COUNT><<<G, 256, 0, st>>>(T, L, in, k, n, cb.as<uint32_t>(), nullptr, none, nullptr,
+                                                 SegOut{}, pick);
+    EHX_HIP(hipGetLastError());
+  }
+  if ((rc = scan_u32_to_u64(ctx, cb.as<uint32_t>(), ob.as<uint64_t>(), n))) return rc;
+  uint64_t nn = 0;
+  if ((rc = copy_d2h(ctx, &nn, ob.as<uint64_t>() + n, 8))) return rc;
+  if (nn * 4ull * (k + 1) > FRONTIER_BUDGET)
+    return fail(EHX_ERR_UNSUPPORTED, "level %d frontier of %llu tuples exceeds the %llu-byte budget", k,
+                (unsigned long long)nn, (unsigned long long)FRONTIER_BUDGET);
+  DevBuf &fb = ctx->scratch[S_F0 + k];
+  if ((rc = fb.ensure(nn * 4ull * (k + 1) + 4))) return rc;
+  OutFrontier out;
+  memset(&out, 0, sizeof(out));
+  for (int j = 0; j <= k; j++) out.col[j] = fb.as<uint32_t>() + (uint64_t)j * nn;
+  if (n && nn) {
+    k_level_plain<MODE_WRITE><<<G, 256, 0, st>>>(T, L, in, k, n, nullptr, ob.as<uint64_t>(), out, nullptr, SegOut{},
+                                                 pick);
+    EHX_HIP(hipGetLastError());
+  }
+  for (int j = 0; j <= k; j++) in.col[j] = out.col[j];
+  *nn_out = nn;
+  return EHX_OK;
+}
+
 int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form, uint64_t begin, uint64_t end,
                int64_t *result, ehx_join_stats *stats, const RecordReq *rec, const SegOut *seg) {
   int nrels = 0, rc;
@@ -198,31 +269,9 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
   EHX_HIP(hipEventRecord(ctx->ev[0], st));
 
   // ---- level 0 -------------------------------------------------------------
-  LevelDesc L0 = level_desc(p, 0);
-  int driver = 0;
-  for (int i = 1; i < L0.nops; i++)
-    if (tries[L0.rel[i]]->nh <= tries[L0.rel[driver]]->nh) driver = i;
-  bool single = true;
-  for (int i = 0; i < L0.nops; i++)
-    if (L0.rel[i] != L0.rel[driver]) single = false;
-  const ehx_trie *td = tries[L0.rel[driver]];
   const uint32_t *f0;
   uint64_t n0;
-  if (single) {
-    f0 = td->head;  // head ∩ head of one trie is the head itself
-    n0 = td->nh;
-  } else {
-    DevBuf &fl = ctx->scratch[S_FLAGS], &ps = ctx->scratch[S_POS], &fb = ctx->scratch[S_F0];
-    if ((rc = fl.ensure(td->nh * 4 + 4)) || (rc = ps.ensure(td->nh * 8 + 8)) || (rc = fb.ensure(td->nh * 4 + 4))) return rc;
-    k_level0_flags<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(T, L0, driver, fl.as<uint32_t>());
-    EHX_HIP(hipGetLastError());
-    if ((rc = scan_u32_to_u64(ctx, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh))) return rc;
-    k_compact_u32<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(td->head, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh,
-                                                                  fb.as<uint32_t>());
-    EHX_HIP(hipGetLastError());
-    if ((rc = copy_d2h(ctx, &n0, ps.as<uint64_t>() + td->nh, 8))) return rc;
-    f0 = fb.as<uint32_t>();
-  }
+  if ((rc = level0(ctx, p, tries, T, &f0, &n0))) return rc;
   if (begin > n0) begin = n0;
   if (end > n0) end = n0;
   if (end < begin) end = begin;
@@ -258,33 +307,8 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
     }
     for (int k = 1; k <= last_mat; k++) {
       LevelDesc L = level_desc(p, k);
-      DevBuf &cb = ctx->scratch[S_COUNTS], &ob = ctx->scratch[S_OFFS];
-      if ((rc = cb.ensure(n * 4 + 4)) || (rc = ob.ensure(n * 8 + 8))) return rc;
-      unsigned G = grid_for(n * 64, 256, 65536);
-      OutFrontier none;
-      memset(&none, 0, sizeof(none));
-      if (n) {
-        k_level_plain<MODE_COUNT><<<G, 256, 0, st>>>(T, L, in, k, n, cb.as<uint32_t>(), nullptr, none, nullptr,
-                                                     SegOut{}, pick);
-        EHX_HIP(hipGetLastError());
-      }
-      if ((rc = scan_u32_to_u64(ctx, cb.as<uint32_t>(), ob.as<uint64_t>(), n))) return rc;
       uint64_t nn = 0;
-      if ((rc = copy_d2h(ctx, &nn, ob.as<uint64_t>() + n, 8))) return rc;
-      if (nn * 4ull * (k + 1) > FRONTIER_BUDGET)
-        return fail(EHX_ERR_UNSUPPORTED, "level %d frontier of %llu tuples exceeds the %llu-byte budget", k,
-                    (unsigned long long)nn, (unsigned long long)FRONTIER_BUDGET);
-      DevBuf &fb = ctx->scratch[S_F0 + k];
-      if ((rc = fb.ensure(nn * 4ull * (k + 1) + 4))) return rc;
-      OutFrontier out;
-      memset(&out, 0, sizeof(out));
-      for (int j = 0; j <= k; j++) out.col[j] = fb.as<uint32_t>() + (uint64_t)j * nn;
-      if (n && nn) {
-        k_level_plain<MODE_WRITE><<<G, 256, 0, st>>>(T, L, in, k, n, nullptr, ob.as<uint64_t>(), out, nullptr,
-                                                     SegOut{}, pick);
-        EHX_HIP(hipGetLastError());
-      }
-      for (int j = 0; j <= k; j++) in.col[j] = out.col[j];
+      if ((rc = expand_level(ctx, T, L, in, k, n, pick, &nn))) return rc;
       n = nn;
       S.frontier[k] = nn;
       EHX_HIP(hipEventRecord(ctx->ev[1 + k], st));
@@ -349,6 +373,58 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
   S.total_ms = ms;
   S.form_used = form_used;
   *result = total;
+  if (stats) *stats = S;
+  return EHX_OK;
+}
+
+// ---- materialized result -------------------------------------------------------------
+// Every result tuple, in lexicographic order: each level is materialized with a
+// count pass, a scan and a write pass (what TrieBuilder::build_set, set_level and
+// allocate_next build level by level, TrieBuilder.cpp:44-221,376-446, read back in
+// Trie::foreach order, Trie.cpp:78-160).
+int join_materialize(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form, uint32_t *const *cols,
+                     uint64_t cap, uint64_t *rows, ehx_join_stats *stats) {
+  int nrels = 0, rc;
+  if (!ctx) return fail(EHX_ERR_INVALID, "null ctx");
+  if ((rc = check_plan(p, tries, &nrels))) return rc;
+  if (!rows) return fail(EHX_ERR_INVALID, "null rows");
+  if (form < EHX_FORM_PLAIN || form > EHX_FORM_LEARNED) return fail(EHX_ERR_INVALID, "bad form %d", form);
+  EHX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  ehx_join_stats S;
+  memset(&S, 0, sizeof(S));
+  Tries T;
+  memset(&T, 0, sizeof(T));
+  for (int r = 0; r < nrels; r++) {
+    if (tries[r]->device != ctx->device) return fail(EHX_ERR_INVALID, "trie %d lives on another device", r);
+    T.t[r] = view(tries[r]);
+  }
+  const int K = p->num_attrs;
+  EHX_HIP(hipEventRecord(ctx->ev[0], st));
+  const uint32_t *f0;
+  uint64_t n;
+  if ((rc = level0(ctx, p, tries, T, &f0, &n))) return rc;
+  S.frontier[0] = n;
+  Frontier in;
+  memset(&in, 0, sizeof(in));
+  in.col[0] = f0;
+  for (int k = 1; k < K; k++) {
+    uint64_t nn = 0;
+    if ((rc = expand_level(ctx, T, level_desc(p, k), in, k, n, form != EHX_FORM_PLAIN, &nn))) return rc;
+    n = nn;
+    S.frontier[k] = nn;
+  }
+  EHX_HIP(hipEventRecord(ctx->ev[1], st));
+  *rows = n;
+  if (n > cap) return fail(EHX_ERR_NOMEM, "result has %llu rows, capacity %llu", (unsigned long long)n,
+                           (unsigned long long)cap);
+  for (int j = 0; j < K && n; j++) {
+    if (!cols || !cols[j]) return fail(EHX_ERR_INVALID, "null output column %d", j);
+    if ((rc = copy_d2h(ctx, cols[j], in.col[j], n * 4))) return rc;
+  }
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) S.total_ms = ms;
+  S.form_used = EHX_FORM_PLAIN;
   if (stats) *stats = S;
   return EHX_OK;
 }

@@ -2,11 +2,13 @@
 //
 // Applies when the last level's operands are one child set of x_{K-2} (in
 // relation x_rel) plus child sets S_1..S_s of earlier attributes, and level K-2
-// intersects exactly those S_i plus head-set filters.  Then for each prefix p
-// (a tuple of levels 0..K-3)
-//     count(p) = sum over x in S(p) ∩ heads of |child(x_rel, x) ∩ S(p)|,
-// with S(p) = S_1 ∩ .. ∩ S_s.  Triangle (S = N(a)) and 4-clique
-// (S = N(a) ∩ N(b)) have this shape.  One persistent kernel drains a queue of
+// intersects head-set filters with either exactly those S_i or one other child
+// set D of an earlier attribute.  Then for each prefix p (a tuple of levels
+// 0..K-3)
+//     count(p) = sum over x in E(p) ∩ heads of |child(x_rel, x) ∩ S(p)|,
+// with S(p) = S_1 ∩ .. ∩ S_s and E(p) = S(p) or D(p).  Triangle (S = N(a)),
+// 4-clique (S = N(a) ∩ N(b)) and the triangle ordered by its middle vertex
+// (E = in-neighbors of b, S = out-neighbors of b) have this shape.  One persistent kernel drains a queue of
 // prefixes binned by estimated work (largest first); each wavefront stages S(p)
 // in LDS once -- as a bitmap when S(p) is a bitset that fits, as an open-
 // addressing hash table otherwise -- and streams every child(x) through it.
@@ -23,6 +25,9 @@ struct TunedPlan {
   int32_t nhf;
   int32_t h_rel[EHX_MAX_OPS];
   int32_t x_rel;
+  // nd = 1: level K-2 enumerates x over one driver set D(p) = child(d_rel,
+  // prefix[d_parent]) instead of over S(p) itself
+  int32_t nd, d_rel, d_parent;
 };
 
 struct TunedArgs {
@@ -44,6 +49,7 @@ struct TuneParams {
   int32_t bitmap_all;     // 1: stage any set whose word window fits as a bitmap
   int32_t use_graph;      // capture the level in a hipGraph and replay it
   int32_t big_work;       // prefixes with at least this much estimated work go to the workgroup bin
+  int32_t stream_lanes;   // lanes that stream one long child set together (a power of two, 4..64)
 };
 
 constexpr int TUNED_MAX_S = 4;  // operand sets of S(p) the tuned kernel accepts

@@ -26,7 +26,7 @@
 
 namespace ehx {
 
-static TuneParams g_params = {4096, 32, 24, 2, 0, 0, 4096};
+static TuneParams g_params = {4096, 32, 24, 2, 0, 0, 4096, 16};
 TuneParams &tune_params() { return g_params; }
 
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
@@ -43,7 +43,7 @@ bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
   }
   if (xcount != 1 || tp->ns < 1 || tp->ns > TUNED_MAX_S) return false;
   bool used[EHX_MAX_OPS] = {false};
-  int matched = 0;
+  int matched = 0, other = 0;
   for (int i = 0; i < p->num_ops[K - 2]; i++) {
     const ehx_op &o = p->ops[K - 2][i];
     if (o.parent < 0) {
@@ -53,10 +53,18 @@ bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
     bool found = false;
     for (int s = 0; s < tp->ns && !found; s++)
       if (!used[s] && tp->s_rel[s] == o.rel && tp->s_parent[s] == o.parent) { used[s] = true; found = true; }
-    if (!found) return false;
-    matched++;
+    if (found) {
+      matched++;
+    } else {
+      tp->d_rel = o.rel;
+      tp->d_parent = o.parent;
+      other++;
+    }
   }
-  return matched == tp->ns;
+  if (other == 0) return matched == tp->ns;
+  // one driver set, and no S set among level K-2's operands
+  tp->nd = 1;
+  return other == 1 && matched == 0;
 }
 
 static constexpr uint32_t EMPTY = 0xFFFFFFFFu;
@@ -79,6 +87,16 @@ __device__ __forceinline__ bool resolve_s(const TunedPlan &P, const TunedArgs &A
   return true;
 }
 
+// The driver set of a prefix (plans with nd = 1); true with nothing to do otherwise.
+__device__ __forceinline__ bool resolve_d(const TunedPlan &P, const TunedArgs &A, uint64_t p, DSet &D) {
+  if (!P.nd) return true;
+  const DTrie &t = A.T.t[P.d_rel];
+  const int32_t h = node_of(t, A.prefix[P.d_parent][p]);
+  if (h < 0) return false;
+  D = set_of(t, (uint32_t)h);
+  return D.n > 0;
+}
+
 __device__ __forceinline__ bool head_ok(const TunedPlan &P, const TunedArgs &A, uint32_t x) {
   for (int i = 0; i < P.nhf; i++)
     if (node_of(A.T.t[P.h_rel[i]], x) < 0) return false;
@@ -97,21 +115,23 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.nprefix;
        p += (uint64_t)gridDim.x * blockDim.x) {
     DSet S[TUNED_MAX_S];
+    DSet Dd;
     uint32_t b = BUCKET_NONE;
-    if (resolve_s<TUNED_MAX_S>(P, A, p, S)) {
+    if (resolve_s<TUNED_MAX_S>(P, A, p, S) && resolve_d(P, A, p, Dd)) {
       int mi = 0;
       for (int i = 1; i < P.ns; i++)
         if (S[i].n < S[mi].n) mi = i;
       const DSet D = S[mi];
-      const uint32_t ns = D.n < 32 ? D.n : 32;  // sample up to 32 elements of the smallest S set
+      const DSet X = P.nd ? Dd : D;  // the set x runs over
+      const uint32_t ns = X.n < 32 ? X.n : 32;  // sample up to 32 of its elements
       uint64_t w = 0;
 #pragma unroll 8
       for (uint32_t k = 0; k < ns; k++) {
-        uint32_t x = D.v[(uint64_t)k * D.n / ns];
+        uint32_t x = X.v[(uint64_t)k * X.n / ns];
         int32_t nx = node_of(tx, x);
         if (nx >= 0) w += tx.off[nx + 1] - tx.off[nx];
       }
-      w = w * D.n / (ns ? ns : 1) + D.n;
+      w = w * X.n / (ns ? ns : 1) + X.n + (P.nd ? D.n : 0u);
       if (P.ns > 1) w = w / 2 + 1;
       // the wave bin needs S(p) to fit a quarter of the LDS (hash) unless it is a fitting bitset
       bool fits_wave = D.n <= wave_hash_cap(slots);
@@ -196,7 +216,7 @@ __device__ __forceinline__ uint32_t probe(const Stage &s, const DSet *S, int ns,
 template <bool BLOCK, int MAXS>
 __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, uint32_t *tab, uint32_t tab_slots,
                                  uint32_t *flt, uint32_t flt_words, uint32_t *sv, uint32_t sv_cap, uint32_t *own,
-                                 int big_deg, int bitmap_all, int &mode_used) {
+                                 int big_deg, int bitmap_all, uint32_t slg, int &mode_used) {
   const uint32_t lane = __lane_id();
   const uint32_t gtid = BLOCK ? threadIdx.x : lane;
   const uint32_t gsize = BLOCK ? blockDim.x : 64u;
@@ -207,7 +227,8 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
     else wave_sync();
   };
   DSet S[MAXS];
-  if (!resolve_s<MAXS>(P, A, p, S)) return 0;
+  DSet Dd;
+  if (!resolve_s<MAXS>(P, A, p, S) || !resolve_d(P, A, p, Dd)) return 0;
   int mi = 0;
 #pragma unroll
   for (int i = 1; i < MAXS; i++)
@@ -306,8 +327,9 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
   mode_used = st.mode;
 
   const DTrie &tx = A.T.t[P.x_rel];
-  const uint32_t *xsrc = src ? src : S[mi].v;
-  const uint32_t xn = src ? sn : S[mi].n;
+  const uint32_t *xsrc = P.nd ? Dd.v : (src ? src : S[mi].v);
+  const uint32_t xn = P.nd ? Dd.n : (src ? sn : S[mi].n);
+  const bool filter_s = !P.nd && !src;  // x runs over one S set: test it against the others
   uint64_t hits = 0;
   // A wave takes XW elements of S(p) per step: 64 on its own; 16 inside a
   // workgroup, so the four waves share even a small S(p) evenly.
@@ -316,7 +338,7 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
     const uint32_t idx = base + lane;
     bool ok = lane < XW && idx < xn;
     const uint32_t x = ok ? xsrc[idx] : 0u;
-    if (ok && !src) {
+    if (ok && filter_s) {
 #pragma unroll
       for (int i = 0; i < MAXS; i++)
         if (i < P.ns && i != mi && ok) ok = set_contains(S[i], x);
@@ -328,20 +350,25 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
       o = tx.off[nx];
       d = tx.off[nx + 1] - o;
     }
-    // long child sets: the whole wave streams each one with 128-bit loads,
-    // two quads per lane in flight per step
+    // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
+    // streamed with 128-bit loads, two quads per lane in flight per step (the
+    // x's of one prefix are adjacent ids, so the lengths in a round are alike)
     uint64_t big = __ballot(d >= (uint32_t)big_deg);
+    const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
     while (big) {
-      const int l = __ffsll((unsigned long long)big) - 1;
-      big &= big - 1;
-      const uint32_t bo = __shfl(o, l, 64), bd = __shfl(d, l, 64);
+      uint64_t mine = big;  // this group's list: the qt-th lowest set bit
+      for (uint32_t k = 0; k < qt; k++) mine &= mine - 1;
+      const int myl = mine ? __ffsll((unsigned long long)mine) - 1 : -1;
+      for (uint32_t k = 0; k < lists; k++) big &= big - 1;
+      const uint32_t so = __shfl(o, myl < 0 ? 0 : myl, 64), sd = __shfl(d, myl < 0 ? 0 : myl, 64);
+      const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
       const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
-      if (lane < head) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + lane]);
+      if (ql < head) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + ql]);
       const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
       const uint32_t n4 = (bd - head) >> 2;
-      uint32_t i = lane;
-      for (; i + 64 < n4; i += 128) {
-        const uint4 v0 = q[i], v1 = q[i + 64];
+      uint32_t i = ql;
+      for (; i + G < n4; i += 2 * G) {
+        const uint4 v0 = q[i], v1 = q[i + G];
         hits += probe<MAXS>(st, S, P.ns, v0.x) + probe<MAXS>(st, S, P.ns, v0.y) +
                 probe<MAXS>(st, S, P.ns, v0.z) + probe<MAXS>(st, S, P.ns, v0.w) +
                 probe<MAXS>(st, S, P.ns, v1.x) + probe<MAXS>(st, S, P.ns, v1.y) +
@@ -353,7 +380,7 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
                 probe<MAXS>(st, S, P.ns, v0.z) + probe<MAXS>(st, S, P.ns, v0.w);
       }
       const uint32_t done = head + 4 * n4;
-      if (lane < bd - done) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + done + lane]);
+      if (ql < bd - done) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + done + ql]);
     }
     // short ones: expand owners into the LDS window, then one element per lane per step
     const uint32_t ds = (d < (uint32_t)big_deg) ? d : 0u;
@@ -378,7 +405,7 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
 template <int SLOTS, int MAXS>
 __global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
                                                unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
-                                               int big_deg, int grab, int bitmap_all) {
+                                               int big_deg, int grab, int bitmap_all, uint32_t slg) {
   constexpr uint32_t WS = SLOTS / 4;                 // per-wave table slots
   constexpr uint32_t FWB = SLOTS / 4, FWW = FWB / 4;  // filter words per block / wave
   constexpr uint32_t VC = (MAXS > 1) ? WS : 1u;      // materialized S capacity per wave
@@ -400,7 +427,7 @@ __global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const u
     const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
     int mode = -1;
     const uint64_t h = tuned_prefix<true, MAXS>(P, A, queue[qi], tab, SLOTS, flt, FWB, sv, 4 * VC, own[w], big_deg,
-                                                bitmap_all, mode);
+                                                bitmap_all, slg, mode);
     hits += h;
     if (A.seg.out) {
       const uint64_t hw = wave_sum(h);
@@ -420,7 +447,7 @@ __global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const u
       const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
       int mode = -1;
       const uint64_t h = tuned_prefix<false, MAXS>(P, A, queue[qi], tab + w * WS, WS, flt + w * FWW, FWW,
-                                                   sv + w * VC, VC, own[w], big_deg, bitmap_all, mode);
+                                                   sv + w * VC, VC, own[w], big_deg, bitmap_all, slg, mode);
       hits += h;
       if (A.seg.out) {
         const uint64_t hw = wave_sum(h);
@@ -437,10 +464,11 @@ __global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const u
 template <int SLOTS>
 static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, const TunedArgs &A, const uint32_t *queue,
                          unsigned *ctl, unsigned long long *acc, const TuneParams &tp) {
+  const uint32_t slg = 31u - (uint32_t)__builtin_clz((unsigned)tp.stream_lanes);  // a power of two in [4, 64]
   if (P.ns == 1)
-    k_tuned<SLOTS, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all);
+    k_tuned<SLOTS, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
   else
-    k_tuned<SLOTS, TUNED_MAX_S><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all);
+    k_tuned<SLOTS, TUNED_MAX_S><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
 }
 
 // One cached graph per process: the key is every value baked into the launches.

@@ -17,6 +17,20 @@ TRIANGLE = [[(0, -1), (0, -1)],
             [(0, 0), (0, -1)],
             [(0, 1), (0, 0)]]
 
+# The same TriangleAgg with attribute order (b, a, c): relation 0 = Edge_0_1,
+# relation 1 = Edge_1_0 (the reverse ordering, which the reference builds for
+# every binary relation: all attribute permutations,
+# query_compiler/src/main/scala/codegen/create/createDB.scala:232-233).
+#   b: head(Edge_0_1) ∩ head(Edge_1_0)
+#   a: child(Edge_1_0, b) ∩ head(Edge_0_1)
+#   c: child(Edge_0_1, a) ∩ child(Edge_0_1, b)
+# Every triangle a < b < c is found once, at its middle vertex b.  With the
+# ascending-degree orientation the last level streams sum_a d+(a)^2 values
+# instead of sum_b d-(b) d+(b) for the (a, b, c) order.
+TRIANGLE_BY_MIDDLE = [[(0, -1), (1, -1)],
+                      [(1, 0), (0, -1)],
+                      [(0, 1), (0, 0)]]
+
 # FliqueAgg(;z) :- Edge(a,b),Edge(b,c),Edge(a,c),Edge(a,d),Edge(b,d),Edge(c,d),COUNT(*).
 #   a: heads of Edge_a_b, Edge_a_c, Edge_a_d
 #   b: child(Edge_a_b, a) ∩ head(Edge_b_c) ∩ head(Edge_b_d)

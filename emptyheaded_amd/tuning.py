@@ -17,6 +17,17 @@ def selector_active():
     return _selector_active
 
 
+def apply_plan(plan_name):
+    """Apply the committed knobs, then the overrides the autotuner found for
+    one plan (tuning.json "plans"); unknown plans get the global knobs."""
+    import emptyheaded_amd as E
+    with open(TUNING_JSON) as f:
+        t = json.load(f)
+    params = dict(t["best"])
+    params.update(t.get("plans", {}).get(plan_name, {}).get("best", {}))
+    return E.tune_set(**params)
+
+
 def apply_committed(device=0, selector_path=None):
     global _selector_active
     import emptyheaded_amd as E

@@ -185,6 +185,15 @@ int ehx_join_count_segments(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const 
                             const uint32_t *bounds, uint64_t nseg, int64_t *counts, int64_t *result,
                             ehx_join_stats *stats);
 
+/* Materialized result: every tuple of the join, in lexicographic order (the
+ * output trie that TrieBuilder::build_set / set_level / allocate_next build,
+ * src/TrieBuilder.cpp:44-221,376-446, read in Trie::foreach order,
+ * src/Trie.cpp:78-160).  cols[j] (host, capacity cap) receives attribute j;
+ * *rows the number of tuples.  When rows > cap nothing is copied and
+ * EHX_ERR_NOMEM is returned with *rows set. */
+int ehx_join_materialize(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form,
+                         uint32_t *const *cols, uint64_t cap, uint64_t *rows, ehx_join_stats *stats);
+
 /* ---- 4-cycle count -------------------------------------------------------- */
 /* Number of 4-cycles (each cycle once) of the undirected graph whose edges the
  * trie `sym` holds in both directions, restricted to cycles whose largest

@@ -23,6 +23,7 @@ typedef struct {
   int32_t bitmap_all;   /* stage every set whose word window fits as a bitmap */
   int32_t use_graph;    /* capture the level in a hipGraph and replay it */
   int32_t big_work;     /* prefixes with at least this much estimated work are staged once per workgroup */
+  int32_t stream_lanes; /* lanes streaming one long child set together: a power of two in [4, 64] */
 } ehx_tune_params;
 
 int ehx_tune_get(ehx_tune_params *out);

@@ -616,6 +616,69 @@ int orc_join_count_per_head(const orc_plan *p, orc_trie *const *tries, int nthre
 }
 
 /* ------------------------------------------------------------------------- */
+/* Materialized result: the generated code's build_set / set_level loop       */
+/* (TrieBuilder.cpp:44-221,376-446) run depth-first on one thread, every      */
+/* level decoded; tuples come out in lexicographic order, the order           */
+/* Trie::foreach reads an output trie (Trie.cpp:78-160).                      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  join_thread th;
+  uint32_t *cols;  /* num_attrs columns of cap entries */
+  uint64_t cap, rows;
+} mat_state;
+
+static void mat_rec(mat_state *ms, int level) {
+  join_thread *th = &ms->th;
+  const join_ctx *jc = th->jc;
+  const orc_plan *p = jc->p;
+  const orc_set *sets[ORC_MAX_OPS];
+  if (!gather(jc, level, th->tuple, sets)) return;
+  orc_set r;
+  if (nway(&r, th->buf[level][0], th->buf[level][1], sets, p->num_ops[level], 0) == 0) return;
+  uint64_t k = orc_set_decode(&r, th->vals[level]);
+  for (uint64_t i = 0; i < k; i++) {
+    th->tuple[level] = th->vals[level][i];
+    if (level == p->num_attrs - 1) {
+      if (ms->rows < ms->cap)
+        for (int j = 0; j < p->num_attrs; j++) ms->cols[(uint64_t)j * ms->cap + ms->rows] = th->tuple[j];
+      ms->rows++;
+    } else {
+      mat_rec(ms, level + 1);
+    }
+  }
+}
+
+uint64_t orc_join_materialize(const orc_plan *p, orc_trie *const *tries, uint32_t *cols, uint64_t cap) {
+  if (!plan_ok(p)) return 0;
+  join_ctx jc;
+  memset(&jc, 0, sizeof(jc));
+  jc.p = p; jc.tries = tries; jc.maxbytes = max_bytes(p, tries, &jc.maxcard);
+  jc.level0 = level0_values(p, tries, jc.maxbytes, &jc.n0);
+  mat_state ms;
+  memset(&ms, 0, sizeof(ms));
+  ms.th.jc = &jc;
+  ms.cols = cols;
+  ms.cap = cap;
+  for (int k = 0; k < p->num_attrs; k++) {
+    ms.th.buf[k][0] = (uint8_t *)malloc(jc.maxbytes + 64);
+    ms.th.buf[k][1] = (uint8_t *)malloc(jc.maxbytes + 64);
+    ms.th.vals[k] = (uint32_t *)malloc(4 * jc.maxcard + 64);
+  }
+  for (uint64_t i = 0; i < jc.n0; i++) {
+    ms.th.tuple[0] = jc.level0[i];
+    if (p->num_attrs == 1) {
+      if (ms.rows < cap) cols[ms.rows] = jc.level0[i];
+      ms.rows++;
+    } else {
+      mat_rec(&ms, 1);
+    }
+  }
+  for (int k = 0; k < p->num_attrs; k++) { free(ms.th.buf[k][0]); free(ms.th.buf[k][1]); free(ms.th.vals[k]); }
+  free(jc.level0);
+  return ms.rows;
+}
+
+/* ------------------------------------------------------------------------- */
 /* 4-cycles of an undirected graph (no counterpart in the reference engine,  */
 /* whose queries have no 4-cycle; the count is the graph-theoretic one).      */
 /* Symmetric CSR: off[n+1], adj sorted per vertex.  Each cycle is charged to  */

@@ -104,6 +104,11 @@ uint64_t orc_join_level0_size(const orc_plan *p, orc_trie *const *tries);
 int orc_join_count_per_head(const orc_plan *p, orc_trie *const *tries, int nthreads,
                             int64_t *out);
 
+/* Every result tuple in lexicographic order into cols (num_attrs columns of
+ * cap entries, column j at cols + j*cap); returns the number of tuples (only
+ * the first cap are written). */
+uint64_t orc_join_materialize(const orc_plan *p, orc_trie *const *tries, uint32_t *cols, uint64_t cap);
+
 /* 4-cycles (each once) of an undirected graph in symmetric CSR form (off[n+1],
  * adjacency sorted per vertex); per_vertex[a] (may be NULL) receives the cycles
  * whose largest vertex is a.  No reference counterpart. */

@@ -88,6 +88,8 @@ def lib():
         L.orc_join_count_per_head.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int,
                                               C.POINTER(C.c_int64)]
         L.orc_join_count_per_head.restype = C.c_int
+        L.orc_join_materialize.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), u32p, u64]
+        L.orc_join_materialize.restype = u64
         L.orc_cycle4_count.argtypes = [C.POINTER(C.c_uint64), u32p, C.c_uint32, C.c_int, C.POINTER(C.c_int64)]
         L.orc_cycle4_count.restype = C.c_int64
         _lib = L
@@ -247,3 +249,12 @@ def cycle4_count(src, dst, num_vertices, nthreads=1, per_vertex=False):
                                  num_vertices, nthreads,
                                  pv.ctypes.data_as(C.POINTER(C.c_int64)) if per_vertex else None)
     return (tot, pv[:num_vertices]) if per_vertex else tot
+
+
+def join_materialize(levels, tries):
+    """Every result tuple in lexicographic order: (rows, num_attrs) uint32."""
+    p = make_plan(levels)
+    n = lib().orc_join_materialize(C.byref(p), _tries_arg(tries), None, 0)
+    out = np.zeros((len(levels), max(n, 1)), dtype=np.uint32)
+    lib().orc_join_materialize(C.byref(p), _tries_arg(tries), out.ctypes.data_as(C.POINTER(C.c_uint32)), max(n, 1))
+    return out[:, :n].T.copy()

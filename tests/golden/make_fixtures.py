@@ -6,7 +6,8 @@ row src > dst) used by test/graph/travis.py:197-358.  The expected results are
 the constants those tests assert:
   TriangleAgg  = 1,612,010   (travis.py:197-213)
   FliqueAgg    = 30,004,668  (travis.py:238-254)
-  Triangle     = 1,612,010 rows (travis.py:302-318)
+  Triangle     = 1,612,010 rows, first row (6,5,2) (travis.py:302-318)
+  Flique       = 30,004,668 rows, first row (9,8,7,0) (travis.py:342-354)
 and, for the graph with both edge directions (facebook_duplicated.tsv),
 9,672,060 triangle tuples (docs/notebooks/Getting Started.ipynb, cell 22).
 
@@ -53,7 +54,10 @@ def main(ref_root):
             "TriangleAgg": 1612010,
             "FliqueAgg": 30004668,
             "Triangle_rows": 1612010,
-            "cite": "test/graph/travis.py:197-213,238-254,302-318",
+            "Triangle_row0": [6, 5, 2],
+            "Flique_rows": 30004668,
+            "Flique_row0": [9, 8, 7, 0],
+            "cite": "test/graph/travis.py:197-213,238-254,302-318,342-354",
         },
         "facebook_duplicated": {
             "source": "test/graph/data/facebook_duplicated.tsv",

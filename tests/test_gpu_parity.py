@@ -189,15 +189,18 @@ def test_graph_replay_matches_eager(ctx):
         E.tune_set(**saved)
 
 
-@pytest.mark.parametrize("lds,big,bitmap_all", [(2048, 16, 0), (8192, 256, 1), (4096, 64, 1)])
-def test_tune_params_never_change_results(ctx, oracle, lds, big, bitmap_all):
+@pytest.mark.parametrize("lds,big,bitmap_all,lanes", [(2048, 16, 0, 4), (8192, 256, 1, 64), (4096, 64, 1, 16),
+                                                      (4096, 4, 0, 8), (2048, 5, 1, 32)])
+def test_tune_params_never_change_results(ctx, oracle, lds, big, bitmap_all, lanes):
     s, d, n = G.rmat_graph(13)
     exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
     T = E.Trie(ctx, s, d)
+    T10 = E.Trie(ctx, d, s)
     saved = E.tune_get()
     try:
-        E.tune_set(lds_slots=lds, big_deg=big, bitmap_all=bitmap_all, grab=3, waves_per_cu=4)
+        E.tune_set(lds_slots=lds, big_deg=big, bitmap_all=bitmap_all, grab=3, waves_per_cu=4, stream_lanes=lanes)
         assert E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_TUNED) == exp
+        assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, T10], E.FORM_TUNED) == exp
         assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_TUNED) == E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_PLAIN)
     finally:
         E.tune_set(**saved)
@@ -225,3 +228,54 @@ def test_rmat18_four_clique_config(ctx, oracle, synthetic_meta):
         for form in FORMS:
             say("rmat18 4-clique policy", policy, "form", form)
             assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], form) == exp
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_materialize_facebook_golden(ctx, oracle, facebook, fixtures_meta, form):
+    """Materialized Triangle and Flique on facebook: the reference's row counts
+    and first rows, and every row equal to the oracle's enumeration."""
+    meta = fixtures_meta["facebook_pruned"]
+    O = oracle.Trie(*facebook)
+    for policy in (E.POLICY_RULE, E.POLICY_ALL_BITSET):
+        T = E.Trie(ctx, *facebook, policy=policy)
+        for levels, rows_key, row0_key in ((plans.TRIANGLE, "Triangle_rows", "Triangle_row0"),
+                                           (plans.FOUR_CLIQUE, "Flique_rows", "Flique_row0")):
+            R = E.join_materialize(ctx, levels, [T], form)
+            assert len(R) == meta[rows_key] and list(R[0]) == meta[row0_key]
+            assert np.array_equal(R, oracle.join_materialize(levels, [O]))
+
+
+def test_materialize_matches_oracle_and_counts(ctx, oracle):
+    s, d, n = G.rmat_graph(12)
+    O, T = oracle.Trie(s, d), E.Trie(ctx, s, d)
+    for levels in (plans.TRIANGLE, plans.FOUR_CLIQUE):
+        R = E.join_materialize(ctx, levels, [T], E.FORM_TUNED)
+        assert len(R) == E.join_count(ctx, levels, [T])
+        assert np.array_equal(R, oracle.join_materialize(levels, [O]))
+    rng = np.random.default_rng(9)
+    r = rng.integers(0, 500, size=(3000, 2)).astype(np.uint32)
+    t = rng.integers(0, 500, size=(2000, 2)).astype(np.uint32)
+    OR, OT = oracle.Trie(r[:, 0], r[:, 1]), oracle.Trie(t[:, 0], t[:, 1])
+    TR_, TT = E.Trie(ctx, r[:, 0], r[:, 1]), E.Trie(ctx, t[:, 0], t[:, 1])
+    assert np.array_equal(E.join_materialize(ctx, plans.TWO_HOP, [TR_, TT]),
+                          oracle.join_materialize(plans.TWO_HOP, [OR, OT]))
+    # an empty result: disjoint relations
+    a = E.Trie(ctx, np.array([1, 2], np.uint32), np.array([3, 4], np.uint32))
+    b = E.Trie(ctx, np.array([7, 8], np.uint32), np.array([9, 9], np.uint32))
+    assert E.join_materialize(ctx, plans.TWO_HOP, [a, b]).shape == (0, 3)
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_triangle_by_middle_vertex(ctx, oracle, scale):
+    """The triangle ordered by its middle vertex over Edge_0_1 and Edge_1_0:
+    the tuned kernel's driver-set shape.  Same count as the (a, b, c) order."""
+    s, d, n = G.rmat_graph(scale)
+    exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
+    assert oracle.join_count(plans.TRIANGLE_BY_MIDDLE, [oracle.Trie(s, d), oracle.Trie(d, s)], 8) == exp
+    for policy in (E.POLICY_RULE, E.POLICY_ALL_UINT, E.POLICY_ALL_BITSET):
+        T01, T10 = E.Trie(ctx, s, d, policy), E.Trie(ctx, d, s, policy)
+        for form in FORMS:
+            c, st = E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T01, T10], form, stats=True)
+            assert c == exp
+            if form != E.FORM_PLAIN:
+                assert st["form_used"] == form  # the driver-set shape runs in the tuned kernel

@@ -251,3 +251,32 @@ def test_range_restriction_sums(oracle, facebook):
     assert sum(parts) == 1612010
     per = oracle.join_count_per_head(plans.TRIANGLE, [T], 2)
     assert int(per.sum()) == 1612010 and len(per) == n0
+
+
+def strictly_lexicographic(R):
+    """Rows strictly increasing in lexicographic order (sorted and unique)."""
+    d = R[1:].astype(np.int64) - R[:-1].astype(np.int64)
+    nz = d != 0
+    first = np.argmax(nz, axis=1)
+    return bool(np.all(nz.any(axis=1)) and np.all(d[np.arange(len(d)), first] > 0))
+
+
+def test_materialized_golden_rows(oracle, facebook, fixtures_meta):
+    """Materialized Triangle and Flique: the reference's row counts and first
+    rows (test/graph/travis.py:302-318,342-354), rows sorted and unique, and
+    every row a clique of the relation."""
+    meta = fixtures_meta["facebook_pruned"]
+    T = oracle.Trie(*facebook)
+    edges = set(zip(facebook[0].tolist(), facebook[1].tolist()))
+    for levels, rows_key, row0_key in ((plans.TRIANGLE, "Triangle_rows", "Triangle_row0"),
+                                       (plans.FOUR_CLIQUE, "Flique_rows", "Flique_row0")):
+        R = oracle.join_materialize(levels, [T])
+        assert len(R) == meta[rows_key]
+        assert list(R[0]) == meta[row0_key]
+        assert strictly_lexicographic(R[:2_000_000])
+        rng = np.random.default_rng(3)
+        for i in rng.choice(len(R), 500, replace=False):
+            t = R[i]
+            for x in range(len(t)):
+                for y in range(x + 1, len(t)):
+                    assert (t[x], t[y]) in edges
