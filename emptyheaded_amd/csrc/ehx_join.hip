@@ -319,8 +319,8 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
     if (tuned) {
       TunedArgs ta;
       memset(&ta, 0, sizeof(ta));
-      ta.T = T;
       for (int j = 0; j < EHX_MAX_ATTRS; j++) ta.prefix[j] = in.col[j];
+      tuned_bind(tp, T, ta);
       ta.nprefix = n;
       ta.force_mode = -1;
       if (seg) ta.seg = *seg;

@@ -146,7 +146,7 @@ __device__ uint32_t wave_nway(const DSet *S, int n, uint32_t *out) {
 // (same result, less traffic for sparse windows); without it the algorithms are
 // exactly the reference's per-pair choice.
 template <bool EMIT>
-__device__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg, bool pick = false) {
+__device__ __forceinline__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg, bool pick = false) {
   for (int i = 0; i < n; i++)
     if (S[i].n == 0) { *alg = ALG_EMPTY; return 0; }
   if (n == 1) {

@@ -30,14 +30,26 @@ struct TunedPlan {
   int32_t nd, d_rel, d_parent;
 };
 
+constexpr int TUNED_MAX_S = 4;  // operand sets of S(p) the tuned kernel accepts
+constexpr int TUNED_MAX_H = 4;  // head-set filters of level K-2
+
+// Kernel arguments.  The plan's operands are bound to fixed slots on the host
+// (tuned_bind), so the kernels index them only with unrolled loop counters: a
+// table of tries indexed by a runtime relation number would be copied to
+// scratch memory.
 struct TunedArgs {
-  Tries T;
   const uint32_t *prefix[EHX_MAX_ATTRS];
   uint64_t nprefix;
   unsigned long long *rec;  // per-prefix shader-clock cycles when recording, else null
   int32_t force_mode;       // -1: staging by layout; 0: bitmap where it fits; 1: hash where it fits
-  int32_t pad;
+  int32_t nh;               // head filters in use
   SegOut seg;               // per-segment counts by first attribute value (seg.out null: off)
+  DTrie ts[TUNED_MAX_S];              // trie of each S operand
+  const uint32_t *sp[TUNED_MAX_S];    // prefix column holding each S operand's parent value
+  DTrie tx;                           // relation of child(x)
+  DTrie td;                           // driver set's relation (nd = 1)
+  const uint32_t *dp;                 // prefix column of the driver's parent
+  DTrie th[TUNED_MAX_H];              // head-set filters
 };
 
 // Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.
@@ -52,8 +64,6 @@ struct TuneParams {
   int32_t stream_lanes;   // lanes that stream one long child set together (a power of two, 4..64)
 };
 
-constexpr int TUNED_MAX_S = 4;  // operand sets of S(p) the tuned kernel accepts
-
 // A request to time every prefix of the tuned level under a forced staging.
 struct RecordReq {
   uint64_t *host_out;  // cycles per prefix (host)
@@ -63,6 +73,8 @@ struct RecordReq {
 };
 
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp);
+// Fills A's operand slots from the plan, the tries and A.prefix.
+void tuned_bind(const TunedPlan &P, const Tries &T, TunedArgs &A);
 int tuned_run(ehx_ctx *ctx, const TunedPlan &tp, const TunedArgs &a, int form, unsigned long long *acc);
 TuneParams &tune_params();
 

@@ -61,6 +61,7 @@ bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
       other++;
     }
   }
+  if (tp->nhf > TUNED_MAX_H) return false;
   if (other == 0) return matched == tp->ns;
   // one driver set, and no S set among level K-2's operands
   tp->nd = 1;
@@ -71,17 +72,38 @@ static constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 static constexpr uint32_t NBUCKETS = 64, BUCKET_NONE = 63, WAVE_BIN = 32;
 static constexpr int OWN = 256;  // owner window per wave
 
+void tuned_bind(const TunedPlan &P, const Tries &T, TunedArgs &A) {
+  for (int i = 0; i < P.ns && i < TUNED_MAX_S; i++) {
+    A.ts[i] = T.t[P.s_rel[i]];
+    A.sp[i] = A.prefix[P.s_parent[i]];
+  }
+  A.tx = T.t[P.x_rel];
+  if (P.nd) {
+    A.td = T.t[P.d_rel];
+    A.dp = A.prefix[P.d_parent];
+  }
+  A.nh = P.nhf < TUNED_MAX_H ? P.nhf : TUNED_MAX_H;
+  for (int i = 0; i < A.nh; i++) A.th[i] = T.t[P.h_rel[i]];
+}
+
+// S[i] for a runtime i, as unrolled selects (a runtime index would put S in scratch).
+template <int MAXS>
+__device__ __forceinline__ DSet sel(const DSet *S, int i) {
+  DSet r = S[0];
+#pragma unroll
+  for (int k = 1; k < MAXS; k++)
+    if (i == k) r = S[k];
+  return r;
+}
+
 template <int MAXS>
 __device__ __forceinline__ bool resolve_s(const TunedPlan &P, const TunedArgs &A, uint64_t p, DSet *S) {
-  uint32_t pv[EHX_MAX_ATTRS];
-  for (int j = 0; j <= P.K - 3; j++) pv[j] = A.prefix[j][p];
 #pragma unroll
   for (int i = 0; i < MAXS; i++) {
     if (i >= P.ns) break;
-    const DTrie &t = A.T.t[P.s_rel[i]];
-    int32_t h = node_of(t, pv[P.s_parent[i]]);
+    const int32_t h = node_of(A.ts[i], A.sp[i][p]);
     if (h < 0) return false;
-    S[i] = set_of(t, (uint32_t)h);
+    S[i] = set_of(A.ts[i], (uint32_t)h);
     if (S[i].n == 0) return false;
   }
   return true;
@@ -90,16 +112,16 @@ __device__ __forceinline__ bool resolve_s(const TunedPlan &P, const TunedArgs &A
 // The driver set of a prefix (plans with nd = 1); true with nothing to do otherwise.
 __device__ __forceinline__ bool resolve_d(const TunedPlan &P, const TunedArgs &A, uint64_t p, DSet &D) {
   if (!P.nd) return true;
-  const DTrie &t = A.T.t[P.d_rel];
-  const int32_t h = node_of(t, A.prefix[P.d_parent][p]);
+  const int32_t h = node_of(A.td, A.dp[p]);
   if (h < 0) return false;
-  D = set_of(t, (uint32_t)h);
+  D = set_of(A.td, (uint32_t)h);
   return D.n > 0;
 }
 
 __device__ __forceinline__ bool head_ok(const TunedPlan &P, const TunedArgs &A, uint32_t x) {
-  for (int i = 0; i < P.nhf; i++)
-    if (node_of(A.T.t[P.h_rel[i]], x) < 0) return false;
+#pragma unroll
+  for (int i = 0; i < TUNED_MAX_H; i++)
+    if (i < A.nh && node_of(A.th[i], x) < 0) return false;
   return true;
 }
 
@@ -111,7 +133,7 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
   __shared__ unsigned h[NBUCKETS];
   if (threadIdx.x < NBUCKETS) h[threadIdx.x] = 0;
   __syncthreads();
-  const DTrie &tx = A.T.t[P.x_rel];
+  const DTrie &tx = A.tx;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.nprefix;
        p += (uint64_t)gridDim.x * blockDim.x) {
     DSet S[TUNED_MAX_S];
@@ -188,13 +210,14 @@ struct Stage {
 __device__ __forceinline__ uint32_t hash_t(uint32_t c, uint32_t shift) { return (c * 0x9E3779B1u) >> shift; }
 __device__ __forceinline__ uint32_t hash_f(uint32_t c, uint32_t shift) { return (c * 0x85EBCA77u) >> shift; }
 
-template <int MAXS>
-__device__ __forceinline__ uint32_t probe(const Stage &s, const DSet *S, int ns, uint32_t c) {
-  if (s.mode == 0) {
+template <int MODE, int MAXS>
+__device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int ns, uint32_t c) {
+  if (MODE == 0) {  // branch-free: clamp the word index, mask the bit
     const uint32_t w = (c >> 5) - s.bm_base;
-    return (w < s.bm_words) ? ((s.tab[w] >> (c & 31)) & 1u) : 0u;
+    const uint32_t bit = (s.tab[min(w, s.bm_words - 1u)] >> (c & 31)) & 1u;
+    return w < s.bm_words ? bit : 0u;
   }
-  if (s.mode == 1) {
+  if (MODE == 1) {
     const uint32_t f = hash_f(c, s.fshift);
     if (!((s.flt[f >> 5] >> (f & 31)) & 1u)) return 0u;
     uint32_t h = hash_t(c, s.hshift);
@@ -211,10 +234,91 @@ __device__ __forceinline__ uint32_t probe(const Stage &s, const DSet *S, int ns,
   return 1u;
 }
 
+// Streams child(x) for every x of E(p) through the staged S(p); MODE is the
+// staging (0 bitmap, 1 hash + filter, 2 sets probed in HBM).
+template <int MODE, bool BLOCK, int MAXS>
+__device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const TunedArgs &A, const Stage &st,
+                                                    const DSet *S, const uint32_t *xsrc, uint32_t xn, bool filter_s,
+                                                    int mi, uint32_t *own, int big_deg, uint32_t slg) {
+  const DTrie &tx = A.tx;
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = BLOCK ? (threadIdx.x >> 6) : 0u;
+  const uint32_t nwaves = BLOCK ? (blockDim.x >> 6) : 1u;
+  uint64_t hits = 0;
+  // A wave takes XW elements of S(p) per step: 64 on its own; 16 inside a
+  // workgroup, so the four waves share even a small S(p) evenly.
+  constexpr uint32_t XW = BLOCK ? 16u : 64u;
+  for (uint32_t base = wave * XW; base < xn; base += nwaves * XW) {
+    const uint32_t idx = base + lane;
+    bool ok = lane < XW && idx < xn;
+    const uint32_t x = ok ? xsrc[idx] : 0u;
+    if (ok && filter_s) {
+#pragma unroll
+      for (int i = 0; i < MAXS; i++)
+        if (i < P.ns && i != mi && ok) ok = set_contains(S[i], x);
+    }
+    if (ok) ok = head_ok(P, A, x);
+    const int32_t nx = ok ? node_of(tx, x) : -1;
+    uint32_t o = 0, d = 0;
+    if (nx >= 0) {
+      o = tx.off[nx];
+      d = tx.off[nx + 1] - o;
+    }
+    // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
+    // streamed with 128-bit loads, two quads per lane in flight per step (the
+    // x's of one prefix are adjacent ids, so the lengths in a round are alike)
+    uint64_t big = __ballot(d >= (uint32_t)big_deg);
+    const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
+    while (big) {
+      uint64_t mine = big;  // this group's list: the qt-th lowest set bit
+      for (uint32_t k = 0; k < qt; k++) mine &= mine - 1;
+      const int myl = mine ? __ffsll((unsigned long long)mine) - 1 : -1;
+      for (uint32_t k = 0; k < lists; k++) big &= big - 1;
+      const uint32_t so = __shfl(o, myl < 0 ? 0 : myl, 64), sd = __shfl(d, myl < 0 ? 0 : myl, 64);
+      const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
+      const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
+      if (ql < head) hits += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + ql]);
+      const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
+      const uint32_t n4 = (bd - head) >> 2;
+      uint32_t i = ql;
+      for (; i + G < n4; i += 2 * G) {
+        const uint4 v0 = q[i], v1 = q[i + G];
+        hits += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
+                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w) +
+                probe_m<MODE, MAXS>(st, S, P.ns, v1.x) + probe_m<MODE, MAXS>(st, S, P.ns, v1.y) +
+                probe_m<MODE, MAXS>(st, S, P.ns, v1.z) + probe_m<MODE, MAXS>(st, S, P.ns, v1.w);
+      }
+      if (i < n4) {
+        const uint4 v0 = q[i];
+        hits += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
+                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w);
+      }
+      const uint32_t done = head + 4 * n4;
+      if (ql < bd - done) hits += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + done + ql]);
+    }
+    // short ones: expand owners into the LDS window, then one element per lane per step
+    const uint32_t ds = (d < (uint32_t)big_deg) ? d : 0u;
+    const uint32_t incl = wave_incl_scan(ds);
+    const uint32_t T = __shfl(incl, 63, 64);
+    const uint32_t s0 = incl - ds;
+    const uint32_t rebase = o - s0;  // element e of the window lives at vals[rebase + e] (mod 2^32)
+    for (uint32_t r0 = 0; r0 < T; r0 += OWN) {
+      const uint32_t lo = s0 > r0 ? s0 : r0;
+      const uint32_t hi = (s0 + ds < r0 + OWN) ? s0 + ds : r0 + OWN;
+      for (uint32_t i = lo; i < hi; i++) own[i - r0] = rebase;
+      wave_sync();
+      const uint32_t lim = (T - r0 < (uint32_t)OWN) ? T - r0 : (uint32_t)OWN;
+      for (uint32_t e = lane; e < lim; e += 64) hits += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[own[e] + r0 + e]);
+      wave_sync();
+    }
+  }
+  return hits;
+}
+
 // One prefix, processed by a group: the whole 4-wave workgroup (BLOCK) or one
 // wave.  tab/flt/sv are the group's LDS; own is the calling wave's window.
 template <bool BLOCK, int MAXS>
-__device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, uint32_t *tab, uint32_t tab_slots,
+__device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, uint32_t *tab, uint32_t tab_slots,
                                  uint32_t *flt, uint32_t flt_words, uint32_t *sv, uint32_t sv_cap, uint32_t *own,
                                  int big_deg, int bitmap_all, uint32_t slg, int &mode_used) {
   const uint32_t lane = __lane_id();
@@ -226,13 +330,15 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
     if (BLOCK) __syncthreads();
     else wave_sync();
   };
+  // the launch picks MAXS = 1 or 2 only for plans with exactly that many S sets
+  const int NS = (MAXS <= 2) ? MAXS : P.ns;
   DSet S[MAXS];
   DSet Dd;
   if (!resolve_s<MAXS>(P, A, p, S) || !resolve_d(P, A, p, Dd)) return 0;
   int mi = 0;
 #pragma unroll
   for (int i = 1; i < MAXS; i++)
-    if (i < P.ns && S[i].n <= S[mi].n) mi = i;
+    if (i < P.ns && S[i].n <= sel<MAXS>(S, mi).n) mi = i;
   const uint32_t *src = nullptr;
   uint32_t sn = 0;
   if (P.ns == 1) {
@@ -249,14 +355,14 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
       if (BLOCK) {
         __shared__ uint32_t s_sn;
         if (wave == 0) {
-          uint32_t c = wave_intersect<true>(S, P.ns, sv, &alg);
+          uint32_t c = wave_intersect<true>(S, NS, sv, &alg);
           if (lane == 0) s_sn = c;
         }
         __syncthreads();
         sn = s_sn;
         __syncthreads();
       } else {
-        sn = wave_intersect<true>(S, P.ns, sv, &alg);
+        sn = wave_intersect<true>(S, NS, sv, &alg);
         wave_sync();
       }
       if (sn == 0) return 0;
@@ -326,84 +432,24 @@ __device__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_
   gsync();
   mode_used = st.mode;
 
-  const DTrie &tx = A.T.t[P.x_rel];
-  const uint32_t *xsrc = P.nd ? Dd.v : (src ? src : S[mi].v);
-  const uint32_t xn = P.nd ? Dd.n : (src ? sn : S[mi].n);
+  const DSet Smi = sel<MAXS>(S, mi);
+  const uint32_t *xsrc = P.nd ? Dd.v : (src ? src : Smi.v);
+  const uint32_t xn = P.nd ? Dd.n : (src ? sn : Smi.n);
   const bool filter_s = !P.nd && !src;  // x runs over one S set: test it against the others
-  uint64_t hits = 0;
-  // A wave takes XW elements of S(p) per step: 64 on its own; 16 inside a
-  // workgroup, so the four waves share even a small S(p) evenly.
-  constexpr uint32_t XW = BLOCK ? 16u : 64u;
-  for (uint32_t base = wave * XW; base < xn; base += nwaves * XW) {
-    const uint32_t idx = base + lane;
-    bool ok = lane < XW && idx < xn;
-    const uint32_t x = ok ? xsrc[idx] : 0u;
-    if (ok && filter_s) {
-#pragma unroll
-      for (int i = 0; i < MAXS; i++)
-        if (i < P.ns && i != mi && ok) ok = set_contains(S[i], x);
-    }
-    if (ok) ok = head_ok(P, A, x);
-    const int32_t nx = ok ? node_of(tx, x) : -1;
-    uint32_t o = 0, d = 0;
-    if (nx >= 0) {
-      o = tx.off[nx];
-      d = tx.off[nx + 1] - o;
-    }
-    // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
-    // streamed with 128-bit loads, two quads per lane in flight per step (the
-    // x's of one prefix are adjacent ids, so the lengths in a round are alike)
-    uint64_t big = __ballot(d >= (uint32_t)big_deg);
-    const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
-    while (big) {
-      uint64_t mine = big;  // this group's list: the qt-th lowest set bit
-      for (uint32_t k = 0; k < qt; k++) mine &= mine - 1;
-      const int myl = mine ? __ffsll((unsigned long long)mine) - 1 : -1;
-      for (uint32_t k = 0; k < lists; k++) big &= big - 1;
-      const uint32_t so = __shfl(o, myl < 0 ? 0 : myl, 64), sd = __shfl(d, myl < 0 ? 0 : myl, 64);
-      const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
-      const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
-      if (ql < head) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + ql]);
-      const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
-      const uint32_t n4 = (bd - head) >> 2;
-      uint32_t i = ql;
-      for (; i + G < n4; i += 2 * G) {
-        const uint4 v0 = q[i], v1 = q[i + G];
-        hits += probe<MAXS>(st, S, P.ns, v0.x) + probe<MAXS>(st, S, P.ns, v0.y) +
-                probe<MAXS>(st, S, P.ns, v0.z) + probe<MAXS>(st, S, P.ns, v0.w) +
-                probe<MAXS>(st, S, P.ns, v1.x) + probe<MAXS>(st, S, P.ns, v1.y) +
-                probe<MAXS>(st, S, P.ns, v1.z) + probe<MAXS>(st, S, P.ns, v1.w);
-      }
-      if (i < n4) {
-        const uint4 v0 = q[i];
-        hits += probe<MAXS>(st, S, P.ns, v0.x) + probe<MAXS>(st, S, P.ns, v0.y) +
-                probe<MAXS>(st, S, P.ns, v0.z) + probe<MAXS>(st, S, P.ns, v0.w);
-      }
-      const uint32_t done = head + 4 * n4;
-      if (ql < bd - done) hits += probe<MAXS>(st, S, P.ns, tx.vals[bo + done + ql]);
-    }
-    // short ones: expand owners into the LDS window, then one element per lane per step
-    const uint32_t ds = (d < (uint32_t)big_deg) ? d : 0u;
-    const uint32_t incl = wave_incl_scan(ds);
-    const uint32_t T = __shfl(incl, 63, 64);
-    const uint32_t s0 = incl - ds;
-    const uint32_t rebase = o - s0;  // element e of the window lives at vals[rebase + e] (mod 2^32)
-    for (uint32_t r0 = 0; r0 < T; r0 += OWN) {
-      const uint32_t lo = s0 > r0 ? s0 : r0;
-      const uint32_t hi = (s0 + ds < r0 + OWN) ? s0 + ds : r0 + OWN;
-      for (uint32_t i = lo; i < hi; i++) own[i - r0] = rebase;
-      wave_sync();
-      const uint32_t lim = (T - r0 < (uint32_t)OWN) ? T - r0 : (uint32_t)OWN;
-      for (uint32_t e = lane; e < lim; e += 64) hits += probe<MAXS>(st, S, P.ns, tx.vals[own[e] + r0 + e]);
-      wave_sync();
-    }
-  }
+  uint64_t hits;
+  // one streaming loop per staging mode: no per-probe branch on the mode
+  if (st.mode == 0)
+    hits = stream_children<0, BLOCK, MAXS>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+  else if (st.mode == 1)
+    hits = stream_children<1, BLOCK, MAXS>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+  else
+    hits = stream_children<2, BLOCK, MAXS>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
   gsync();
   return hits;
 }
 
 template <int SLOTS, int MAXS>
-__global__ __launch_bounds__(256) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
+__global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
                                                unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
                                                int big_deg, int grab, int bitmap_all, uint32_t slg) {
   constexpr uint32_t WS = SLOTS / 4;                 // per-wave table slots
@@ -467,6 +513,8 @@ static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, cons
   const uint32_t slg = 31u - (uint32_t)__builtin_clz((unsigned)tp.stream_lanes);  // a power of two in [4, 64]
   if (P.ns == 1)
     k_tuned<SLOTS, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+  else if (P.ns == 2)
+    k_tuned<SLOTS, 2><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
   else
     k_tuned<SLOTS, TUNED_MAX_S><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
 }
