@@ -41,9 +41,10 @@ def measure(E, ctx, tries, plan, form, reps):
     return total, best
 
 
-def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID):
+def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID, form=None):
     """workloads: list of (name, tries, plan).  Objective: sum of best-of-reps
     wall ms per query over the workloads."""
+    form = E.FORM_TUNED if form is None else form
     cur = E.tune_get()
     ref = {}
     history = []
@@ -52,7 +53,7 @@ def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID):
         E.tune_set(**params)
         tot = 0.0
         for name, tries, plan in workloads:
-            cnt, ms = measure(E, ctx, tries, plan, E.FORM_TUNED, reps)
+            cnt, ms = measure(E, ctx, tries, plan, form, reps)
             if name in ref and ref[name] != cnt:
                 raise AssertionError("knobs changed the result of %s: %d vs %d" % (name, cnt, ref[name]))
             ref.setdefault(name, cnt)
@@ -90,32 +91,42 @@ def main():
     ap.add_argument("--scales", default="20")
     ap.add_argument("--clique-scale", type=int, default=17)
     ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--layouts", default="learned", choices=("learned", "rule"),
+                    help="tries built with the committed selector (and timed in the learned form) or the density rule")
+    ap.add_argument("--fixed", default="bitmap_all=0",
+                    help="knobs held at a value, e.g. bitmap_all=0 (the learned layouts decide staging)")
     args = ap.parse_args()
+    fixed = dict((kv.split("=")[0], int(kv.split("=")[1])) for kv in args.fixed.split(",") if kv)
     import emptyheaded_amd as E
     from emptyheaded_amd import graphs as G
     from emptyheaded_amd import plans, tuning
-    tuning.apply_committed()  # start from the committed knobs
+    applied = tuning.apply_committed()  # start from the committed knobs and selector
+    E.tune_set(**fixed)
+    learned = args.layouts == "learned" and applied.get("selector")
+    pol = E.POLICY_LEARNED if learned else E.POLICY_RULE
+    form = E.FORM_LEARNED if learned else E.FORM_TUNED
+    grid = {k: v for k, v in GRID.items() if k not in fixed}
     ctx = E.Context(0)
     workloads, keep = [], []
     if args.plan in ("mixed", "triangle", "triangle_by_middle"):
         for s in [int(x) for x in args.scales.split(",")]:
             src, dst, nv = G.rmat_graph(s)
-            T = E.Trie(ctx, src, dst)
+            T = E.Trie(ctx, src, dst, pol)
             keep.append(T)
             if args.plan == "triangle_by_middle":
-                T10 = E.Trie(ctx, dst, src)
+                T10 = E.Trie(ctx, dst, src, pol)
                 keep.append(T10)
                 workloads.append(("triangle_by_middle_rmat%d" % s, [T, T10], plans.TRIANGLE_BY_MIDDLE))
             else:
                 workloads.append(("triangle_rmat%d" % s, [T], plans.TRIANGLE))
     if args.plan in ("mixed", "four_clique") and args.clique_scale:
         src, dst, nv = G.rmat_graph(args.clique_scale)
-        T = E.Trie(ctx, src, dst)
+        T = E.Trie(ctx, src, dst, pol)
         keep.append(T)
         workloads.append(("clique4_rmat%d" % args.clique_scale, [T], plans.FOUR_CLIQUE))
-    best, ms, hist, ref = tune(E, ctx, workloads, passes=args.passes)
+    best, ms, hist, ref = tune(E, ctx, workloads, passes=args.passes, grid=grid, form=form)
     out = {"plan": args.plan, "best": best, "objective_ms": ms, "workloads": [w[0] for w in workloads],
-           "counts": ref, "history": hist}
+           "counts": ref, "fixed": fixed, "layouts": "learned" if learned else "rule", "history": hist}
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)
     log("best", json.dumps(best), "%.3f ms" % ms)

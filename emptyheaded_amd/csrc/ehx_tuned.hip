@@ -155,9 +155,9 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
       }
       w = w * X.n / (ns ? ns : 1) + X.n + (P.nd ? D.n : 0u);
       if (P.ns > 1) w = w / 2 + 1;
-      // the wave bin needs S(p) to fit a quarter of the LDS (hash) unless it is a fitting bitset
+      // the wave bin needs S(p) to fit a quarter of the LDS, as a hash table or as a bitmap
       bool fits_wave = D.n <= wave_hash_cap(slots);
-      if (!fits_wave && P.ns == 1 && D.lay == EHX_RANGE_BITSET && D.n)
+      if (!fits_wave && P.ns == 1 && D.n)  // a wave can stage it as a bitmap
         fits_wave = ((D.v[D.n - 1] >> 5) - (D.v[0] >> 5) + 1) <= (uint32_t)slots / 4;
       const bool big = w >= (uint64_t)big_work || !fits_wave;
       uint32_t lg = 63 - __clzll(w | 1);
@@ -386,6 +386,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
     else if (A.force_mode == 1) st.mode = hash_fits ? 1 : (bm_fits ? 0 : 2);
     else if (bm_fits && (is_bitset || bitmap_all)) st.mode = 0;
     else if (hash_fits) st.mode = 1;
+    else if (bm_fits) st.mode = 0;  // a bitmap still beats probing the sets in HBM
     if (st.mode == 0) {
       st.bm_base = first >> 5;
       st.bm_words = w32;

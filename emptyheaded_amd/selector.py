@@ -55,30 +55,46 @@ def features(ctx, T):
     return out
 
 
-def record(ctx, T, force_mode, plan=plans.TRIANGLE):
+def record(ctx, T, force_mode, plan=plans.TRIANGLE, tries=None, nprefix=None):
     """Cycles per prefix and the staging actually used (-1: no work)."""
     L = _sigs()
     p = make_plan(plan)
-    nh = T.info()["num_heads"]
-    cyc = np.zeros(nh, np.uint64)
+    tries = tries or [T]
+    nh = T.info()["num_heads"] if nprefix is None else nprefix
+    cyc = np.zeros(max(nh, 1), np.uint64)
     n = C.c_uint64(0)
     res = C.c_int64(0)
-    arr = (C.c_void_p * 1)(T.h.value)
+    arr = (C.c_void_p * len(tries))(*[t.h.value for t in tries])
     check(L.ehx_join_record(ctx.h, C.byref(p), arr, force_mode, cyc.ctypes.data_as(C.POINTER(C.c_uint64)), nh,
                             C.byref(n), C.byref(res)))
     assert n.value == nh, (n.value, nh)
+    cyc = cyc[:nh]
     mode = np.where(cyc == 0, -1, (cyc >> np.uint64(MODE_SHIFT)).astype(np.int64))
     cycles = (cyc & np.uint64((1 << MODE_SHIFT) - 1)).astype(np.float64)
     return cycles, mode, res.value
 
 
-def dataset(ctx, T, reps=2):
-    """X, Y (log2 cycles: bitmap, hash), M (masks), plus raw cycles."""
+def dataset(ctx, T, reps=2, plan="triangle", src=None, dst=None):
+    """X, Y (log2 cycles: bitmap, hash), M (masks), plus raw cycles.
+
+    plan "triangle": the prefix is a head vertex a of T and the staged set is
+    N+(a).  plan "triangle_by_middle": the prefix is a middle vertex b (a head of
+    both T and its reverse, built here from src, dst) and the staged set is
+    N+(b), again a set of T, so every sample is one set of T either way."""
     X = features(ctx, T)
+    if plan == "triangle_by_middle":
+        import emptyheaded_amd as E
+        R = E.Trie(ctx, dst, src)
+        heads = np.unique(src)
+        mids = np.intersect1d(heads, np.unique(dst))
+        X = X[np.searchsorted(heads, mids)]
+        rec = dict(plan=plans.TRIANGLE_BY_MIDDLE, tries=[T, R], nprefix=len(mids))
+    else:
+        rec = {}
     c0s, c1s = [], []
     for _ in range(reps):
-        c0, m0, r0 = record(ctx, T, 0)
-        c1, m1, r1 = record(ctx, T, 1)
+        c0, m0, r0 = record(ctx, T, 0, **rec)
+        c1, m1, r1 = record(ctx, T, 1, **rec)
         assert r0 == r1, "staging changed the count"
         c0s.append(c0)
         c1s.append(c1)
@@ -88,6 +104,8 @@ def dataset(ctx, T, reps=2):
     v1 = (m1 == 1) & (c1 > 0)
     Y = np.stack([np.log2(c0 + 1), np.log2(c1 + 1)], 1).astype(np.float32)
     M = np.stack([v0, v1], 1).astype(np.float32)
+    if plan == "triangle_by_middle":
+        R.close()
     return X, Y, M, c0, c1, r0
 
 
@@ -203,8 +221,9 @@ def regret(c0, c1, M, choose_bitmap):
             "regret": float((chosen.sum() - best.sum()) / max(best.sum(), 1.0))}
 
 
-def rule_choice(ctx, T):
+def rule_choice(ctx, T, rows=None):
     types, cards = T.node_info()
+    types = types if rows is None else types[rows]
     return types == 0  # bitset layout -> staged as a bitmap
 
 
@@ -243,7 +262,11 @@ def main():
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--weighting", choices=("uniform", "cost"), default="cost")
     ap.add_argument("--target", choices=("costs", "diff"), default="diff")
+    ap.add_argument("--plans", default="triangle_by_middle",
+                    help="comma list of plans whose recorded costs train the selector "
+                         "(triangle, triangle_by_middle); regret is reported for each")
     args = ap.parse_args()
+    train_plans = [x for x in args.plans.split(",") if x]
     import emptyheaded_amd as E
     from emptyheaded_amd import graphs as G
     from emptyheaded_amd import tuning
@@ -256,16 +279,18 @@ def main():
     for s in [int(x) for x in args.train_scales.split(",")]:
         src, dst, nv = G.rmat_graph(s)
         T = E.Trie(ctx, src, dst)
-        X, Y, M, c0, c1, cnt = dataset(ctx, T)
-        log("scale %d: %d sets, %d with both stagings" % (s, len(X), int(((M[:, 0] > 0) & (M[:, 1] > 0)).sum())))
-        if args.target == "diff":
-            Y, M = difference_targets(Y, M, c0, c1)
-        elif args.weighting == "cost":
-            M = cost_weights(M, c0, c1)
-        keep = M.sum(1) > 0
-        Xs.append(X[keep])
-        Ys.append(Y[keep])
-        Ms.append(M[keep])
+        for plan in train_plans:
+            X, Y, M, c0, c1, cnt = dataset(ctx, T, plan=plan, src=src, dst=dst)
+            log("scale %d %s: %d sets, %d with both stagings" %
+                (s, plan, len(X), int(((M[:, 0] > 0) & (M[:, 1] > 0)).sum())))
+            if args.target == "diff":
+                Y, M = difference_targets(Y, M, c0, c1)
+            elif args.weighting == "cost":
+                M = cost_weights(M, c0, c1)
+            keep = M.sum(1) > 0
+            Xs.append(X[keep])
+            Ys.append(Y[keep])
+            Ms.append(M[keep])
         T.close()
     X, Y, M = np.concatenate(Xs), np.concatenate(Ys), np.concatenate(Ms)
     log("training on %d samples (%.1fs of recording)" % (len(X), time.perf_counter() - t0))
@@ -273,19 +298,25 @@ def main():
     # evaluation graph
     src, dst, nv = G.rmat_graph(args.eval_scale)
     T = E.Trie(ctx, src, dst)
-    Xe, Ye, Me, c0, c1, cnt = dataset(ctx, T)
-    rule = rule_choice(ctx, T)
-    learned, pred = learned_choice(ctx, params, norm, Xe, max_words)
+    regrets = {"eval_scale": args.eval_scale}
+    for plan in ("triangle_by_middle", "triangle"):
+        Xe, Ye, Me, c0, c1, cnt = dataset(ctx, T, plan=plan, src=src, dst=dst)
+        rows = None
+        if plan == "triangle_by_middle":
+            heads = np.unique(src)
+            rows = np.searchsorted(heads, np.intersect1d(heads, np.unique(dst)))
+        rule = rule_choice(ctx, T, rows)
+        learned, pred = learned_choice(ctx, params, norm, Xe, max_words)
+        regrets[plan] = {"rule": regret(c0, c1, Me, rule), "learned": regret(c0, c1, Me, learned),
+                         "bitmap_when_fits": regret(c0, c1, Me, Me[:, 0] > 0),
+                         "bitmap_sets": {"rule": int(rule.sum()), "learned": int(learned.sum())}}
     out = {
         "params": params.tolist(), "norm": norm.tolist(), "max_words": max_words,
-        "trained_on": {"scales": args.train_scales, "samples": int(len(X)), "steps": args.steps,
-                       "weighting": args.weighting, "target": args.target,
+        "trained_on": {"scales": args.train_scales, "plans": train_plans, "samples": int(len(X)),
+                       "steps": args.steps, "weighting": args.weighting, "target": args.target,
                        "final_loss": hist[-1][1]},
         "loss_history": hist,
-        "regret": {"eval_scale": args.eval_scale, "rule": regret(c0, c1, Me, rule),
-                   "learned": regret(c0, c1, Me, learned),
-                   "bitmap_when_fits": regret(c0, c1, Me, Me[:, 0] > 0),
-                   "bitmap_sets": {"rule": int(rule.sum()), "learned": int(learned.sum())}},
+        "regret": regrets,
     }
     log(json.dumps(out["regret"], indent=1))
     with open(args.out, "w") as f:

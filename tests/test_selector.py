@@ -81,3 +81,24 @@ def test_training_on_recorded_costs_and_learned_layouts_exact(ctx, oracle):
             E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_PLAIN)
     finally:
         E.lib().ehx_selector_clear()
+
+
+@pytest.mark.gpu
+def test_recording_the_middle_vertex_plan(ctx, oracle):
+    """Recorded staging costs for the triangle ordered by its middle vertex: one
+    sample per middle vertex, its staged set a set of Edge_0_1; the recorded
+    runs count every triangle."""
+    import emptyheaded_amd as E
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import plans
+    from emptyheaded_amd import selector as S
+    s, d, n = G.rmat_graph(13)
+    T = E.Trie(ctx, s, d)
+    X, Y, M, c0, c1, cnt = S.dataset(ctx, T, reps=1, plan="triangle_by_middle", src=s, dst=d)
+    assert cnt == oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
+    mids = np.intersect1d(np.unique(s), np.unique(d))
+    assert X.shape == (len(mids), S.IN) and len(c0) == len(mids)
+    assert (M.sum(0) > 0).all()
+    # features of middle vertex b are those of b's set in Edge_0_1
+    allx = S.features(ctx, T)
+    assert np.array_equal(X, allx[np.searchsorted(np.unique(s), mids)])
