@@ -53,9 +53,10 @@ __device__ __forceinline__ uint64_t c4_wedge_bound(const DTrie &S, uint32_t b0, 
 }
 
 // Calls f(c) for every wedge endpoint c < a of the lower neighbors vals[b0..b1)
-// (b1 - b0 <= 64).  Lane l owns b = vals[b0 + l]; the 64 neighbor lists are laid
-// end to end and the positions dealt out over the lanes, so short lists do not
-// idle the wave.
+// (b1 - b0 <= 64).  Lane l owns b = vals[b0 + l] and N(b) cut at a (a binary
+// search: the lists are sorted, so the endpoints below a are a prefix); the 64
+// cut lists are laid end to end and the positions dealt out over the lanes, so
+// short lists do not idle the wave.
 template <class F>
 __device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t b0, uint32_t b1, uint32_t *s_start,
                                             uint32_t *s_off, F f) {
@@ -66,7 +67,7 @@ __device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t
     const int32_t hb = node_of(S, S.vals[bi]);
     if (hb >= 0) {
       p0 = S.off[hb];
-      len = S.off[hb + 1] - p0;
+      len = lower_bound(S.vals, p0, S.off[hb + 1], a) - p0;
     }
   }
   const uint32_t incl = wave_incl_scan(len);
@@ -93,7 +94,7 @@ __device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t
     }
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (c[u] < a) f(c[u]);
+      if (c[u] != C4_EMPTY) f(c[u]);  // every loaded endpoint is below a
   }
   wave_sync();
 }

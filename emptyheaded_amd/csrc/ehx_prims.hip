@@ -67,10 +67,13 @@ __global__ __launch_bounds__(256) void k_scan_apply(const Tin *__restrict__ in, 
   }
 }
 
+template <class T> __global__ void k_fill(T *p, T v, uint64_t n);
+
 template <class Tin, class Tout>
 static int scan_impl(ehx_ctx *ctx, const Tin *in, Tout *out, uint64_t n, int level) {
-  if (n == 0) {
-    EHX_HIP(hipMemsetAsync(out, 0, sizeof(Tout), ctx->stream));
+  if (n == 0) {  // out[0] = 0 by a fill kernel (memset nodes misbehave under graph replay)
+    k_fill<Tout><<<1, 64, 0, ctx->stream>>>(out, (Tout)0, 1);
+    EHX_HIP(hipGetLastError());
     return EHX_OK;
   }
   uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;

@@ -530,8 +530,10 @@ struct GraphCache {
 };
 static GraphCache g_cache;
 
+// events: time the persistent kernel (eager launches); a captured level is timed
+// around its graph launch instead, since event records are not graph nodes here
 static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, unsigned long long *acc,
-                         uint32_t *bucket, unsigned *ctl, uint32_t *queue, const TuneParams &tp) {
+                         uint32_t *bucket, unsigned *ctl, uint32_t *queue, const TuneParams &tp, bool events = true) {
   hipStream_t st = ctx->stream;
   int rc;
   if ((rc = fill_u32(ctx, ctl, 0u, 132))) return rc;
@@ -543,14 +545,14 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
   EHX_HIP(hipGetLastError());
   const unsigned per_cu = (unsigned)(tp.waves_per_cu > 4 ? tp.waves_per_cu / 4 : 1);
   const unsigned grid = (unsigned)ctx->num_cus * per_cu;
-  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
+  if (events) EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
   switch (tp.lds_slots) {
     case 2048: launch_tuned<2048>(st, grid, P, A, queue, ctl, acc, tp); break;
     case 8192: launch_tuned<8192>(st, grid, P, A, queue, ctl, acc, tp); break;
     default: launch_tuned<4096>(st, grid, P, A, queue, ctl, acc, tp); break;
   }
   EHX_HIP(hipGetLastError());
-  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));
+  if (events) EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));
   return EHX_OK;
 }
 
@@ -579,7 +581,7 @@ int tuned_run(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, int form, un
     if (g_cache.graph) { (void)hipGraphDestroy(g_cache.graph); g_cache.graph = nullptr; }
     g_cache.valid = false;
     EHX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_level(ctx, P, A, acc, bucket, ctl, queue, tp);
+    rc = enqueue_level(ctx, P, A, acc, bucket, ctl, queue, tp, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc || e != hipSuccess) {
@@ -596,7 +598,9 @@ int tuned_run(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, int form, un
     g_cache.device = ctx->device;
     g_cache.valid = true;
   }
+  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], ctx->stream));  // the whole captured level
   EHX_HIP(hipGraphLaunch(g_cache.exec, ctx->stream));
+  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], ctx->stream));
   return EHX_OK;
 }
 
