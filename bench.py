@@ -96,17 +96,23 @@ def bench_egonets(E, ctx, steps):
     ss, sd = G.symmetric(es, ed)
     nets = len(base) - 1
     log("ego batch: %d nets, %d edges, %d vertices (%.1fs)" % (nets, len(es), int(base[-1]), time.perf_counter() - t))
-    T = E.Trie(ctx, es, ed, E.POLICY_LEARNED if _selector_loaded(E) else E.POLICY_RULE)
+    from emptyheaded_amd import tuning as TU
+    pol = E.POLICY_LEARNED if _selector_loaded(E) else E.POLICY_RULE
+    T = E.Trie(ctx, es, ed, pol)
     S = E.Trie(ctx, ss, sd)
+    # the reference attribute order: per net it is faster here than the middle-vertex order,
+    # whose level 0 joins two head sets
+    TRI, TT = plans.TRIANGLE, [T]
+    TU.apply_plan("triangle")
     saved = E.tune_get()
     # batched: per-segment counts, the tuned triangle level replayed from a hipGraph
     E.tune_set(use_graph=1)
-    tri, _ = E.join_count_segments(ctx, plans.TRIANGLE, [T], base, E.FORM_LEARNED)
+    tri, _ = E.join_count_segments(ctx, TRI, TT, base, E.FORM_LEARNED)
     cyc, _ = E.cycle4_count(ctx, S, bounds=base)
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        E.join_count_segments(ctx, plans.TRIANGLE, [T], base, E.FORM_LEARNED)
+        E.join_count_segments(ctx, TRI, TT, base, E.FORM_LEARNED)
     t1 = time.perf_counter()
     for _ in range(steps):
         E.cycle4_count(ctx, S, bounds=base)
@@ -121,7 +127,7 @@ def bench_egonets(E, ctx, steps):
     pq_cyc = np.zeros(nets, dtype=np.int64)
     t0 = time.perf_counter()
     for q in range(nets):
-        pq_tri[q] = E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_LEARNED, int(lo[q]), int(hi[q]))
+        pq_tri[q] = E.join_count(ctx, TRI, TT, E.FORM_LEARNED, int(lo[q]), int(hi[q]))
     t1 = time.perf_counter()
     for q in range(nets):
         pq_cyc[q] = E.cycle4_count(ctx, S, int(base[q]), int(base[q + 1]))
@@ -139,8 +145,9 @@ def bench_egonets(E, ctx, steps):
     out["speedup_batched_vs_per_query"] = out["per_query_ms"] / out["batched_ms"]
     log("ego batch: batched %.2f ms (tri %.2f, 4-cycle %.2f); per-net launches %.1f ms" %
         (out["batched_ms"], batched_tri, batched_cyc, out["per_query_ms"]))
-    T.close()
-    S.close()
+    TU.apply_plan("triangle")
+    for x in (T, S):
+        x.close()
     return out
 
 

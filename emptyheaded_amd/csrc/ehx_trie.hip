@@ -152,7 +152,7 @@ static void free_trie_arrays(ehx_trie *t) {
 int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t rows, int policy, int on_device,
                ehx_trie **out) {
   if (!ctx || !out) return fail(EHX_ERR_INVALID, "ehx_trie_build: null argument");
-  if (rows == 0) return fail(EHX_ERR_INVALID, "ehx_trie_build: empty relation (Trie.cpp:397 asserts num_rows != 0)");
+  if (rows == 0) return fail(EHX_ERR_INVALID, "ehx_trie_build: empty relation (Trie.cpp:394 asserts num_rows != 0)");
   if (rows >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "ehx_trie_build: at most 2^32-1 rows");
   if (policy < EHX_POLICY_RULE || policy > EHX_POLICY_LEARNED) return fail(EHX_ERR_INVALID, "bad layout policy %d", policy);
   EHX_HIP(hipSetDevice(ctx->device));

@@ -246,9 +246,13 @@ def test_egonet_batch_full_size(ctx, oracle, synthetic_meta):
     cyc_exp = per_net(pv, np.arange(int(base[-1])), base)
     assert int(tri_exp.sum()) == meta["triangles"] and int(cyc_exp.sum()) == meta["cycles4"]
     T = E.Trie(ctx, es, ed)
+    R = E.Trie(ctx, ed, es)
     S = E.Trie(ctx, ss, sd)
     for form in (E.FORM_PLAIN, E.FORM_LEARNED):
         tri, _ = E.join_count_segments(ctx, plans.TRIANGLE, [T], base, form)
         assert np.array_equal(tri, tri_exp)
+    # the middle vertex of a triangle lies in the same net: same per-net counts
+    tri, _ = E.join_count_segments(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], base, E.FORM_LEARNED)
+    assert np.array_equal(tri, tri_exp)
     cyc, tot = E.cycle4_count(ctx, S, bounds=base)
     assert np.array_equal(cyc, cyc_exp)
