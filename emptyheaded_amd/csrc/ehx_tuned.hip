@@ -158,7 +158,7 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
       // the wave bin needs S(p) to fit a quarter of the LDS, as a hash table or as a bitmap
       bool fits_wave = D.n <= wave_hash_cap(slots);
       if (!fits_wave && P.ns == 1 && D.n)  // a wave can stage it as a bitmap
-        fits_wave = ((D.v[D.n - 1] >> 5) - (D.v[0] >> 5) + 1) <= (uint32_t)slots / 4;
+        fits_wave = ((D.v[D.n - 1] >> 5) - (D.v[0] >> 5) + 1) < (uint32_t)slots / 4;
       const bool big = w >= (uint64_t)big_work || !fits_wave;
       uint32_t lg = 63 - __clzll(w | 1);
       b = (big ? 0u : WAVE_BIN) + (31 - (lg > 31 ? 31 : lg));
@@ -212,10 +212,9 @@ __device__ __forceinline__ uint32_t hash_f(uint32_t c, uint32_t shift) { return 
 
 template <int MODE, int MAXS>
 __device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int ns, uint32_t c) {
-  if (MODE == 0) {  // branch-free: clamp the word index, mask the bit
-    const uint32_t w = (c >> 5) - s.bm_base;
-    const uint32_t bit = (s.tab[min(w, s.bm_words - 1u)] >> (c & 31)) & 1u;
-    return w < s.bm_words ? bit : 0u;
+  if (MODE == 0) {  // branch-free: an index past the window clamps to the zero word tab[bm_words]
+    const uint32_t w = min((c >> 5) - s.bm_base, s.bm_words);
+    return __builtin_amdgcn_ubfe(s.tab[w], c, 1u);  // bit c & 31 of the word
   }
   if (MODE == 1) {
     const uint32_t f = hash_f(c, s.fshift);
@@ -277,24 +276,26 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       const uint32_t so = __shfl(o, myl < 0 ? 0 : myl, 64), sd = __shfl(d, myl < 0 ? 0 : myl, 64);
       const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
       const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
-      if (ql < head) hits += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + ql]);
+      uint32_t h32 = 0;  // this round's hits of the lane (< 2^32)
+      if (ql < head) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + ql]);
       const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
       const uint32_t n4 = (bd - head) >> 2;
       uint32_t i = ql;
       for (; i + G < n4; i += 2 * G) {
         const uint4 v0 = q[i], v1 = q[i + G];
-        hits += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
-                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w) +
-                probe_m<MODE, MAXS>(st, S, P.ns, v1.x) + probe_m<MODE, MAXS>(st, S, P.ns, v1.y) +
-                probe_m<MODE, MAXS>(st, S, P.ns, v1.z) + probe_m<MODE, MAXS>(st, S, P.ns, v1.w);
+        h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v1.x) + probe_m<MODE, MAXS>(st, S, P.ns, v1.y) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v1.z) + probe_m<MODE, MAXS>(st, S, P.ns, v1.w);
       }
       if (i < n4) {
         const uint4 v0 = q[i];
-        hits += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
-                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w);
+        h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w);
       }
       const uint32_t done = head + 4 * n4;
-      if (ql < bd - done) hits += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + done + ql]);
+      if (ql < bd - done) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + done + ql]);
+      hits += h32;
     }
     // short ones: expand owners into the LDS window, then one element per lane per step
     const uint32_t ds = (d < (uint32_t)big_deg) ? d : 0u;
@@ -308,7 +309,9 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       for (uint32_t i = lo; i < hi; i++) own[i - r0] = rebase;
       wave_sync();
       const uint32_t lim = (T - r0 < (uint32_t)OWN) ? T - r0 : (uint32_t)OWN;
-      for (uint32_t e = lane; e < lim; e += 64) hits += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[own[e] + r0 + e]);
+      uint32_t h32 = 0;
+      for (uint32_t e = lane; e < lim; e += 64) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[own[e] + r0 + e]);
+      hits += h32;
       wave_sync();
     }
   }
@@ -381,7 +384,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
     const uint32_t w32 = (last >> 5) - (first >> 5) + 1;
     const bool is_bitset = (P.ns == 1) ? (S[0].lay == EHX_RANGE_BITSET)
                                        : (sn > 2 && (uint64_t)(last - first) <= 128ull * sn);
-    const bool bm_fits = w32 <= tab_slots, hash_fits = sn <= tab_slots / 2 && last != EMPTY;
+    const bool bm_fits = w32 < tab_slots, hash_fits = sn <= tab_slots / 2 && last != EMPTY;  // bitmap + zero word
     if (A.force_mode == 0) st.mode = bm_fits ? 0 : (hash_fits ? 1 : 2);
     else if (A.force_mode == 1) st.mode = hash_fits ? 1 : (bm_fits ? 0 : 2);
     else if (bm_fits && (is_bitset || bitmap_all)) st.mode = 0;
@@ -390,7 +393,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
     if (st.mode == 0) {
       st.bm_base = first >> 5;
       st.bm_words = w32;
-      for (uint32_t i = gtid; i < w32; i += gsize) tab[i] = 0;
+      for (uint32_t i = gtid; i <= w32; i += gsize) tab[i] = 0;  // tab[w32]: the zero word
       gsync();
       if (P.ns == 1 && S[0].lay == EHX_RANGE_BITSET) {
         // copy range_bitset words; 32-bit word i of the bitmap covers values [(bm_base + i) * 32, +32)
