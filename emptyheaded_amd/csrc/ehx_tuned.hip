@@ -266,13 +266,14 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
     // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
     // streamed with 128-bit loads, two quads per lane in flight per step (the
     // x's of one prefix are adjacent ids, so the lengths in a round are alike)
-    uint64_t big = __ballot(d >= (uint32_t)big_deg);
+    const uint64_t big = __ballot(d >= (uint32_t)big_deg);
+    const uint32_t nbig = (uint32_t)__popcll(big);
     const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
-    while (big) {
-      uint64_t mine = big;  // this group's list: the qt-th lowest set bit
-      for (uint32_t k = 0; k < qt; k++) mine &= mine - 1;
-      const int myl = mine ? __ffsll((unsigned long long)mine) - 1 : -1;
-      for (uint32_t k = 0; k < lists; k++) big &= big - 1;
+    // own[k] = the lane holding the k-th long set (the owner window is free until below)
+    if (d >= (uint32_t)big_deg) own[wave_popc_below(big)] = lane;
+    wave_sync();
+    for (uint32_t r0 = 0; r0 < nbig; r0 += lists) {
+      const int myl = (r0 + qt < nbig) ? (int)own[r0 + qt] : -1;
       const uint32_t so = __shfl(o, myl < 0 ? 0 : myl, 64), sd = __shfl(d, myl < 0 ? 0 : myl, 64);
       const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
       const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
@@ -297,6 +298,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       if (ql < bd - done) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + done + ql]);
       hits += h32;
     }
+    wave_sync();  // the owner window below reuses own[]
     // short ones: expand owners into the LDS window, then one element per lane per step
     const uint32_t ds = (d < (uint32_t)big_deg) ? d : 0u;
     const uint32_t incl = wave_incl_scan(ds);
