@@ -235,7 +235,7 @@ __device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int n
 
 // Streams child(x) for every x of E(p) through the staged S(p); MODE is the
 // staging (0 bitmap, 1 hash + filter, 2 sets probed in HBM).
-template <int MODE, bool BLOCK, int MAXS>
+template <int MODE, bool BLOCK, int MAXS, int DEEP>
 __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const TunedArgs &A, const Stage &st,
                                                     const DSet *S, const uint32_t *xsrc, uint32_t xn, bool filter_s,
                                                     int mi, uint32_t *own, int big_deg, uint32_t slg) {
@@ -282,6 +282,17 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
       const uint32_t n4 = (bd - head) >> 2;
       uint32_t i = ql;
+      for (; DEEP && i + 3 * G < n4; i += 4 * G) {  // four quads in flight
+        const uint4 v0 = q[i], v1 = q[i + G], v2 = q[i + 2 * G], v3 = q[i + 3 * G];
+        h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v1.x) + probe_m<MODE, MAXS>(st, S, P.ns, v1.y) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v1.z) + probe_m<MODE, MAXS>(st, S, P.ns, v1.w) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v2.x) + probe_m<MODE, MAXS>(st, S, P.ns, v2.y) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v2.z) + probe_m<MODE, MAXS>(st, S, P.ns, v2.w) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v3.x) + probe_m<MODE, MAXS>(st, S, P.ns, v3.y) +
+               probe_m<MODE, MAXS>(st, S, P.ns, v3.z) + probe_m<MODE, MAXS>(st, S, P.ns, v3.w);
+      }
       for (; i + G < n4; i += 2 * G) {
         const uint4 v0 = q[i], v1 = q[i + G];
         h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
@@ -322,7 +333,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
 
 // One prefix, processed by a group: the whole 4-wave workgroup (BLOCK) or one
 // wave.  tab/flt/sv are the group's LDS; own is the calling wave's window.
-template <bool BLOCK, int MAXS>
+template <bool BLOCK, int MAXS, int DEEP>
 __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, uint32_t *tab, uint32_t tab_slots,
                                  uint32_t *flt, uint32_t flt_words, uint32_t *sv, uint32_t sv_cap, uint32_t *own,
                                  int big_deg, int bitmap_all, uint32_t slg, int &mode_used) {
@@ -445,16 +456,16 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   uint64_t hits;
   // one streaming loop per staging mode: no per-probe branch on the mode
   if (st.mode == 0)
-    hits = stream_children<0, BLOCK, MAXS>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+    hits = stream_children<0, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
   else if (st.mode == 1)
-    hits = stream_children<1, BLOCK, MAXS>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+    hits = stream_children<1, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
   else
-    hits = stream_children<2, BLOCK, MAXS>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+    hits = stream_children<2, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
   gsync();
   return hits;
 }
 
-template <int SLOTS, int MAXS>
+template <int SLOTS, int MAXS, int DEEP>
 __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
                                                unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
                                                int big_deg, int grab, int bitmap_all, uint32_t slg) {
@@ -478,7 +489,7 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, T
     if (qi >= nbig) break;
     const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
     int mode = -1;
-    const uint64_t h = tuned_prefix<true, MAXS>(P, A, queue[qi], tab, SLOTS, flt, FWB, sv, 4 * VC, own[w], big_deg,
+    const uint64_t h = tuned_prefix<true, MAXS, DEEP>(P, A, queue[qi], tab, SLOTS, flt, FWB, sv, 4 * VC, own[w], big_deg,
                                                 bitmap_all, slg, mode);
     hits += h;
     if (A.seg.out) {
@@ -498,7 +509,7 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, T
     for (uint32_t qi = q0; qi < q1; qi++) {
       const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
       int mode = -1;
-      const uint64_t h = tuned_prefix<false, MAXS>(P, A, queue[qi], tab + w * WS, WS, flt + w * FWW, FWW,
+      const uint64_t h = tuned_prefix<false, MAXS, DEEP>(P, A, queue[qi], tab + w * WS, WS, flt + w * FWW, FWW,
                                                    sv + w * VC, VC, own[w], big_deg, bitmap_all, slg, mode);
       hits += h;
       if (A.seg.out) {
@@ -517,12 +528,17 @@ template <int SLOTS>
 static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, const TunedArgs &A, const uint32_t *queue,
                          unsigned *ctl, unsigned long long *acc, const TuneParams &tp) {
   const uint32_t slg = 31u - (uint32_t)__builtin_clz((unsigned)tp.stream_lanes);  // a power of two in [4, 64]
-  if (P.ns == 1)
-    k_tuned<SLOTS, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+  // four quads in flight per lane pay off for narrow lane groups (many short lists); wider
+  // groups keep two, and the registers they would cost
+  if (P.ns == 1 && slg <= 2)
+    k_tuned<SLOTS, 1, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+  else if (P.ns == 1)
+    k_tuned<SLOTS, 1, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
   else if (P.ns == 2)
-    k_tuned<SLOTS, 2><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+    k_tuned<SLOTS, 2, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
   else
-    k_tuned<SLOTS, TUNED_MAX_S><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+    k_tuned<SLOTS, TUNED_MAX_S, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all,
+                                                         slg);
 }
 
 // One cached graph per process: the key is every value baked into the launches.
