@@ -54,12 +54,12 @@ __device__ __forceinline__ uint64_t c4_wedge_bound(const DTrie &S, uint32_t b0, 
 
 // Calls f(c) for every wedge endpoint c < a of the lower neighbors vals[b0..b1)
 // (b1 - b0 <= 64).  Lane l owns b = vals[b0 + l] and N(b) cut at a (a binary
-// search: the lists are sorted, so the endpoints below a are a prefix); the 64
-// cut lists are laid end to end and the positions dealt out over the lanes, so
-// short lists do not idle the wave.
+// search: the lists are sorted, so the endpoints below a are a prefix).  The
+// lists are streamed by groups of G lanes, 64/G lists at a time, G sized to
+// about four endpoints per lane per list (adjacent b's have similar degrees),
+// with four loads in flight per lane.
 template <class F>
-__device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t b0, uint32_t b1, uint32_t *s_start,
-                                            uint32_t *s_off, F f) {
+__device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t b0, uint32_t b1, F f) {
   const uint32_t lane = __lane_id();
   const uint32_t bi = b0 + lane;
   uint32_t p0 = 0, len = 0;
@@ -70,33 +70,28 @@ __device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t
       len = lower_bound(S.vals, p0, S.off[hb + 1], a) - p0;
     }
   }
-  const uint32_t incl = wave_incl_scan(len);
-  const uint32_t tot = __shfl(incl, 63, 64);
-  s_start[lane] = incl - len;
-  s_off[lane] = p0;
-  wave_sync();
-  // four positions per lane per round: their loads are independent and overlap
-  for (uint32_t f0 = 0; f0 < tot; f0 += 256) {
-    uint32_t c[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const uint32_t fi = f0 + 64 * u + lane;
-      c[u] = C4_EMPTY;
-      if (fi < tot) {
-        uint32_t lo = 0, hi = 63;  // last owner whose list starts at or before fi
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi + 1) >> 1;
-          if (s_start[mid] <= fi) lo = mid;
-          else hi = mid - 1;
-        }
-        c[u] = S.vals[s_off[lo] + (fi - s_start[lo])];
-      }
+  const uint32_t nl = b1 - b0;
+  const uint32_t avg = wave_sum(len) / nl;
+  uint32_t slg = 2;
+  while (slg < 6 && (4u << slg) < avg) slg++;
+  const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
+  for (uint32_t r0 = 0; r0 < nl; r0 += lists) {
+    const uint32_t k = r0 + qt;
+    // shuffles on every lane: a source lane outside the active set would read as 0
+    const uint32_t po = __shfl(p0, k < nl ? k : 0, 64);
+    const uint32_t lk = __shfl(len, k < nl ? k : 0, 64);
+    const uint32_t ln = k < nl ? lk : 0u;
+    for (uint32_t i = ql; i < ln; i += 4 * G) {
+      const uint32_t c0 = S.vals[po + i];
+      const uint32_t c1 = i + G < ln ? S.vals[po + i + G] : C4_EMPTY;
+      const uint32_t c2 = i + 2 * G < ln ? S.vals[po + i + 2 * G] : C4_EMPTY;
+      const uint32_t c3 = i + 3 * G < ln ? S.vals[po + i + 3 * G] : C4_EMPTY;
+      f(c0);  // every loaded endpoint is below a
+      if (c1 != C4_EMPTY) f(c1);
+      if (c2 != C4_EMPTY) f(c2);
+      if (c3 != C4_EMPTY) f(c3);
     }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (c[u] != C4_EMPTY) f(c[u]);  // every loaded endpoint is below a
   }
-  wave_sync();
 }
 
 // Open-addressing insert (count + 1) into a table of mask + 1 slots.
@@ -119,7 +114,6 @@ __global__ __launch_bounds__(256) void k_c4_wave(DTrie S, uint64_t begin, uint64
                                                  uint32_t *__restrict__ ctl) {
   __shared__ uint32_t keys[4][C4_SLOTS];
   __shared__ uint32_t cnts[4][C4_SLOTS];
-  __shared__ uint32_t s_start[4][64], s_off[4][64];
   const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
   uint32_t *K = keys[w], *Cn = cnts[w];
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
@@ -144,8 +138,9 @@ __global__ __launch_bounds__(256) void k_c4_wave(DTrie S, uint64_t begin, uint64
     }
     wave_sync();
     for (uint32_t b0 = o0; b0 < lb; b0 += 64)
-      c4_wedges64(S, a, b0, min(b0 + 64, lb), s_start[w], s_off[w],
+      c4_wedges64(S, a, b0, min(b0 + 64, lb),
                   [&](uint32_t c) { lost |= !c4_insert(K, Cn, ts - 1, c); });
+    wave_sync();  // every lane's counts are in before they are summed
     uint64_t s = 0;
     for (uint32_t j = lane; j < ts; j += 64) s += pairs_of(Cn[j]);
     s = wave_sum(s);
@@ -183,7 +178,6 @@ __global__ __launch_bounds__(256) void k_c4_block(DTrie S, const uint32_t *__res
   __shared__ uint32_t H[C4_DENSE];
   __shared__ uint32_t s_item;
   __shared__ unsigned long long red[4];
-  __shared__ uint32_t s_start[4][64], s_off[4][64];
   const uint32_t w = threadIdx.x >> 6;
   uint32_t *K = gkeys + (uint64_t)blockIdx.x * C4_GSLOTS, *Cn = gcnts + (uint64_t)blockIdx.x * C4_GSLOTS;
   const uint32_t n_big = ctl[C4_BIG_COUNT];
@@ -217,7 +211,7 @@ __global__ __launch_bounds__(256) void k_c4_block(DTrie S, const uint32_t *__res
       for (uint32_t j = threadIdx.x; j < R; j += 256) H[j] = 0;
       __syncthreads();
       for (uint32_t b0 = o0 + 64 * w; b0 < lb; b0 += 256)
-        c4_wedges64(S, a, b0, min(b0 + 64, lb), s_start[w], s_off[w],
+        c4_wedges64(S, a, b0, min(b0 + 64, lb),
                     [&](uint32_t c) { atomicAdd(&H[c - lowest], 1u); });
       __syncthreads();
       uint64_t t = 0;
@@ -237,7 +231,7 @@ __global__ __launch_bounds__(256) void k_c4_block(DTrie S, const uint32_t *__res
         __threadfence();
         __syncthreads();
         for (uint32_t b0 = o0 + 64 * w; b0 < lb; b0 += 256)
-          c4_wedges64(S, a, b0, min(b0 + 64, lb), s_start[w], s_off[w], [&](uint32_t c) {
+          c4_wedges64(S, a, b0, min(b0 + 64, lb), [&](uint32_t c) {
             if (P == 1 || c4_hash(c ^ 0x5BD1E995u) % P == p) lost |= !c4_insert(K, Cn, ts - 1, c);
           });
         __threadfence();
