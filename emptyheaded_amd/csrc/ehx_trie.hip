@@ -83,33 +83,32 @@ __global__ void k_layouts(DTrie t, int policy, SelectorWeights sel, uint8_t *__r
   }
 }
 
-// One wave per bitset set, one lane per 64-bit word: the word's elements are
-// v[lo, hi) with lo = lower_bound(64 * word); rank[word] = lo.  No atomics.
-__global__ __launch_bounds__(256) void k_fill_words(DTrie t, uint64_t *__restrict__ words,
+// One lane per 64-bit word of the whole pool (the head set's window can be tens
+// of thousands of words, so sets are not given a wave each): the word's set is
+// the last h with bs_off[h] <= j; its elements are v[lo, hi) with
+// lo = lower_bound(64 * word); rank[word] = lo.  No atomics.
+__global__ __launch_bounds__(256) void k_fill_words(DTrie t, uint64_t nwords, uint64_t *__restrict__ words,
                                                     uint32_t *__restrict__ rank) {
   const uint32_t nh = t.nh;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t h = wave; h <= nh; h += nwaves) {
-    if (t.layout[h] != EHX_RANGE_BITSET) continue;
-    const DSet s = set_of(t, (uint32_t)h);
-    const uint32_t b = t.bs_off[h];
-    for (uint32_t i = __lane_id(); i < s.W; i += 64) {
-      const uint64_t lo_val = 64ull * (s.start + i);
-      uint32_t lo = 0, hi = s.n;
-      while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if ((uint64_t)s.v[mid] < lo_val) lo = mid + 1; else hi = mid;
-      }
-      uint64_t wv = 0;
-      uint32_t k = lo;
-      while (k < s.n && (uint64_t)s.v[k] < lo_val + 64) {
-        wv |= 1ull << (s.v[k] & 63);
-        k++;
-      }
-      words[b + i] = wv;
-      rank[b + i] = lo;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nwords;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = nh + 1;  // sets 0..nh; bs_off has nh + 2 entries
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if ((uint64_t)t.bs_off[mid] <= j) lo = mid; else hi = mid;
     }
+    const DSet s = set_of(t, lo);
+    const uint32_t i = (uint32_t)(j - t.bs_off[lo]);
+    const uint64_t lo_val = 64ull * (s.start + i);
+    uint32_t a = 0, b = s.n;
+    while (a < b) {
+      const uint32_t mid = (a + b) >> 1;
+      if ((uint64_t)s.v[mid] < lo_val) a = mid + 1; else b = mid;
+    }
+    uint64_t wv = 0;
+    for (uint32_t k = a; k < s.n && (uint64_t)s.v[k] < lo_val + 64; k++) wv |= 1ull << (s.v[k] & 63);
+    words[j] = wv;
+    rank[j] = a;
   }
 }
 
@@ -254,7 +253,9 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   t->dense = dense ? (int32_t *)((uint8_t *)blk2 + wbytes + rbytes) : nullptr;
   t->dense_len = dense ? (uint32_t)dense_len : 0;
   dv = view(t);
-  k_fill_words<<<grid_for((nh + 1) * 64, 256, 16384), 256, 0, st>>>(dv, t->words, t->rank);
+  if (W) {
+    k_fill_words<<<grid_for(W, 256, 16384), 256, 0, st>>>(dv, W, t->words, t->rank);
+  }
   EHX_HIP(hipGetLastError());
   if (dense) {
     if ((rc = fill_u32(ctx, (uint32_t *)t->dense, 0xFFFFFFFFu, dense_len))) return rc;
