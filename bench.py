@@ -50,6 +50,40 @@ def time_form(E, ctx, T, plan, form, steps, warmup, tries=None):
     return total, (t1 - t0) * 1e3 / steps, sum(kernel_ms) / len(kernel_ms), st
 
 
+def bench_small_graphs(E, ctx, steps, cpu=True):
+    """Erdos-Renyi (65,536 vertices, 2^20 samples; uint layouts) and R-MAT-20
+    triangle counts: the plain GPU form and the tuned+learned form next to the
+    CPU oracle (16 threads), all on the same graph."""
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import plans
+    from emptyheaded_amd import tuning as TU
+    out = {}
+    for name, (s, d, n), pol in (("er_65536", G.er_graph(), E.POLICY_ALL_UINT), ("rmat20", G.rmat_graph(20), None)):
+        T = E.Trie(ctx, s, d, pol if pol is not None else E.POLICY_RULE)
+        TL = E.Trie(ctx, s, d, pol if pol is not None else
+                    (E.POLICY_LEARNED if _selector_loaded(E) else E.POLICY_RULE))
+        TU.apply_plan("triangle")
+        pc, pms, _, _ = time_form(E, ctx, T, plans.TRIANGLE, E.FORM_PLAIN, steps, 1)
+        lc, lms, _, _ = time_form(E, ctx, TL, plans.TRIANGLE, E.FORM_LEARNED, steps, 1)
+        assert pc == lc
+        r = {"oriented_edges": len(s), "triangles": pc, "plain_ms": pms, "learned_ms": lms,
+             "layouts": "all uint" if pol is not None else "density rule (plain), learned (tuned+learned)"}
+        if cpu:
+            from oracle import oracle as O
+            OT = O.Trie(s, d)
+            t = time.perf_counter()
+            oc = O.join_count(plans.TRIANGLE, [OT], min(16, os.cpu_count() or 1))
+            r["cpu_oracle_ms"] = (time.perf_counter() - t) * 1e3
+            r["cpu_threads"] = min(16, os.cpu_count() or 1)
+            assert oc == pc
+        out[name] = r
+        log("%s: %d triangles; plain %.3f ms, tuned+learned %.3f ms%s" %
+            (name, pc, pms, lms, (", CPU oracle %.1f ms" % r["cpu_oracle_ms"]) if cpu else ""))
+        T.close()
+        TL.close()
+    return out
+
+
 def bench_four_clique(E, ctx, steps, scale=18):
     """4-clique COUNT on R-MAT scale 18: plain vs tuned vs tuned+learned."""
     from emptyheaded_amd import graphs as G
@@ -413,7 +447,8 @@ def main():
                                                          res["learned_reference_order"][1])
         out["plain_pairs_by_alg"] = res["plain_reference_order"][3]["pairs_by_alg"]
     if not args.no_configs and not args.headline_only:
-        out["configs"] = {"rmat18_four_clique": bench_four_clique(E, ctx, max(2, args.steps // 2)),
+        out["configs"] = {"small_graphs": bench_small_graphs(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
+                          "rmat18_four_clique": bench_four_clique(E, ctx, max(2, args.steps // 2)),
                           "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2)),
                           "triple_star_path": bench_triples(E, ctx, max(2, args.steps // 2)),
                           "dropin_cpp_triangle": bench_dropin(src, dst, count)}
