@@ -9,6 +9,7 @@ never change results).  Writes a timing DB:
     python -m emptyheaded_amd.autotune --out gpurun_out/tuning.json
 """
 import argparse
+import itertools
 import json
 import sys
 import time
@@ -20,9 +21,14 @@ GRID = {
     "grab": [1, 2, 4, 8],
     "big_work": [1024, 4096, 16384, 65536, 262144],
     "stream_lanes": [4, 8, 16, 32, 64],
+    "gap_coded": [0, 1],
+    "split_work": [0, 1 << 19, 1 << 20, 1 << 21, 1 << 22],
     "bitmap_all": [0, 1],
     "use_graph": [0, 1],
 }
+
+# knobs whose best values depend on each other
+JOINT = [("stream_lanes", "gap_coded"), ("lds_slots", "big_deg")]
 
 
 def log(*a):
@@ -74,6 +80,18 @@ def tune(E, ctx, workloads, passes=2, reps=3, grid=GRID, form=None):
                 ms = evaluate(trial)
                 if ms < best_ms * 0.99:
                     best_ms, cur, improved = ms, trial, True
+        # coupled knobs, moved together (one at a time can stall between them)
+        for group in JOINT:
+            if not all(k in grid for k in group):
+                continue
+            for vals in itertools.product(*(grid[k] for k in group)):
+                if all(cur[k] == v for k, v in zip(group, vals)):
+                    continue
+                trial = dict(cur)
+                trial.update(zip(group, vals))
+                ms = evaluate(trial)
+                if ms < best_ms * 0.99:
+                    best_ms, cur, improved = ms, trial, True
         if not improved:
             break
     E.tune_set(**cur)
@@ -101,6 +119,8 @@ def main():
     from emptyheaded_amd import graphs as G
     from emptyheaded_amd import plans, tuning
     applied = tuning.apply_committed()  # start from the committed knobs and selector
+    if args.plan != "mixed":
+        tuning.apply_plan(args.plan)  # and from the plan's own committed knobs
     E.tune_set(**fixed)
     learned = args.layouts == "learned" and applied.get("selector")
     pol = E.POLICY_LEARNED if learned else E.POLICY_RULE

@@ -68,6 +68,18 @@ struct ehx_trie {
   uint64_t *words = nullptr;
   int32_t *dense = nullptr;
   uint32_t dense_len = 0;
+  // gap-coded copy of long child lists, streamed by the tuned level: list h
+  // owns chunks pk_off[h] .. pk_off[h+1] - 1 of pk (none: not coded); a chunk is
+  // 32 bytes holding PK_VALS values, the first as a uint32 and then the 16-bit
+  // gaps between consecutive values
+  uint32_t *pk_off = nullptr;
+  uint4 *pk = nullptr;
+  uint64_t npk = 0;
+  // child lookup by head value (dense tries with a compact head domain):
+  // nav[x] = {off, length, first chunk, chunks} of x's child list, zeros when x
+  // is not a head
+  uint4 *nav = nullptr;
+  uint32_t nav_len = 0;
   ehx_trie_meta info{};
   void *block = nullptr;  // single allocation holding most arrays
 };
@@ -81,16 +93,27 @@ enum Slot {
   S_F0, S_F1, S_F2, S_F3, S_F4, S_F5, S_F6, S_F7,  // frontier columns
   S_COUNTS, S_OFFS, S_QUEUE, S_ACC, S_IN0, S_IN1,
   S_SEGB, S_SEGO,  // segment bounds and per-segment counters
+  S_DESC,          // per-prefix operand descriptors of the tuned level
   S_NUM_SLOTS
 };
+static_assert(S_NUM_SLOTS <= (int)(sizeof(ehx_ctx::scratch) / sizeof(ehx_ctx::scratch[0])), "context scratch slots");
 
 // ---- device view of a trie --------------------------------------------------
+// values per 32-byte gap-coded chunk: a uint32 first value and 14 16-bit gaps
+constexpr uint32_t PK_VALS = 15;
+// child lists shorter than this are not gap-coded
+constexpr uint32_t PK_MIN = 16;
+
 struct DTrie {
   const uint32_t *head, *off, *vals;
   const uint8_t *layout;
   const uint32_t *bs_off, *bs_start, *rank;
   const uint64_t *words;
   const int32_t *dense;
+  const uint32_t *pk_off;
+  const uint4 *pk;
+  const uint4 *nav;
+  uint32_t nav_len;
   uint32_t dense_len, nh;
 };
 
@@ -99,6 +122,8 @@ inline DTrie view(const ehx_trie *t) {
   d.head = t->head; d.off = t->off; d.vals = t->vals; d.layout = t->layout;
   d.bs_off = t->bs_off; d.bs_start = t->bs_start; d.rank = t->rank; d.words = t->words;
   d.dense = t->dense; d.dense_len = t->dense_len; d.nh = (uint32_t)t->nh;
+  d.pk_off = t->pk_off; d.pk = t->pk;
+  d.nav = t->nav; d.nav_len = t->nav_len;
   return d;
 }
 

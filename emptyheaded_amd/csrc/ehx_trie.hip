@@ -112,6 +112,64 @@ __global__ __launch_bounds__(256) void k_fill_words(DTrie t, uint64_t nwords, ui
   }
 }
 
+// Gap-coded lists: a child list of at least PK_MIN values whose consecutive
+// gaps all fit 16 bits is copied into ceil(n / PK_VALS) 32-byte chunks; chunk
+// c holds value c * PK_VALS as a uint32, then the gaps to the next 14 values
+// (zeros past the end).  One wavefront per list.
+__global__ __launch_bounds__(256) void k_pk_count(DTrie t, uint32_t *__restrict__ cnt) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < t.nh; h += nwaves) {
+    const uint32_t a = t.off[h], n = t.off[h + 1] - a;
+    uint32_t mx = 0;
+    if (n >= PK_MIN)
+      for (uint32_t k = 1 + __lane_id(); k < n; k += 64) mx = max(mx, t.vals[a + k] - t.vals[a + k - 1]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
+    if (__lane_id() == 0) cnt[h] = (n >= PK_MIN && mx < 65536u) ? (n + PK_VALS - 1) / PK_VALS : 0u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pk_fill(DTrie t, const uint32_t *__restrict__ pk_off,
+                                                 uint4 *__restrict__ pk) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < t.nh; h += nwaves) {
+    const uint32_t o = pk_off[h], nc = pk_off[h + 1] - o;
+    if (!nc) continue;
+    const uint32_t a = t.off[h], n = t.off[h + 1] - a;
+    for (uint32_t c = __lane_id(); c < nc; c += 64) {
+      const uint32_t k0 = c * PK_VALS;
+      uint32_t w[8];
+      w[0] = t.vals[a + k0];
+      uint32_t prev = w[0];
+#pragma unroll
+      for (int j = 1; j < 8; j++) {
+        uint32_t g[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const uint32_t k = k0 + 2 * j - 1 + u;  // value k is reached by gap 2j - 2 + u
+          g[u] = 0;
+          if (k < n) {
+            const uint32_t v = t.vals[a + k];
+            g[u] = v - prev;
+            prev = v;
+          }
+        }
+        w[j] = g[0] | (g[1] << 16);
+      }
+      uint4 *q = pk + 2ull * (o + c);
+      q[0] = make_uint4(w[0], w[1], w[2], w[3]);
+      q[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+  }
+}
+
+__global__ void k_nav(DTrie t, uint4 *__restrict__ nav) {
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < t.nh; h += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = t.off[h], c = t.pk_off[h];
+    nav[t.head[h]] = make_uint4(a, t.off[h + 1] - a, c, t.pk_off[h + 1] - c);
+  }
+}
+
 __global__ void k_dense(const uint32_t *__restrict__ head, uint64_t nh, int32_t *__restrict__ dense) {
   for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += (uint64_t)gridDim.x * blockDim.x)
     dense[head[h]] = (int32_t)h;
@@ -142,6 +200,8 @@ static int bits_of(uint32_t v) {
   while (b < 32 && (v >> b)) b++;
   return b;
 }
+
+int trie_destroy(ehx_trie *t);
 
 static void free_trie_arrays(ehx_trie *t) {
   if (t->block) (void)hipFree(t->block);
@@ -262,6 +322,36 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
     k_dense<<<grid_for(nh, 256, 16384), 256, 0, st>>>(t->head, nh, t->dense);
     EHX_HIP(hipGetLastError());
   }
+  // gap-coded copy of the long child lists
+  {
+    DevBuf &sc = ctx->scratch[S_COUNTS];
+    hipError_t e1 = hipMalloc((void **)&t->pk_off, (nh + 1) * 4);
+    if (e1 != hipSuccess) { trie_destroy(t); return fail(EHX_ERR_NOMEM, "chunk offsets: %s", hipGetErrorString(e1)); }
+    if ((rc = sc.ensure((nh + 1) * 4))) { trie_destroy(t); return rc; }
+    k_pk_count<<<grid_for(nh * 64, 256, 16384), 256, 0, st>>>(dv, sc.as<uint32_t>());
+    EHX_HIP(hipGetLastError());
+    if ((rc = scan_u32(ctx, sc.as<uint32_t>(), t->pk_off, nh))) { trie_destroy(t); return rc; }
+    uint32_t npk = 0;  // at most m / 15 + m / 16 chunks: fits 32 bits
+    if ((rc = copy_d2h(ctx, &npk, t->pk_off + nh, 4))) { trie_destroy(t); return rc; }
+    t->npk = npk;
+    e1 = hipMalloc((void **)&t->pk, (uint64_t)npk * 32 + 256);
+    if (e1 != hipSuccess) { trie_destroy(t); return fail(EHX_ERR_NOMEM, "coded lists: %s", hipGetErrorString(e1)); }
+    dv = view(t);
+    if (npk) {
+      k_pk_fill<<<grid_for(nh * 64, 256, 16384), 256, 0, st>>>(dv, t->pk_off, t->pk);
+      EHX_HIP(hipGetLastError());
+    }
+  }
+  // child lookup by value: 16 bytes per id of the head domain, when that domain is compact
+  if (dense && dense_len <= 8ull * nh + (1ull << 20)) {
+    hipError_t e1 = hipMalloc((void **)&t->nav, dense_len * 16);
+    if (e1 != hipSuccess) { trie_destroy(t); return fail(EHX_ERR_NOMEM, "child lookup: %s", hipGetErrorString(e1)); }
+    t->nav_len = (uint32_t)dense_len;
+    if ((rc = fill_u32(ctx, (uint32_t *)t->nav, 0u, dense_len * 4))) { trie_destroy(t); return rc; }
+    dv = view(t);
+    k_nav<<<grid_for(nh, 256, 16384), 256, 0, st>>>(dv, t->nav);
+    EHX_HIP(hipGetLastError());
+  }
   // stats
   DevBuf &ac = ctx->scratch[S_ACC];
   if ((rc = ac.ensure(64))) return rc;
@@ -295,6 +385,9 @@ int trie_destroy(ehx_trie *t) {
   EHX_HIP(hipSetDevice(t->device));
   if (t->block) EHX_HIP(hipFree(t->block));
   if (t->words) EHX_HIP(hipFree(t->words));
+  if (t->pk_off) EHX_HIP(hipFree(t->pk_off));
+  if (t->pk) EHX_HIP(hipFree(t->pk));
+  if (t->nav) EHX_HIP(hipFree(t->nav));
   delete t;
   return EHX_OK;
 }

@@ -50,6 +50,10 @@ struct TunedArgs {
   DTrie td;                           // driver set's relation (nd = 1)
   const uint32_t *dp;                 // prefix column of the driver's parent
   DTrie th[TUNED_MAX_H];              // head-set filters
+  // single-S plans: the operands of queue entry i, resolved by the binning pass,
+  // as desc[2i] = {S offset, |S|, D offset, |D|}, desc[2i+1] = {first of S,
+  // last of S, S's bitset word offset, S's layout}; null: resolved per prefix
+  const uint4 *desc;
 };
 
 // Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.
@@ -62,6 +66,8 @@ struct TuneParams {
   int32_t use_graph;      // capture the level in a hipGraph and replay it
   int32_t big_work;       // prefixes with at least this much estimated work go to the workgroup bin
   int32_t stream_lanes;   // lanes that stream one long child set together (a power of two, 4..64)
+  int32_t gap_coded;      // 1: stream the gap-coded copy of a child list where the trie has one
+  int32_t split_work;     // workgroup-bin prefixes are split into parts of about this much work (0: never)
 };
 
 // A request to time every prefix of the tuned level under a forced staging.

@@ -26,7 +26,7 @@
 
 namespace ehx {
 
-static TuneParams g_params = {4096, 32, 24, 2, 0, 0, 4096, 16};
+static TuneParams g_params = {4096, 32, 24, 2, 0, 0, 4096, 16, 0, 1 << 21};
 TuneParams &tune_params() { return g_params; }
 
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
@@ -70,6 +70,10 @@ bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
 
 static constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 static constexpr uint32_t NBUCKETS = 64, BUCKET_NONE = 63, WAVE_BIN = 32;
+// A workgroup-bin prefix of estimated work w is split into up to SPLIT_MAX
+// parts of about split_work each, over disjoint ranges of E(p) (each part
+// stages S(p) itself); at most SPLIT_EXTRA queue entries are added per level.
+static constexpr uint32_t SPLIT_MAX = 64, SPLIT_EXTRA = 1u << 16;
 static constexpr int OWN = 256;  // owner window per wave
 
 void tuned_bind(const TunedPlan &P, const Tries &T, TunedArgs &A) {
@@ -128,8 +132,9 @@ __device__ __forceinline__ bool head_ok(const TunedPlan &P, const TunedArgs &A, 
 // Wave capacity of the hash table for a given LDS slot budget.
 __host__ __device__ constexpr uint32_t wave_hash_cap(int slots) { return (uint32_t)slots / 8; }
 
+// bucket[p] = bin | parts << 6; hist[b] counts queue entries (parts)
 __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ bucket, unsigned *__restrict__ hist,
-                           int slots, int big_work) {
+                           int slots, int big_work, uint4 *__restrict__ pdesc, uint32_t split_work) {
   __shared__ unsigned h[NBUCKETS];
   if (threadIdx.x < NBUCKETS) h[threadIdx.x] = 0;
   __syncthreads();
@@ -138,7 +143,7 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
        p += (uint64_t)gridDim.x * blockDim.x) {
     DSet S[TUNED_MAX_S];
     DSet Dd;
-    uint32_t b = BUCKET_NONE;
+    uint32_t b = BUCKET_NONE, parts = 1;
     if (resolve_s<TUNED_MAX_S>(P, A, p, S) && resolve_d(P, A, p, Dd)) {
       int mi = 0;
       for (int i = 1; i < P.ns; i++)
@@ -162,16 +167,32 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
       const bool big = w >= (uint64_t)big_work || !fits_wave;
       uint32_t lg = 63 - __clzll(w | 1);
       b = (big ? 0u : WAVE_BIN) + (31 - (lg > 31 ? 31 : lg));
+      if (pdesc) {  // single-S plans (S[0] = D here); the third and fourth words locate E(p)
+        pdesc[2 * p] = make_uint4((uint32_t)(D.v - A.ts[0].vals), D.n,
+                                  (uint32_t)(X.v - (P.nd ? A.td.vals : A.ts[0].vals)), X.n);
+        pdesc[2 * p + 1] = make_uint4(D.v[0], D.v[D.n - 1], (uint32_t)(D.w - A.ts[0].words), D.lay);
+        if (big && split_work && !A.rec && w > split_work) {
+          uint64_t want = (w + split_work - 1) / split_work;
+          const uint32_t by_x = X.n / 64u;  // at least one full step of a workgroup per part
+          if (want > by_x) want = by_x;
+          if (want > SPLIT_MAX) want = SPLIT_MAX;
+          if (want > 1) {
+            const unsigned got = atomicAdd(&hist[132], (unsigned)want - 1u);  // the split budget
+            if (got + (unsigned)want - 1u <= SPLIT_EXTRA) parts = (uint32_t)want;
+          }
+        }
+      }
     }
-    bucket[p] = b;
-    atomicAdd(&h[b], 1u);
+    bucket[p] = b | (parts << 6);
+    atomicAdd(&h[b], parts);
   }
   __syncthreads();
   if (threadIdx.x < NBUCKETS && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
 // ctl: [0..63] hist, [64..127] bin base / cursor, [128] queue length,
-//      [129] block-bin cursor, [130] wave-bin cursor, [131] block-bin size
+//      [129] block-bin cursor, [130] wave-bin cursor, [131] block-bin size,
+//      [132] queue entries added by splits
 __global__ void k_bucket_scan(unsigned *__restrict__ ctl) {
   const uint32_t lane = threadIdx.x;
   unsigned c = (lane < BUCKET_NONE) ? ctl[lane] : 0u;
@@ -182,19 +203,36 @@ __global__ void k_bucket_scan(unsigned *__restrict__ ctl) {
 }
 
 __global__ __launch_bounds__(256) void k_enqueue(const uint32_t *__restrict__ bucket, uint64_t n,
-                                                 unsigned *__restrict__ ctl, uint32_t *__restrict__ queue) {
+                                                 unsigned *__restrict__ ctl, uint32_t *__restrict__ queue,
+                                                 const uint4 *__restrict__ pdesc, uint4 *__restrict__ qdesc) {
   __shared__ unsigned cnt[NBUCKETS], base[NBUCKETS];
   for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x; t0 < n; t0 += (uint64_t)gridDim.x * blockDim.x) {
     if (threadIdx.x < NBUCKETS) cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t p = t0 + threadIdx.x;
-    const uint32_t b = (p < n) ? bucket[p] : BUCKET_NONE;
+    const uint32_t bp = (p < n) ? bucket[p] : BUCKET_NONE;
+    const uint32_t b = bp & 63u, parts = bp >> 6;
     unsigned local = 0;
-    if (b != BUCKET_NONE) local = atomicAdd(&cnt[b], 1u);
+    if (b != BUCKET_NONE) local = atomicAdd(&cnt[b], parts);
     __syncthreads();
     if (threadIdx.x < NBUCKETS && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&ctl[64 + threadIdx.x], cnt[threadIdx.x]);
     __syncthreads();
-    if (b != BUCKET_NONE) queue[base[b] + local] = (uint32_t)p;
+    if (b != BUCKET_NONE) {
+      const uint32_t at = base[b] + local;
+      uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
+      if (pdesc) {
+        d0 = pdesc[2 * p];
+        d1 = pdesc[2 * p + 1];
+      }
+      for (uint32_t k = 0; k < parts; k++) {  // part k takes E(p)[xn * k / parts, xn * (k + 1) / parts)
+        queue[at + k] = (uint32_t)p;
+        if (pdesc) {
+          const uint32_t lo = (uint32_t)((uint64_t)d0.w * k / parts), hi = (uint32_t)((uint64_t)d0.w * (k + 1) / parts);
+          qdesc[2ull * (at + k)] = make_uint4(d0.x, d0.y, d0.z + lo, hi - lo);
+          qdesc[2ull * (at + k) + 1] = d1;
+        }
+      }
+    }
     __syncthreads();
   }
 }
@@ -233,36 +271,95 @@ __device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int n
   return 1u;
 }
 
+// Probes the values of one gap-coded chunk (a, b): the first value, then 14
+// values reached by the 16-bit gaps; only values below `lim` count (the last
+// chunk of a list is zero-padded).
+template <int MODE, int MAXS, bool TAIL>
+__device__ __forceinline__ uint32_t probe_chunk(const Stage &st, const DSet *S, int ns, const uint4 a, const uint4 b,
+                                                uint32_t lim) {
+  const uint32_t w[7] = {a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  uint32_t v = a.x;
+  uint32_t h = probe_m<MODE, MAXS>(st, S, ns, v);
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    v += w[j] & 0xFFFFu;
+    if (!TAIL || 2u * j + 1u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+    v += w[j] >> 16;
+    if (!TAIL || 2u * j + 2u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+  }
+  return h;
+}
+
+// The child list of one x of E(p) held by a lane: values vals[o, o + d), and
+// its gap-coded chunks pk[ko, ko + kn) when streamed from that copy.
+struct XList {
+  uint32_t o, d, ko, kn;
+};
+
+// A wave takes XW elements of E(p) per step: 64 on its own; 16 inside a
+// workgroup, so the four waves share even a small E(p) evenly.
+template <bool BLOCK>
+__device__ __forceinline__ uint32_t x_width() { return BLOCK ? 16u : 64u; }
+
+// The lane's x at position base + lane of E(p), and its child list.
+template <bool BLOCK, int MAXS>
+__device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, const DSet *S, const uint32_t *xsrc,
+                                        uint32_t xn, uint32_t base, bool filter_s, int mi, int big_deg, int gap) {
+  const DTrie &tx = A.tx;
+  const uint32_t lane = __lane_id(), idx = base + lane;
+  bool ok = lane < x_width<BLOCK>() && idx < xn;
+  const uint32_t x = ok ? xsrc[idx] : 0u;
+  if (ok && filter_s) {
+#pragma unroll
+    for (int i = 0; i < MAXS; i++)
+      if (i < P.ns && i != mi && ok) ok = set_contains(S[i], x);
+  }
+  if (ok) ok = head_ok(P, A, x);
+  XList r = {0u, 0u, 0u, 0u};
+  if (ok && tx.nav) {  // one 16-byte lookup by value
+    if (x < tx.nav_len) {
+      const uint4 nv = tx.nav[x];
+      r.o = nv.x;
+      r.d = nv.y;
+      if (gap && r.d >= (uint32_t)big_deg) {
+        r.ko = nv.z;
+        r.kn = nv.w;
+      }
+    }
+  } else if (ok) {
+    const int32_t nx = node_of(tx, x);
+    if (nx >= 0) {
+      r.o = tx.off[nx];
+      r.d = tx.off[nx + 1] - r.o;
+      if (gap && r.d >= (uint32_t)big_deg && r.d >= PK_MIN) {
+        r.ko = tx.pk_off[nx];
+        r.kn = tx.pk_off[nx + 1] - r.ko;
+      }
+    }
+  }
+  return r;
+}
+
 // Streams child(x) for every x of E(p) through the staged S(p); MODE is the
-// staging (0 bitmap, 1 hash + filter, 2 sets probed in HBM).
+// staging (0 bitmap, 1 hash + filter, 2 sets probed in HBM).  With `gap`, a
+// long list that the trie holds gap-coded is read from that copy (about 2.1
+// bytes per value instead of 4).  `first` is the caller's x_list of the wave's
+// first step, issued before the staging so that its loads overlap it.
 template <int MODE, bool BLOCK, int MAXS, int DEEP>
 __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const TunedArgs &A, const Stage &st,
                                                     const DSet *S, const uint32_t *xsrc, uint32_t xn, bool filter_s,
-                                                    int mi, uint32_t *own, int big_deg, uint32_t slg) {
+                                                    int mi, uint32_t *own, int big_deg, uint32_t slg, int gap,
+                                                    const XList first) {
   const DTrie &tx = A.tx;
   const uint32_t lane = __lane_id();
   const uint32_t wave = BLOCK ? (threadIdx.x >> 6) : 0u;
   const uint32_t nwaves = BLOCK ? (blockDim.x >> 6) : 1u;
   uint64_t hits = 0;
-  // A wave takes XW elements of S(p) per step: 64 on its own; 16 inside a
-  // workgroup, so the four waves share even a small S(p) evenly.
   constexpr uint32_t XW = BLOCK ? 16u : 64u;
   for (uint32_t base = wave * XW; base < xn; base += nwaves * XW) {
-    const uint32_t idx = base + lane;
-    bool ok = lane < XW && idx < xn;
-    const uint32_t x = ok ? xsrc[idx] : 0u;
-    if (ok && filter_s) {
-#pragma unroll
-      for (int i = 0; i < MAXS; i++)
-        if (i < P.ns && i != mi && ok) ok = set_contains(S[i], x);
-    }
-    if (ok) ok = head_ok(P, A, x);
-    const int32_t nx = ok ? node_of(tx, x) : -1;
-    uint32_t o = 0, d = 0;
-    if (nx >= 0) {
-      o = tx.off[nx];
-      d = tx.off[nx + 1] - o;
-    }
+    XList xl = first;
+    if (base != wave * XW) xl = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, filter_s, mi, big_deg, gap);
+    const uint32_t o = xl.o, d = xl.d, ko = xl.ko, kn = xl.kn;
     // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
     // streamed with 128-bit loads, two quads per lane in flight per step (the
     // x's of one prefix are adjacent ids, so the lengths in a round are alike)
@@ -274,10 +371,31 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
     wave_sync();
     for (uint32_t r0 = 0; r0 < nbig; r0 += lists) {
       const int myl = (r0 + qt < nbig) ? (int)own[r0 + qt] : -1;
-      const uint32_t so = __shfl(o, myl < 0 ? 0 : myl, 64), sd = __shfl(d, myl < 0 ? 0 : myl, 64);
+      const int src = myl < 0 ? 0 : myl;
+      const uint32_t so = __shfl(o, src, 64), sd = __shfl(d, src, 64);
+      const uint32_t sko = __shfl(ko, src, 64), skn = __shfl(kn, src, 64);
       const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
-      const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
       uint32_t h32 = 0;  // this round's hits of the lane (< 2^32)
+      if (myl >= 0 && skn) {  // gap-coded: chunks of PK_VALS values, one per lane per step
+        const uint4 *q = tx.pk + 2ull * sko;
+        const uint32_t nfull = bd / PK_VALS;
+        uint32_t c = ql;
+        for (; c + G < nfull; c += 2 * G) {  // two chunks (30 values) in flight
+          const uint4 a0 = q[2 * c], b0 = q[2 * c + 1], a1 = q[2 * (c + G)], b1 = q[2 * (c + G) + 1];
+          h32 += probe_chunk<MODE, MAXS, false>(st, S, P.ns, a0, b0, 0) +
+                 probe_chunk<MODE, MAXS, false>(st, S, P.ns, a1, b1, 0);
+        }
+        if (c < nfull) {
+          h32 += probe_chunk<MODE, MAXS, false>(st, S, P.ns, q[2 * c], q[2 * c + 1], 0);
+          c += G;
+        }
+        // exactly one lane of the group reaches c == nfull: the partial last chunk
+        const uint32_t rem = bd - nfull * PK_VALS;
+        if (c == nfull && rem) h32 += probe_chunk<MODE, MAXS, true>(st, S, P.ns, q[2 * c], q[2 * c + 1], rem);
+        hits += h32;
+        continue;
+      }
+      const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
       if (ql < head) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + ql]);
       const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
       const uint32_t n4 = (bd - head) >> 2;
@@ -333,10 +451,12 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
 
 // One prefix, processed by a group: the whole 4-wave workgroup (BLOCK) or one
 // wave.  tab/flt/sv are the group's LDS; own is the calling wave's window.
+// dsc0/dsc1: the prefix's descriptor (A.desc) when the plan has one S set.
 template <bool BLOCK, int MAXS, int DEEP>
-__device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, uint32_t *tab, uint32_t tab_slots,
+__device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, const uint4 dsc0,
+                                 const uint4 dsc1, uint32_t *tab, uint32_t tab_slots,
                                  uint32_t *flt, uint32_t flt_words, uint32_t *sv, uint32_t sv_cap, uint32_t *own,
-                                 int big_deg, int bitmap_all, uint32_t slg, int &mode_used) {
+                                 int big_deg, int bitmap_all, uint32_t slg, int gap, int &mode_used) {
   const uint32_t lane = __lane_id();
   const uint32_t gtid = BLOCK ? threadIdx.x : lane;
   const uint32_t gsize = BLOCK ? blockDim.x : 64u;
@@ -350,7 +470,20 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   const int NS = (MAXS <= 2) ? MAXS : P.ns;
   DSet S[MAXS];
   DSet Dd;
-  if (!resolve_s<MAXS>(P, A, p, S) || !resolve_d(P, A, p, Dd)) return 0;
+  const bool use_desc = MAXS == 1 && A.desc;
+  if (use_desc) {
+    S[0].v = A.ts[0].vals + dsc0.x;
+    S[0].n = dsc0.y;
+    S[0].w = A.ts[0].words + dsc1.z;
+    S[0].lay = dsc1.w;
+    S[0].start = dsc1.x >> 6;
+    S[0].W = (dsc1.w == EHX_RANGE_BITSET) ? (dsc1.y >> 6) - (dsc1.x >> 6) + 1u : 0u;
+    // E(p) (or this part of it): in D's trie for driver plans, else in S's own
+    Dd.v = (P.nd ? A.td.vals : A.ts[0].vals) + dsc0.z;
+    Dd.n = dsc0.w;
+  } else if (!resolve_s<MAXS>(P, A, p, S) || !resolve_d(P, A, p, Dd)) {
+    return 0;
+  }
   int mi = 0;
 #pragma unroll
   for (int i = 1; i < MAXS; i++)
@@ -385,6 +518,16 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
       src = sv;
     }
   }
+  // one S set: E(p) is known now, so the first step's child lookups are issued
+  // ahead of the staging
+  XList first = {0u, 0u, 0u, 0u};
+  const uint32_t wave_x0 = wave * x_width<BLOCK>();
+  const bool x_in_d = P.nd || use_desc;  // E(p) is Dd
+  if (MAXS == 1) {
+    const uint32_t *xs = x_in_d ? Dd.v : src;
+    const uint32_t xn1 = x_in_d ? Dd.n : sn;
+    first = x_list<BLOCK, MAXS>(P, A, S, xs, xn1, wave_x0, false, 0, big_deg, gap);
+  }
   // staging decision (group-uniform)
   Stage st;
   st.mode = 2;
@@ -393,7 +536,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   st.flt = flt;
   st.bm_base = st.bm_words = st.hshift = st.hmask = st.fshift = 0;
   if (src) {
-    const uint32_t first = src[0], last = src[sn - 1];
+    const uint32_t first = use_desc ? dsc1.x : src[0], last = use_desc ? dsc1.y : src[sn - 1];
     const uint32_t w32 = (last >> 5) - (first >> 5) + 1;
     const bool is_bitset = (P.ns == 1) ? (S[0].lay == EHX_RANGE_BITSET)
                                        : (sn > 2 && (uint64_t)(last - first) <= 128ull * sn);
@@ -450,17 +593,21 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   mode_used = st.mode;
 
   const DSet Smi = sel<MAXS>(S, mi);
-  const uint32_t *xsrc = P.nd ? Dd.v : (src ? src : Smi.v);
-  const uint32_t xn = P.nd ? Dd.n : (src ? sn : Smi.n);
+  const uint32_t *xsrc = x_in_d ? Dd.v : (src ? src : Smi.v);
+  const uint32_t xn = x_in_d ? Dd.n : (src ? sn : Smi.n);
   const bool filter_s = !P.nd && !src;  // x runs over one S set: test it against the others
+  if (MAXS > 1) first = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, wave_x0, filter_s, mi, big_deg, gap);
   uint64_t hits;
   // one streaming loop per staging mode: no per-probe branch on the mode
   if (st.mode == 0)
-    hits = stream_children<0, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+    hits = stream_children<0, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
+                                                       first);
   else if (st.mode == 1)
-    hits = stream_children<1, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+    hits = stream_children<1, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
+                                                       first);
   else
-    hits = stream_children<2, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg);
+    hits = stream_children<2, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
+                                                       first);
   gsync();
   return hits;
 }
@@ -468,7 +615,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
 template <int SLOTS, int MAXS, int DEEP>
 __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
                                                unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
-                                               int big_deg, int grab, int bitmap_all, uint32_t slg) {
+                                               int big_deg, int grab, int bitmap_all, uint32_t slg, int gap) {
   constexpr uint32_t WS = SLOTS / 4;                 // per-wave table slots
   constexpr uint32_t FWB = SLOTS / 4, FWW = FWB / 4;  // filter words per block / wave
   constexpr uint32_t VC = (MAXS > 1) ? WS : 1u;      // materialized S capacity per wave
@@ -489,8 +636,13 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, T
     if (qi >= nbig) break;
     const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
     int mode = -1;
-    const uint64_t h = tuned_prefix<true, MAXS, DEEP>(P, A, queue[qi], tab, SLOTS, flt, FWB, sv, 4 * VC, own[w], big_deg,
-                                                bitmap_all, slg, mode);
+    uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
+    if (MAXS == 1 && A.desc) {
+      d0 = A.desc[2ull * qi];
+      d1 = A.desc[2ull * qi + 1];
+    }
+    const uint64_t h = tuned_prefix<true, MAXS, DEEP>(P, A, queue[qi], d0, d1, tab, SLOTS, flt, FWB, sv, 4 * VC, own[w],
+                                                      big_deg, bitmap_all, slg, gap, mode);
     hits += h;
     if (A.seg.out) {
       const uint64_t hw = wave_sum(h);
@@ -506,11 +658,22 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, T
     q0 = __shfl(q0, 0, 64) + nbig;
     if (q0 >= qlen) break;
     const uint32_t q1 = min(q0 + (uint32_t)grab, qlen);
+    // the next entry's descriptor is loaded while this one runs
+    uint4 n0 = make_uint4(0, 0, 0, 0), n1 = n0;
+    if (MAXS == 1 && A.desc) {
+      n0 = A.desc[2ull * q0];
+      n1 = A.desc[2ull * q0 + 1];
+    }
     for (uint32_t qi = q0; qi < q1; qi++) {
+      const uint4 d0 = n0, d1 = n1;
+      if (MAXS == 1 && A.desc && qi + 1 < q1) {
+        n0 = A.desc[2ull * (qi + 1)];
+        n1 = A.desc[2ull * (qi + 1) + 1];
+      }
       const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
       int mode = -1;
-      const uint64_t h = tuned_prefix<false, MAXS, DEEP>(P, A, queue[qi], tab + w * WS, WS, flt + w * FWW, FWW,
-                                                   sv + w * VC, VC, own[w], big_deg, bitmap_all, slg, mode);
+      const uint64_t h = tuned_prefix<false, MAXS, DEEP>(P, A, queue[qi], d0, d1, tab + w * WS, WS, flt + w * FWW, FWW,
+                                                         sv + w * VC, VC, own[w], big_deg, bitmap_all, slg, gap, mode);
       hits += h;
       if (A.seg.out) {
         const uint64_t hw = wave_sum(h);
@@ -528,17 +691,18 @@ template <int SLOTS>
 static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, const TunedArgs &A, const uint32_t *queue,
                          unsigned *ctl, unsigned long long *acc, const TuneParams &tp) {
   const uint32_t slg = 31u - (uint32_t)__builtin_clz((unsigned)tp.stream_lanes);  // a power of two in [4, 64]
+  const int gap = (tp.gap_coded && A.tx.pk) ? 1 : 0;
   // four quads in flight per lane pay off for narrow lane groups (many short lists); wider
   // groups keep two, and the registers they would cost
   if (P.ns == 1 && slg <= 2)
-    k_tuned<SLOTS, 1, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+    k_tuned<SLOTS, 1, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg, gap);
   else if (P.ns == 1)
-    k_tuned<SLOTS, 1, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+    k_tuned<SLOTS, 1, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg, gap);
   else if (P.ns == 2)
-    k_tuned<SLOTS, 2, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg);
+    k_tuned<SLOTS, 2, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg, gap);
   else
     k_tuned<SLOTS, TUNED_MAX_S, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all,
-                                                         slg);
+                                                         slg, gap);
 }
 
 // One cached graph per process: the key is every value baked into the launches.
@@ -557,12 +721,16 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
                          uint32_t *bucket, unsigned *ctl, uint32_t *queue, const TuneParams &tp, bool events = true) {
   hipStream_t st = ctx->stream;
   int rc;
-  if ((rc = fill_u32(ctx, ctl, 0u, 132))) return rc;
-  k_classify<<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(P, A, bucket, ctl, tp.lds_slots, tp.big_work);
+  if ((rc = fill_u32(ctx, ctl, 0u, 133))) return rc;
+  // descriptors by prefix after the queue-ordered ones (A.desc)
+  uint4 *pdesc = A.desc ? const_cast<uint4 *>(A.desc) + 2 * (A.nprefix + SPLIT_EXTRA) : nullptr;
+  k_classify<<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(P, A, bucket, ctl, tp.lds_slots, tp.big_work, pdesc,
+                                                               (uint32_t)tp.split_work);
   EHX_HIP(hipGetLastError());
   k_bucket_scan<<<1, 64, 0, st>>>(ctl);
   EHX_HIP(hipGetLastError());
-  k_enqueue<<<grid_for(A.nprefix, 256, 8192), 256, 0, st>>>(bucket, A.nprefix, ctl, queue);
+  k_enqueue<<<grid_for(A.nprefix, 256, 8192), 256, 0, st>>>(bucket, A.nprefix, ctl, queue, pdesc,
+                                                            const_cast<uint4 *>(A.desc));
   EHX_HIP(hipGetLastError());
   const unsigned per_cu = (unsigned)(tp.waves_per_cu > 4 ? tp.waves_per_cu / 4 : 1);
   const unsigned grid = (unsigned)ctx->num_cus * per_cu;
@@ -577,14 +745,21 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
   return EHX_OK;
 }
 
-int tuned_run(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, int form, unsigned long long *acc) {
+int tuned_run(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A_in, int form, unsigned long long *acc) {
   (void)form;
-  if (A.nprefix == 0) return EHX_OK;
-  if (A.nprefix >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "tuned level: more than 2^32 prefixes");
+  if (A_in.nprefix == 0) return EHX_OK;
+  if (A_in.nprefix >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "tuned level: more than 2^32 prefixes");
   const TuneParams tp = g_params;
-  DevBuf &bb = ctx->scratch[S_QUEUE], &cb = ctx->scratch[S_MISC];
+  DevBuf &bb = ctx->scratch[S_QUEUE], &cb = ctx->scratch[S_MISC], &db = ctx->scratch[S_DESC];
   int rc;
-  if ((rc = bb.ensure(A.nprefix * 8 + 64)) || (rc = cb.ensure(256 * 4))) return rc;
+  // bucket[nprefix], then the queue (split parts add up to SPLIT_EXTRA entries)
+  if ((rc = bb.ensure((A_in.nprefix * 2 + SPLIT_EXTRA) * 4 + 64)) || (rc = cb.ensure(256 * 4))) return rc;
+  TunedArgs A = A_in;
+  A.desc = nullptr;
+  if (P.ns == 1) {  // descriptors: queue order, then by prefix (32 bytes each)
+    if ((rc = db.ensure((A.nprefix * 2 + SPLIT_EXTRA) * 32))) return rc;
+    A.desc = db.as<uint4>();
+  }
   uint32_t *bucket = bb.as<uint32_t>(), *queue = bucket + A.nprefix;
   unsigned *ctl = cb.as<unsigned>();
   if (!tp.use_graph) return enqueue_level(ctx, P, A, acc, bucket, ctl, queue, tp);

@@ -102,11 +102,52 @@ def test_hub_beyond_every_staging_area(ctx, oracle, policy):
     T, R = E.Trie(ctx, s, d, policy), E.Trie(ctx, d, s, policy)
     saved = E.tune_get()
     try:
-        for lanes in (4, 64):
-            E.tune_set(stream_lanes=lanes)
+        for lanes, gap, split in ((4, 0, 0), (4, 1, 1 << 21), (64, 1, 4096)):
+            E.tune_set(stream_lanes=lanes, gap_coded=gap, split_work=split)
             for form in FORMS:
                 assert E.join_count(ctx, plans.TRIANGLE, [T], form) == tri
                 assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], form) == tri
                 assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], form) == k4
+    finally:
+        E.tune_set(**saved)
+
+
+@pytest.mark.parametrize("lanes", [4, 16, 64])
+def test_gap_coded_lists_at_their_limits(ctx, oracle, lanes):
+    """Child lists of 16 to 5,000 values (15-value chunks: lengths on, just
+    under and just over a chunk boundary), some ending in a 65,535 gap (coded)
+    and some in a 65,536 gap (not coded), streamed with and without the coded
+    copy; every triangle and 4-clique goes through such a list."""
+    rng = np.random.default_rng(31)
+    A = np.arange(1, 301)  # the hub's neighbours, pairwise joined through the lists below
+    edges = [(0, int(a)) for a in A]
+    lengths = [16, 29, 30, 31, 44, 45, 46, 60, 61, 100, 1000, 5000]
+    for i, b in enumerate(A[:240]):
+        L = lengths[i % len(lengths)]
+        mine = A[A > b]
+        take = rng.choice(mine, size=min(len(mine), L // 3), replace=False) if len(mine) else np.zeros(0, int)
+        rest = L - len(take) - 2
+        far = np.unique(rng.integers(1000, 1000 + 60 * max(rest, 1), size=max(rest, 0)))
+        top = int(far.max()) if len(far) else 1000
+        last_gap = 65535 if i % 3 else 65536
+        tail = [top + 7, top + 7 + last_gap]
+        for c in list(take) + list(far) + tail:
+            edges.append((int(b), int(c)))
+    s, d = oriented(edges)
+    O = oracle.Trie(s, d)
+    tri = oracle.join_count(plans.TRIANGLE, [O], 8)
+    k4 = oracle.join_count(plans.FOUR_CLIQUE, [O], 8)
+    say("gap-coded lists: triangles", tri, "4-cliques", k4)
+    assert tri > 1000 and k4 > 1000
+    T, R = E.Trie(ctx, s, d), E.Trie(ctx, d, s)
+    saved = E.tune_get()
+    try:
+        for big_deg in (4, 16):
+            for gap in (0, 1):
+                E.tune_set(stream_lanes=lanes, big_deg=big_deg, gap_coded=gap)
+                for form in (E.FORM_TUNED, E.FORM_LEARNED):
+                    assert E.join_count(ctx, plans.TRIANGLE, [T], form) == tri
+                    assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], form) == tri
+                    assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], form) == k4
     finally:
         E.tune_set(**saved)
