@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/ehx.h"
 
@@ -80,6 +81,16 @@ struct ehx_trie {
   // is not a head
   uint4 *nav = nullptr;
   uint32_t nav_len = 0;
+  // child pointers into other tries (or this one), built on first use: for
+  // target trie u, arr[e] = u's nav entry of vals[e] (the reference's tries keep
+  // a child pointer beside every set element; here the child may live in a
+  // different relation's trie)
+  struct Link {
+    uint64_t target;
+    uint4 *arr;
+  };
+  std::vector<Link> links;
+  uint64_t uid = 0;  // unique per process, never reused
   ehx_trie_meta info{};
   void *block = nullptr;  // single allocation holding most arrays
 };
@@ -210,6 +221,9 @@ int scan_u32(ehx_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
 int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits);
 int reduce_max_u32(ehx_ctx *ctx, const uint32_t *in, uint64_t n, uint32_t *host_out);
 int fill_u32(ehx_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
+// src's child pointers into dst (built and cached on src on first use); null in
+// *out when dst has no child lookup by value
+int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out);
 int fill_u64(ehx_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
 int copy_d2h(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int copy_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);

@@ -16,6 +16,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "ehx_internal.h"
 #include "ehx_selector.h"
 
@@ -170,6 +173,13 @@ __global__ void k_nav(DTrie t, uint4 *__restrict__ nav) {
   }
 }
 
+__global__ void k_link(const uint32_t *__restrict__ vals, uint64_t m, DTrie dst, uint4 *__restrict__ arr) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = vals[e];
+    arr[e] = (x < dst.nav_len) ? dst.nav[x] : make_uint4(0, 0, 0, 0);
+  }
+}
+
 __global__ void k_dense(const uint32_t *__restrict__ head, uint64_t nh, int32_t *__restrict__ dense) {
   for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += (uint64_t)gridDim.x * blockDim.x)
     dense[head[h]] = (int32_t)h;
@@ -202,6 +212,9 @@ static int bits_of(uint32_t v) {
 }
 
 int trie_destroy(ehx_trie *t);
+
+static std::mutex g_link_mu;
+static std::atomic<uint64_t> g_next_uid{1};
 
 static void free_trie_arrays(ehx_trie *t) {
   if (t->block) (void)hipFree(t->block);
@@ -259,6 +272,7 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
 
   ehx_trie *t = new ehx_trie();
   t->device = ctx->device;
+  t->uid = g_next_uid++;
   t->m = m;
   t->nh = nh;
   // dense navigation array when it costs at most ~64 entries per head plus 4 MiB
@@ -380,9 +394,33 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   return EHX_OK;
 }
 
+int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out) {
+  *out = nullptr;
+  if (!dst->nav || !src->m) return EHX_OK;
+  std::lock_guard<std::mutex> lk(g_link_mu);
+  for (const auto &l : src->links)
+    if (l.target == dst->uid) {
+      *out = l.arr;
+      return EHX_OK;
+    }
+  uint4 *arr = nullptr;
+  hipError_t e = hipMalloc((void **)&arr, src->m * 16);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return EHX_OK;  // no room: the level looks children up by value instead
+  }
+  k_link<<<grid_for(src->m, 256, 16384), 256, 0, ctx->stream>>>(src->vals, src->m, view(dst), arr);
+  EHX_HIP(hipGetLastError());
+  EHX_HIP(hipStreamSynchronize(ctx->stream));  // other contexts may use it next
+  src->links.push_back({dst->uid, arr});
+  *out = arr;
+  return EHX_OK;
+}
+
 int trie_destroy(ehx_trie *t) {
   if (!t) return EHX_OK;
   EHX_HIP(hipSetDevice(t->device));
+  for (const auto &l : t->links) EHX_HIP(hipFree(l.arr));
   if (t->block) EHX_HIP(hipFree(t->block));
   if (t->words) EHX_HIP(hipFree(t->words));
   if (t->pk_off) EHX_HIP(hipFree(t->pk_off));

@@ -54,6 +54,9 @@ struct TunedArgs {
   // as desc[2i] = {S offset, |S|, D offset, |D|}, desc[2i+1] = {first of S,
   // last of S, S's bitset word offset, S's layout}; null: resolved per prefix
   const uint4 *desc;
+  // with desc: xlink[e] = x's child list {off, length, first chunk, chunks} in
+  // the x trie for the value at position e of E(p)'s trie; null: looked up by value
+  const uint4 *xlink;
 };
 
 // Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.

@@ -277,15 +277,26 @@ __device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int n
 template <int MODE, int MAXS, bool TAIL>
 __device__ __forceinline__ uint32_t probe_chunk(const Stage &st, const DSet *S, int ns, const uint4 a, const uint4 b,
                                                 uint32_t lim) {
-  const uint32_t w[7] = {a.y, a.z, a.w, b.x, b.y, b.z, b.w};
   uint32_t v = a.x;
   uint32_t h = probe_m<MODE, MAXS>(st, S, ns, v);
+  if (MODE != 2) {
+    const uint32_t w[7] = {a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-  for (int j = 0; j < 7; j++) {
-    v += w[j] & 0xFFFFu;
-    if (!TAIL || 2u * j + 1u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
-    v += w[j] >> 16;
-    if (!TAIL || 2u * j + 2u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+    for (int j = 0; j < 7; j++) {
+      v += w[j] & 0xFFFFu;
+      if (!TAIL || 2u * j + 1u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+      v += w[j] >> 16;
+      if (!TAIL || 2u * j + 2u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+    }
+  } else {  // probes of sets in HBM (rare): a rolled loop keeps the code small
+#pragma unroll 1
+    for (uint32_t j = 0; j < 7; j++) {
+      const uint32_t wv = j == 0 ? a.y : j == 1 ? a.z : j == 2 ? a.w : j == 3 ? b.x : j == 4 ? b.y : j == 5 ? b.z : b.w;
+      v += wv & 0xFFFFu;
+      if (!TAIL || 2u * j + 1u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+      v += wv >> 16;
+      if (!TAIL || 2u * j + 2u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+    }
   }
   return h;
 }
@@ -301,13 +312,28 @@ struct XList {
 template <bool BLOCK>
 __device__ __forceinline__ uint32_t x_width() { return BLOCK ? 16u : 64u; }
 
-// The lane's x at position base + lane of E(p), and its child list.
+// The lane's x at position base + lane of E(p), and its child list; xl: E(p)'s
+// child pointers (A.xlink at E(p)'s offset), or null.
 template <bool BLOCK, int MAXS>
 __device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, const DSet *S, const uint32_t *xsrc,
-                                        uint32_t xn, uint32_t base, bool filter_s, int mi, int big_deg, int gap) {
+                                        uint32_t xn, uint32_t base, bool filter_s, int mi, int big_deg, int gap,
+                                        const uint4 *xl) {
   const DTrie &tx = A.tx;
   const uint32_t lane = __lane_id(), idx = base + lane;
   bool ok = lane < x_width<BLOCK>() && idx < xn;
+  XList r = {0u, 0u, 0u, 0u};
+  if (xl && !filter_s && A.nh == 0) {  // one 16-byte load, the value itself not needed
+    if (ok) {
+      const uint4 nv = xl[idx];
+      r.o = nv.x;
+      r.d = nv.y;
+      if (gap && r.d >= (uint32_t)big_deg) {
+        r.ko = nv.z;
+        r.kn = nv.w;
+      }
+    }
+    return r;
+  }
   const uint32_t x = ok ? xsrc[idx] : 0u;
   if (ok && filter_s) {
 #pragma unroll
@@ -315,7 +341,6 @@ __device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, 
       if (i < P.ns && i != mi && ok) ok = set_contains(S[i], x);
   }
   if (ok) ok = head_ok(P, A, x);
-  XList r = {0u, 0u, 0u, 0u};
   if (ok && tx.nav) {  // one 16-byte lookup by value
     if (x < tx.nav_len) {
       const uint4 nv = tx.nav[x];
@@ -349,17 +374,20 @@ template <int MODE, bool BLOCK, int MAXS, int DEEP>
 __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const TunedArgs &A, const Stage &st,
                                                     const DSet *S, const uint32_t *xsrc, uint32_t xn, bool filter_s,
                                                     int mi, uint32_t *own, int big_deg, uint32_t slg, int gap,
-                                                    const XList first) {
+                                                    const XList first, const uint4 *xl) {
   const DTrie &tx = A.tx;
   const uint32_t lane = __lane_id();
   const uint32_t wave = BLOCK ? (threadIdx.x >> 6) : 0u;
   const uint32_t nwaves = BLOCK ? (blockDim.x >> 6) : 1u;
   uint64_t hits = 0;
   constexpr uint32_t XW = BLOCK ? 16u : 64u;
+  // several loads in flight per lane for the LDS stagings; sets probed in HBM
+  // (rare) stay one quad (chunk) per step, which keeps the kernel's code small
+  constexpr bool UNROLL = MODE != 2;
   for (uint32_t base = wave * XW; base < xn; base += nwaves * XW) {
-    XList xl = first;
-    if (base != wave * XW) xl = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, filter_s, mi, big_deg, gap);
-    const uint32_t o = xl.o, d = xl.d, ko = xl.ko, kn = xl.kn;
+    XList xi = first;
+    if (base != wave * XW) xi = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, filter_s, mi, big_deg, gap, xl);
+    const uint32_t o = xi.o, d = xi.d, ko = xi.ko, kn = xi.kn;
     // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
     // streamed with 128-bit loads, two quads per lane in flight per step (the
     // x's of one prefix are adjacent ids, so the lengths in a round are alike)
@@ -380,15 +408,12 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
         const uint4 *q = tx.pk + 2ull * sko;
         const uint32_t nfull = bd / PK_VALS;
         uint32_t c = ql;
-        for (; c + G < nfull; c += 2 * G) {  // two chunks (30 values) in flight
+        for (; UNROLL && c + G < nfull; c += 2 * G) {  // two chunks (30 values) in flight
           const uint4 a0 = q[2 * c], b0 = q[2 * c + 1], a1 = q[2 * (c + G)], b1 = q[2 * (c + G) + 1];
           h32 += probe_chunk<MODE, MAXS, false>(st, S, P.ns, a0, b0, 0) +
                  probe_chunk<MODE, MAXS, false>(st, S, P.ns, a1, b1, 0);
         }
-        if (c < nfull) {
-          h32 += probe_chunk<MODE, MAXS, false>(st, S, P.ns, q[2 * c], q[2 * c + 1], 0);
-          c += G;
-        }
+        for (; c < nfull; c += G) h32 += probe_chunk<MODE, MAXS, false>(st, S, P.ns, q[2 * c], q[2 * c + 1], 0);
         // exactly one lane of the group reaches c == nfull: the partial last chunk
         const uint32_t rem = bd - nfull * PK_VALS;
         if (c == nfull && rem) h32 += probe_chunk<MODE, MAXS, true>(st, S, P.ns, q[2 * c], q[2 * c + 1], rem);
@@ -400,7 +425,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
       const uint32_t n4 = (bd - head) >> 2;
       uint32_t i = ql;
-      for (; DEEP && i + 3 * G < n4; i += 4 * G) {  // four quads in flight
+      for (; UNROLL && DEEP && i + 3 * G < n4; i += 4 * G) {  // four quads in flight
         const uint4 v0 = q[i], v1 = q[i + G], v2 = q[i + 2 * G], v3 = q[i + 3 * G];
         h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w) +
@@ -411,14 +436,14 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
                probe_m<MODE, MAXS>(st, S, P.ns, v3.x) + probe_m<MODE, MAXS>(st, S, P.ns, v3.y) +
                probe_m<MODE, MAXS>(st, S, P.ns, v3.z) + probe_m<MODE, MAXS>(st, S, P.ns, v3.w);
       }
-      for (; i + G < n4; i += 2 * G) {
+      for (; UNROLL && i + G < n4; i += 2 * G) {
         const uint4 v0 = q[i], v1 = q[i + G];
         h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w) +
                probe_m<MODE, MAXS>(st, S, P.ns, v1.x) + probe_m<MODE, MAXS>(st, S, P.ns, v1.y) +
                probe_m<MODE, MAXS>(st, S, P.ns, v1.z) + probe_m<MODE, MAXS>(st, S, P.ns, v1.w);
       }
-      if (i < n4) {
+      for (; i < n4; i += G) {
         const uint4 v0 = q[i];
         h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w);
@@ -451,7 +476,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
 
 // One prefix, processed by a group: the whole 4-wave workgroup (BLOCK) or one
 // wave.  tab/flt/sv are the group's LDS; own is the calling wave's window.
-// dsc0/dsc1: the prefix's descriptor (A.desc) when the plan has one S set.
+// dsc0/dsc1: the prefix's descriptor (A.desc, always present for single-S plans).
 template <bool BLOCK, int MAXS, int DEEP>
 __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const TunedArgs &A, uint32_t p, const uint4 dsc0,
                                  const uint4 dsc1, uint32_t *tab, uint32_t tab_slots,
@@ -470,7 +495,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   const int NS = (MAXS <= 2) ? MAXS : P.ns;
   DSet S[MAXS];
   DSet Dd;
-  const bool use_desc = MAXS == 1 && A.desc;
+  constexpr bool use_desc = MAXS == 1;  // tuned_run gives single-S plans descriptors
   if (use_desc) {
     S[0].v = A.ts[0].vals + dsc0.x;
     S[0].n = dsc0.y;
@@ -523,10 +548,11 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   XList first = {0u, 0u, 0u, 0u};
   const uint32_t wave_x0 = wave * x_width<BLOCK>();
   const bool x_in_d = P.nd || use_desc;  // E(p) is Dd
+  const uint4 *xl = (use_desc && A.xlink) ? A.xlink + dsc0.z : nullptr;
   if (MAXS == 1) {
     const uint32_t *xs = x_in_d ? Dd.v : src;
     const uint32_t xn1 = x_in_d ? Dd.n : sn;
-    first = x_list<BLOCK, MAXS>(P, A, S, xs, xn1, wave_x0, false, 0, big_deg, gap);
+    first = x_list<BLOCK, MAXS>(P, A, S, xs, xn1, wave_x0, false, 0, big_deg, gap, xl);
   }
   // staging decision (group-uniform)
   Stage st;
@@ -596,22 +622,31 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   const uint32_t *xsrc = x_in_d ? Dd.v : (src ? src : Smi.v);
   const uint32_t xn = x_in_d ? Dd.n : (src ? sn : Smi.n);
   const bool filter_s = !P.nd && !src;  // x runs over one S set: test it against the others
-  if (MAXS > 1) first = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, wave_x0, filter_s, mi, big_deg, gap);
+  if (MAXS > 1) first = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, wave_x0, filter_s, mi, big_deg, gap, nullptr);
   uint64_t hits;
   // one streaming loop per staging mode: no per-probe branch on the mode
   if (st.mode == 0)
     hits = stream_children<0, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
-                                                       first);
+                                                       first, xl);
   else if (st.mode == 1)
     hits = stream_children<1, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
-                                                       first);
+                                                       first, xl);
   else
     hits = stream_children<2, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
-                                                       first);
+                                                       first, xl);
   gsync();
   return hits;
 }
 
+// A wave-uniform uint4 moved to scalar registers.
+__device__ __forceinline__ uint4 uniform4(const uint4 v) {
+  return make_uint4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                    __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
+
+// Single-S plans ask for 6 wavefronts per SIMD: 80 VGPRs, with some spilled to
+// scratch, which still beats 4 or 5 wavefronts with every value in registers (the
+// level is bound by memory latency, and more resident waves hide more of it).
 template <int SLOTS, int MAXS, int DEEP>
 __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
                                                unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
@@ -631,15 +666,15 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, T
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(&ctl[129], 1u);
     __syncthreads();
-    const uint32_t qi = s_item;
+    const uint32_t qi = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
     if (qi >= nbig) break;
     const uint64_t t0 = A.rec ? __builtin_amdgcn_s_memtime() : 0;
     int mode = -1;
     uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
-    if (MAXS == 1 && A.desc) {
-      d0 = A.desc[2ull * qi];
-      d1 = A.desc[2ull * qi + 1];
+    if (MAXS == 1) {
+      d0 = uniform4(A.desc[2ull * qi]);
+      d1 = uniform4(A.desc[2ull * qi + 1]);
     }
     const uint64_t h = tuned_prefix<true, MAXS, DEEP>(P, A, queue[qi], d0, d1, tab, SLOTS, flt, FWB, sv, 4 * VC, own[w],
                                                       big_deg, bitmap_all, slg, gap, mode);
@@ -655,18 +690,18 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, T
   for (;;) {
     uint32_t q0 = 0;
     if (__lane_id() == 0) q0 = atomicAdd(&ctl[130], (unsigned)grab);
-    q0 = __shfl(q0, 0, 64) + nbig;
+    q0 = __builtin_amdgcn_readfirstlane(q0) + nbig;  // lane 0's: the whole wave is active here
     if (q0 >= qlen) break;
     const uint32_t q1 = min(q0 + (uint32_t)grab, qlen);
     // the next entry's descriptor is loaded while this one runs
     uint4 n0 = make_uint4(0, 0, 0, 0), n1 = n0;
-    if (MAXS == 1 && A.desc) {
+    if (MAXS == 1) {
       n0 = A.desc[2ull * q0];
       n1 = A.desc[2ull * q0 + 1];
     }
     for (uint32_t qi = q0; qi < q1; qi++) {
-      const uint4 d0 = n0, d1 = n1;
-      if (MAXS == 1 && A.desc && qi + 1 < q1) {
+      const uint4 d0 = uniform4(n0), d1 = uniform4(n1);
+      if (MAXS == 1 && qi + 1 < q1) {
         n0 = A.desc[2ull * (qi + 1)];
         n1 = A.desc[2ull * (qi + 1) + 1];
       }
@@ -694,15 +729,16 @@ static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, cons
   const int gap = (tp.gap_coded && A.tx.pk) ? 1 : 0;
   // four quads in flight per lane pay off for narrow lane groups (many short lists); wider
   // groups keep two, and the registers they would cost
+#define EHX_TUNED_ARGS P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg, gap
   if (P.ns == 1 && slg <= 2)
-    k_tuned<SLOTS, 1, 1><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg, gap);
+    k_tuned<SLOTS, 1, 1><<<grid, 256, 0, st>>>(EHX_TUNED_ARGS);
   else if (P.ns == 1)
-    k_tuned<SLOTS, 1, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg, gap);
+    k_tuned<SLOTS, 1, 0><<<grid, 256, 0, st>>>(EHX_TUNED_ARGS);
   else if (P.ns == 2)
-    k_tuned<SLOTS, 2, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all, slg, gap);
+    k_tuned<SLOTS, 2, 0><<<grid, 256, 0, st>>>(EHX_TUNED_ARGS);
   else
-    k_tuned<SLOTS, TUNED_MAX_S, 0><<<grid, 256, 0, st>>>(P, A, queue, ctl, acc, tp.big_deg, tp.grab, tp.bitmap_all,
-                                                         slg, gap);
+    k_tuned<SLOTS, TUNED_MAX_S, 0><<<grid, 256, 0, st>>>(EHX_TUNED_ARGS);
+#undef EHX_TUNED_ARGS
 }
 
 // One cached graph per process: the key is every value baked into the launches.
