@@ -397,6 +397,8 @@ def main():
         assert gc == count, "graph replay disagrees"
         log("learned_by_middle, hipGraph replay: %.3f ms/step (graph %s)" % (gms, "used" if graph_used else "refused"))
 
+    TU.apply_plan(MID)
+    headline_knobs = E.tune_get()
     achieved = balg / (kms * 1e-3)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_k_tuned_rmat22.json")
@@ -437,6 +439,7 @@ def main():
         "last_level_ms_by_form": {k: r[2] for k, r in res.items()},
         "graph_replay_ms_per_step": gms if graph_used else None,
         "tuning": tuning,
+        "knobs_of_headline_plan": headline_knobs,
     }
     if "plain_reference_order" in res:
         out["plain_ms_per_step"] = res["plain_reference_order"][1]

@@ -404,7 +404,9 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       const uint32_t sko = __shfl(ko, src, 64), skn = __shfl(kn, src, 64);
       const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
       uint32_t h32 = 0;  // this round's hits of the lane (< 2^32)
-      if (myl >= 0 && skn) {  // gap-coded: chunks of PK_VALS values, one per lane per step
+      // gap-coded: chunks of PK_VALS values, one per lane per step (single-S plans only:
+      // the 4-clique's kernel keeps its registers for the materialized S(p))
+      if (MAXS == 1 && myl >= 0 && skn) {
         const uint4 *q = tx.pk + 2ull * sko;
         const uint32_t nfull = bd / PK_VALS;
         uint32_t c = ql;
@@ -647,8 +649,10 @@ __device__ __forceinline__ uint4 uniform4(const uint4 v) {
 // Single-S plans ask for 6 wavefronts per SIMD: 80 VGPRs, with some spilled to
 // scratch, which still beats 4 or 5 wavefronts with every value in registers (the
 // level is bound by memory latency, and more resident waves hide more of it).
+// Two-S plans (4-clique) ask for 5 (96 VGPRs): left free, the allocator takes
+// 129 and three waves, 40 % slower.
 template <int SLOTS, int MAXS, int DEEP>
-__global__ __launch_bounds__(256, MAXS == 1 ? 6 : 2) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
+__global__ __launch_bounds__(256, MAXS == 1 ? 6 : (MAXS == 2 ? 5 : 2)) void k_tuned(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
                                                unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc,
                                                int big_deg, int grab, int bitmap_all, uint32_t slg, int gap) {
   constexpr uint32_t WS = SLOTS / 4;                 // per-wave table slots
