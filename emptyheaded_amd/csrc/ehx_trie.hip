@@ -118,28 +118,38 @@ __global__ __launch_bounds__(256) void k_fill_words(DTrie t, uint64_t nwords, ui
 // Gap-coded lists: a child list of at least PK_MIN values whose consecutive
 // gaps all fit 16 bits is copied into ceil(n / PK_VALS) 32-byte chunks; chunk
 // c holds value c * PK_VALS as a uint32, then the gaps to the next 14 values
-// (zeros past the end).  One wavefront per list.
+// (zeros past the end).  A group of PK_GROUP lanes per list (most lists are short).
+constexpr uint32_t PK_GROUP = 8;
+
 __global__ __launch_bounds__(256) void k_pk_count(DTrie t, uint32_t *__restrict__ cnt) {
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < t.nh; h += nwaves) {
-    const uint32_t a = t.off[h], n = t.off[h + 1] - a;
-    uint32_t mx = 0;
-    if (n >= PK_MIN)
-      for (uint32_t k = 1 + __lane_id(); k < n; k += 64) mx = max(mx, t.vals[a + k] - t.vals[a + k - 1]);
+  const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / PK_GROUP;
+  const uint32_t gl = threadIdx.x % PK_GROUP;
+  // every lane runs the same number of outer steps, so the group shuffles see all lanes
+  const uint64_t steps = (t.nh + ngroups - 1) / ngroups;
+  const uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / PK_GROUP;
+  for (uint64_t s = 0; s < steps; s++) {
+    const uint64_t h = g0 + s * ngroups;
+    uint32_t mx = 0, n = 0;
+    if (h < t.nh) {
+      const uint32_t a = t.off[h];
+      n = t.off[h + 1] - a;
+      if (n >= PK_MIN)
+        for (uint32_t k = 1 + gl; k < n; k += PK_GROUP) mx = max(mx, t.vals[a + k] - t.vals[a + k - 1]);
+    }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
-    if (__lane_id() == 0) cnt[h] = (n >= PK_MIN && mx < 65536u) ? (n + PK_VALS - 1) / PK_VALS : 0u;
+    for (uint32_t d = PK_GROUP / 2; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, PK_GROUP));
+    if (h < t.nh && gl == 0) cnt[h] = (n >= PK_MIN && mx < 65536u) ? (n + PK_VALS - 1) / PK_VALS : 0u;
   }
 }
 
 __global__ __launch_bounds__(256) void k_pk_fill(DTrie t, const uint32_t *__restrict__ pk_off,
                                                  uint4 *__restrict__ pk) {
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < t.nh; h += nwaves) {
+  const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / PK_GROUP;
+  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / PK_GROUP; h < t.nh; h += ngroups) {
     const uint32_t o = pk_off[h], nc = pk_off[h + 1] - o;
     if (!nc) continue;
     const uint32_t a = t.off[h], n = t.off[h + 1] - a;
-    for (uint32_t c = __lane_id(); c < nc; c += 64) {
+    for (uint32_t c = threadIdx.x % PK_GROUP; c < nc; c += PK_GROUP) {
       const uint32_t k0 = c * PK_VALS;
       uint32_t w[8];
       w[0] = t.vals[a + k0];
@@ -342,7 +352,7 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
     hipError_t e1 = hipMalloc((void **)&t->pk_off, (nh + 1) * 4);
     if (e1 != hipSuccess) { trie_destroy(t); return fail(EHX_ERR_NOMEM, "chunk offsets: %s", hipGetErrorString(e1)); }
     if ((rc = sc.ensure((nh + 1) * 4))) { trie_destroy(t); return rc; }
-    k_pk_count<<<grid_for(nh * 64, 256, 16384), 256, 0, st>>>(dv, sc.as<uint32_t>());
+    k_pk_count<<<grid_for(nh * PK_GROUP, 256, 16384), 256, 0, st>>>(dv, sc.as<uint32_t>());
     EHX_HIP(hipGetLastError());
     if ((rc = scan_u32(ctx, sc.as<uint32_t>(), t->pk_off, nh))) { trie_destroy(t); return rc; }
     uint32_t npk = 0;  // at most m / 15 + m / 16 chunks: fits 32 bits
@@ -352,7 +362,7 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
     if (e1 != hipSuccess) { trie_destroy(t); return fail(EHX_ERR_NOMEM, "coded lists: %s", hipGetErrorString(e1)); }
     dv = view(t);
     if (npk) {
-      k_pk_fill<<<grid_for(nh * 64, 256, 16384), 256, 0, st>>>(dv, t->pk_off, t->pk);
+      k_pk_fill<<<grid_for(nh * PK_GROUP, 256, 16384), 256, 0, st>>>(dv, t->pk_off, t->pk);
       EHX_HIP(hipGetLastError());
     }
   }
