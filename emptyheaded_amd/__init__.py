@@ -56,7 +56,8 @@ class JoinStats(C.Structure):
 class TuneParams(C.Structure):
     _fields_ = [("lds_slots", C.c_int32), ("big_deg", C.c_int32), ("waves_per_cu", C.c_int32),
                 ("grab", C.c_int32), ("bitmap_all", C.c_int32), ("use_graph", C.c_int32), ("big_work", C.c_int32),
-                ("stream_lanes", C.c_int32), ("gap_coded", C.c_int32), ("split_work", C.c_int32)]
+                ("stream_lanes", C.c_int32), ("gap_coded", C.c_int32), ("split_work", C.c_int32),
+                ("tiny_work", C.c_int32)]
 
     def as_dict(self):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

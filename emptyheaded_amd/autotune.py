@@ -23,6 +23,7 @@ GRID = {
     "stream_lanes": [4, 8, 16, 32, 64],
     "gap_coded": [0, 1],
     "split_work": [0, 1 << 19, 1 << 20, 1 << 21, 1 << 22],
+    "tiny_work": [0, 256, 1024, 4096, 16384],
     "bitmap_all": [0, 1],
     "use_graph": [0, 1],
 }

@@ -246,6 +246,7 @@ int ehx_tune_get(ehx_tune_params *out) {
   out->stream_lanes = p.stream_lanes;
   out->gap_coded = p.gap_coded;
   out->split_work = p.split_work;
+  out->tiny_work = p.tiny_work;
   return EHX_OK;
 }
 
@@ -254,7 +255,7 @@ int ehx_tune_set(const ehx_tune_params *in) {
   if (in->lds_slots != 2048 && in->lds_slots != 4096 && in->lds_slots != 8192)
     return fail(EHX_ERR_INVALID, "lds_slots must be 2048, 4096 or 8192");
   if (in->big_deg < 1 || in->waves_per_cu < 1 || in->waves_per_cu > 64 || in->grab < 1 || in->grab > 64 ||
-      in->big_work < 1 || in->split_work < 0)
+      in->big_work < 1 || in->split_work < 0 || in->tiny_work < 0)
     return fail(EHX_ERR_INVALID, "tune parameter out of range");
   // >= 4: a group reads the up-to-3-element unaligned head and tail of a set one element per lane
   if (in->stream_lanes < 4 || in->stream_lanes > 64 || (in->stream_lanes & (in->stream_lanes - 1)))
@@ -270,6 +271,7 @@ int ehx_tune_set(const ehx_tune_params *in) {
   p.stream_lanes = in->stream_lanes;
   p.gap_coded = in->gap_coded ? 1 : 0;
   p.split_work = in->split_work;
+  p.tiny_work = in->tiny_work;
   return EHX_OK;
 }
 

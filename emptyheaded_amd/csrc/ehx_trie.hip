@@ -117,61 +117,40 @@ __global__ __launch_bounds__(256) void k_fill_words(DTrie t, uint64_t nwords, ui
 
 // Gap-coded lists: a child list of at least PK_MIN values whose consecutive
 // gaps all fit 16 bits is copied into ceil(n / PK_VALS) 32-byte chunks; chunk
-// c holds value c * PK_VALS as a uint32, then the gaps to the next 14 values
-// (zeros past the end).  A group of PK_GROUP lanes per list (most lists are short).
-constexpr uint32_t PK_GROUP = 8;
+// c holds value c * PK_VALS as a uint32, then the 16-bit gaps to the next 14
+// values (zeros past the end).  One lane per value (the split's run flags and
+// list numbers say where a value's list starts and which list it is in), so
+// hub lists of 10^5 values cost no more than many short ones.
 
-__global__ __launch_bounds__(256) void k_pk_count(DTrie t, uint32_t *__restrict__ cnt) {
-  const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / PK_GROUP;
-  const uint32_t gl = threadIdx.x % PK_GROUP;
-  // every lane runs the same number of outer steps, so the group shuffles see all lanes
-  const uint64_t steps = (t.nh + ngroups - 1) / ngroups;
-  const uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / PK_GROUP;
-  for (uint64_t s = 0; s < steps; s++) {
-    const uint64_t h = g0 + s * ngroups;
-    uint32_t mx = 0, n = 0;
-    if (h < t.nh) {
-      const uint32_t a = t.off[h];
-      n = t.off[h + 1] - a;
-      if (n >= PK_MIN)
-        for (uint32_t k = 1 + gl; k < n; k += PK_GROUP) mx = max(mx, t.vals[a + k] - t.vals[a + k - 1]);
-    }
-#pragma unroll
-    for (uint32_t d = PK_GROUP / 2; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, PK_GROUP));
-    if (h < t.nh && gl == 0) cnt[h] = (n >= PK_MIN && mx < 65536u) ? (n + PK_VALS - 1) / PK_VALS : 0u;
+// wide[h] = 1 when list h has a gap of 2^16 or more.  list(k) = pos[k + 1] - 1.
+__global__ void k_pk_wide(const uint32_t *__restrict__ vals, const uint32_t *__restrict__ starts,
+                          const uint64_t *__restrict__ pos, uint64_t m, uint8_t *__restrict__ wide) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x)
+    if (!starts[k] && vals[k] - vals[k - 1] >= 65536u) wide[pos[k + 1] - 1] = 1;
+}
+
+__global__ void k_pk_count(DTrie t, const uint8_t *__restrict__ wide, uint32_t *__restrict__ cnt) {
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < t.nh; h += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t n = t.off[h + 1] - t.off[h];
+    cnt[h] = (n >= PK_MIN && !wide[h]) ? (n + PK_VALS - 1) / PK_VALS : 0u;
   }
 }
 
-__global__ __launch_bounds__(256) void k_pk_fill(DTrie t, const uint32_t *__restrict__ pk_off,
-                                                 uint4 *__restrict__ pk) {
-  const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) / PK_GROUP;
-  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / PK_GROUP; h < t.nh; h += ngroups) {
-    const uint32_t o = pk_off[h], nc = pk_off[h + 1] - o;
-    if (!nc) continue;
-    const uint32_t a = t.off[h], n = t.off[h + 1] - a;
-    for (uint32_t c = threadIdx.x % PK_GROUP; c < nc; c += PK_GROUP) {
-      const uint32_t k0 = c * PK_VALS;
-      uint32_t w[8];
-      w[0] = t.vals[a + k0];
-      uint32_t prev = w[0];
-#pragma unroll
-      for (int j = 1; j < 8; j++) {
-        uint32_t g[2];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const uint32_t k = k0 + 2 * j - 1 + u;  // value k is reached by gap 2j - 2 + u
-          g[u] = 0;
-          if (k < n) {
-            const uint32_t v = t.vals[a + k];
-            g[u] = v - prev;
-            prev = v;
-          }
-        }
-        w[j] = g[0] | (g[1] << 16);
-      }
-      uint4 *q = pk + 2ull * (o + c);
-      q[0] = make_uint4(w[0], w[1], w[2], w[3]);
-      q[1] = make_uint4(w[4], w[5], w[6], w[7]);
+// pk16 is the chunk pool as 16-bit words (zeroed first): 16 per chunk, the
+// first value in words 0-1, gap j (j = 1..14) in word j + 1.
+__global__ void k_pk_fill(DTrie t, uint64_t m, const uint64_t *__restrict__ pos, uint16_t *__restrict__ pk16) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t h = pos[k + 1] - 1;
+    const uint32_t c0 = t.pk_off[h];
+    if (t.pk_off[h + 1] == c0) continue;
+    const uint32_t r = (uint32_t)(k - t.off[h]), j = r % PK_VALS;
+    uint16_t *q = pk16 + 16ull * (c0 + r / PK_VALS);
+    if (j == 0) {
+      const uint32_t v = t.vals[k];
+      q[0] = (uint16_t)v;
+      q[1] = (uint16_t)(v >> 16);
+    } else {
+      q[j + 1] = (uint16_t)(t.vals[k] - t.vals[k - 1]);
     }
   }
 }
@@ -348,11 +327,15 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   }
   // gap-coded copy of the long child lists
   {
-    DevBuf &sc = ctx->scratch[S_COUNTS];
+    DevBuf &sc = ctx->scratch[S_COUNTS], &wb = ctx->scratch[S_OFFS];
     hipError_t e1 = hipMalloc((void **)&t->pk_off, (nh + 1) * 4);
     if (e1 != hipSuccess) { trie_destroy(t); return fail(EHX_ERR_NOMEM, "chunk offsets: %s", hipGetErrorString(e1)); }
-    if ((rc = sc.ensure((nh + 1) * 4))) { trie_destroy(t); return rc; }
-    k_pk_count<<<grid_for(nh * PK_GROUP, 256, 16384), 256, 0, st>>>(dv, sc.as<uint32_t>());
+    if ((rc = sc.ensure((nh + 1) * 4)) || (rc = wb.ensure(nh + 4))) { trie_destroy(t); return rc; }
+    uint8_t *wide = wb.as<uint8_t>();
+    if ((rc = fill_u32(ctx, (uint32_t *)wide, 0u, (nh + 3) / 4))) { trie_destroy(t); return rc; }
+    k_pk_wide<<<grid_for(m, 256, 16384), 256, 0, st>>>(t->vals, flags, pos, m, wide);
+    EHX_HIP(hipGetLastError());
+    k_pk_count<<<grid_for(nh, 256, 16384), 256, 0, st>>>(dv, wide, sc.as<uint32_t>());
     EHX_HIP(hipGetLastError());
     if ((rc = scan_u32(ctx, sc.as<uint32_t>(), t->pk_off, nh))) { trie_destroy(t); return rc; }
     uint32_t npk = 0;  // at most m / 15 + m / 16 chunks: fits 32 bits
@@ -362,7 +345,8 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
     if (e1 != hipSuccess) { trie_destroy(t); return fail(EHX_ERR_NOMEM, "coded lists: %s", hipGetErrorString(e1)); }
     dv = view(t);
     if (npk) {
-      k_pk_fill<<<grid_for(nh * PK_GROUP, 256, 16384), 256, 0, st>>>(dv, t->pk_off, t->pk);
+      if ((rc = fill_u32(ctx, (uint32_t *)t->pk, 0u, (uint64_t)npk * 8))) { trie_destroy(t); return rc; }
+      k_pk_fill<<<grid_for(m, 256, 16384), 256, 0, st>>>(dv, m, pos, (uint16_t *)t->pk);
       EHX_HIP(hipGetLastError());
     }
   }

@@ -71,6 +71,7 @@ struct TuneParams {
   int32_t stream_lanes;   // lanes that stream one long child set together (a power of two, 4..64)
   int32_t gap_coded;      // 1: stream the gap-coded copy of a child list where the trie has one (single-S plans)
   int32_t split_work;     // workgroup-bin prefixes are split into parts of about this much work (0: never)
+  int32_t tiny_work;      // single-S prefixes below this estimated work run up to 64 per wavefront (0: never)
 };
 
 // A request to time every prefix of the tuned level under a forced staging.

@@ -26,7 +26,7 @@
 
 namespace ehx {
 
-static TuneParams g_params = {4096, 32, 24, 2, 0, 0, 4096, 16, 0, 1 << 21};
+static TuneParams g_params = {4096, 32, 24, 2, 0, 0, 4096, 16, 0, 1 << 21, 0};
 TuneParams &tune_params() { return g_params; }
 
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
@@ -74,6 +74,12 @@ static constexpr uint32_t NBUCKETS = 64, BUCKET_NONE = 63, WAVE_BIN = 32;
 // parts of about split_work each, over disjoint ranges of E(p) (each part
 // stages S(p) itself); at most SPLIT_EXTRA queue entries are added per level.
 static constexpr uint32_t SPLIT_MAX = 64, SPLIT_EXTRA = 1u << 16;
+// Single-S prefixes whose estimated work is below tiny_work and whose S(p) holds
+// at most TINY_S values go to the tiny bin (bucket TINY_BUCKET, the end of the
+// queue), which k_tiny runs up to 64 prefixes per wavefront at a time.
+static constexpr uint32_t TINY_BUCKET = 62, TINY_S = 128;
+static constexpr uint32_t TINY_U = 4;       // elements per lane per step in k_tiny
+static constexpr uint32_t TINY_BATCH = 32;  // queue entries a wavefront takes at once (<= 64)
 static constexpr int OWN = 256;  // owner window per wave
 
 void tuned_bind(const TunedPlan &P, const Tries &T, TunedArgs &A) {
@@ -134,7 +140,8 @@ __host__ __device__ constexpr uint32_t wave_hash_cap(int slots) { return (uint32
 
 // bucket[p] = bin | parts << 6; hist[b] counts queue entries (parts)
 __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ bucket, unsigned *__restrict__ hist,
-                           int slots, int big_work, uint4 *__restrict__ pdesc, uint32_t split_work) {
+                           int slots, int big_work, uint4 *__restrict__ pdesc, uint32_t split_work,
+                           uint32_t tiny_work) {
   __shared__ unsigned h[NBUCKETS];
   if (threadIdx.x < NBUCKETS) h[threadIdx.x] = 0;
   __syncthreads();
@@ -167,6 +174,8 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
       const bool big = w >= (uint64_t)big_work || !fits_wave;
       uint32_t lg = 63 - __clzll(w | 1);
       b = (big ? 0u : WAVE_BIN) + (31 - (lg > 31 ? 31 : lg));
+      if (!big && b > TINY_BUCKET - 1) b = TINY_BUCKET - 1;  // the last wave bucket holds w < 4
+      if (pdesc && !big && !A.rec && w < tiny_work && D.n <= TINY_S && D.v[D.n - 1] != EMPTY) b = TINY_BUCKET;
       if (pdesc) {  // single-S plans (S[0] = D here); the third and fourth words locate E(p)
         pdesc[2 * p] = make_uint4((uint32_t)(D.v - A.ts[0].vals), D.n,
                                   (uint32_t)(X.v - (P.nd ? A.td.vals : A.ts[0].vals)), X.n);
@@ -192,13 +201,14 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
 
 // ctl: [0..63] hist, [64..127] bin base / cursor, [128] queue length,
 //      [129] block-bin cursor, [130] wave-bin cursor, [131] block-bin size,
-//      [132] queue entries added by splits
+//      [132] queue entries added by splits, [133] tiny-bin start
 __global__ void k_bucket_scan(unsigned *__restrict__ ctl) {
   const uint32_t lane = threadIdx.x;
   unsigned c = (lane < BUCKET_NONE) ? ctl[lane] : 0u;
   unsigned incl = wave_incl_scan(c);
   ctl[64 + lane] = incl - c;
   if (lane == WAVE_BIN - 1) ctl[131] = incl;
+  if (lane == TINY_BUCKET) ctl[133] = incl - c;  // where the tiny bin starts
   if (lane == 63) { ctl[128] = incl; ctl[129] = 0; ctl[130] = 0; }
 }
 
@@ -258,10 +268,10 @@ __device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int n
     const uint32_t f = hash_f(c, s.fshift);
     if (!((s.flt[f >> 5] >> (f & 31)) & 1u)) return 0u;
     uint32_t h = hash_t(c, s.hshift);
-    for (;;) {
+    for (;;) {  // EMPTY first: a probed 0xFFFFFFFF (never staged) must not match a free slot
       const uint32_t v = s.tab[h];
-      if (v == c) return 1u;
       if (v == EMPTY) return 0u;
+      if (v == c) return 1u;
       h = (h + 1) & s.hmask;
     }
   }
@@ -640,6 +650,226 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   return hits;
 }
 
+// ---- tiny prefixes, many per wavefront ---------------------------------------
+// Most prefixes of a skewed graph carry almost no work (on R-MAT-22, middle
+// vertex first, 85 % of them hold 1 % of the work) but took a full round of
+// staging and child lookups each in k_tuned.  Here a wavefront takes TINY_BATCH
+// tiny-bin entries at once: each prefix's S(p) becomes its own small
+// open-addressing table in the wave's LDS, and the x's of all of them, then the
+// child lists of those x's, are flattened and streamed TINY_W elements per step
+// (four consecutive elements per lane).  An element's owner (its prefix, or its
+// x) comes from an LDS owner map: each run marks its first element, and a
+// max-scan carries the marks forward, so no step waits on a chain of searches.
+__device__ __forceinline__ uint32_t tiny_hash(uint32_t c) { return (c * 0x9E3779B1u) >> 20; }
+
+constexpr uint32_t TINY_W = 256;  // flattened elements per step: 4 per lane
+
+// Owners of elements [r0, r0 + TINY_W) of consecutive runs (run ids ascend with
+// their starts; each lane holds RPL runs {start, length, id}): out[u] = id of
+// the run holding element r0 + 4 * lane + u (past the last run: the last id).
+// own is TINY_W words of the wave's LDS.
+template <int RPL>
+__device__ __forceinline__ void tiny_owners(uint32_t *own, uint32_t r0, const uint32_t (&st)[RPL],
+                                            const uint32_t (&ln)[RPL], const uint32_t (&id)[RPL],
+                                            uint32_t (&out)[4]) {
+  const uint32_t lane = __lane_id();
+  reinterpret_cast<uint4 *>(own)[lane] = make_uint4(0, 0, 0, 0);  // 0: no run starts here
+  wave_sync();
+#pragma unroll
+  for (int u = 0; u < RPL; u++) {
+    if (!ln[u]) continue;
+    if (st[u] >= r0 && st[u] - r0 < TINY_W) own[st[u] - r0] = id[u] + 1;
+    else if (st[u] < r0 && st[u] + ln[u] > r0) own[0] = id[u] + 1;  // the run that crosses r0
+  }
+  wave_sync();
+  uint4 v = reinterpret_cast<uint4 *>(own)[lane];
+  wave_sync();  // own is free again
+  v.y = max(v.y, v.x);
+  v.z = max(v.z, v.y);
+  v.w = max(v.w, v.z);
+  uint32_t incl = v.w;
+#pragma unroll
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = max(incl, y);
+  }
+  uint32_t carry = __shfl_up(incl, 1, 64);
+  if (lane == 0) carry = 0;
+  out[0] = max(v.x, carry) - 1u;
+  out[1] = max(v.y, carry) - 1u;
+  out[2] = max(v.z, carry) - 1u;
+  out[3] = max(v.w, carry) - 1u;
+}
+
+// x's child list {off, length}: from E(p)'s child pointers when there are any,
+// else by value.
+__device__ __forceinline__ uint2 tiny_child(const TunedPlan &P, const TunedArgs &A, const uint32_t *xvals,
+                                            uint32_t pos) {
+  if (A.xlink && A.nh == 0) {
+    const uint4 nv = A.xlink[pos];
+    return make_uint2(nv.x, nv.y);
+  }
+  const uint32_t x = xvals[pos];
+  if (!head_ok(P, A, x)) return make_uint2(0, 0);
+  const DTrie &tx = A.tx;
+  if (tx.nav) {
+    if (x >= tx.nav_len) return make_uint2(0, 0);
+    const uint4 nv = tx.nav[x];
+    return make_uint2(nv.x, nv.y);
+  }
+  const int32_t nx = node_of(tx, x);
+  if (nx < 0) return make_uint2(0, 0);
+  const uint32_t o = tx.off[nx];
+  return make_uint2(o, tx.off[nx + 1] - o);
+}
+
+template <int SLOTS>
+__global__ __launch_bounds__(256) void k_tiny(TunedPlan P, TunedArgs A, const uint32_t *__restrict__ queue,
+                                              unsigned *__restrict__ ctl, unsigned long long *__restrict__ acc) {
+  constexpr uint32_t WS = SLOTS / 4;  // table words per wave (>= 512: a 256-slot table always fits)
+  __shared__ uint32_t tab_all[4][WS];
+  __shared__ __attribute__((aligned(16))) uint32_t own_all[4][TINY_W];  // read as uint4
+  // per prefix of the batch: S offset, first staged element, table base, table
+  // mask, E offset, first x, hits; per x of a step: list offset, first element, prefix
+  __shared__ uint32_t meta_all[4][7][64];
+  __shared__ uint32_t run_all[4][3][TINY_W];
+  const uint32_t w = threadIdx.x >> 6, lane = __lane_id();
+  uint32_t *tab = tab_all[w], *own = own_all[w];
+  uint32_t *mSo = meta_all[w][0], *mSs = meta_all[w][1], *mTb = meta_all[w][2], *mMk = meta_all[w][3];
+  uint32_t *mEo = meta_all[w][4], *mXs = meta_all[w][5], *mHit = meta_all[w][6];
+  uint32_t *cO = run_all[w][0], *cS = run_all[w][1], *cK = run_all[w][2];
+  const uint32_t qt = ctl[133], qlen = ctl[128];
+  const bool segs = A.seg.out != nullptr;
+  const uint32_t *svals = A.ts[0].vals, *xvals = P.nd ? A.td.vals : A.ts[0].vals, *cvals = A.tx.vals;
+  uint64_t hits = 0;
+  // batches are alike, so they are dealt out by wave number (one shared atomic
+  // cursor, hit by every wave at once when the kernel starts, cost more than the
+  // tiny bin's work)
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t bi = blockIdx.x * (blockDim.x >> 6) + w;; bi += nwaves) {
+    const uint64_t q064 = (uint64_t)qt + (uint64_t)bi * TINY_BATCH;
+    if (q064 >= qlen) break;
+    const uint32_t q0 = (uint32_t)q064;
+    const uint32_t nb = min(TINY_BATCH, qlen - q0);
+    uint4 d0 = make_uint4(0, 0, 0, 0);
+    uint32_t p = 0;
+    if (lane < nb) {
+      d0 = A.desc[2ull * (q0 + lane)];  // {S offset, |S|, E offset, |E|}
+      p = queue[q0 + lane];
+    }
+    uint32_t sz = 0;  // this prefix's table: a power of two >= 4/3 |S| (load <= 3/4)
+    if (lane < nb) {
+      sz = 4;
+      while (3 * sz < 4 * d0.y) sz <<= 1;
+    }
+    for (uint32_t start = 0; start < nb;) {
+      // the batch's prefixes [start, end) whose tables fit the wave's LDS together
+      const uint32_t mine = (lane >= start) ? sz : 0u;
+      const uint32_t tincl = wave_incl_scan(mine);
+      const bool in = lane >= start && lane < nb && tincl <= WS;
+      const uint32_t end = start + (uint32_t)__popcll(__ballot(in));
+      const uint32_t tsum = __shfl(tincl, (int)end - 1, 64);
+      const uint32_t sn = in ? d0.y : 0u, xn = in ? d0.w : 0u;
+      const uint32_t sincl = wave_incl_scan(sn), xincl = wave_incl_scan(xn);
+      const uint32_t stot = __shfl(sincl, 63, 64), xtot = __shfl(xincl, 63, 64);
+      mSo[lane] = d0.x;
+      mSs[lane] = sincl - sn;
+      mTb[lane] = tincl - mine;
+      mMk[lane] = sz - 1;
+      mEo[lane] = d0.z;
+      mXs[lane] = xincl - xn;
+      mHit[lane] = 0;
+      for (uint32_t i = lane; i < tsum; i += 64) tab[i] = EMPTY;
+      wave_sync();
+      // stage every S(p) of the sub-batch in its own table
+      const uint32_t sst[1] = {sincl - sn}, sln[1] = {sn}, sid[1] = {lane};
+      for (uint32_t r0 = 0; r0 < stot; r0 += TINY_W) {
+        uint32_t k[4], c[4];
+        tiny_owners<1>(own, r0, sst, sln, sid, k);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t e = r0 + 4 * lane + u;
+          if (e < stot) c[u] = svals[mSo[k[u]] + e - mSs[k[u]]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t e = r0 + 4 * lane + u;
+          if (e >= stot) continue;
+          const uint32_t base = mTb[k[u]], mask = mMk[k[u]];
+          uint32_t h = tiny_hash(c[u]) & mask;
+          while (atomicCAS(&tab[base + h], EMPTY, c[u]) != EMPTY) h = (h + 1) & mask;
+        }
+      }
+      wave_sync();
+      // the sub-batch's x's, TINY_W per step, then their child lists' elements
+      uint32_t h32 = 0;
+      const uint32_t xst[1] = {xincl - xn}, xln[1] = {xn}, xid[1] = {lane};
+      for (uint32_t x0 = 0; x0 < xtot; x0 += TINY_W) {
+        uint32_t k[4], o[4], dl[4];
+        tiny_owners<1>(own, x0, xst, xln, xid, k);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const uint32_t t = x0 + 4 * lane + u;
+          o[u] = dl[u] = 0;
+          if (t < xtot) {
+            const uint2 cl = tiny_child(P, A, xvals, mEo[k[u]] + t - mXs[k[u]]);
+            o[u] = cl.x;
+            dl[u] = cl.y;
+          }
+        }
+        // runs of the step's x's: run 4 * lane + u, in x order
+        const uint32_t l4 = dl[0] + dl[1] + dl[2] + dl[3];
+        const uint32_t lincl = wave_incl_scan(l4);
+        const uint32_t ctot = __shfl(lincl, 63, 64);
+        uint32_t cst[4], cid[4];
+        cst[0] = lincl - l4;
+        cst[1] = cst[0] + dl[0];
+        cst[2] = cst[1] + dl[1];
+        cst[3] = cst[2] + dl[2];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          cid[u] = 4 * lane + u;
+          cO[cid[u]] = o[u];
+          cS[cid[u]] = cst[u];
+          cK[cid[u]] = k[u];
+        }
+        wave_sync();
+        for (uint32_t e0 = 0; e0 < ctot; e0 += TINY_W) {
+          uint32_t r[4], c[4];
+          tiny_owners<4>(own, e0, cst, dl, cid, r);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const uint32_t e = e0 + 4 * lane + u;
+            if (e < ctot) c[u] = cvals[cO[r[u]] + e - cS[r[u]]];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const uint32_t e = e0 + 4 * lane + u;
+            if (e >= ctot) continue;
+            const uint32_t kk = cK[r[u]], base = mTb[kk], mask = mMk[kk];
+            uint32_t h = tiny_hash(c[u]) & mask, hit = 0;
+            for (;;) {  // EMPTY first: 0xFFFFFFFF is never staged
+              const uint32_t v = tab[base + h];
+              if (v == EMPTY) break;
+              if (v == c[u]) { hit = 1; break; }
+              h = (h + 1) & mask;
+            }
+            h32 += hit;
+            if (segs && hit) atomicAdd(&mHit[kk], 1u);
+          }
+        }
+        wave_sync();  // cO, cS, cK are the next step's
+      }
+      hits += h32;
+      if (segs && in && mHit[lane]) seg_add(A.seg, A.prefix[0][p], mHit[lane]);
+      wave_sync();  // the tables and meta are the next sub-batch's
+      start = end;
+    }
+  }
+  hits = wave_sum(hits);
+  if (lane == 0) atomicAdd(acc, (unsigned long long)hits);
+}
+
 // A wave-uniform uint4 moved to scalar registers.
 __device__ __forceinline__ uint4 uniform4(const uint4 v) {
   return make_uint4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
@@ -664,11 +894,13 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : (MAXS == 2 ? 5 : 2)) void k_tu
   __shared__ uint32_t sv[4 * VC];
   __shared__ uint32_t s_item;
   const uint32_t w = threadIdx.x >> 6;
-  const uint32_t qlen = ctl[128], nbig = ctl[131];
+  const uint32_t qlen = ctl[133], nbig = ctl[131];  // the tiny bin after qlen is k_tiny's
   uint64_t hits = 0;
-  // phase 1: block bin, one prefix per workgroup
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(&ctl[129], 1u);
+  // phase 1: block bin, one prefix per workgroup.  Each workgroup's first item
+  // and each wave's first grab are dealt out by number: a shared atomic cursor
+  // hit by every workgroup at once serializes the kernel's start.
+  for (bool first = true;; first = false) {
+    if (threadIdx.x == 0) s_item = first ? blockIdx.x : gridDim.x + atomicAdd(&ctl[129], 1u);
     __syncthreads();
     const uint32_t qi = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
@@ -691,11 +923,18 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : (MAXS == 2 ? 5 : 2)) void k_tu
       A.rec[queue[qi]] = (__builtin_amdgcn_s_memtime() - t0) | ((unsigned long long)mode << 62);
   }
   // phase 2: wave bin, each wave on its own
-  for (;;) {
-    uint32_t q0 = 0;
-    if (__lane_id() == 0) q0 = atomicAdd(&ctl[130], (unsigned)grab);
-    q0 = __builtin_amdgcn_readfirstlane(q0) + nbig;  // lane 0's: the whole wave is active here
-    if (q0 >= qlen) break;
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6), wid = blockIdx.x * (blockDim.x >> 6) + w;
+  for (bool first = true;; first = false) {
+    uint64_t q064 = nbig;
+    if (first) {
+      q064 += (uint64_t)wid * (uint32_t)grab;
+    } else {
+      uint32_t q0 = 0;
+      if (__lane_id() == 0) q0 = atomicAdd(&ctl[130], (unsigned)grab);
+      q064 += (uint64_t)nwaves * (uint32_t)grab + __builtin_amdgcn_readfirstlane(q0);  // lane 0's: all lanes are active
+    }
+    if (q064 >= qlen) break;
+    const uint32_t q0 = (uint32_t)q064;
     const uint32_t q1 = min(q0 + (uint32_t)grab, qlen);
     // the next entry's descriptor is loaded while this one runs
     uint4 n0 = make_uint4(0, 0, 0, 0), n1 = n0;
@@ -745,6 +984,20 @@ static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, cons
 #undef EHX_TUNED_ARGS
 }
 
+// k_tiny deals its batches out by wave number, so its grid is what stays resident.
+template <int SLOTS>
+static void launch_tiny(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, const uint32_t *queue, unsigned *ctl,
+                        unsigned long long *acc) {
+  static int per_cu[3][64] = {};  // by SLOTS (2048, 4096, 8192) and device
+  const int dev = ctx->device & 63, si = SLOTS == 2048 ? 0 : (SLOTS == 4096 ? 1 : 2);
+  if (!per_cu[si][dev]) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tiny<SLOTS>, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    per_cu[si][dev] = nb;
+  }
+  k_tiny<SLOTS><<<(unsigned)(ctx->num_cus * per_cu[si][dev]), 256, 0, ctx->stream>>>(P, A, queue, ctl, acc);
+}
+
 // One cached graph per process: the key is every value baked into the launches.
 struct GraphCache {
   bool valid = false;
@@ -761,11 +1014,11 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
                          uint32_t *bucket, unsigned *ctl, uint32_t *queue, const TuneParams &tp, bool events = true) {
   hipStream_t st = ctx->stream;
   int rc;
-  if ((rc = fill_u32(ctx, ctl, 0u, 133))) return rc;
+  if ((rc = fill_u32(ctx, ctl, 0u, 134))) return rc;
   // descriptors by prefix after the queue-ordered ones (A.desc)
   uint4 *pdesc = A.desc ? const_cast<uint4 *>(A.desc) + 2 * (A.nprefix + SPLIT_EXTRA) : nullptr;
   k_classify<<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(P, A, bucket, ctl, tp.lds_slots, tp.big_work, pdesc,
-                                                               (uint32_t)tp.split_work);
+                                                               (uint32_t)tp.split_work, (uint32_t)tp.tiny_work);
   EHX_HIP(hipGetLastError());
   k_bucket_scan<<<1, 64, 0, st>>>(ctl);
   EHX_HIP(hipGetLastError());
@@ -782,6 +1035,15 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
   }
   EHX_HIP(hipGetLastError());
   if (events) EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));
+  if (A.desc && tp.tiny_work > 0) {  // the tiny bin (empty unless single-S and tiny_work > 0)
+    // (on a second stream it displaced the persistent kernel's workgroups: slower)
+    switch (tp.lds_slots) {
+      case 2048: launch_tiny<2048>(ctx, P, A, queue, ctl, acc); break;
+      case 8192: launch_tiny<8192>(ctx, P, A, queue, ctl, acc); break;
+      default: launch_tiny<4096>(ctx, P, A, queue, ctl, acc); break;
+    }
+    EHX_HIP(hipGetLastError());
+  }
   return EHX_OK;
 }
 

@@ -26,6 +26,7 @@ typedef struct {
   int32_t stream_lanes; /* lanes streaming one long child set together: a power of two in [4, 64] */
   int32_t gap_coded;    /* 1: stream the 16-bit gap-coded copy of a long child list where one exists (plans whose last level has one operand besides x) */
   int32_t split_work;   /* split staged-per-workgroup prefixes into parts of about this much work; 0: never */
+  int32_t tiny_work;    /* prefixes below this estimated work (single-operand plans) run up to 64 per wavefront; 0: never */
 } ehx_tune_params;
 
 int ehx_tune_get(ehx_tune_params *out);

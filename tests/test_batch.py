@@ -147,11 +147,14 @@ def test_segment_counts_match_oracle(ctx, oracle, form):
     if form != E.FORM_PLAIN:  # prefixes split into parts add to their segment part by part
         saved = E.tune_get()
         try:
-            E.tune_set(big_work=64, split_work=64)
+            E.tune_set(big_work=64, split_work=64, tiny_work=0)
             c1, t1 = E.join_count_segments(ctx, plans.TRIANGLE, [T], base, form)
+            E.tune_set(big_work=1 << 20, tiny_work=1 << 20)  # every prefix with |S| <= 128 many per wavefront
+            c2, t2 = E.join_count_segments(ctx, plans.TRIANGLE, [T], base, form)
         finally:
             E.tune_set(**saved)
         assert t1 == total and np.array_equal(c1, expect)
+        assert t2 == total and np.array_equal(c2, expect)
     # 4-clique per net through the same entry point
     ph4 = oracle.join_count_per_head(plans.FOUR_CLIQUE, [O], 8)
     c4, t4 = E.join_count_segments(ctx, plans.FOUR_CLIQUE, [T], base, form)

@@ -64,6 +64,12 @@ def test_ids_at_the_top_of_the_range(ctx, oracle):
     k4 = oracle.join_count(plans.FOUR_CLIQUE, [O], 4)
     assert tri >= 20 + 10 and k4 >= 15 + 5
     assert all_counts(ctx, s, d) == [tri, tri, k4] * 3
+    saved = E.tune_get()
+    try:  # every small prefix through the many-per-wavefront bin, 0xFFFFFFFF probed there
+        E.tune_set(tiny_work=1 << 20, big_work=1 << 20)
+        assert all_counts(ctx, s, d) == [tri, tri, k4] * 3
+    finally:
+        E.tune_set(**saved)
     ss, sd = G.symmetric(s, d)
     # 4-cycles on ids near the top: relabel-free oracle needs a dense id range, so shift both
     base = int(min(ss.min(), sd.min()))
@@ -102,8 +108,8 @@ def test_hub_beyond_every_staging_area(ctx, oracle, policy):
     T, R = E.Trie(ctx, s, d, policy), E.Trie(ctx, d, s, policy)
     saved = E.tune_get()
     try:
-        for lanes, gap, split in ((4, 0, 0), (4, 1, 1 << 21), (64, 1, 4096)):
-            E.tune_set(stream_lanes=lanes, gap_coded=gap, split_work=split)
+        for lanes, gap, split, tiny in ((4, 0, 0, 0), (4, 1, 1 << 21, 4096), (64, 1, 4096, 1 << 20)):
+            E.tune_set(stream_lanes=lanes, gap_coded=gap, split_work=split, tiny_work=tiny)
             for form in FORMS:
                 assert E.join_count(ctx, plans.TRIANGLE, [T], form) == tri
                 assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], form) == tri
@@ -144,7 +150,7 @@ def test_gap_coded_lists_at_their_limits(ctx, oracle, lanes):
     try:
         for big_deg in (4, 16):
             for gap in (0, 1):
-                E.tune_set(stream_lanes=lanes, big_deg=big_deg, gap_coded=gap)
+                E.tune_set(stream_lanes=lanes, big_deg=big_deg, gap_coded=gap, tiny_work=gap * 4096)
                 for form in (E.FORM_TUNED, E.FORM_LEARNED):
                     assert E.join_count(ctx, plans.TRIANGLE, [T], form) == tri
                     assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], form) == tri

@@ -189,12 +189,13 @@ def test_graph_replay_matches_eager(ctx):
         E.tune_set(**saved)
 
 
-@pytest.mark.parametrize("lds,big,bitmap_all,lanes,big_work,split", [
-    (2048, 16, 0, 4, 16384, 0), (8192, 256, 1, 64, 256, 1024), (4096, 64, 1, 16, 4096, 1 << 21),
-    (4096, 4, 0, 8, 64, 64), (2048, 5, 1, 32, 512, 4096)])
-def test_tune_params_never_change_results(ctx, oracle, lds, big, bitmap_all, lanes, big_work, split):
+@pytest.mark.parametrize("lds,big,bitmap_all,lanes,big_work,split,tiny", [
+    (2048, 16, 0, 4, 16384, 0, 0), (8192, 256, 1, 64, 256, 1024, 1 << 20), (4096, 64, 1, 16, 4096, 1 << 21, 1024),
+    (4096, 4, 0, 8, 64, 64, 64), (2048, 5, 1, 32, 512, 4096, 16384)])
+def test_tune_params_never_change_results(ctx, oracle, lds, big, bitmap_all, lanes, big_work, split, tiny):
     """Every knob setting gives the same counts; small big_work and split_work
-    send many prefixes to the workgroup bin and split them into parts."""
+    send many prefixes to the workgroup bin and split them into parts; tiny_work
+    sends the small ones to the many-per-wavefront bin (all of them at 2^20)."""
     s, d, n = G.rmat_graph(13)
     exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
     T = E.Trie(ctx, s, d)
@@ -204,7 +205,7 @@ def test_tune_params_never_change_results(ctx, oracle, lds, big, bitmap_all, lan
         k4 = E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_PLAIN)
         for gap in (0, 1):
             E.tune_set(lds_slots=lds, big_deg=big, bitmap_all=bitmap_all, grab=3, waves_per_cu=4, stream_lanes=lanes,
-                       gap_coded=gap, big_work=big_work, split_work=split)
+                       gap_coded=gap, big_work=big_work, split_work=split, tiny_work=tiny)
             assert E.join_count(ctx, plans.TRIANGLE, [T], E.FORM_TUNED) == exp
             assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, T10], E.FORM_TUNED) == exp
             assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_TUNED) == k4
