@@ -20,7 +20,7 @@ GRID = {
     "waves_per_cu": [8, 16, 24, 32, 48],
     "grab": [1, 2, 4, 8],
     "big_work": [1024, 4096, 16384, 65536, 262144],
-    "stream_lanes": [4, 8, 16, 32, 64],
+    "stream_lanes": [2, 4, 8, 16, 32, 64],
     "gap_coded": [0, 1],
     "split_work": [0, 1 << 19, 1 << 20, 1 << 21, 1 << 22],
     "tiny_work": [0, 256, 1024, 4096, 16384],

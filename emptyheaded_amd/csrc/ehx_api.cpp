@@ -257,9 +257,8 @@ int ehx_tune_set(const ehx_tune_params *in) {
   if (in->big_deg < 1 || in->waves_per_cu < 1 || in->waves_per_cu > 64 || in->grab < 1 || in->grab > 64 ||
       in->big_work < 1 || in->split_work < 0 || in->tiny_work < 0)
     return fail(EHX_ERR_INVALID, "tune parameter out of range");
-  // >= 4: a group reads the up-to-3-element unaligned head and tail of a set one element per lane
-  if (in->stream_lanes < 4 || in->stream_lanes > 64 || (in->stream_lanes & (in->stream_lanes - 1)))
-    return fail(EHX_ERR_INVALID, "stream_lanes must be a power of two in [4, 64]");
+  if (in->stream_lanes < 2 || in->stream_lanes > 64 || (in->stream_lanes & (in->stream_lanes - 1)))
+    return fail(EHX_ERR_INVALID, "stream_lanes must be a power of two in [2, 64]");
   TuneParams &p = tune_params();
   p.lds_slots = in->lds_slots;
   p.big_deg = in->big_deg;

@@ -68,7 +68,7 @@ struct TuneParams {
   int32_t bitmap_all;     // 1: stage any set whose word window fits as a bitmap
   int32_t use_graph;      // capture the level in a hipGraph and replay it
   int32_t big_work;       // prefixes with at least this much estimated work go to the workgroup bin
-  int32_t stream_lanes;   // lanes that stream one long child set together (a power of two, 4..64)
+  int32_t stream_lanes;   // lanes that stream one long child set together (a power of two, 2..64)
   int32_t gap_coded;      // 1: stream the gap-coded copy of a child list where the trie has one (single-S plans)
   int32_t split_work;     // workgroup-bin prefixes are split into parts of about this much work (0: never)
   int32_t tiny_work;      // single-S prefixes below this estimated work run up to 64 per wavefront (0: never)

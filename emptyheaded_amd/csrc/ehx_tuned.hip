@@ -159,11 +159,17 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
       const DSet X = P.nd ? Dd : D;  // the set x runs over
       const uint32_t ns = X.n < 32 ? X.n : 32;  // sample up to 32 of its elements
       uint64_t w = 0;
+      if (pdesc && A.xlink) {  // child pointers: one independent load per sample
+        const uint4 *xl = A.xlink + (X.v - (P.nd ? A.td.vals : A.ts[0].vals));
 #pragma unroll 8
-      for (uint32_t k = 0; k < ns; k++) {
-        uint32_t x = X.v[(uint64_t)k * X.n / ns];
-        int32_t nx = node_of(tx, x);
-        if (nx >= 0) w += tx.off[nx + 1] - tx.off[nx];
+        for (uint32_t k = 0; k < ns; k++) w += xl[(uint64_t)k * X.n / ns].y;
+      } else {
+#pragma unroll 8
+        for (uint32_t k = 0; k < ns; k++) {
+          uint32_t x = X.v[(uint64_t)k * X.n / ns];
+          int32_t nx = node_of(tx, x);
+          if (nx >= 0) w += tx.off[nx + 1] - tx.off[nx];
+        }
       }
       w = w * X.n / (ns ? ns : 1) + X.n + (P.nd ? D.n : 0u);
       if (P.ns > 1) w = w / 2 + 1;
@@ -433,7 +439,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
         continue;
       }
       const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
-      if (ql < head) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + ql]);
+      for (uint32_t j = ql; j < head; j += G) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + j]);
       const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
       const uint32_t n4 = (bd - head) >> 2;
       uint32_t i = ql;
@@ -461,7 +467,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
                probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w);
       }
       const uint32_t done = head + 4 * n4;
-      if (ql < bd - done) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + done + ql]);
+      for (uint32_t j = ql; j < bd - done; j += G) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + done + j]);
       hits += h32;
     }
     wave_sync();  // the owner window below reuses own[]
@@ -968,7 +974,7 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : (MAXS == 2 ? 5 : 2)) void k_tu
 template <int SLOTS>
 static void launch_tuned(hipStream_t st, unsigned grid, const TunedPlan &P, const TunedArgs &A, const uint32_t *queue,
                          unsigned *ctl, unsigned long long *acc, const TuneParams &tp) {
-  const uint32_t slg = 31u - (uint32_t)__builtin_clz((unsigned)tp.stream_lanes);  // a power of two in [4, 64]
+  const uint32_t slg = 31u - (uint32_t)__builtin_clz((unsigned)tp.stream_lanes);  // a power of two in [2, 64]
   const int gap = (tp.gap_coded && A.tx.pk) ? 1 : 0;
   // four quads in flight per lane pay off for narrow lane groups (many short lists); wider
   // groups keep two, and the registers they would cost

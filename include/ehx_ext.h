@@ -23,7 +23,7 @@ typedef struct {
   int32_t bitmap_all;   /* stage every set whose word window fits as a bitmap */
   int32_t use_graph;    /* capture the level in a hipGraph and replay it */
   int32_t big_work;     /* prefixes with at least this much estimated work are staged once per workgroup */
-  int32_t stream_lanes; /* lanes streaming one long child set together: a power of two in [4, 64] */
+  int32_t stream_lanes; /* lanes streaming one long child set together: a power of two in [2, 64] */
   int32_t gap_coded;    /* 1: stream the 16-bit gap-coded copy of a long child list where one exists (plans whose last level has one operand besides x) */
   int32_t split_work;   /* split staged-per-workgroup prefixes into parts of about this much work; 0: never */
   int32_t tiny_work;    /* prefixes below this estimated work (single-operand plans) run up to 64 per wavefront; 0: never */

@@ -108,7 +108,7 @@ def test_plan_encoding():
 def test_tune_params_are_checked():
     saved = E.tune_get()
     try:
-        for bad in ({"stream_lanes": 2}, {"stream_lanes": 12}, {"stream_lanes": 128}, {"lds_slots": 1000}):
+        for bad in ({"stream_lanes": 1}, {"stream_lanes": 12}, {"stream_lanes": 128}, {"lds_slots": 1000}):
             with pytest.raises(E.EhxError):
                 E.tune_set(**bad)
         assert E.tune_set(stream_lanes=4)["stream_lanes"] == 4

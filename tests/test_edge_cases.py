@@ -118,7 +118,7 @@ def test_hub_beyond_every_staging_area(ctx, oracle, policy):
         E.tune_set(**saved)
 
 
-@pytest.mark.parametrize("lanes", [4, 16, 64])
+@pytest.mark.parametrize("lanes", [2, 4, 16, 64])
 def test_gap_coded_lists_at_their_limits(ctx, oracle, lanes):
     """Child lists of 16 to 5,000 values (15-value chunks: lengths on, just
     under and just over a chunk boundary), some ending in a 65,535 gap (coded)

@@ -191,7 +191,7 @@ def test_graph_replay_matches_eager(ctx):
 
 @pytest.mark.parametrize("lds,big,bitmap_all,lanes,big_work,split,tiny", [
     (2048, 16, 0, 4, 16384, 0, 0), (8192, 256, 1, 64, 256, 1024, 1 << 20), (4096, 64, 1, 16, 4096, 1 << 21, 1024),
-    (4096, 4, 0, 8, 64, 64, 64), (2048, 5, 1, 32, 512, 4096, 16384)])
+    (4096, 4, 0, 8, 64, 64, 64), (2048, 5, 1, 32, 512, 4096, 16384), (4096, 16, 0, 2, 4096, 1 << 21, 256)])
 def test_tune_params_never_change_results(ctx, oracle, lds, big, bitmap_all, lanes, big_work, split, tiny):
     """Every knob setting gives the same counts; small big_work and split_work
     send many prefixes to the workgroup bin and split them into parts; tiny_work
