@@ -27,7 +27,7 @@ class EhxError(RuntimeError):
 
 
 class Op(C.Structure):
-    _fields_ = [("rel", C.c_int32), ("parent", C.c_int32)]
+    _fields_ = [("rel", C.c_int32), ("parent", C.c_int32), ("value", C.c_uint32)]
 
 
 class Plan(C.Structure):
@@ -267,16 +267,18 @@ def intersect(ctx, ta, keys_a, tb, keys_b):
 
 
 def make_plan(levels):
-    """levels: one list per attribute of (rel, parent) operands; parent -1 = head set."""
+    """levels: one list per attribute of (rel, parent) operands; parent -1 = head
+    set; (rel, -2, value) = the child set of the constant value (a selection)."""
     if not 1 <= len(levels) <= MAX_ATTRS:
         raise ValueError("1..%d attributes" % MAX_ATTRS)
     p = Plan()
     p.num_attrs = len(levels)
     for k, ops in enumerate(levels):
         p.num_ops[k] = len(ops)
-        for i, (rel, parent) in enumerate(ops):
-            p.ops[k][i].rel = rel
-            p.ops[k][i].parent = parent
+        for i, op in enumerate(ops):
+            p.ops[k][i].rel = op[0]
+            p.ops[k][i].parent = op[1]
+            p.ops[k][i].value = op[2] if len(op) > 2 else 0
     return p
 
 

@@ -85,10 +85,10 @@ __global__ __launch_bounds__(256) void k_level_card(Tries T, LevelDesc L, Fronti
   unsigned long long sum = 0, nonempty = 0, empty = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t c;
-    if (par < 0) {
+    if (par == EHX_PARENT_HEAD) {
       c = t.nh;
     } else {
-      const int32_t h = node_of(t, in.col[par][i]);
+      const int32_t h = node_of(t, par == EHX_PARENT_CONST ? L.value[0] : in.col[par][i]);
       c = h >= 0 ? t.off[h + 1] - t.off[h] : 0u;
     }
     sum += c;
@@ -116,20 +116,32 @@ __global__ __launch_bounds__(256) void k_level_card(Tries T, LevelDesc L, Fronti
   }
 }
 
-// Level 0: flags over the driver head set, membership in the other head sets.
-__global__ void k_level0_flags(Tries T, LevelDesc L, int driver, uint32_t *__restrict__ flags) {
-  const DTrie &td = T.t[L.rel[driver]];
-  DSet D = set_of(td, td.nh);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D.n; i += (uint64_t)gridDim.x * blockDim.x) {
-    uint32_t x = D.v[i];
+// Level 0: flags over the driver's set dv[0, dn), membership in the other
+// operands' sets (a head set, or the child set of a selection constant).
+__global__ void k_level0_flags(Tries T, LevelDesc L, int driver, const uint32_t *__restrict__ dv, uint64_t dn,
+                               uint32_t *__restrict__ flags) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dn; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t x = dv[i];
     bool hit = true;
     for (int o = 0; o < L.nops && hit; o++) {
       if (o == driver) continue;
       const DTrie &t = T.t[L.rel[o]];
-      hit = node_of(t, x) >= 0;  // membership in a head set = a child exists
+      if (L.parent[o] == EHX_PARENT_CONST) {
+        const int32_t h = node_of(t, L.value[o]);
+        hit = h >= 0 && set_contains(set_of(t, (uint32_t)h), x);
+      } else {
+        hit = node_of(t, x) >= 0;  // membership in a head set = a child exists
+      }
     }
     flags[i] = hit ? 1u : 0u;
   }
+}
+
+// {offset in vals, length} of the child set of a constant (0, 0 when absent).
+__global__ void k_const_set(DTrie t, uint32_t value, uint32_t *__restrict__ out) {
+  const int32_t h = node_of(t, value);
+  out[0] = h >= 0 ? t.off[h] : 0u;
+  out[1] = h >= 0 ? t.off[h + 1] - t.off[h] : 0u;
 }
 
 __global__ void k_compact_u32(const uint32_t *__restrict__ v, const uint32_t *__restrict__ flags,
@@ -149,6 +161,7 @@ static int check_plan(const ehx_plan *p, ehx_trie *const *tries, int *nrels) {
       const ehx_op &o = p->ops[k][i];
       if (o.rel < 0 || o.rel >= EHX_MAX_RELS) return fail(EHX_ERR_INVALID, "level %d op %d: rel %d", k, i, o.rel);
       if (o.parent >= k) return fail(EHX_ERR_INVALID, "level %d op %d: parent %d not before level", k, i, o.parent);
+      if (o.parent < EHX_PARENT_CONST) return fail(EHX_ERR_INVALID, "level %d op %d: parent %d", k, i, o.parent);
       if (o.rel > mr) mr = o.rel;
     }
   }
@@ -165,6 +178,7 @@ static LevelDesc level_desc(const ehx_plan *p, int k) {
   for (int i = 0; i < L.nops; i++) {
     L.rel[i] = p->ops[k][i].rel;
     L.parent[i] = p->ops[k][i].parent;
+    L.value[i] = p->ops[k][i].value;
   }
   return L;
 }
@@ -179,33 +193,58 @@ __global__ void k_seg_level0(const uint32_t *__restrict__ f0, uint64_t n, SegOut
 
 // Level 0: ParTrieBuilder::build_aggregated_set over the head sets
 // (TrieBuilder.cpp:478-578).  One relation: its head set; several: the driver
-// (smallest) head filtered by membership in the others, then compacted.
+// (a selection's child set if there is one, else the smallest head) filtered by
+// membership in the others, then compacted.
 static int level0(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, const Tries &T, const uint32_t **f0_out,
                   uint64_t *n0_out) {
   hipStream_t st = ctx->stream;
   int rc;
   LevelDesc L0 = level_desc(p, 0);
-  int driver = 0;
-  for (int i = 1; i < L0.nops; i++)
-    if (tries[L0.rel[i]]->nh <= tries[L0.rel[driver]]->nh) driver = i;
-  bool single = true;
-  for (int i = 0; i < L0.nops; i++)
-    if (L0.rel[i] != L0.rel[driver]) single = false;
-  const ehx_trie *td = tries[L0.rel[driver]];
-  if (single) {
-    *f0_out = td->head;  // head ∩ head of one trie is the head itself
-    *n0_out = td->nh;
-    return EHX_OK;
+  int driver = -1;
+  for (int i = 0; i < L0.nops && driver < 0; i++)
+    if (L0.parent[i] == EHX_PARENT_CONST) driver = i;
+  const uint32_t *dv = nullptr;
+  uint64_t dn = 0;
+  if (driver >= 0) {  // the selection's child set: {offset, length} from the device
+    DevBuf &cb = ctx->scratch[S_MISC];
+    if ((rc = cb.ensure(64))) return rc;
+    const ehx_trie *tc = tries[L0.rel[driver]];
+    k_const_set<<<1, 1, 0, st>>>(T.t[L0.rel[driver]], L0.value[driver], cb.as<uint32_t>());
+    EHX_HIP(hipGetLastError());
+    uint32_t ol[2] = {0, 0};
+    if ((rc = copy_d2h(ctx, ol, cb.p, 8))) return rc;
+    dv = tc->vals + ol[0];
+    dn = ol[1];
+    if (L0.nops == 1 || dn == 0) {
+      *f0_out = dv;
+      *n0_out = dn;
+      return EHX_OK;
+    }
+  } else {
+    driver = 0;
+    for (int i = 1; i < L0.nops; i++)
+      if (tries[L0.rel[i]]->nh <= tries[L0.rel[driver]]->nh) driver = i;
+    bool single = true;
+    for (int i = 0; i < L0.nops; i++)
+      if (L0.rel[i] != L0.rel[driver]) single = false;
+    const ehx_trie *td = tries[L0.rel[driver]];
+    if (single) {
+      *f0_out = td->head;  // head ∩ head of one trie is the head itself
+      *n0_out = td->nh;
+      return EHX_OK;
+    }
+    dv = td->head;
+    dn = td->nh;
   }
   DevBuf &fl = ctx->scratch[S_FLAGS], &ps = ctx->scratch[S_POS], &fb = ctx->scratch[S_F0];
-  if ((rc = fl.ensure(td->nh * 4 + 4)) || (rc = ps.ensure(td->nh * 8 + 8)) || (rc = fb.ensure(td->nh * 4 + 4))) return rc;
-  k_level0_flags<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(T, L0, driver, fl.as<uint32_t>());
+  if ((rc = fl.ensure(dn * 4 + 4)) || (rc = ps.ensure(dn * 8 + 8)) || (rc = fb.ensure(dn * 4 + 4))) return rc;
+  k_level0_flags<<<grid_for(dn, 256, 16384), 256, 0, st>>>(T, L0, driver, dv, dn, fl.as<uint32_t>());
   EHX_HIP(hipGetLastError());
-  if ((rc = scan_u32_to_u64(ctx, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh))) return rc;
-  k_compact_u32<<<grid_for(td->nh, 256, 16384), 256, 0, st>>>(td->head, fl.as<uint32_t>(), ps.as<uint64_t>(), td->nh,
-                                                                fb.as<uint32_t>());
+  if ((rc = scan_u32_to_u64(ctx, fl.as<uint32_t>(), ps.as<uint64_t>(), dn))) return rc;
+  k_compact_u32<<<grid_for(dn, 256, 16384), 256, 0, st>>>(dv, fl.as<uint32_t>(), ps.as<uint64_t>(), dn,
+                                                          fb.as<uint32_t>());
   EHX_HIP(hipGetLastError());
-  if ((rc = copy_d2h(ctx, n0_out, ps.as<uint64_t>() + td->nh, 8))) return rc;
+  if ((rc = copy_d2h(ctx, n0_out, ps.as<uint64_t>() + dn, 8))) return rc;
   *f0_out = fb.as<uint32_t>();
   return EHX_OK;
 }

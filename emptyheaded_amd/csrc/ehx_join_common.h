@@ -192,6 +192,7 @@ struct LevelDesc {
   int32_t nops;
   int32_t rel[EHX_MAX_OPS];
   int32_t parent[EHX_MAX_OPS];
+  uint32_t value[EHX_MAX_OPS];  // the constant of an EHX_PARENT_CONST operand
 };
 
 struct Tries {
@@ -202,10 +203,11 @@ struct Tries {
 __device__ __forceinline__ bool resolve(const Tries &T, const LevelDesc &L, const uint32_t *tuple, DSet *S) {
   for (int i = 0; i < L.nops; i++) {
     const DTrie &t = T.t[L.rel[i]];
-    if (L.parent[i] < 0) {
+    if (L.parent[i] == EHX_PARENT_HEAD) {
       S[i] = set_of(t, t.nh);
     } else {
-      int32_t h = node_of(t, tuple[L.parent[i]]);
+      const uint32_t key = L.parent[i] == EHX_PARENT_CONST ? L.value[i] : tuple[L.parent[i]];
+      int32_t h = node_of(t, key);
       if (h < 0) return false;  // NULL TrieBlock: empty (TrieBuilder.cpp:249-254)
       S[i] = set_of(t, (uint32_t)h);
     }

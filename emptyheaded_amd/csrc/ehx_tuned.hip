@@ -32,6 +32,9 @@ TuneParams &tune_params() { return g_params; }
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp) {
   const int K = p->num_attrs;
   if (K < 3) return false;
+  for (int k = K - 2; k < K; k++)  // selections in the fused levels run in the plain form
+    for (int i = 0; i < p->num_ops[k]; i++)
+      if (p->ops[k][i].parent == EHX_PARENT_CONST) return false;
   memset(tp, 0, sizeof(*tp));
   tp->K = K;
   int xcount = 0;

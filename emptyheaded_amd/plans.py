@@ -2,7 +2,8 @@
 
 A plan lists, per attribute in attribute order, the operand sets intersected
 at that level: (rel, parent) with parent = -1 for the head set of relation
-`rel` and parent = j for the child set of attribute j's value.  The operand
+`rel` and parent = j for the child set of attribute j's value, or
+(rel, -2, value) for the child set of a constant (an equality selection).  The operand
 order follows the accessor order of the generated code
 (query_compiler/src/main/scala/codegen/query/EHGenerator.scala:273-838); the
 attribute order is the sorted attribute names (parsers/Datalog.scala:150-188).
@@ -40,6 +41,39 @@ FOUR_CLIQUE = [[(0, -1), (0, -1), (0, -1)],
                [(0, 0), (0, -1), (0, -1)],
                [(0, 1), (0, 0), (0, -1)],
                [(0, 0), (0, 1), (0, 2)]]
+
+HEAD, CONST = -1, -2
+
+
+# Equality selections.  The generated code puts a selected attribute first and,
+# when it has one accessor, replaces its level by one lookup:
+# Iterators_<acc>.get_next_block(selection_x_0), so the next attribute reads
+# the selected value's child set (EHGenerator.scala:657-701).  Here that child
+# set is an operand (rel, CONST, value); relation 1 is the reverse ordering
+# Edge_1_0 (the accessor Edge(a,x) ordered (x, a)).
+
+def flique_sel(node):
+    """FliqueSelAgg(;z) :- Edge(a,b),Edge(b,c),Edge(a,c),Edge(a,d),Edge(b,d),
+    Edge(c,d),Edge(a,x),x=node,z:long<-[COUNT(*)] (test/graph/travis.py:385-400):
+    the 4-clique plan with a restricted to child(Edge_1_0, node)."""
+    return [[(0, HEAD), (0, HEAD), (0, HEAD), (1, CONST, node)],
+            [(0, 0), (0, HEAD), (0, HEAD)],
+            [(0, 1), (0, 0), (0, HEAD)],
+            [(0, 0), (0, 1), (0, 2)]]
+
+
+def barbell_sel(node):
+    """BarbellSelAgg(;w) :- Edge(a,b),Edge(b,c),Edge(a,c),Edge(a,p),Edge(p,x),
+    Edge(x,y),Edge(y,z),Edge(x,z),p=node,w:long<-[COUNT(*)] (test/graph/
+    travis.py:473-489), as one bag over (a, b, c, x, y, z): a in
+    child(Edge_1_0, node), x in child(Edge_0_1, node)."""
+    return [[(0, HEAD), (0, HEAD), (1, CONST, node)],
+            [(0, 0), (0, HEAD)],
+            [(0, 1), (0, 0)],
+            [(0, HEAD), (0, HEAD), (0, CONST, node)],
+            [(0, 3), (0, HEAD)],
+            [(0, 4), (0, 3)]]
+
 
 # Closed 4-walks over a symmetric relation S (both edge directions):
 #   Q(;z) :- S(a,b),S(b,c),S(c,d),S(a,d),COUNT(*)

@@ -53,13 +53,21 @@ extern "C" {
 typedef struct ehx_ctx ehx_ctx;
 typedef struct ehx_trie ehx_trie;
 
-/* One operand of a join level: parent < 0 is the head set of relation `rel`
- * (ParTrieIterator::head / TrieIterator::get_block(0)); parent = j is the child
- * set of attribute x_j in `rel` (TrieIterator::get_next_block(0, x_j) then
- * get_block(1), TrieIterator.cpp:23-42). */
+/* One operand of a join level: parent = EHX_PARENT_HEAD is the head set of
+ * relation `rel` (ParTrieIterator::head / TrieIterator::get_block(0)); parent =
+ * j >= 0 is the child set of attribute x_j in `rel` (TrieIterator::
+ * get_next_block(0, x_j) then get_block(1), TrieIterator.cpp:23-42); parent =
+ * EHX_PARENT_CONST is the child set of the constant `value` in `rel`: an
+ * equality selection such as Edge(a,x), x=0 over the trie ordered (x, a) (the
+ * generated selection path, query_compiler/src/main/scala/codegen/
+ * EHGenerator.scala:657-701; empty when `value` is not a head).  `value` is
+ * ignored for the other parents. */
+#define EHX_PARENT_HEAD (-1)
+#define EHX_PARENT_CONST (-2)
 typedef struct {
   int32_t rel;
   int32_t parent;
+  uint32_t value;
 } ehx_op;
 
 /* A single-bag COUNT(*) plan over binary relations, in attribute order: the

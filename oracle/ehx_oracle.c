@@ -432,9 +432,9 @@ static int gather(const join_ctx *jc, int level, const uint32_t *tuple, const or
   for (int i = 0; i < jc->p->num_ops[level]; i++) {
     orc_op op = jc->p->ops[level][i];
     const orc_trie *t = jc->tries[op.rel];
-    if (op.parent < 0) sets[i] = &t->head;
+    if (op.parent == ORC_PARENT_HEAD) sets[i] = &t->head;
     else {
-      sets[i] = trie_child(t, tuple[op.parent]);
+      sets[i] = trie_child(t, op.parent == ORC_PARENT_CONST ? op.value : tuple[op.parent]);
       if (!sets[i]) return 0; /* NULL TrieBlock -> empty (TrieBuilder.cpp:278-304) */
     }
   }
@@ -519,7 +519,7 @@ static int plan_ok(const orc_plan *p) {
   for (int k = 0; k < p->num_attrs; k++) {
     if (p->num_ops[k] < 1 || p->num_ops[k] > ORC_MAX_OPS) return 0;
     for (int i = 0; i < p->num_ops[k]; i++) {
-      if (p->ops[k][i].parent >= k) return 0;
+      if (p->ops[k][i].parent >= k || p->ops[k][i].parent < ORC_PARENT_CONST) return 0;
       if (k == 0 && p->ops[k][i].parent >= 0) return 0;
     }
   }
@@ -548,7 +548,10 @@ static uint64_t max_bytes(const orc_plan *p, orc_trie *const *tries, uint64_t *m
 static uint32_t *level0_values(const orc_plan *p, orc_trie *const *tries, uint64_t mx, uint64_t *n0) {
   const orc_set *sets[ORC_MAX_OPS];
   uint32_t dummy[1] = {0};
-  gather(&(join_ctx){.p = p, .tries = tries}, 0, dummy, sets);
+  if (!gather(&(join_ctx){.p = p, .tries = tries}, 0, dummy, sets)) {  /* a selection of an absent value */
+    *n0 = 0;
+    return (uint32_t *)malloc(4);
+  }
   uint8_t *b0 = (uint8_t *)malloc(mx + 64), *b1 = (uint8_t *)malloc(mx + 64);
   orc_set r;
   nway(&r, b0, b1, sets, p->num_ops[0], 0);

@@ -79,11 +79,15 @@ uint64_t orc_trie_tuples(const orc_trie *t, uint32_t *c0, uint32_t *c1);
 void orc_trie_node_info(const orc_trie *t, uint8_t *types, uint32_t *cards);
 
 /* Single-bag generic-join COUNT plan over binary relations.  Level k's
- * candidate set is the intersection of its operands: parent < 0 names the head
- * set of relation `rel`, parent = j < k the child set of x_j in `rel`. */
+ * candidate set is the intersection of its operands: parent = -1 names the head
+ * set of relation `rel`, parent = j < k the child set of x_j in `rel`, parent =
+ * -2 the child set of the constant `value` in `rel` (an equality selection,
+ * query_compiler/src/main/scala/codegen/EHGenerator.scala:657-701). */
 #define ORC_MAX_ATTRS 8
 #define ORC_MAX_OPS 8
-typedef struct { int32_t rel; int32_t parent; } orc_op;
+#define ORC_PARENT_HEAD (-1)
+#define ORC_PARENT_CONST (-2)
+typedef struct { int32_t rel; int32_t parent; uint32_t value; } orc_op;
 typedef struct {
   int32_t num_attrs;
   int32_t num_ops[ORC_MAX_ATTRS];

@@ -24,7 +24,7 @@ class OrcSet(C.Structure):
 
 
 class OrcOp(C.Structure):
-    _fields_ = [("rel", C.c_int32), ("parent", C.c_int32)]
+    _fields_ = [("rel", C.c_int32), ("parent", C.c_int32), ("value", C.c_uint32)]
 
 
 class OrcPlan(C.Structure):
@@ -195,14 +195,16 @@ class Trie:
 
 
 def make_plan(levels):
-    """levels: list (one per attribute) of lists of (rel, parent) pairs."""
+    """levels: list (one per attribute) of lists of (rel, parent) pairs, or
+    (rel, -2, value) triples for the child set of a constant (a selection)."""
     p = OrcPlan()
     p.num_attrs = len(levels)
     for k, ops in enumerate(levels):
         p.num_ops[k] = len(ops)
-        for i, (rel, parent) in enumerate(ops):
-            p.ops[k][i].rel = rel
-            p.ops[k][i].parent = parent
+        for i, op in enumerate(ops):
+            p.ops[k][i].rel = op[0]
+            p.ops[k][i].parent = op[1]
+            p.ops[k][i].value = op[2] if len(op) > 2 else 0
     return p
 
 

@@ -9,7 +9,10 @@ the constants those tests assert:
   Triangle     = 1,612,010 rows, first row (6,5,2) (travis.py:302-318)
   Flique       = 30,004,668 rows, first row (9,8,7,0) (travis.py:342-354)
 and, for the graph with both edge directions (facebook_duplicated.tsv),
-9,672,060 triangle tuples (docs/notebooks/Getting Started.ipynb, cell 22).
+9,672,060 triangle tuples (docs/notebooks/Getting Started.ipynb, cell 22),
+and the selections test_duplicated runs (travis.py:647-693):
+  FliqueSelAgg, x=0  = 3,759,972  (travis.py:385-400; FliqueSel rows, :429-444)
+  BarbellSelAgg, p=6 = 26,936,100 (travis.py:473-489; BarbellSel rows, :519-530)
 
 Run once in a container that holds the reference tree:
     python tests/golden/make_fixtures.py /root/reference
@@ -65,7 +68,9 @@ def main(ref_root):
             "rows": int(dup.shape[0]),
             "symmetric": bool(dup_is_sym),
             "Triangle_rows": 9672060,
-            "cite": "docs/notebooks/Getting Started.ipynb cell 22",
+            "FliqueSelAgg_x0": 3759972,
+            "BarbellSelAgg_p6": 26936100,
+            "cite": "docs/notebooks/Getting Started.ipynb cell 22; test/graph/travis.py:385-400,429-444,473-489,519-530",
         },
         "simple": {
             "source": "test/graph/data/simple.tsv",

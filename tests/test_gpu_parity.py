@@ -286,3 +286,50 @@ def test_triangle_by_middle_vertex(ctx, oracle, scale):
             assert c == exp
             if form != E.FORM_PLAIN:
                 assert st["form_used"] == form  # the driver-set shape runs in the tuned kernel
+
+
+# ---- equality selections ------------------------------------------------------
+
+@pytest.mark.parametrize("form", FORMS)
+def test_selection_goldens(ctx, facebook_dup, fixtures_meta, form):
+    """FliqueSelAgg (x=0) and BarbellSelAgg (p=6) on the symmetric facebook
+    graph: the counts travis.py asserts (385-400, 473-489)."""
+    s, d = facebook_dup
+    T, R = E.Trie(ctx, s, d), E.Trie(ctx, d, s)
+    meta = fixtures_meta["facebook_duplicated"]
+    assert E.join_count(ctx, plans.flique_sel(0), [T, R], form) == meta["FliqueSelAgg_x0"]
+    assert E.join_count(ctx, plans.barbell_sel(6), [T, R], form) == meta["BarbellSelAgg_p6"]
+
+
+def test_selection_rows(ctx, oracle, facebook_dup, fixtures_meta):
+    """FliqueSel(a,b,c,d) rows (travis.py:429-444: 3,759,972) equal the oracle's, in order."""
+    s, d = facebook_dup
+    levels = plans.flique_sel(0)
+    rows = E.join_materialize(ctx, levels, [E.Trie(ctx, s, d), E.Trie(ctx, d, s)], E.FORM_TUNED)
+    assert rows.shape == (fixtures_meta["facebook_duplicated"]["FliqueSelAgg_x0"], 4)
+    assert np.array_equal(rows, oracle.join_materialize(levels, [oracle.Trie(s, d), oracle.Trie(d, s)]))
+
+
+def test_selections_vs_oracle(ctx, oracle):
+    """Selections on a symmetric R-MAT graph at a hub, at ordinary vertices and
+    at an absent value, at level 0, at a later level, as a level's only operand
+    (the driver) and as the last level's only operand."""
+    s, d, n = G.rmat_graph(12)
+    s, d = G.symmetric(s, d)
+    T, R = E.Trie(ctx, s, d), E.Trie(ctx, d, s)
+    O = [oracle.Trie(s, d), oracle.Trie(d, s)]
+    deg = np.bincount(s.astype(np.int64), minlength=n)
+    nodes = [int(np.argmax(deg)), int(deg.argsort()[len(deg) // 2]), 7, n + 1000]
+    for node in nodes:
+        cases = [plans.flique_sel(node),
+                 [[(0, -1), (0, -1)], [(0, 0), (0, -1), (1, -2, node)], [(0, 1), (0, 0)]],  # triangles, b ~ node
+                 [[(1, -2, node)], [(0, 0)]],                                             # 2-paths from node
+                 [[(0, -1)], [(1, -2, node)]]]                                            # heads x N(node)
+        # the barbell enumerates (ordered triangles at N(node))^2 results: at the hub
+        # of this graph 7.7e12, so only where that is small (vertex 7: 2.8e8)
+        if node in (7, n + 1000):
+            cases.append(plans.barbell_sel(node))
+        for levels in cases:
+            exp = oracle.join_count(levels, O, 8)
+            for form in FORMS:
+                assert E.join_count(ctx, levels, [T, R], form) == exp, (node, levels, form)

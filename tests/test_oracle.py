@@ -87,6 +87,40 @@ def test_triangle_and_clique_brute_force(oracle, seed):
     assert plans.four_cycles(walks, A.sum(1)) == c4
 
 
+def test_facebook_selection_goldens(oracle, facebook_dup, fixtures_meta):
+    """FliqueSelAgg (x=0) and BarbellSelAgg (p=6) on the symmetric facebook
+    graph, the constants travis.py asserts (385-400, 473-489)."""
+    s, d = facebook_dup
+    T, R = oracle.Trie(s, d), oracle.Trie(d, s)
+    meta = fixtures_meta["facebook_duplicated"]
+    assert oracle.join_count(plans.flique_sel(0), [T, R], 4) == meta["FliqueSelAgg_x0"]
+    assert oracle.join_count(plans.barbell_sel(6), [T, R], 4) == meta["BarbellSelAgg_p6"]
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_selections_brute_force(oracle, seed):
+    """Selections on a small symmetric graph against enumeration: ordered
+    4-cliques with a adjacent to the node, and the barbell count, which
+    factors into (ordered triangles at neighbours of p) squared."""
+    rng = np.random.default_rng(seed)
+    n = 20
+    pairs = rng.integers(0, n, size=(110, 2))
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    A = np.zeros((n, n), dtype=np.int64)
+    A[pairs[:, 0], pairs[:, 1]] = 1
+    A = ((A + A.T) > 0).astype(np.int64)
+    both = np.argwhere(A)
+    s, d = both[:, 0].astype(np.uint32), both[:, 1].astype(np.uint32)
+    T, R = oracle.Trie(s, d), oracle.Trie(d, s)
+    for node in list(range(0, n, 3)) + [n + 5]:  # n + 5: not a vertex, empty selection
+        adj = A[node] if node < n else np.zeros(n, np.int64)
+        fs = sum(1 for q in itertools.permutations(range(n), 4)
+                 if adj[q[0]] and all(A[x, y] for x, y in itertools.combinations(q, 2)))
+        assert oracle.join_count(plans.flique_sel(node), [T, R], 2) == fs
+        t6 = sum(1 for a, b, c in itertools.permutations(range(n), 3) if adj[a] and A[a, b] and A[b, c] and A[a, c])
+        assert oracle.join_count(plans.barbell_sel(node), [T, R], 2) == t6 * t6
+
+
 def test_two_hop_closed_form(oracle):
     rng = np.random.default_rng(7)
     r = np.unique(rng.integers(0, 50, size=(400, 2)), axis=0)
