@@ -84,6 +84,38 @@ def bench_small_graphs(E, ctx, steps, cpu=True):
     return out
 
 
+def bench_selections(E, ctx, steps, cpu=True):
+    """Equality selections on the symmetric facebook graph the reference's
+    tests use (FliqueSelAgg x=0, BarbellSelAgg p=6; expected counts from the
+    committed fixtures), every GPU form next to the CPU oracle."""
+    import numpy as np
+    from emptyheaded_amd import plans
+    z = np.load(os.path.join(ROOT, "tests", "golden", "facebook_duplicated.npz"))
+    with open(os.path.join(ROOT, "tests", "golden", "fixtures.json")) as f:
+        meta = json.load(f)["facebook_duplicated"]
+    s, d = z["src"], z["dst"]
+    T, R = E.Trie(ctx, s, d), E.Trie(ctx, d, s)
+    out = {"workload": "facebook_duplicated (176,468 rows, both directions): FliqueSelAgg x=0, BarbellSelAgg p=6"}
+    for name, levels, key in (("flique_sel_x0", plans.flique_sel(0), "FliqueSelAgg_x0"),
+                              ("barbell_sel_p6", plans.barbell_sel(6), "BarbellSelAgg_p6")):
+        r = {"expected": meta[key]}
+        for fname, form in (("plain", E.FORM_PLAIN), ("tuned", E.FORM_TUNED), ("learned", E.FORM_LEARNED)):
+            c, ms, _, _ = time_form(E, ctx, T, levels, form, steps, 1, tries=[T, R])
+            assert c == meta[key], "%s %s: %d" % (name, fname, c)
+            r[fname + "_ms"] = ms
+        if cpu:
+            from oracle import oracle as O
+            t = time.perf_counter()
+            assert O.join_count(levels, [O.Trie(s, d), O.Trie(d, s)], min(16, os.cpu_count() or 1)) == meta[key]
+            r["cpu_oracle_ms_including_trie_build"] = (time.perf_counter() - t) * 1e3
+        out[name] = r
+        log("%s: %d; plain %.3f ms, tuned %.3f ms, learned %.3f ms" % (name, meta[key], r["plain_ms"], r["tuned_ms"],
+                                                                     r["learned_ms"]))
+    T.close()
+    R.close()
+    return out
+
+
 def bench_four_clique(E, ctx, steps, scale=18):
     """4-clique COUNT on R-MAT scale 18: plain vs tuned vs tuned+learned."""
     from emptyheaded_amd import graphs as G
@@ -454,6 +486,7 @@ def main():
                           "rmat18_four_clique": bench_four_clique(E, ctx, max(2, args.steps // 2)),
                           "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2)),
                           "triple_star_path": bench_triples(E, ctx, max(2, args.steps // 2)),
+                          "selections": bench_selections(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "dropin_cpp_triangle": bench_dropin(src, dst, count)}
     if not args.no_cpu:
         log("cpu baseline ...")
