@@ -116,6 +116,56 @@ def bench_selections(E, ctx, steps, cpu=True):
     return out
 
 
+def bench_two_bag(E, ctx, steps, cpu=True):
+    """LollipopAgg and BarbellAgg (triangle bag per vertex, then the weighted
+    edge fold) on the symmetric facebook graph, checked against the committed
+    reference constants, and on R-MAT-20 with both edge directions, checked
+    against the CPU oracle's per-vertex triangle counts folded in numpy."""
+    import numpy as np
+    from emptyheaded_amd import ghd, plans
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import tuning as TU
+    z = np.load(os.path.join(ROOT, "tests", "golden", "facebook_duplicated.npz"))
+    with open(os.path.join(ROOT, "tests", "golden", "fixtures.json")) as f:
+        meta = json.load(f)["facebook_duplicated"]
+    rs, rd, _ = G.rmat_graph(20)
+    rs, rd = G.symmetric(rs, rd)
+    out = {}
+    TU.apply_plan("triangle")
+    for name, (s, d) in (("facebook_duplicated", (z["src"], z["dst"])), ("rmat20_symmetric", (rs, rd))):
+        S = E.Trie(ctx, s, d)
+        r = {"rows": len(s)}
+        for fname, form in (("plain", E.FORM_PLAIN), ("learned", E.FORM_LEARNED)):
+            for q, fn in (("lollipop", ghd.lollipop_count), ("barbell", ghd.barbell_count)):
+                fn(ctx, S, form)
+                ctx.synchronize()
+                t = time.perf_counter()
+                for _ in range(steps):
+                    v = fn(ctx, S, form)
+                ctx.synchronize()
+                r[q] = v
+                r["%s_%s_ms" % (q, fname)] = (time.perf_counter() - t) * 1e3 / steps
+        if name == "facebook_duplicated":
+            assert r["lollipop"] == meta["LollipopAgg"] and r["barbell"] == meta["BarbellAgg"]
+        if cpu:
+            from oracle import oracle as O
+            t = time.perf_counter()
+            OT = O.Trie(s, d)
+            nv = int(max(s.max(), d.max())) + 1
+            ph = np.zeros(nv, np.int64)
+            ph[np.unique(s)] = O.join_count_per_head(plans.TRIANGLE, [OT], min(16, os.cpu_count() or 1))
+            s64, d64 = s.astype(np.int64), d.astype(np.int64)
+            lol, bar = int(ph[s64].sum()), int((ph[s64] * ph[d64]).sum())
+            r["cpu_oracle_ms_including_trie_build"] = (time.perf_counter() - t) * 1e3
+            assert lol == r["lollipop"] and bar == r["barbell"]
+        out[name] = r
+        log("%s: lollipop %d, barbell %d; learned %.3f / %.3f ms, plain %.3f / %.3f ms" %
+            (name, r["lollipop"], r["barbell"], r["lollipop_learned_ms"], r["barbell_learned_ms"],
+             r["lollipop_plain_ms"], r["barbell_plain_ms"]))
+        S.close()
+    return out
+
+
 def bench_four_clique(E, ctx, steps, scale=18):
     """4-clique COUNT on R-MAT scale 18: plain vs tuned vs tuned+learned."""
     from emptyheaded_amd import graphs as G
@@ -487,6 +537,7 @@ def main():
                           "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2)),
                           "triple_star_path": bench_triples(E, ctx, max(2, args.steps // 2)),
                           "selections": bench_selections(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
+                          "two_bag": bench_two_bag(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "dropin_cpp_triangle": bench_dropin(src, dst, count)}
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -97,6 +97,8 @@ SIGNATURES = {
     "ehx_join_materialize": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, C.POINTER(_u32p), C.c_uint64, _u64p,
                               C.POINTER(JoinStats)], C.c_int),
     "ehx_cycle4_count": ([_vp, _vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
+    "ehx_weighted_pairs": ([_vp, _vp, C.POINTER(C.c_int64), C.c_uint64, C.POINTER(C.c_int64), C.c_uint64,
+                            C.POINTER(C.c_int64)], C.c_int),
     "ehx_cycle4_count_segments": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(JoinStats)], C.c_int),
     "ehx_gen_egonets": ([_u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
@@ -341,6 +343,17 @@ def cycle4_count(ctx, sym, begin=None, end=None, bounds=None, stats=False):
                                           counts.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(res), C.byref(st)))
     counts = counts[: len(b) - 1]
     return (counts, res.value, st.as_dict()) if stats else (counts, res.value)
+
+
+def weighted_pairs(ctx, trie, wa, wb):
+    """Sum over the tuples (a, x) of trie of wa[a] * wb[x] (int64 arrays indexed
+    by value; values past an array's end weigh 0), modulo 2^64."""
+    a = np.ascontiguousarray(wa, dtype=np.int64)
+    b = np.ascontiguousarray(wb, dtype=np.int64)
+    res = C.c_int64(0)
+    check(lib().ehx_weighted_pairs(ctx.h, trie.h, a.ctypes.data_as(C.POINTER(C.c_int64)), len(a),
+                                   b.ctypes.data_as(C.POINTER(C.c_int64)), len(b), C.byref(res)))
+    return res.value
 
 
 def tune_get():

@@ -26,6 +26,7 @@ SOURCES = [
     "ehx_selector.hip",
     "ehx_mlp.hip",
     "ehx_cycles.hip",
+    "ehx_ghd.hip",
 ]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

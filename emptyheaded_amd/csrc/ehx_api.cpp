@@ -47,6 +47,7 @@ int join_count(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint64_t, ui
 int join_count_segments(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, const uint32_t *, uint64_t, int64_t *,
                         int64_t *, ehx_join_stats *);
 int cycle4_count(ehx_ctx *, const ehx_trie *, uint64_t, uint64_t, const SegOut *, int64_t *, ehx_join_stats *);
+int weighted_pairs(ehx_ctx *, const ehx_trie *, const int64_t *, uint64_t, const int64_t *, uint64_t, int64_t *);
 int join_materialize(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint32_t *const *, uint64_t, uint64_t *,
                      ehx_join_stats *);
 int cycle4_count_segments(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *, int64_t *,
@@ -213,6 +214,13 @@ int ehx_join_count_segments(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const 
 int ehx_join_materialize(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form, uint32_t *const *cols,
                          uint64_t cap, uint64_t *rows, ehx_join_stats *stats) {
   return join_materialize(ctx, plan, tries, form, cols, cap, rows, stats);
+}
+
+int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint64_t na, const int64_t *wb,
+                       uint64_t nb, int64_t *result) {
+  if (!ctx || !t || !result || (na && !wa) || (nb && !wb)) return fail(EHX_ERR_INVALID, "null argument");
+  if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
+  return weighted_pairs(ctx, t, wa, na, wb, nb, result);
 }
 
 int ehx_cycle4_count(ehx_ctx *ctx, const ehx_trie *sym, uint64_t begin, uint64_t end, int64_t *result,

@@ -202,6 +202,16 @@ int ehx_join_count_segments(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const 
 int ehx_join_materialize(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int form,
                          uint32_t *const *cols, uint64_t cap, uint64_t *rows, ehx_join_stats *stats);
 
+/* ---- two-bag plans ----------------------------------------------------------
+ * Top bag of a plan whose lower bags were run grouped by one attribute: the sum
+ * over the tuples (a, x) of trie t of wa[a] * wb[x] (host arrays indexed by
+ * value; values >= na / nb weigh 0), exact modulo 2^64.  Replaces the
+ * generated annotated fold over the intermediate bag tries
+ * (query_compiler/src/main/scala/codegen/query/EHGenerator.scala:399-434,
+ * 711-726) for LollipopAgg / BarbellAgg-shaped plans. */
+int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint64_t na, const int64_t *wb,
+                       uint64_t nb, int64_t *result);
+
 /* ---- 4-cycle count -------------------------------------------------------- */
 /* Number of 4-cycles (each cycle once) of the undirected graph whose edges the
  * trie `sym` holds in both directions, restricted to cycles whose largest

@@ -13,6 +13,8 @@ and, for the graph with both edge directions (facebook_duplicated.tsv),
 and the selections test_duplicated runs (travis.py:647-693):
   FliqueSelAgg, x=0  = 3,759,972  (travis.py:385-400; FliqueSel rows, :429-444)
   BarbellSelAgg, p=6 = 26,936,100 (travis.py:473-489; BarbellSel rows, :519-530)
+  LollipopAgg        = 1,426,911,480      (travis.py:14-30)
+  BarbellAgg         = 20,371,831,447,136 (travis.py:55-70)
 
 Run once in a container that holds the reference tree:
     python tests/golden/make_fixtures.py /root/reference
@@ -70,7 +72,9 @@ def main(ref_root):
             "Triangle_rows": 9672060,
             "FliqueSelAgg_x0": 3759972,
             "BarbellSelAgg_p6": 26936100,
-            "cite": "docs/notebooks/Getting Started.ipynb cell 22; test/graph/travis.py:385-400,429-444,473-489,519-530",
+            "LollipopAgg": 1426911480,
+            "BarbellAgg": 20371831447136,
+            "cite": "docs/notebooks/Getting Started.ipynb cell 22; test/graph/travis.py:14-30,55-70,385-400,429-444,473-489,519-530",
         },
         "simple": {
             "source": "test/graph/data/simple.tsv",

@@ -333,3 +333,36 @@ def test_selections_vs_oracle(ctx, oracle):
             exp = oracle.join_count(levels, O, 8)
             for form in FORMS:
                 assert E.join_count(ctx, levels, [T, R], form) == exp, (node, levels, form)
+
+
+# ---- two-bag plans (lollipop, barbell) -----------------------------------------
+
+@pytest.mark.parametrize("form", FORMS)
+def test_two_bag_goldens(ctx, facebook_dup, fixtures_meta, form):
+    """LollipopAgg = 1,426,911,480 and BarbellAgg = 20,371,831,447,136 on the
+    symmetric facebook graph (travis.py:14-70): triangle bag per vertex, then
+    the weighted edge fold."""
+    from emptyheaded_amd import ghd
+    s, d = facebook_dup
+    S = E.Trie(ctx, s, d)
+    meta = fixtures_meta["facebook_duplicated"]
+    assert ghd.lollipop_count(ctx, S, form) == meta["LollipopAgg"]
+    assert ghd.barbell_count(ctx, S, form) == meta["BarbellAgg"]
+
+
+def test_weighted_pairs_against_numpy(ctx):
+    """The fold over random int64 weights (wrapping mod 2^64), weights shorter
+    than the value range (missing values weigh 0), on a graph with hubs."""
+    s, d, n = G.rmat_graph(12)
+    T = E.Trie(ctx, s, d)
+    rng = np.random.default_rng(41)
+    for na, nb in ((n, n), (n // 3, n), (n, 17), (0, n)):
+        wa = rng.integers(-2**40, 2**40, size=na, dtype=np.int64)
+        wb = rng.integers(-2**40, 2**40, size=nb, dtype=np.int64)
+        fa = np.zeros(n, np.int64)
+        fa[:na] = wa
+        fb = np.zeros(n, np.int64)
+        fb[:nb] = wb
+        with np.errstate(over="ignore"):
+            exp = int((fa[s.astype(np.int64)] * fb[d.astype(np.int64)]).sum())
+        assert E.weighted_pairs(ctx, T, wa, wb) == exp

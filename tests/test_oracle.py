@@ -121,6 +121,20 @@ def test_selections_brute_force(oracle, seed):
         assert oracle.join_count(plans.barbell_sel(node), [T, R], 2) == t6 * t6
 
 
+def test_facebook_two_bag_goldens(oracle, facebook_dup, fixtures_meta):
+    """LollipopAgg and BarbellAgg (travis.py:14-70) from the oracle's triangle
+    counts grouped by first attribute, folded over the edges."""
+    s, d = facebook_dup
+    T = oracle.Trie(s, d)
+    nv = int(max(s.max(), d.max())) + 1
+    heads = np.unique(s)
+    ph = np.zeros(nv, np.int64)
+    ph[heads] = oracle.join_count_per_head(plans.TRIANGLE, [T], 4)
+    meta = fixtures_meta["facebook_duplicated"]
+    assert int(ph[s].sum()) == meta["LollipopAgg"]
+    assert int((ph[s] * ph[d]).sum()) == meta["BarbellAgg"]
+
+
 def test_two_hop_closed_form(oracle):
     rng = np.random.default_rng(7)
     r = np.unique(rng.integers(0, 50, size=(400, 2)), axis=0)
