@@ -203,12 +203,15 @@ struct Tries {
 __device__ __forceinline__ bool resolve(const Tries &T, const LevelDesc &L, const uint32_t *tuple, DSet *S) {
   for (int i = 0; i < L.nops; i++) {
     const DTrie &t = T.t[L.rel[i]];
-    if (L.parent[i] == EHX_PARENT_HEAD) {
-      S[i] = set_of(t, t.nh);
-    } else {
-      const uint32_t key = L.parent[i] == EHX_PARENT_CONST ? L.value[i] : tuple[L.parent[i]];
-      int32_t h = node_of(t, key);
+    if (L.parent[i] >= 0) {
+      int32_t h = node_of(t, tuple[L.parent[i]]);
       if (h < 0) return false;  // NULL TrieBlock: empty (TrieBuilder.cpp:249-254)
+      S[i] = set_of(t, (uint32_t)h);
+    } else if (L.parent[i] == EHX_PARENT_HEAD) {
+      S[i] = set_of(t, t.nh);
+    } else {  // EHX_PARENT_CONST: a selection
+      int32_t h = node_of(t, L.value[i]);
+      if (h < 0) return false;
       S[i] = set_of(t, (uint32_t)h);
     }
   }
