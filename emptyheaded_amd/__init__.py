@@ -99,6 +99,7 @@ SIGNATURES = {
     "ehx_cycle4_count": ([_vp, _vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(JoinStats)], C.c_int),
     "ehx_weighted_pairs": ([_vp, _vp, C.POINTER(C.c_int64), C.c_uint64, C.POINTER(C.c_int64), C.c_uint64,
                             C.POINTER(C.c_int64)], C.c_int),
+    "ehx_sssp_unweighted": ([_vp, _vp, C.c_uint32, C.c_int32, _u32p, C.c_uint64, _u32p], C.c_int),
     "ehx_cycle4_count_segments": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(JoinStats)], C.c_int),
     "ehx_gen_egonets": ([_u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
@@ -354,6 +355,18 @@ def weighted_pairs(ctx, trie, wa, wb):
     check(lib().ehx_weighted_pairs(ctx.h, trie.h, a.ctypes.data_as(C.POINTER(C.c_int64)), len(a),
                                    b.ctypes.data_as(C.POINTER(C.c_int64)), len(b), C.byref(res)))
     return res.value
+
+
+UNREACHED = 0xFFFFFFFF
+
+
+def sssp_unweighted(ctx, trie, start, n, iterations=-1):
+    """BFS levels from `start` over the trie's child sets, for values 0..n-1
+    (UNREACHED where never reached), and the number of levels run."""
+    dist = np.empty(n, dtype=np.uint32)
+    lv = C.c_uint32(0)
+    check(lib().ehx_sssp_unweighted(ctx.h, trie.h, start, iterations, ptr_u32(dist), n, C.byref(lv)))
+    return dist, lv.value
 
 
 def tune_get():

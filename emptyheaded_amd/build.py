@@ -27,6 +27,7 @@ SOURCES = [
     "ehx_mlp.hip",
     "ehx_cycles.hip",
     "ehx_ghd.hip",
+    "ehx_closure.hip",
 ]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

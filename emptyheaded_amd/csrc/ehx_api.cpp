@@ -48,6 +48,7 @@ int join_count_segments(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, con
                         int64_t *, ehx_join_stats *);
 int cycle4_count(ehx_ctx *, const ehx_trie *, uint64_t, uint64_t, const SegOut *, int64_t *, ehx_join_stats *);
 int weighted_pairs(ehx_ctx *, const ehx_trie *, const int64_t *, uint64_t, const int64_t *, uint64_t, int64_t *);
+int sssp_unweighted(ehx_ctx *, const ehx_trie *, uint32_t, int, uint32_t *, uint64_t, uint32_t *);
 int join_materialize(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint32_t *const *, uint64_t, uint64_t *,
                      ehx_join_stats *);
 int cycle4_count_segments(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *, int64_t *,
@@ -221,6 +222,13 @@ int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint6
   if (!ctx || !t || !result || (na && !wa) || (nb && !wb)) return fail(EHX_ERR_INVALID, "null argument");
   if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
   return weighted_pairs(ctx, t, wa, na, wb, nb, result);
+}
+
+int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t iterations, uint32_t *dist_out,
+                        uint64_t nd, uint32_t *levels) {
+  if (!ctx || !t || !dist_out) return fail(EHX_ERR_INVALID, "null argument");
+  if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
+  return sssp_unweighted(ctx, t, start, iterations, dist_out, nd, levels);
 }
 
 int ehx_cycle4_count(ehx_ctx *ctx, const ehx_trie *sym, uint64_t begin, uint64_t end, int64_t *result,

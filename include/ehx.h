@@ -212,6 +212,17 @@ int ehx_join_materialize(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tr
 int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint64_t na, const int64_t *wb,
                        uint64_t nb, int64_t *result);
 
+/* ---- single-source closure --------------------------------------------------
+ * Unweighted single-source shortest paths over trie t (x's neighbours = its
+ * child set): dist_out[v] (nd host entries) = the BFS level at which v is first
+ * reached from `start` (0 for start itself), 0xFFFFFFFF if never; at most
+ * `iterations` levels (< 0: until the frontier empties); *levels = levels run.
+ * Replaces tc::unweighted_single_source (storage_engine/include/
+ * TransitiveClosure.hpp:12-123), which recursive queries such as SSSP
+ * (test/graph/travis.py:556-567) run. */
+int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t iterations, uint32_t *dist_out,
+                        uint64_t nd, uint32_t *levels);
+
 /* ---- 4-cycle count -------------------------------------------------------- */
 /* Number of 4-cycles (each cycle once) of the undirected graph whose edges the
  * trie `sym` holds in both directions, restricted to cycles whose largest

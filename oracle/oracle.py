@@ -260,3 +260,32 @@ def join_materialize(levels, tries):
     out = np.zeros((len(levels), max(n, 1)), dtype=np.uint32)
     lib().orc_join_materialize(C.byref(p), _tries_arg(tries), out.ctypes.data_as(C.POINTER(C.c_uint32)), max(n, 1))
     return out[:, :n].T.copy()
+
+
+def sssp_unweighted(src, dst, start, n, iterations=-1):
+    """Unweighted single-source closure, level by level as
+    tc::unweighted_single_source does (storage_engine/include/
+    TransitiveClosure.hpp:12-123): the start is visited at level 0; each level
+    visits the unvisited children of the frontier.  Returns (dist, levels) with
+    dist[v] = level of first visit (0xFFFFFFFF never), for v < n."""
+    src = np.asarray(src, dtype=np.int64)
+    dst = np.asarray(dst, dtype=np.int64)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(off, src[src < n] + 1, 1)
+    off = np.cumsum(off)
+    dist = np.full(n, 0xFFFFFFFF, dtype=np.uint32)
+    dist[start] = 0
+    frontier = np.array([start], dtype=np.int64)
+    level = 0
+    while len(frontier) and (iterations < 0 or level < iterations):
+        lo, hi = off[frontier], off[frontier + 1]
+        idx = np.concatenate([np.arange(a, b) for a, b in zip(lo, hi)]) if len(frontier) else np.zeros(0, np.int64)
+        kids = np.unique(dst[idx]) if len(idx) else np.zeros(0, np.int64)
+        kids = kids[(kids < n)]
+        kids = kids[dist[kids] == 0xFFFFFFFF]
+        level += 1
+        dist[kids] = level
+        frontier = kids
+    return dist, level

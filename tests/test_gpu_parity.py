@@ -366,3 +366,32 @@ def test_weighted_pairs_against_numpy(ctx):
         with np.errstate(over="ignore"):
             exp = int((fa[s.astype(np.int64)] * fb[d.astype(np.int64)]).sum())
         assert E.weighted_pairs(ctx, T, wa, wb) == exp
+
+
+# ---- single-source closure (SSSP) ------------------------------------------------
+
+def test_sssp_golden_and_oracle(ctx, oracle, facebook_dup):
+    """BFS levels: the SSSP check of travis.py:556-567 on the symmetric facebook
+    graph, and the oracle's levels on R-MAT graphs (directed and both
+    directions) from a hub, an ordinary vertex, a vertex with no children and a
+    value that is not a head, with and without an iteration limit."""
+    s, d = facebook_dup
+    n = int(max(s.max(), d.max())) + 1
+    dist, _ = E.sssp_unweighted(ctx, E.Trie(ctx, s, d), 0, n)
+    assert np.array_equal(dist, oracle.sssp_unweighted(s, d, 0, n)[0])
+    visited = np.flatnonzero(dist != E.UNREACHED)
+    assert dist[visited[1000]] == 2
+    for sym in (False, True):
+        s, d, n = G.rmat_graph(16)
+        if sym:
+            s, d = G.symmetric(s, d)
+        T = E.Trie(ctx, s, d)
+        deg = np.bincount(s.astype(np.int64), minlength=n)
+        starts = [int(np.argmax(deg)), int(np.flatnonzero(deg)[len(np.flatnonzero(deg)) // 2])]
+        starts += [int(np.flatnonzero(deg == 0)[0])] if (deg == 0).any() else []
+        for start in starts:
+            for it in (-1, 1, 3):
+                got, lv = E.sssp_unweighted(ctx, T, start, n, it)
+                exp, elv = oracle.sssp_unweighted(s, d, start, n, it)
+                assert np.array_equal(got, exp), (sym, start, it)
+                assert lv == elv

@@ -135,6 +135,32 @@ def test_facebook_two_bag_goldens(oracle, facebook_dup, fixtures_meta):
     assert int((ph[s] * ph[d]).sum()) == meta["BarbellAgg"]
 
 
+def test_sssp_against_scipy_and_golden(oracle, facebook_dup):
+    """The closure restated level by level against scipy's unweighted shortest
+    paths (directed, random graphs), and the SSSP check travis.py makes
+    (556-567): on the symmetric facebook graph from vertex 0, the 1001st
+    visited vertex (rows ordered by value) is at distance 2."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import shortest_path
+    rng = np.random.default_rng(5)
+    for n, m in ((50, 120), (300, 700), (1000, 1500)):
+        s = rng.integers(0, n, m)
+        d = rng.integers(0, n, m)
+        A = sp.csr_matrix((np.ones(m), (s, d)), shape=(n, n))
+        for start in (0, n // 2):
+            ref = shortest_path(A, directed=True, unweighted=True, indices=start)
+            dist, _ = oracle.sssp_unweighted(s, d, start, n)
+            exp = np.where(np.isinf(ref), 0xFFFFFFFF, ref).astype(np.uint32)
+            assert np.array_equal(dist, exp)
+            lim, _ = oracle.sssp_unweighted(s, d, start, n, iterations=2)
+            assert np.array_equal(lim, np.where(exp <= 2, exp, 0xFFFFFFFF).astype(np.uint32))
+    s, d = facebook_dup
+    n = int(max(s.max(), d.max())) + 1
+    dist, _ = oracle.sssp_unweighted(s, d, 0, n)
+    visited = np.flatnonzero(dist != 0xFFFFFFFF)
+    assert dist[visited[1000]] == 2
+
+
 def test_two_hop_closed_form(oracle):
     rng = np.random.default_rng(7)
     r = np.unique(rng.integers(0, 50, size=(400, 2)), axis=0)
