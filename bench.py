@@ -166,6 +166,41 @@ def bench_two_bag(E, ctx, steps, cpu=True):
     return out
 
 
+def bench_sssp(E, ctx, steps, cpu=True, scale=22):
+    """Unweighted single-source closure (BFS levels) on R-MAT scale 22 with both
+    edge directions (128 M rows) from the highest-degree vertex: traversed
+    edges per second (the rows of every reached vertex's child set), next to
+    the oracle's level-by-level restatement on the host."""
+    import numpy as np
+    from emptyheaded_amd import graphs as G
+    s, d, n = G.rmat_graph(scale)
+    s, d = G.symmetric(s, d)
+    T = E.Trie(ctx, s, d)
+    deg = np.bincount(s.astype(np.int64), minlength=n)
+    start = int(np.argmax(deg))
+    E.sssp_unweighted(ctx, T, start, n)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        dist, levels = E.sssp_unweighted(ctx, T, start, n)
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    reached = dist != E.UNREACHED
+    traversed = int(deg[reached].sum())
+    out = {"workload": "BFS levels from the highest-degree vertex, R-MAT scale %d both directions" % scale,
+           "rows": len(s), "reached": int(reached.sum()), "levels": levels, "ms": ms,
+           "traversed_edges_per_s": traversed / (ms * 1e-3)}
+    if cpu:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        od, _ = O.sssp_unweighted(s, d, start, n)
+        out["cpu_oracle_ms"] = (time.perf_counter() - t) * 1e3
+        assert np.array_equal(od, dist)
+    log("sssp: %d reached in %d levels, %.3f ms (%.3g traversed edges/s)" % (out["reached"], levels, ms,
+                                                                          out["traversed_edges_per_s"]))
+    T.close()
+    return out
+
+
 def bench_four_clique(E, ctx, steps, scale=18):
     """4-clique COUNT on R-MAT scale 18: plain vs tuned vs tuned+learned."""
     from emptyheaded_amd import graphs as G
@@ -538,6 +573,7 @@ def main():
                           "triple_star_path": bench_triples(E, ctx, max(2, args.steps // 2)),
                           "selections": bench_selections(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "two_bag": bench_two_bag(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
+                          "sssp": bench_sssp(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "dropin_cpp_triangle": bench_dropin(src, dst, count)}
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -281,7 +281,8 @@ def sssp_unweighted(src, dst, start, n, iterations=-1):
     level = 0
     while len(frontier) and (iterations < 0 or level < iterations):
         lo, hi = off[frontier], off[frontier + 1]
-        idx = np.concatenate([np.arange(a, b) for a, b in zip(lo, hi)]) if len(frontier) else np.zeros(0, np.int64)
+        cnt = hi - lo
+        idx = np.repeat(lo - np.cumsum(cnt) + cnt, cnt) + np.arange(int(cnt.sum()))  # the frontier's child rows
         kids = np.unique(dst[idx]) if len(idx) else np.zeros(0, np.int64)
         kids = kids[(kids < n)]
         kids = kids[dist[kids] == 0xFFFFFFFF]
