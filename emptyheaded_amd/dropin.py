@@ -8,7 +8,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DROPIN_LIB = os.path.join(HERE, "engine", "libehx_dropin.so")
-QUERIES = ("TriangleAgg", "FliqueAgg")
+QUERIES = ("TriangleAgg", "FliqueAgg", "Triangle", "TriangleCounts")
+# Materializing queries: output columns and whether rows carry an annotation.
+MATERIALIZED = {"Triangle": (3, False), "TriangleCounts": (1, True)}
 
 _lib = None
 
@@ -24,6 +26,11 @@ def lib():
         for q in QUERIES:
             f = getattr(L, "c_run_" + q)
             f.argtypes = [C.c_char_p, C.POINTER(C.c_ulong)]
+            f.restype = C.c_long
+        for q, (nc, annotated) in MATERIALIZED.items():
+            f = getattr(L, "c_rows_" + q)
+            f.argtypes = ([C.c_char_p, C.POINTER(C.c_uint32)] + ([C.POINTER(C.c_double)] if annotated else []) +
+                          [C.c_ulong, C.POINTER(C.c_ulong)])
             f.restype = C.c_long
         _lib = L
     return _lib
@@ -51,3 +58,20 @@ def run(query, db):
     rows = C.c_ulong(0)
     v = getattr(lib(), "c_run_" + query)(db.encode(), C.byref(rows))
     return v, rows.value
+
+
+def rows(query, db, cap):
+    """Runs a materializing query and reads its output trie back through
+    Trie::foreach: returns (rows [r, ncols] uint32, annotations or None,
+    num_rows, rows read).  At most `cap` rows are copied."""
+    nc, annotated = MATERIALIZED[query]
+    out = np.zeros((max(1, cap), nc), np.uint32)
+    ann = np.zeros(max(1, cap), np.float64) if annotated else None
+    num_rows = C.c_ulong(0)
+    args = [db.encode(), out.ctypes.data_as(C.POINTER(C.c_uint32))]
+    if annotated:
+        args.append(ann.ctypes.data_as(C.POINTER(C.c_double)))
+    args += [cap, C.byref(num_rows)]
+    got = getattr(lib(), "c_rows_" + query)(*args)
+    k = min(got, cap)
+    return out[:k], (ann[:k] if annotated else None), num_rows.value, got

@@ -21,19 +21,23 @@ struct Trie {
   std::vector<void *> encodings;
   A annotation;
   ehx_dropin::HostTrie *host;
+  ehx_dropin::OutRows *out;  // rows of a materialized output trie
 
-  Trie() : annotated(false), num_rows(0), num_columns(0), memoryBuffers(nullptr), annotation((A)0), host(nullptr) {}
+  Trie()
+      : annotated(false), num_rows(0), num_columns(0), memoryBuffers(nullptr), annotation((A)0), host(nullptr),
+        out(nullptr) {}
 
-  // An output trie of a bag (aggregate holder), Trie.hpp:46-50.
+  // An output trie of a bag, Trie.hpp:46-50: the aggregate holder of a scalar
+  // bag, or the rows its builders materialize.
   Trie(std::string path, size_t num_columns_in, bool annotated_in)
       : annotated(annotated_in), num_rows(0), num_columns(num_columns_in), memoryBuffers(new M(path, 2)),
-        annotation((A)0), host(nullptr) {}
+        annotation((A)0), host(nullptr), out(nullptr) {}
 
   // Build from encoded columns (Trie.cpp:380-510).  Binary relations.
   Trie(std::string path, std::vector<uint32_t> *max_set_sizes, std::vector<std::vector<uint32_t>> *attr_in,
        std::vector<A> *annotation_in)
       : annotated(annotation_in && !annotation_in->empty()), num_rows(0), num_columns(attr_in->size()),
-        memoryBuffers(new M(path, 2)), annotation((A)0), host(nullptr) {
+        memoryBuffers(new M(path, 2)), annotation((A)0), host(nullptr), out(nullptr) {
     (void)max_set_sizes;
     if (num_columns != 2) ehx_dropin::not_implemented("tries over relations of arity other than 2");
     assert(attr_in->at(0).size() == attr_in->at(1).size() && !attr_in->at(0).empty());
@@ -44,10 +48,33 @@ struct Trie {
   ~Trie() {
     delete memoryBuffers;
     if (host) ehx_dropin::free_host_trie(host);
+    delete out;
   }
 
-  // Trie::foreach (Trie.cpp:130-160): every tuple in lexicographic order.
+  // The row store the builders of this output trie fill.
+  ehx_dropin::OutRows *output_rows() {
+    if (!out) {
+      out = new ehx_dropin::OutRows();
+      out->ncols = (int)num_columns;
+      out->annotated = annotated;
+    }
+    return out;
+  }
+
+  // Trie::foreach (Trie.cpp:78-160): every tuple in lexicographic order, with
+  // its annotation when the trie is annotated.
   void foreach (const std::function<void(std::vector<uint32_t> *, A)> body) {
+    if (out && out->ncols > 0) {
+      const size_t nc = out->ncols, nr = out->num_rows();
+      std::vector<uint32_t> t(nc);
+      for (size_t r = 0; r < nr; r++) {
+        for (size_t c = 0; c < nc; c++) t[c] = out->rows[r * nc + c];
+        A a = (A)0;
+        if (r < out->annos.size()) memcpy(&a, &out->annos[r], sizeof(A) < 8 ? sizeof(A) : 8);
+        body(&t, a);
+      }
+      return;
+    }
     if (!host) return;
     std::vector<uint32_t> c0, c1, t(2);
     ehx_dropin::host_trie_tuples(host, c0, c1);
@@ -63,6 +90,15 @@ struct Trie {
   }
 
   TrieBlock<layout, M> *getHead() {
+    if (!host && out && out->ncols == 2 && out->num_rows()) {
+      // a materialized binary output read as the input of a later bag
+      std::vector<uint32_t> c0(out->num_rows()), c1(out->num_rows());
+      for (size_t r = 0; r < c0.size(); r++) {
+        c0[r] = out->rows[2 * r];
+        c1[r] = out->rows[2 * r + 1];
+      }
+      host = ehx_dropin::build_host_trie(c0.data(), c1.data(), c0.size());
+    }
     if (!host) return nullptr;
     return (TrieBlock<layout, M> *)ehx_dropin::block_of(host, ehx_dropin::num_heads(host));
   }

@@ -22,15 +22,36 @@ struct Operands {
   const void *blocks[8];
 };
 
-// ParTrieBuilder::build_aggregated_set (level 0): computed at once.
+// The tuples of a materialized output trie, row-major and lexicographic, with
+// one annotation per row (the value's bytes, zero-extended to 64 bits) when
+// the trie is annotated.
+struct OutRows {
+  int ncols = 0;
+  bool annotated = false;
+  std::vector<uint32_t> rows;
+  std::vector<uint64_t> annos;
+  size_t num_rows() const { return ncols ? rows.size() / ncols : 0; }
+};
+
+// ParTrieBuilder::build_aggregated_set / build_set (level 0): computed at once.
 size_t aggregate_head(const Operands &ops, Set<hybrid> *out);
-// TrieBuilder::build_aggregated_set at builder level `level` of a query with
-// `num_attrs` attributes: recorded or replayed by the pass driver.
-size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybrid> *out);
-// ParTrieBuilder::par_foreach_aggregate: runs `body` over the level-0 values
-// once per deferred level plus a final replay.
+// TrieBuilder::build_aggregated_set / build_set at builder depth `depth`
+// (0 = the level below the head) of a query with `num_attrs` attributes:
+// recorded or replayed by the pass driver.  A materialized call (build_set)
+// always yields the result's values; an aggregated call at the innermost
+// depth yields its cardinality only.
+size_t aggregate(size_t num_attrs, uint32_t depth, const Operands &ops, Set<hybrid> *out, bool materialize = false);
+// ParTrieBuilder::par_foreach_aggregate / par_foreach_builder: runs `body`
+// (tid, index, value) over the level-0 values once per deferred level plus a
+// final replay; rows emitted during the replay are gathered into `out`.
 void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
-                const std::function<void(size_t, uint32_t)> &body);
+                const std::function<void(size_t, uint32_t, uint32_t)> &body, OutRows *out = nullptr);
+// True during the final replay, the only run whose effects count.
+bool replaying();
+// Output rows of the current level-0 chunk (replay only): `prefix` followed by
+// each value of `leaf`, or one row with an annotation.
+void emit_rows(const uint32_t *prefix, int plen, const Set<hybrid> &leaf);
+void emit_row(const uint32_t *row, int ncols, uint64_t anno_bits);
 
 [[noreturn]] void not_implemented(const char *what);
 

@@ -226,6 +226,9 @@ struct Session {
   std::vector<std::vector<LevelLog>> logs;  // [chunk][level]
   std::vector<LevelBatch> batches;          // [level]
   std::vector<std::once_flag> sig_once;
+  OutRows *out = nullptr;                   // materialized output (par_foreach_builder)
+  std::vector<std::vector<uint32_t>> rows;  // [chunk] rows emitted by the replay
+  std::vector<std::vector<uint64_t>> annos; // [chunk]
 };
 
 static Session *g_sess = nullptr;
@@ -233,15 +236,36 @@ static thread_local uint64_t t_chunk = 0;
 static const uint64_t CHUNK = 128;  // par::for_range dynamic chunk (parallel.cpp:65-79)
 
 bool recording() { return g_sess && g_sess->record_level >= 0; }
+bool replaying() { return g_sess && g_sess->record_level < 0; }
 
-size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybrid> *out) {
+void emit_rows(const uint32_t *prefix, int plen, const Set<hybrid> &leaf) {
   Session *S = g_sess;
-  if (!S) not_implemented("TrieBuilder::build_aggregated_set outside par_foreach_aggregate");
-  if (ops.n < 1 || ops.n > 8) not_implemented("build_aggregated_set arity");
+  if (!S || S->record_level >= 0 || !S->out) return;
+  if (plen + 1 != S->out->ncols) not_implemented("rows of an arity other than the output trie's");
+  if (leaf.cardinality && !leaf.values) not_implemented("leaf set without host values");
+  std::vector<uint32_t> &R = S->rows[t_chunk];
+  for (uint32_t k = 0; k < leaf.cardinality; k++) {
+    R.insert(R.end(), prefix, prefix + plen);
+    R.push_back(leaf.values[k]);
+  }
+}
+
+void emit_row(const uint32_t *row, int ncols, uint64_t anno_bits) {
+  Session *S = g_sess;
+  if (!S || S->record_level >= 0 || !S->out) return;
+  if (ncols != S->out->ncols) not_implemented("rows of an arity other than the output trie's");
+  S->rows[t_chunk].insert(S->rows[t_chunk].end(), row, row + ncols);
+  S->annos[t_chunk].push_back(anno_bits);
+}
+
+size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybrid> *out, bool materialize) {
+  Session *S = g_sess;
+  if (!S) not_implemented("TrieBuilder::build_aggregated_set / build_set outside par_foreach_aggregate / builder");
+  if (ops.n < 1 || ops.n > 8) not_implemented("build_aggregated_set / build_set arity");
   for (int i = 0; i < ops.n; i++)
     if (!blk(ops, i)) { set_empty(out); return 0; }  // a NULL TrieBlock is empty (TrieBuilder.cpp:249-254)
   const bool innermost = level + 2 == num_attrs;
-  if (ops.n == 1 && (innermost || blk(ops, 0)->set.values)) {
+  if (ops.n == 1 && ((innermost && !materialize) || blk(ops, 0)->set.values)) {
     *out = blk(ops, 0)->set;  // the set itself (TrieBuilder.cpp:223-240)
     return out->cardinality;
   }
@@ -252,9 +276,11 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
     std::call_once(S->sig_once[level], [&]() {
       B.nops = ops.n;
       for (int i = 0; i < ops.n; i++) B.tries[i] = blk(ops, i)->trie;
-      B.materialized = !innermost;
+      B.materialized = materialize || !innermost;
     });
     if (B.nops != ops.n) not_implemented("two call sites with different arity at one builder level");
+    if (B.materialized != (materialize || !innermost))
+      not_implemented("build_set and build_aggregated_set at one builder level");
     for (int i = 0; i < ops.n; i++) {
       if (B.tries[i] != blk(ops, i)->trie) not_implemented("two call sites on different tries at one builder level");
       L.nodes.push_back(blk(ops, i)->node);
@@ -275,7 +301,8 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
   return out->cardinality;
 }
 
-static void run_chunks(const Set<hybrid> *level0, size_t nthreads, const std::function<void(size_t, uint32_t)> &body) {
+static void run_chunks(const Set<hybrid> *level0, size_t nthreads,
+                       const std::function<void(size_t, uint32_t, uint32_t)> &body) {
   const uint64_t n0 = level0->cardinality;
   const uint64_t nchunks = (n0 + CHUNK - 1) / CHUNK;
   std::atomic<uint64_t> next(0);
@@ -285,7 +312,7 @@ static void run_chunks(const Set<hybrid> *level0, size_t nthreads, const std::fu
       if (c >= nchunks) break;
       t_chunk = c;
       const uint64_t e = std::min(n0, (c + 1) * CHUNK);
-      for (uint64_t i = c * CHUNK; i < e; i++) body(tid, level0->values[i]);
+      for (uint64_t i = c * CHUNK; i < e; i++) body(tid, (uint32_t)i, level0->values[i]);
     }
   };
   std::vector<std::thread> th;
@@ -295,8 +322,8 @@ static void run_chunks(const Set<hybrid> *level0, size_t nthreads, const std::fu
 }
 
 void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
-                const std::function<void(size_t, uint32_t)> &body) {
-  if (g_sess) not_implemented("nested par_foreach_aggregate");
+                const std::function<void(size_t, uint32_t, uint32_t)> &body, OutRows *out) {
+  if (g_sess) not_implemented("nested par_foreach_aggregate / par_foreach_builder");
   if (level0->cardinality && !level0->values) not_implemented("level-0 set without host values");
   const uint64_t nchunks = (level0->cardinality + CHUNK - 1) / CHUNK;
   const size_t levels = num_attrs >= 2 ? num_attrs - 1 : 0;  // builder levels 0..num_attrs-2
@@ -305,6 +332,11 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
   S.logs.assign(nchunks, std::vector<LevelLog>(levels));
   S.batches.resize(levels);
   S.sig_once = std::vector<std::once_flag>(levels);
+  S.out = out;
+  if (out) {
+    S.rows.resize(nchunks);
+    S.annos.resize(nchunks);
+  }
   g_sess = &S;
   for (size_t p = 0; p < levels; p++) {
     S.record_level = (int)p;
@@ -333,6 +365,15 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
     for (auto &l : cl) l.cursor = 0;
   run_chunks(level0, nthreads, body);
   g_sess = nullptr;
+  if (out) {  // chunks follow the level-0 set, so their concatenation is lexicographic
+    size_t nr = 0, na = 0;
+    for (auto &r : S.rows) nr += r.size();
+    for (auto &a : S.annos) na += a.size();
+    out->rows.reserve(out->rows.size() + nr);
+    out->annos.reserve(out->annos.size() + na);
+    for (auto &r : S.rows) out->rows.insert(out->rows.end(), r.begin(), r.end());
+    for (auto &a : S.annos) out->annos.insert(out->annos.end(), a.begin(), a.end());
+  }
 }
 
 }  // namespace ehx_dropin

@@ -114,3 +114,15 @@ def test_tune_params_are_checked():
         assert E.tune_set(stream_lanes=4)["stream_lanes"] == 4
     finally:
         E.tune_set(**saved)
+
+
+def test_dropin_library_exports_every_query():
+    """The drop-in runtime with the restated generated queries loads without a
+    GPU and exports each query's c_run_<id> (and the row readers of the
+    materializing ones)."""
+    from emptyheaded_amd import dropin
+    L = dropin.lib()
+    for q in dropin.QUERIES:
+        assert hasattr(L, "c_run_" + q)
+    for q in dropin.MATERIALIZED:
+        assert hasattr(L, "c_rows_" + q)

@@ -51,3 +51,59 @@ def test_empty_and_missing_children(tmp_path):
     dropin.write_relation(db, "Edge", np.array([5, 5, 6], np.uint32), np.array([1, 2, 3], np.uint32))
     assert dropin.run("TriangleAgg", db)[0] == 0
     assert dropin.run("FliqueAgg", db)[0] == 0
+
+
+def test_triangle_materialized_golden(tmp_path, oracle, facebook, fixtures_meta):
+    """Triangle(a,b,c): build_set / allocate_next / foreach_builder / set_level
+    produce the output trie; Trie::foreach reads the reference's 1,612,010 rows
+    with first row (6,5,2) (test/graph/travis.py:206-220), every row equal to
+    the oracle's enumeration."""
+    meta = fixtures_meta["facebook_pruned"]
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", *facebook)
+    R, _, num_rows, got = dropin.rows("Triangle", db, meta["Triangle_rows"] + 1)
+    assert num_rows == got == meta["Triangle_rows"]
+    assert list(R[0]) == meta["Triangle_row0"]
+    assert np.array_equal(R, dropin_oracle_rows(oracle, facebook, plans.TRIANGLE))
+
+
+def dropin_oracle_rows(oracle, graph, levels):
+    return oracle.join_materialize(levels, [oracle.Trie(*graph)])
+
+
+@pytest.mark.parametrize("scale", [10, 13])
+def test_triangle_materialized_rmat(tmp_path, oracle, scale):
+    s, d, n = G.rmat_graph(scale)
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", s, d)
+    exp = dropin_oracle_rows(oracle, (s, d), plans.TRIANGLE)
+    R, _, num_rows, got = dropin.rows("Triangle", db, len(exp) + 1)
+    assert num_rows == got == len(exp)
+    assert np.array_equal(R, exp)
+
+
+def test_triangle_counts_annotated(tmp_path, oracle, facebook, fixtures_meta):
+    """TriangleCounts(a;z) z<-COUNT(b,c): build_set + allocate_annotation +
+    set_annotation give one row per head value of Edge with its triangle count
+    (the oracle's per-head counts; they sum to the reference's 1,612,010);
+    num_rows is what the generated code folds: one per (a,b) iteration."""
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", *facebook)
+    src, dst = facebook
+    heads = np.unique(src)
+    R, ann, num_rows, got = dropin.rows("TriangleCounts", db, len(heads) + 1)
+    assert got == len(heads)
+    assert np.array_equal(R[:, 0], heads)
+    per_head = oracle.join_count_per_head(plans.TRIANGLE, [oracle.Trie(src, dst)])
+    assert np.array_equal(ann.astype(np.int64), per_head)
+    assert int(ann.sum()) == fixtures_meta["facebook_pruned"]["TriangleAgg"]
+    assert num_rows == int(np.isin(dst, heads).sum())
+
+
+def test_materialized_empty_children(tmp_path):
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", np.array([5, 5, 6], np.uint32), np.array([1, 2, 3], np.uint32))
+    R, _, num_rows, got = dropin.rows("Triangle", db, 4)
+    assert got == 0 and num_rows == 0
+    R, ann, num_rows, got = dropin.rows("TriangleCounts", db, 4)
+    assert list(R[:, 0]) == [5, 6] and list(ann) == [0, 0] and num_rows == 0
