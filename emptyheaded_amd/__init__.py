@@ -100,6 +100,8 @@ SIGNATURES = {
     "ehx_weighted_pairs": ([_vp, _vp, C.POINTER(C.c_int64), C.c_uint64, C.POINTER(C.c_int64), C.c_uint64,
                             C.POINTER(C.c_int64)], C.c_int),
     "ehx_sssp_unweighted": ([_vp, _vp, C.c_uint32, C.c_int32, _u32p, C.c_uint64, _u32p], C.c_int),
+    "ehx_pagerank": ([_vp, _vp, _u32p, C.POINTER(C.c_float), C.c_uint64, C.c_void_p, C.c_int,
+                      C.POINTER(C.c_float), C.POINTER(C.c_double)], C.c_int),
     "ehx_cycle4_count_segments": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(JoinStats)], C.c_int),
     "ehx_gen_egonets": ([_u32p, _u32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
@@ -367,6 +369,29 @@ def sssp_unweighted(ctx, trie, start, n, iterations=-1):
     lv = C.c_uint32(0)
     check(lib().ehx_sssp_unweighted(ctx.h, trie.h, start, iterations, ptr_u32(dist), n, C.byref(lv)))
     return dist, lv.value
+
+
+class RankRule(C.Structure):
+    """ehx_rank_rule: R(x;y) <- base; R(x;y)*[i=iterations] <- add + mul*SUM(z;init)."""
+    _fields_ = [("base", C.c_double), ("add", C.c_double), ("mul", C.c_double), ("init", C.c_double),
+                ("iterations", C.c_int32), ("pad", C.c_int32)]
+
+
+def pagerank(ctx, trie, w_keys, w_vals, base, add=0.15, mul=0.85, init=1.0, iterations=5, form=FORM_TUNED):
+    """The reference's recursive PageRank rule over the trie's child sets with
+    the unary float relation W (sorted distinct keys): per-head float32 ranks
+    (head order) and the device time of the iterations in ms."""
+    k = np.ascontiguousarray(w_keys, dtype=np.uint32)
+    v = np.ascontiguousarray(w_vals, dtype=np.float32)
+    assert len(k) == len(v)
+    nh = trie.info()["num_heads"]
+    out = np.empty(max(1, nh), dtype=np.float32)
+    rule = RankRule(float(base), float(add), float(mul), float(init), int(iterations), 0)
+    ms = C.c_double(0.0)
+    check(lib().ehx_pagerank(ctx.h, trie.h, ptr_u32(k), v.ctypes.data_as(C.POINTER(C.c_float)), len(k),
+                             C.cast(C.pointer(rule), C.c_void_p), form, out.ctypes.data_as(C.POINTER(C.c_float)),
+                             C.byref(ms)))
+    return out[:nh], ms.value
 
 
 def tune_get():

@@ -28,6 +28,7 @@ SOURCES = [
     "ehx_cycles.hip",
     "ehx_ghd.hip",
     "ehx_closure.hip",
+    "ehx_rank.hip",
 ]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -49,6 +49,8 @@ int join_count_segments(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, con
 int cycle4_count(ehx_ctx *, const ehx_trie *, uint64_t, uint64_t, const SegOut *, int64_t *, ehx_join_stats *);
 int weighted_pairs(ehx_ctx *, const ehx_trie *, const int64_t *, uint64_t, const int64_t *, uint64_t, int64_t *);
 int sssp_unweighted(ehx_ctx *, const ehx_trie *, uint32_t, int, uint32_t *, uint64_t, uint32_t *);
+int rank_recursive(ehx_ctx *, const ehx_trie *, const uint32_t *, const float *, uint64_t, const ehx_rank_rule *, int,
+                   float *, double *);
 int join_materialize(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint32_t *const *, uint64_t, uint64_t *,
                      ehx_join_stats *);
 int cycle4_count_segments(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *, int64_t *,
@@ -229,6 +231,13 @@ int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t
   if (!ctx || !t || !dist_out) return fail(EHX_ERR_INVALID, "null argument");
   if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
   return sssp_unweighted(ctx, t, start, iterations, dist_out, nd, levels);
+}
+
+int ehx_pagerank(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *w_keys, const float *w_vals, uint64_t n_w,
+                 const ehx_rank_rule *rule, int form, float *rank_out, double *ms) {
+  if (!ctx || !t || !rule || !rank_out) return fail(EHX_ERR_INVALID, "null argument");
+  if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
+  return rank_recursive(ctx, t, w_keys, w_vals, n_w, rule, form, rank_out, ms);
 }
 
 int ehx_cycle4_count(ehx_ctx *ctx, const ehx_trie *sym, uint64_t begin, uint64_t end, int64_t *result,

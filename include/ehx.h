@@ -223,6 +223,34 @@ int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint6
 int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t iterations, uint32_t *dist_out,
                         uint64_t nd, uint32_t *levels);
 
+/* ---- recursive annotated SUM (PageRank) -----------------------------------
+ * The reference's recursive float-annotated rule
+ *   R(x;y) :- Edge(x,z),y:float<-[base].
+ *   R(x;y)*[i=iterations] :- Edge(x,z),R(z),W(z),y:float<-[add+mul*SUM(z;init)].
+ * (PageRank, test/graph/travis.py:585-598, with base = 1.0/N, add 0.15,
+ * mul 0.85, init 1.0, 5 iterations), as the generated code computes it
+ * (query_compiler/src/main/scala/codegen/query/EHGenerator.scala:398-502 and
+ * the recursion loop :752-764,820-828, reading annotations through
+ * TrieIterator::get_annotation, storage_engine/src/TrieIterator.cpp:44-59).
+ * R's keys are the head values of trie t (Edge, x's neighbours = its child
+ * set); W is a unary float relation given as sorted distinct w_keys[n_w] with
+ * annotations w_vals[n_w].  Per head x, over z ascending in child(x) ∩ R ∩ W:
+ * acc(float) += (float)((init * R(z)) * W(z)); then R'(x) = (float)(add + mul*acc).
+ * rank_out receives num_heads floats, by head node.  EHX_FORM_PLAIN folds in
+ * that order (bit-identical to the reference's fold); EHX_FORM_TUNED (and
+ * LEARNED) sums each list in parallel lanes with a fixed combining order
+ * (deterministic, rounded differently).  *ms (may be NULL) = device time of the
+ * iterations.  Values must be < 2^31. */
+typedef struct {
+  double base;        /* value of the base case (1.0/N for PageRank) */
+  double add, mul;    /* y = add + mul * SUM */
+  double init;        /* the SUM's per-term factor (SUM(z;init)) */
+  int32_t iterations; /* recursion count ([i=iterations]); 0 = the base case */
+  int32_t pad;
+} ehx_rank_rule;
+int ehx_pagerank(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *w_keys, const float *w_vals, uint64_t n_w,
+                 const ehx_rank_rule *rule, int form, float *rank_out, double *ms);
+
 /* ---- 4-cycle count -------------------------------------------------------- */
 /* Number of 4-cycles (each cycle once) of the undirected graph whose edges the
  * trie `sym` holds in both directions, restricted to cycles whose largest

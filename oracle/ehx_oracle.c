@@ -9,6 +9,7 @@
 #define _GNU_SOURCE
 #include "ehx_oracle.h"
 
+#include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -749,4 +750,98 @@ int64_t orc_cycle4_count(const uint64_t *off, const uint32_t *adj, uint32_t n, i
   free(th);
   free(tid);
   return total;
+}
+
+
+/* ---- recursive annotated SUM (PageRank) ------------------------------------ */
+typedef struct {
+  const orc_trie *t;
+  const orc_set *w;
+  const float *w_vals, *cur;
+  float *nxt;
+  double add, mul, init;
+  uint64_t next;  /* dynamic chunks of 128 heads (src/utils/parallel.cpp:65-79) */
+} rank_ctx;
+
+static void *rank_worker(void *arg) {
+  rank_ctx *c = (rank_ctx *)arg;
+  const orc_trie *t = c->t;
+  uint64_t cap = 0, zcap = 0;
+  uint8_t *b1 = NULL, *b2 = NULL;
+  uint32_t *zs = NULL;
+  for (;;) {
+    const uint64_t h0 = __atomic_fetch_add(&c->next, 128, __ATOMIC_RELAXED);
+    if (h0 >= t->nheads) break;
+    const uint64_t h1 = h0 + 128 < t->nheads ? h0 + 128 : t->nheads;
+    for (uint64_t h = h0; h < h1; h++) {
+      const orc_set *s = &t->child[h];
+      const uint64_t need = s->number_of_bytes + t->head.number_of_bytes + c->w->number_of_bytes + 16;
+      if (need > cap) {
+        cap = need;
+        b1 = (uint8_t *)realloc(b1, cap);
+        b2 = (uint8_t *)realloc(b2, cap);
+      }
+      if (s->cardinality + 64 > zcap) {  /* the z set is a subset of child(x) */
+        zcap = s->cardinality + 64;
+        zs = (uint32_t *)realloc(zs, 4 * zcap);
+      }
+      /* z set: child(x) ∩ head(R) ∩ W, smallest operand first (TrieBuilder.cpp:276-348) */
+      orc_set r1, r2;
+      const orc_set *ops[3] = {s, &t->head, c->w};
+      int mi = 0;
+      for (int i = 1; i < 3; i++)
+        if (ops[i]->cardinality < ops[mi]->cardinality) mi = i;
+      const orc_set *first = ops[mi], *rest[2];
+      for (int i = 0, k = 0; i < 3; i++)
+        if (i != mi) rest[k++] = ops[i];
+      orc_intersect(&r1, b1, first, rest[0]);
+      orc_intersect(&r2, b2, &r1, rest[1]);
+      const uint64_t nz = orc_set_decode(&r2, zs);
+      float acc = 0.0f;
+      for (uint64_t k = 0; k < nz; k++) {
+        const uint32_t z = zs[k];
+        const double rz = (double)c->cur[orc_set_find(&t->head, z)];
+        const double wz = (double)c->w_vals[orc_set_find(c->w, z)];
+        const float term = (float)(((double)1.0f * c->init) * rz * wz);
+        acc = acc + term;
+      }
+      /* g++ -O3 -march=native (cython/query/setup.py:10-20) fuses add + mul*acc
+         into one double FMA (C++'s default -ffp-contract=fast) */
+      c->nxt[h] = (float)fma(c->mul, (double)acc, c->add);
+    }
+  }
+  free(b1);
+  free(b2);
+  free(zs);
+  return NULL;
+}
+
+int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_vals, uint64_t n_w, double base,
+                 double add, double mul, double init, int iterations, int nthreads, float *out) {
+  if (nthreads < 1) nthreads = 1;
+  for (uint64_t i = 1; i < n_w; i++)
+    if (w_keys[i] <= w_keys[i - 1]) return -1;
+  const uint64_t nh = edge->nheads;
+  orc_set w;
+  const int wl = orc_layout_of(w_keys, n_w);
+  uint8_t *wbytes = (uint8_t *)malloc(orc_bytes_needed(n_w, w_keys, wl) + 16);
+  orc_set_build(wbytes, w_keys, n_w, wl, &w);
+  float *cur = (float *)malloc(4 * (nh ? nh : 1)), *nxt = (float *)malloc(4 * (nh ? nh : 1));
+  for (uint64_t h = 0; h < nh; h++) cur[h] = (float)base;
+  pthread_t *tid = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+  for (int it = 0; it < iterations; it++) {
+    rank_ctx c = {edge, &w, w_vals, cur, nxt, add, mul, init, 0};
+    for (int k = 1; k < nthreads; k++) pthread_create(&tid[k], NULL, rank_worker, &c);
+    rank_worker(&c);
+    for (int k = 1; k < nthreads; k++) pthread_join(tid[k], NULL);
+    float *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+  }
+  memcpy(out, cur, 4 * nh);
+  free(tid);
+  free(cur);
+  free(nxt);
+  free(wbytes);
+  return 0;
 }

@@ -113,6 +113,21 @@ int orc_join_count_per_head(const orc_plan *p, orc_trie *const *tries, int nthre
  * the first cap are written). */
 uint64_t orc_join_materialize(const orc_plan *p, orc_trie *const *tries, uint32_t *cols, uint64_t cap);
 
+/* Recursive annotated SUM (the reference's PageRank rule, test/graph/
+ * travis.py:585-598): R(x) = (float)base for every head x of `edge`, then
+ * `iterations` times, per head x, over z ascending in child(x) ∩ head(R) ∩ W
+ * (the N-way builder, src/TrieBuilder.cpp:276-348):
+ *   acc(float) += (float)((((double)1.0f * init) * R(z)) * W(z))
+ * with R(z), W(z) read as TrieIterator::get_annotation does (find the key's
+ * index, src/TrieIterator.cpp:44-59), then R'(x) = (float)(add + mul * acc)
+ * (query_compiler/.../EHGenerator.scala:398-502) with add + mul*acc one
+ * double FMA, as g++ -O3 -march=native -mavx2 compiles the generated code
+ * (cython/query/setup.py:10-20; C++'s default -ffp-contract=fast).  Nothing
+ * else is contracted (built with -ffp-contract=off).  W: sorted distinct keys
+ * with float annotations.  out[h] = R of head node h.  0 on success. */
+int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_vals, uint64_t n_w, double base,
+                 double add, double mul, double init, int iterations, int nthreads, float *out);
+
 /* 4-cycles (each once) of an undirected graph in symmetric CSR form (off[n+1],
  * adjacency sorted per vertex); per_vertex[a] (may be NULL) receives the cycles
  * whose largest vertex is a.  No reference counterpart. */

@@ -15,6 +15,10 @@ and the selections test_duplicated runs (travis.py:647-693):
   BarbellSelAgg, p=6 = 26,936,100 (travis.py:473-489; BarbellSel rows, :519-530)
   LollipopAgg        = 1,426,911,480      (travis.py:14-30)
   BarbellAgg         = 20,371,831,447,136 (travis.py:55-70)
+and PageRank over facebook_duplicated with test/graph/data/inv_degree.tsv
+(4,039 float32 inverse degrees): the rank of the first vertex after the base
+case and 5 iterations is 15.227079960463206, checked to 0.0001
+(travis.py:585-598; the relation is loaded as float32, travis.py:659-667).
 
 Run once in a container that holds the reference tree:
     python tests/golden/make_fixtures.py /root/reference
@@ -49,6 +53,11 @@ def main(ref_root):
     # original SNAP vertex ids (facebook_pruned is a relabelled copy).
     np.savez_compressed(os.path.join(HERE, "facebook_duplicated.npz"),
                         src=dup[:, 0].astype(np.uint32), dst=dup[:, 1].astype(np.uint32))
+    rawi = open(os.path.join(src, "inv_degree.tsv"), "rb").read()
+    inv = np.loadtxt(os.path.join(src, "inv_degree.tsv"), dtype=np.float64, delimiter="\t", ndmin=2)
+    assert inv.shape == (4039, 2)
+    np.savez_compressed(os.path.join(HERE, "inv_degree.npz"), key=inv[:, 0].astype(np.uint32),
+                        val=inv[:, 1].astype(np.float32))
     pairs = {tuple(r) for r in dup.tolist()}
     dup_is_sym = all((y, x) in pairs for (x, y) in pairs)
     meta = {
@@ -75,6 +84,16 @@ def main(ref_root):
             "LollipopAgg": 1426911480,
             "BarbellAgg": 20371831447136,
             "cite": "docs/notebooks/Getting Started.ipynb cell 22; test/graph/travis.py:14-30,55-70,385-400,429-444,473-489,519-530",
+        },
+        "inv_degree": {
+            "source": "test/graph/data/inv_degree.tsv",
+            "sha256": hashlib.sha256(rawi).hexdigest(),
+            "rows": int(inv.shape[0]),
+            "PageRank_N": int(len(np.unique(dup[:, 0]))),
+            "PageRank_iterations": 5,
+            "PageRank_row0": 15.227079960463206,
+            "PageRank_tolerance": 0.0001,
+            "cite": "test/graph/travis.py:585-598,659-667",
         },
         "simple": {
             "source": "test/graph/data/simple.tsv",

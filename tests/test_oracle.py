@@ -354,3 +354,90 @@ def test_materialized_golden_rows(oracle, facebook, fixtures_meta):
             for x in range(len(t)):
                 for y in range(x + 1, len(t)):
                     assert (t[x], t[y]) in edges
+
+
+def _inv_degree():
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inv_degree.npz"))
+    return d["key"], d["val"]
+
+
+def pagerank_f64(s, d, w_keys, w_vals, base, add, mul, init, iterations):
+    """The rule in float64 sparse algebra (an independent evaluation): R lives
+    on the head values (sources); a term exists where z is a head and in W."""
+    import scipy.sparse as sp
+    s = np.asarray(s, np.int64)
+    d = np.asarray(d, np.int64)
+    pairs = np.unique(np.stack([s, d], 1), axis=0)
+    n = int(max(pairs.max(), int(np.max(w_keys)) if len(w_keys) else 0)) + 1
+    A = sp.csr_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(n, n))
+    heads = np.unique(pairs[:, 0])
+    is_head = np.zeros(n, bool)
+    is_head[heads] = True
+    w = np.zeros(n)
+    w[np.asarray(w_keys, np.int64)] = np.asarray(w_vals, np.float64)
+    r = np.where(is_head, float(np.float32(base)), 0.0)
+    for _ in range(iterations):
+        acc = A @ (init * r * w)
+        r = np.where(is_head, add + mul * acc, 0.0)
+    return r[heads]
+
+
+def test_pagerank_oracle_golden(oracle, facebook_dup, fixtures_meta):
+    """PageRank over facebook_duplicated with the reference's inv_degree.tsv:
+    vertex 0's rank after the base case 1/N and 5 iterations is the value
+    travis.py:585-598 checks, to its tolerance 0.0001."""
+    meta = fixtures_meta["inv_degree"]
+    keys, vals = _inv_degree()
+    s, d = facebook_dup
+    N = len(np.unique(s))
+    assert N == meta["PageRank_N"]
+    r = oracle.pagerank(s, d, keys, vals, 1.0 / N, iterations=meta["PageRank_iterations"], nthreads=4)
+    assert abs(float(r[0]) - meta["PageRank_row0"]) < meta["PageRank_tolerance"]
+    # float64 sparse algebra agrees to float32 accumulation error
+    ref = pagerank_f64(s, d, keys, vals, 1.0 / N, 0.15, 0.85, 1.0, 5)
+    assert np.allclose(r, ref, rtol=2e-5, atol=0)
+
+
+def test_pagerank_oracle_against_sparse_algebra(oracle):
+    """Directed random graphs (a transposed product or a wrong key set fails),
+    W on a subset of the values, several rule constants and iteration counts."""
+    rng = np.random.default_rng(11)
+    for n, m in ((40, 150), (400, 4000), (2000, 9000)):
+        s = rng.integers(0, n, m).astype(np.uint32)
+        d = rng.integers(0, n, m).astype(np.uint32)
+        wk = np.unique(rng.integers(0, n + 20, n // 2)).astype(np.uint32)
+        wv = rng.uniform(0.01, 1.0, len(wk)).astype(np.float32)
+        for base, add, mul, init, it in ((1.0 / n, 0.15, 0.85, 1.0, 5), (0.5, 0.1, 0.5, 2.0, 3), (1.0, 0.0, 1.0, 1.0, 0)):
+            r = oracle.pagerank(s, d, wk, wv, base, add, mul, init, it, nthreads=3)
+            ref = pagerank_f64(s, d, wk, wv, base, add, mul, init, it)
+            assert np.allclose(r, ref, rtol=2e-5, atol=1e-30), (n, base, it)
+
+
+def test_pagerank_oracle_closed_form_complete_graph(oracle):
+    """On the complete directed graph K_n with W = 1/(n-1) every rank follows
+    r_k = 1 + (r_0 - 1) * 0.85^k."""
+    n = 50
+    s, d = np.nonzero(~np.eye(n, dtype=bool))
+    keys = np.arange(n, dtype=np.uint32)
+    vals = np.full(n, 1.0 / (n - 1), np.float32)
+    for k in (0, 1, 4, 9):
+        r = oracle.pagerank(s.astype(np.uint32), d.astype(np.uint32), keys, vals, 1.0 / n, iterations=k)
+        exp = 1 + (1.0 / n - 1) * 0.85 ** k
+        assert np.allclose(r, exp, rtol=1e-5)
+
+
+def test_pagerank_oracle_sequential_fold(oracle):
+    """The fold is the generated code's: float32 terms added one by one in
+    ascending z.  Terms 1, 2^-24, 2^-24 (ascending z) sum to 1 in float32 and
+    would give 1 + 2^-23 in the other order."""
+    s = np.array([0, 0, 0, 1, 2, 3], np.uint32)
+    d = np.array([1, 2, 3, 0, 0, 0], np.uint32)
+    keys = np.array([1, 2, 3], np.uint32)
+    vals = np.array([1.0, 2.0 ** -24, 2.0 ** -24], np.float32)
+    r = oracle.pagerank(s, d, keys, vals, 1.0, add=0.0, mul=1.0, init=1.0, iterations=1)
+    assert r[0] == np.float32(1.0)
+    s2 = np.array([0, 0, 0, 1, 2, 3], np.uint32)
+    d2 = np.array([3, 2, 1, 0, 0, 0], np.uint32)
+    vals2 = np.array([2.0 ** -24, 2.0 ** -24, 1.0], np.float32)  # big term last in z order
+    r2 = oracle.pagerank(s2, d2, keys, vals2, 1.0, add=0.0, mul=1.0, init=1.0, iterations=1)
+    assert r2[0] == np.float32(1.0 + 2.0 ** -23)
