@@ -201,6 +201,56 @@ def bench_sssp(E, ctx, steps, cpu=True, scale=22):
     return out
 
 
+def bench_pagerank(E, ctx, steps, cpu=True, scale=22, iterations=5):
+    """The reference's PageRank rule (base 1/N, 5 iterations of
+    0.15 + 0.85*SUM(z;1.0) over Edge(x,z),PageRank(z),InvDegree(z)) on R-MAT
+    scale 22 with both edge directions, InvDegree = 1/out-degree as float32:
+    device time of the iterations for the plain (reference-order) and tuned
+    folds, rows*iterations per second, and the HBM rate on algorithmic bytes
+    (per iteration 16 B per value of the domain for the term pass, 8 B per row
+    and 12 B per head for the fold).  The oracle times one iteration on the
+    host cores (trie built beforehand) and checks the plain result bit for bit."""
+    import numpy as np
+    from emptyheaded_amd import graphs as G
+    s, d, n = G.rmat_graph(scale)
+    s, d = G.symmetric(s, d)
+    T = E.Trie(ctx, s, d)
+    heads, deg = np.unique(s, return_counts=True)
+    keys, vals = heads.astype(np.uint32), (1.0 / deg).astype(np.float32)
+    base = 1.0 / len(heads)
+    rows, nh, V = len(s), len(heads), int(max(s.max(), d.max())) + 1
+    alg_bytes = iterations * (16.0 * V + 8.0 * rows + 12.0 * nh)
+    out = {"workload": "PageRank, R-MAT scale %d both directions, %d iterations" % (scale, iterations),
+           "rows": rows, "heads": nh, "iterations": iterations, "algorithmic_bytes": alg_bytes}
+    res = {}
+    for name, form in (("plain", E.FORM_PLAIN), ("tuned", E.FORM_TUNED)):
+        E.pagerank(ctx, T, keys, vals, base, iterations=iterations, form=form)
+        times = []
+        for _ in range(steps):
+            r, ms = E.pagerank(ctx, T, keys, vals, base, iterations=iterations, form=form)
+            times.append(ms)
+        ms = float(np.median(times))
+        res[name] = r
+        out[name + "_ms"] = ms
+        out[name + "_rows_per_s"] = rows * iterations / (ms * 1e-3)
+        out[name + "_hbm_gbs"] = alg_bytes / (ms * 1e-3) / 1e9
+    out["tuned_vs_plain_max_rel_diff"] = float(np.max(np.abs(res["tuned"] - res["plain"]) / res["plain"]))
+    if cpu:
+        from oracle import oracle as O
+        OT = O.Trie(s, d)
+        r1, _ = E.pagerank(ctx, T, keys, vals, base, iterations=1, form=E.FORM_PLAIN)
+        t = time.perf_counter()
+        o1 = O.pagerank(s, d, keys, vals, base, iterations=1, nthreads=16, trie=OT)
+        out["cpu_oracle_ms_per_iteration"] = (time.perf_counter() - t) * 1e3
+        out["cpu_oracle_threads"] = 16
+        out["plain_bit_identical_to_oracle"] = bool(np.array_equal(r1.view(np.uint32), o1.view(np.uint32)))
+        del OT
+    log("pagerank: plain %.3f ms, tuned %.3f ms for %d iterations (%.3g rows/s, %.0f GB/s)" % (
+        out["plain_ms"], out["tuned_ms"], iterations, out["tuned_rows_per_s"], out["tuned_hbm_gbs"]))
+    T.close()
+    return out
+
+
 def bench_four_clique(E, ctx, steps, scale=18):
     """4-clique COUNT on R-MAT scale 18: plain vs tuned vs tuned+learned."""
     from emptyheaded_amd import graphs as G
@@ -574,6 +624,7 @@ def main():
                           "selections": bench_selections(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "two_bag": bench_two_bag(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "sssp": bench_sssp(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
+                          "pagerank": bench_pagerank(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "dropin_cpp_triangle": bench_dropin(src, dst, count)}
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -19,9 +19,11 @@
 // the fold is a gather over each child list:
 //   plain form: one lane per head, terms added in ascending z -- the
 //     reference's order, so the result is bit-identical to its fold;
-//   tuned form: 16-lane groups per head, waves for heads of more than 512
-//     children; each lane sums a strided subsequence and the lanes combine by
-//     a fixed butterfly, so results are deterministic but rounded differently.
+//   tuned form: 16-lane groups per head; heads of more than 512 children are
+//     cut into chunks of 4096 children, one wavefront each, whose partial sums
+//     are added in chunk order.  Each lane sums a strided subsequence and the
+//     lanes combine by a fixed butterfly, so results are deterministic (the
+//     cuts depend on the degrees only) but rounded differently.
 // Nothing else may be contracted: fp contraction is off in this file, and the
 // one FMA is written out.
 #include <hip/hip_runtime.h>
@@ -34,7 +36,8 @@
 namespace ehx {
 
 constexpr uint32_t RANK_GROUP = 16;   // lanes per head in the tuned group kernel
-constexpr uint32_t RANK_BIG = 512;    // heads with more children take a wavefront
+constexpr uint32_t RANK_BIG = 512;    // heads with more children are cut into chunks
+constexpr uint32_t RANK_CHUNK = 4096; // children per chunk, one wavefront each
 
 // hv[v] = head node of value v, or -1 (the PageRank relation's keys are the heads)
 __global__ void k_rank_heads(DTrie t, uint32_t V, int32_t *__restrict__ hv) {
@@ -94,24 +97,47 @@ __global__ __launch_bounds__(256) void k_rank_group(DTrie t, const float *__rest
   for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / RANK_GROUP; h < t.nh; h += ngroups) {
     const uint32_t b = t.off[h], e = t.off[h + 1];
     if (e - b > RANK_BIG) continue;  // uniform across the group
-    float acc = 0.0f;
-    for (uint32_t k = b + g; k < e; k += RANK_GROUP) acc = __fadd_rn(acc, term[t.vals[k]]);
+    float a0 = 0.0f, a1 = 0.0f;
+    uint32_t k = b + g;
+    for (; k + RANK_GROUP < e; k += 2 * RANK_GROUP) {  // two gathers in flight per lane
+      const uint32_t z0 = t.vals[k], z1 = t.vals[k + RANK_GROUP];
+      a0 = __fadd_rn(a0, term[z0]);
+      a1 = __fadd_rn(a1, term[z1]);
+    }
+    if (k < e) a0 = __fadd_rn(a0, term[t.vals[k]]);
+    float acc = __fadd_rn(a0, a1);
 #pragma unroll
     for (uint32_t s = RANK_GROUP / 2; s >= 1; s >>= 1) acc = __fadd_rn(acc, __shfl_xor(acc, (int)s, RANK_GROUP));
     if (g == 0) out[h] = rank_finish(acc, add, mul);
   }
 }
 
-// Tuned form, heads of more than RANK_BIG children: one wavefront each, four
-// loads in flight per lane.
-__global__ __launch_bounds__(256) void k_rank_wave(DTrie t, const uint32_t *__restrict__ bigs, uint32_t nb,
-                                                   const float *__restrict__ term, double add, double mul,
-                                                   float *__restrict__ out) {
+// Tuned form, heads of more than RANK_BIG children: cnt[i] = chunks of big head i.
+__global__ void k_rank_chunk_counts(DTrie t, const uint32_t *__restrict__ bigs, uint32_t nb,
+                                    uint32_t *__restrict__ cnt) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t h = bigs[i];
+    cnt[i] = (t.off[h + 1] - t.off[h] + RANK_CHUNK - 1) / RANK_CHUNK;
+  }
+}
+
+// One wavefront per chunk j (< choff[nb]): the big head i with choff[i] <= j <
+// choff[i+1], its children [c*RANK_CHUNK, (c+1)*RANK_CHUNK) with c = j - choff[i],
+// four loads in flight per lane; partial[j] = their sum.
+__global__ __launch_bounds__(256) void k_rank_chunk(DTrie t, const uint32_t *__restrict__ bigs,
+                                                    const uint32_t *__restrict__ choff, uint32_t nb, uint32_t nch,
+                                                    const float *__restrict__ term, float *__restrict__ partial) {
   const uint32_t lane = __lane_id();
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < nb; i += nwaves) {
-    const uint32_t h = bigs[i];
-    const uint32_t b = t.off[h], e = t.off[h + 1];
+  for (uint64_t j = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; j < nch; j += nwaves) {
+    uint32_t lo = 0, hi = nb;  // last i with choff[i] <= j
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (choff[mid] <= j) lo = mid; else hi = mid;
+    }
+    const uint32_t h = bigs[lo];
+    const uint32_t b = t.off[h] + ((uint32_t)j - choff[lo]) * RANK_CHUNK;
+    const uint32_t e = min(t.off[h + 1], b + RANK_CHUNK);
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
     uint32_t k = b + lane;
     for (; k + 192 < e; k += 256) {
@@ -125,7 +151,17 @@ __global__ __launch_bounds__(256) void k_rank_wave(DTrie t, const uint32_t *__re
     float acc = __fadd_rn(__fadd_rn(a0, a1), __fadd_rn(a2, a3));
 #pragma unroll
     for (uint32_t s = 32; s >= 1; s >>= 1) acc = __fadd_rn(acc, __shfl_xor(acc, (int)s, 64));
-    if (lane == 0) out[h] = rank_finish(acc, add, mul);
+    if (lane == 0) partial[j] = acc;
+  }
+}
+
+// One lane per big head: its chunks' partial sums in chunk order, then the finish.
+__global__ void k_rank_combine(const uint32_t *__restrict__ bigs, const uint32_t *__restrict__ choff, uint32_t nb,
+                               const float *__restrict__ partial, double add, double mul, float *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (uint64_t)gridDim.x * blockDim.x) {
+    float acc = 0.0f;
+    for (uint32_t j = choff[i]; j < choff[i + 1]; j++) acc = __fadd_rn(acc, partial[j]);
+    out[bigs[i]] = rank_finish(acc, add, mul);
   }
 }
 
@@ -184,12 +220,21 @@ int rank_recursive(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *w_keys, cons
     k_rank_scatter<<<grid_for(nw_in, 256, 8192), 256, 0, st>>>(bk.as<uint32_t>(), bv.as<float>(), nw_in, wd);
     EHX_HIP(hipGetLastError());
   }
-  uint32_t nb = 0;
+  uint32_t nb = 0, nch = 0;
+  DevBuf &bn = ctx->scratch[S_COUNTS], &bo = ctx->scratch[S_OFFS], &bpart = ctx->scratch[S_ACC];
   if (form != EHX_FORM_PLAIN) {
     if ((rc = fill_u32(ctx, bc.as<uint32_t>(), 0u, 1))) return rc;
     k_rank_bigs<<<gH, 256, 0, st>>>(dv, bb.as<uint32_t>(), bc.as<unsigned>());
     EHX_HIP(hipGetLastError());
     if ((rc = copy_d2h(ctx, &nb, bc.p, 4))) return rc;
+    if (nb) {
+      if ((rc = bn.ensure((uint64_t)nb * 4 + 4)) || (rc = bo.ensure((uint64_t)nb * 4 + 4))) return rc;
+      k_rank_chunk_counts<<<grid_for(nb, 256, 8192), 256, 0, st>>>(dv, bb.as<uint32_t>(), nb, bn.as<uint32_t>());
+      EHX_HIP(hipGetLastError());
+      if ((rc = scan_u32(ctx, bn.as<uint32_t>(), bo.as<uint32_t>(), nb))) return rc;
+      if ((rc = copy_d2h(ctx, &nch, bo.as<uint32_t>() + nb, 4))) return rc;
+      if ((rc = bpart.ensure((uint64_t)nch * 4 + 4))) return rc;
+    }
   }
   k_rank_fill<<<gH, 256, 0, st>>>(cur, nh, (float)rule->base);
   EHX_HIP(hipGetLastError());
@@ -201,8 +246,12 @@ int rank_recursive(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *w_keys, cons
       k_rank_plain<<<gH, 256, 0, st>>>(dv, term, rule->add, rule->mul, nxt);
     } else {
       k_rank_group<<<grid_for(nh * RANK_GROUP, 256, 16384), 256, 0, st>>>(dv, term, rule->add, rule->mul, nxt);
-      if (nb) k_rank_wave<<<grid_for((uint64_t)nb * 64, 256, 4096), 256, 0, st>>>(dv, bb.as<uint32_t>(), nb, term,
-                                                                                rule->add, rule->mul, nxt);
+      if (nb) {
+        k_rank_chunk<<<grid_for((uint64_t)nch * 64, 256, 16384), 256, 0, st>>>(
+            dv, bb.as<uint32_t>(), bo.as<uint32_t>(), nb, nch, term, bpart.as<float>());
+        k_rank_combine<<<grid_for(nb, 256, 8192), 256, 0, st>>>(bb.as<uint32_t>(), bo.as<uint32_t>(), nb,
+                                                                 bpart.as<float>(), rule->add, rule->mul, nxt);
+      }
     }
     EHX_HIP(hipGetLastError());
     float *tmp = cur;

@@ -296,12 +296,12 @@ def sssp_unweighted(src, dst, start, n, iterations=-1):
 
 
 def pagerank(src, dst, w_keys, w_vals, base, add=0.15, mul=0.85, init=1.0, iterations=5, nthreads=1,
-             policy=POLICY_RULE):
+             policy=POLICY_RULE, trie=None):
     """The reference's recursive PageRank rule over Edge(src,dst) with the
     unary float relation W, folded per head in ascending order with float32
     accumulation as the generated code does (orc_pagerank): float32 ranks per
     head value of Edge, in head order."""
-    T = Trie(src, dst, policy)
+    T = trie if trie is not None else Trie(src, dst, policy)
     nh = int(T.head().cardinality)
     k = np.ascontiguousarray(w_keys, dtype=np.uint32)
     v = np.ascontiguousarray(w_vals, dtype=np.float32)

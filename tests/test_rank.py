@@ -111,3 +111,24 @@ def test_pagerank_edge_cases(ctx, oracle):
     vals = np.array([1.0, 2.0 ** -24, 2.0 ** -24], np.float32)
     r, _ = E.pagerank(ctx, T, keys, vals, 1.0, add=0.0, mul=1.0, init=1.0, iterations=1, form=E.FORM_PLAIN)
     assert r[0] == np.float32(1.0)
+
+
+def test_pagerank_hubs_across_chunks(ctx, oracle):
+    """Hubs of 600, 5,000 and 20,000 children (one, two and five chunks of
+    4,096 in the tuned form) beside small heads, both edge directions."""
+    rng = np.random.default_rng(8)
+    n = 30000
+    parts = [(np.full(k, h), rng.choice(n, k, replace=False)) for h, k in ((7, 600), (11, 5000), (29999, 20000))]
+    s = np.concatenate([p[0] for p in parts] + [rng.integers(0, n, 60000)]).astype(np.uint32)
+    d = np.concatenate([p[1] for p in parts] + [rng.integers(0, n, 60000)]).astype(np.uint32)
+    S, D = np.concatenate([s, d]), np.concatenate([d, s])
+    heads, deg = np.unique(S, return_counts=True)
+    keys, vals = heads.astype(np.uint32), (1.0 / deg).astype(np.float32)
+    T = E.Trie(ctx, S, D)
+    exp = oracle.pagerank(S, D, keys, vals, 1.0 / len(heads), iterations=5, nthreads=8)
+    r, _ = E.pagerank(ctx, T, keys, vals, 1.0 / len(heads), iterations=5, form=E.FORM_PLAIN)
+    assert np.array_equal(r.view(np.uint32), exp.view(np.uint32))
+    r, _ = E.pagerank(ctx, T, keys, vals, 1.0 / len(heads), iterations=5, form=E.FORM_TUNED)
+    assert tuned_close(r, exp, 5)
+    r2, _ = E.pagerank(ctx, T, keys, vals, 1.0 / len(heads), iterations=5, form=E.FORM_TUNED)
+    assert np.array_equal(r.view(np.uint32), r2.view(np.uint32))
