@@ -482,6 +482,8 @@ def main():
                     help="time only the headline form (for profiling its kernel alone)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one copy kernel of this many bytes (traffic-counter reference)")
+    ap.add_argument("--config", default=None, choices=["pagerank", "sssp", "two_bag", "selections"],
+                    help="run only this secondary workload and print its JSON (for profiling it alone)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the join's multi-GPU form (replicated trie, first attribute "
@@ -493,6 +495,14 @@ def main():
     import emptyheaded_amd as E
     from emptyheaded_amd import graphs as G
     from emptyheaded_amd import plans
+
+    if args.config:
+        ctx = E.Context(0)
+        fn = {"pagerank": bench_pagerank, "sssp": bench_sssp, "two_bag": bench_two_bag,
+              "selections": bench_selections}[args.config]
+        print(json.dumps(fn(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu)))
+        ctx.close()
+        return
 
     t = time.perf_counter()
     src, dst, nv = G.rmat_graph(args.scale, args.edge_factor)
