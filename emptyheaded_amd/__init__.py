@@ -100,6 +100,11 @@ SIGNATURES = {
     "ehx_weighted_pairs": ([_vp, _vp, C.POINTER(C.c_int64), C.c_uint64, C.POINTER(C.c_int64), C.c_uint64,
                             C.POINTER(C.c_int64)], C.c_int),
     "ehx_sssp_unweighted": ([_vp, _vp, C.c_uint32, C.c_int32, _u32p, C.c_uint64, _u32p], C.c_int),
+    "ehx_refdb_open": ([C.c_char_p, C.c_int, C.POINTER(_vp)], C.c_int),
+    "ehx_refdb_info": ([_vp, _u32p, _u64p, C.POINTER(C.c_int32)], C.c_int),
+    "ehx_refdb_columns": ([_vp, C.POINTER(_u32p), _u64p], C.c_int),
+    "ehx_refdb_close": ([_vp], C.c_int),
+    "ehx_trie_load_reference": ([_vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
     "ehx_pagerank": ([_vp, _vp, _u32p, C.POINTER(C.c_float), C.c_uint64, C.c_void_p, C.c_int,
                       C.POINTER(C.c_float), C.POINTER(C.c_double)], C.c_int),
     "ehx_cycle4_count_segments": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -207,6 +212,16 @@ class Trie:
         self.h = h
         self.ctx = ctx
         self.policy = policy
+
+    @classmethod
+    def load_reference(cls, ctx, path, anno_bytes=8, policy=POLICY_RULE):
+        """A relation the reference engine saved to disk (ram/ or mmap/ image),
+        as a device trie: Trie<A,M>::load."""
+        h = _vp()
+        check(lib().ehx_trie_load_reference(ctx.h, path.encode(), anno_bytes, policy, C.byref(h)))
+        t = cls.__new__(cls)
+        t.h, t.ctx, t.policy = h, ctx, policy
+        return t
 
     def close(self):
         if getattr(self, "h", None):
@@ -369,6 +384,23 @@ def sssp_unweighted(ctx, trie, start, n, iterations=-1):
     lv = C.c_uint32(0)
     check(lib().ehx_sssp_unweighted(ctx.h, trie.h, start, iterations, ptr_u32(dist), n, C.byref(lv)))
     return dist, lv.value
+
+
+def refdb_read(path, anno_bytes=8):
+    """Rows (columns of uint32, lexicographic) and annotations (raw 64-bit
+    values, or None) of a relation the reference engine saved to disk."""
+    h = _vp()
+    check(lib().ehx_refdb_open(path.encode(), anno_bytes, C.byref(h)))
+    try:
+        nc, rows, ann = C.c_uint32(0), C.c_uint64(0), C.c_int32(0)
+        check(lib().ehx_refdb_info(h, C.byref(nc), C.byref(rows), C.byref(ann)))
+        cols = [np.zeros(max(1, rows.value), dtype=np.uint32) for _ in range(nc.value)]
+        ptrs = (_u32p * max(1, nc.value))(*[ptr_u32(c) for c in cols])
+        annos = np.zeros(max(1, rows.value), dtype=np.uint64) if ann.value else None
+        check(lib().ehx_refdb_columns(h, ptrs, annos.ctypes.data_as(_u64p) if annos is not None else None))
+        return [c[:rows.value] for c in cols], (annos[:rows.value] if annos is not None else None)
+    finally:
+        lib().ehx_refdb_close(h)
 
 
 class RankRule(C.Structure):

@@ -19,6 +19,7 @@ ARCH = os.environ.get("EHX_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     "ehx_api.cpp",
     "ehx_graph.cpp",
+    "ehx_refdb.cpp",
     "ehx_prims.hip",
     "ehx_trie.hip",
     "ehx_join.hip",

@@ -103,12 +103,13 @@ struct Trie {
     return (TrieBlock<layout, M> *)ehx_dropin::block_of(host, ehx_dropin::num_heads(host));
   }
 
-  // Trie::load (Trie.cpp:37-75) from a relation folder written by save().
+  // Trie::load (Trie.cpp:37-75): a relation folder the reference saved, or
+  // one written by save() here.
   static Trie<A, M> *load(std::string path) {
     Trie<A, M> *t = new Trie<A, M>();
     t->memoryBuffers = new M(path, 2);
     t->num_columns = 2;
-    t->host = ehx_dropin::load_host_trie(path);
+    t->host = ehx_dropin::load_host_trie(path, (int)(sizeof(A) < 8 ? sizeof(A) : 8));
     t->num_rows = ehx_dropin::host_trie_rows(t->host);
     return t;
   }

@@ -10,7 +10,9 @@ namespace ehx_dropin {
 
 // A device-resident trie over a binary relation plus its host-side head mirror.
 HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows);
-HostTrie *load_host_trie(const std::string &path);
+// From a relation folder: the reference's saved image (ram/ or mmap/
+// trieinfo.bin and buffer files; anno_bytes = sizeof(A)), else ehx_rows.bin.
+HostTrie *load_host_trie(const std::string &path, int anno_bytes);
 void save_host_trie(const HostTrie *t, const std::string &path);
 void free_host_trie(HostTrie *t);
 size_t host_trie_rows(const HostTrie *t);

@@ -91,7 +91,28 @@ HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
 
 static std::string rows_file(const std::string &path) { return path + "/ehx_rows.bin"; }
 
-HostTrie *load_host_trie(const std::string &path) {
+static bool exists(const std::string &f) {
+  std::ifstream in(f, std::ios::binary);
+  return (bool)in;
+}
+
+HostTrie *load_host_trie(const std::string &path, int anno_bytes) {
+  if (exists(path + "/ram/trieinfo.bin") || exists(path + "/mmap/trieinfo.bin")) {
+    // a relation the reference's loader saved (Trie<A,M>::load, src/Trie.cpp:37-75)
+    ehx_refdb *db = nullptr;
+    check(ehx_refdb_open(path.c_str(), anno_bytes, &db), "ehx_refdb_open");
+    uint32_t nc = 0;
+    uint64_t rows = 0;
+    int32_t annotated = 0;
+    check(ehx_refdb_info(db, &nc, &rows, &annotated), "ehx_refdb_info");
+    if (nc != 2) not_implemented("tries over relations of arity other than 2");
+    if (annotated) not_implemented("annotated relations (TrieIterator::get_annotation)");
+    std::vector<uint32_t> c0(rows), c1(rows);
+    uint32_t *cols[2] = {c0.data(), c1.data()};
+    check(ehx_refdb_columns(db, cols, nullptr), "ehx_refdb_columns");
+    ehx_refdb_close(db);
+    return build_host_trie(c0.data(), c1.data(), rows);
+  }
   std::ifstream in(rows_file(path), std::ios::binary);
   if (!in) {
     std::cerr << "ERROR: cannot open " << rows_file(path) << std::endl;

@@ -223,6 +223,26 @@ int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint6
 int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t iterations, uint32_t *dist_out,
                         uint64_t nd, uint32_t *levels);
 
+/* ---- the reference's on-disk relations --------------------------------------
+ * A relation the reference's loader saved (Trie<A,M>::save, src/Trie.cpp:15-35;
+ * ParMemoryBuffer::save, src/utils/ParMemoryBuffer.cpp:87-99; MemoryBuffer::save,
+ * src/utils/MemoryBuffer.cpp:104-114): <dir>/ram/ (or <dir>/mmap/) trieinfo.bin,
+ * data_head.bin and data_<i>.bin, read back as Trie<A,M>::load does
+ * (src/Trie.cpp:37-75) and walked as Trie::foreach does (src/Trie.cpp:78-160).
+ * anno_bytes = sizeof(A) of the relation's annotation type (8 for void*, long
+ * and double, 4 for int and float).  Host-only: no GPU is touched.
+ * ehx_refdb_columns copies the rows (lexicographic) into num_columns host arrays
+ * of `rows` entries and, for an annotated relation, each row's annotation bytes
+ * (zero-extended to 8) into annos (may be NULL). */
+typedef struct ehx_refdb ehx_refdb;
+int ehx_refdb_open(const char *dir, int anno_bytes, ehx_refdb **out);
+int ehx_refdb_info(const ehx_refdb *db, uint32_t *num_columns, uint64_t *rows, int32_t *annotated);
+int ehx_refdb_columns(const ehx_refdb *db, uint32_t *const *cols, uint64_t *annos);
+int ehx_refdb_close(ehx_refdb *db);
+/* The same, built into a device trie (binary relations) with layout `policy`:
+ * Trie<A,M>::load of a reference database directory. */
+int ehx_trie_load_reference(ehx_ctx *ctx, const char *dir, int anno_bytes, int policy, ehx_trie **out);
+
 /* ---- recursive annotated SUM (PageRank) -----------------------------------
  * The reference's recursive float-annotated rule
  *   R(x;y) :- Edge(x,z),y:float<-[base].

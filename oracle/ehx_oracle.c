@@ -10,6 +10,7 @@
 #include "ehx_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -844,4 +845,100 @@ int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_va
   free(nxt);
   free(wbytes);
   return 0;
+}
+
+
+/* ---- the reference's on-disk relation ------------------------------------- */
+typedef struct {
+  uint8_t *p;
+  uint64_t n, cap;
+} bytebuf;
+
+static uint64_t bb_put(bytebuf *b, const void *src, uint64_t n) {
+  if (b->n + n > b->cap) {
+    b->cap = (b->n + n) * 2 + 4096;
+    b->p = (uint8_t *)realloc(b->p, b->cap);
+  }
+  const uint64_t at = b->n;
+  if (src) memcpy(b->p + at, src, n);
+  else memset(b->p + at, 0, n);
+  b->n += n;
+  return at;
+}
+
+/* TrieBlock (1 byte) + Set header (24 bytes) + the set's bytes; returns the
+ * block's offset and the offset just past the set. */
+static uint64_t put_block(bytebuf *b, const orc_set *s, uint64_t *tail) {
+  uint8_t hdr[25];
+  memset(hdr, 0, sizeof(hdr));
+  hdr[0] = (uint8_t)orc_is_sparse(s->cardinality, s->range);
+  memcpy(hdr + 1, &s->cardinality, 4);
+  memcpy(hdr + 5, &s->range, 4);
+  memcpy(hdr + 9, &s->number_of_bytes, 8);
+  hdr[17] = s->type;
+  const uint64_t at = bb_put(b, hdr, 25);
+  bb_put(b, s->data, s->number_of_bytes);
+  *tail = b->n;
+  return at;
+}
+
+static int write_buffer(const char *path, const bytebuf *b, int mmapped) {
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  if (!mmapped) fprintf(f, "%llu %llu ", (unsigned long long)b->n, (unsigned long long)b->n);
+  if (b->n && fwrite(b->p, 1, b->n, f) != b->n) { fclose(f); return -1; }
+  fclose(f);
+  return 0;
+}
+
+int orc_write_refdb(const orc_trie *t, const char *dir, int nbuffers, int mmapped) {
+  if (nbuffers < 1) return -1;
+  char path[4096];
+  snprintf(path, sizeof(path), "mkdir -p '%s/%s'", dir, mmapped ? "mmap" : "ram");
+  if (system(path) != 0) return -1;
+  bytebuf head = {0}, *bufs = (bytebuf *)calloc((size_t)nbuffers, sizeof(bytebuf));
+  uint64_t tail;
+  put_block(&head, &t->head, &tail);
+  const int sp = orc_is_sparse(t->head.cardinality, t->head.range);
+  const uint64_t next_size = t->head.cardinality == 0 ? 0 : (sp ? t->head.cardinality : (uint64_t)t->head.range + 1);
+  const uint64_t next_at = bb_put(&head, NULL, 16 * next_size);
+  for (uint64_t e = 0; e < next_size; e++) {  /* slots of absent values: no child */
+    const int32_t none = -1;
+    memcpy(head.p + next_at + 16 * e, &none, 4);
+  }
+  for (uint64_t h = 0; h < t->nheads; h++) {
+    const int b = (int)((h / 128) % (uint64_t)nbuffers);
+    uint64_t ctail;
+    const uint64_t off = put_block(&bufs[b], &t->child[h], &ctail);
+    const uint64_t slot = sp ? h : t->head_vals[h];
+    const int32_t idx = b;
+    memcpy(head.p + next_at + 16 * slot, &idx, 4);
+    memcpy(head.p + next_at + 16 * slot + 8, &off, 8);
+  }
+  /* trieinfo.bin: annotated, annotation (void*), num_rows, num_columns,
+     num_buffers, head size, per-buffer sizes (Trie.cpp:15-35) */
+  snprintf(path, sizeof(path), "%s/%s/trieinfo.bin", dir, mmapped ? "mmap" : "ram");
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  const uint8_t annotated = 0;
+  const uint64_t annotation = 0, rows = t->rows, ncols = 2, nb = (uint64_t)nbuffers;
+  fwrite(&annotated, 1, 1, f);
+  fwrite(&annotation, 8, 1, f);
+  fwrite(&rows, 8, 1, f);
+  fwrite(&ncols, 8, 1, f);
+  fwrite(&nb, 8, 1, f);
+  fwrite(&head.n, 8, 1, f);
+  for (int b = 0; b < nbuffers; b++) fwrite(&bufs[b].n, 8, 1, f);
+  fclose(f);
+  int rc = 0;
+  snprintf(path, sizeof(path), "%s/%s/data_head.bin", dir, mmapped ? "mmap" : "ram");
+  rc |= write_buffer(path, &head, mmapped);
+  for (int b = 0; b < nbuffers; b++) {
+    snprintf(path, sizeof(path), "%s/%s/data_%d.bin", dir, mmapped ? "mmap" : "ram", b);
+    rc |= write_buffer(path, &bufs[b], mmapped);
+    free(bufs[b].p);
+  }
+  free(bufs);
+  free(head.p);
+  return rc;
 }

@@ -128,6 +128,14 @@ uint64_t orc_join_materialize(const orc_plan *p, orc_trie *const *tries, uint32_
 int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_vals, uint64_t n_w, double base,
                  double add, double mul, double init, int iterations, int nthreads, float *out);
 
+/* Writes trie t as the reference engine saves a relation (src/Trie.cpp:15-35
+ * trieinfo.bin; the TrieBlock layout of build_head / recursive_build,
+ * src/Trie.cpp:226-377; MemoryBuffer::save, src/utils/MemoryBuffer.cpp:104-114,
+ * under <dir>/ram/, or raw buffer files under <dir>/mmap/ when mmapped != 0).
+ * Child blocks go to nbuffers thread buffers, chunks of 128 head values dealt
+ * round-robin as the dynamic scheduler would.  0 on success. */
+int orc_write_refdb(const orc_trie *t, const char *dir, int nbuffers, int mmapped);
+
 /* 4-cycles (each once) of an undirected graph in symmetric CSR form (off[n+1],
  * adjacency sorted per vertex); per_vertex[a] (may be NULL) receives the cycles
  * whose largest vertex is a.  No reference counterpart. */

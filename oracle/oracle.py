@@ -95,6 +95,8 @@ def lib():
         L.orc_pagerank.argtypes = [C.c_void_p, u32p, C.POINTER(C.c_float), u64, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.orc_pagerank.restype = C.c_int
+        L.orc_write_refdb.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
+        L.orc_write_refdb.restype = C.c_int
         _lib = L
     return _lib
 
@@ -311,3 +313,13 @@ def pagerank(src, dst, w_keys, w_vals, base, add=0.15, mul=0.85, init=1.0, itera
                             int(iterations), int(nthreads), out.ctypes.data_as(C.POINTER(C.c_float)))
     assert rc == 0
     return out[:nh]
+
+
+def write_refdb(src, dst, path, nbuffers=4, mmapped=False, policy=POLICY_RULE):
+    """Saves the relation (src, dst) as the reference engine does
+    (orc_write_refdb): <path>/ram/ or <path>/mmap/ with trieinfo.bin and the
+    buffer images."""
+    T = Trie(src, dst, policy)
+    rc = lib().orc_write_refdb(T.h, path.encode(), int(nbuffers), 1 if mmapped else 0)
+    assert rc == 0
+    return T

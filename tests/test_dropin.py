@@ -5,6 +5,7 @@ own call sequence."""
 import numpy as np
 import pytest
 
+import emptyheaded_amd as E
 from emptyheaded_amd import dropin
 from emptyheaded_amd import graphs as G
 from emptyheaded_amd import plans
@@ -107,3 +108,44 @@ def test_materialized_empty_children(tmp_path):
     assert got == 0 and num_rows == 0
     R, ann, num_rows, got = dropin.rows("TriangleCounts", db, 4)
     assert list(R[:, 0]) == [5, 6] and list(ann) == [0, 0] and num_rows == 0
+
+
+def write_reference_db(oracle, db, src, dst, **kw):
+    """A database folder as the reference's loader lays it out: the relation's
+    saved trie image and the uint32_t encoding (size_t count, then each key's
+    value, src/Encoding.cpp:81-104; here the value of key k is 3k + 1)."""
+    import os
+    oracle.write_refdb(src, dst, os.path.join(db, "relations", "Edge", "Edge_0_1"), **kw)
+    enc = os.path.join(db, "encodings", "uint32_t")
+    os.makedirs(enc, exist_ok=True)
+    n = int(max(src.max(), dst.max())) + 1
+    with open(os.path.join(enc, "encoding.bin"), "wb") as f:
+        f.write(np.uint64(n).tobytes())
+        f.write((np.arange(n, dtype=np.uint32) * 3 + 1).tobytes())
+
+
+@pytest.mark.parametrize("mmapped", [False, True])
+def test_reference_format_database(tmp_path, oracle, facebook, fixtures_meta, mmapped):
+    """The generated code's own Trie::load and Encoding::from_binary open a
+    database in the reference's on-disk format: TriangleAgg and FliqueAgg
+    goldens, and Triangle's rows (first row (6,5,2))."""
+    meta = fixtures_meta["facebook_pruned"]
+    db = str(tmp_path / "db")
+    write_reference_db(oracle, db, *facebook, nbuffers=4, mmapped=mmapped)
+    assert dropin.run("TriangleAgg", db)[0] == meta["TriangleAgg"]
+    assert dropin.run("FliqueAgg", db)[0] == meta["FliqueAgg"]
+    R, _, num_rows, got = dropin.rows("Triangle", db, meta["Triangle_rows"] + 1)
+    assert got == num_rows == meta["Triangle_rows"] and list(R[0]) == meta["Triangle_row0"]
+
+
+def test_trie_load_reference_device(ctx, tmp_path, oracle):
+    """ehx_trie_load_reference builds the device trie of a saved relation:
+    its tuples equal the relation's."""
+    s, d, n = G.rmat_graph(12)
+    p = str(tmp_path / "Edge_0_1")
+    T = oracle.write_refdb(s, d, p, nbuffers=3, policy=2)
+    D = E.Trie.load_reference(ctx, p)
+    a, b = D.tuples()
+    ea, eb = T.tuples()
+    assert np.array_equal(a, ea) and np.array_equal(b, eb)
+    D.close()

@@ -1,0 +1,58 @@
+"""Relations saved in the reference engine's on-disk format (trieinfo.bin and
+the bump-buffer images, src/Trie.cpp:15-75, src/utils/MemoryBuffer.cpp:104-170)
+read back by libehx's host reader.  The images are written by the oracle's
+restatement of the reference's build and save layout; the reader walks them as
+Trie::foreach does.  No GPU."""
+import os
+
+import numpy as np
+import pytest
+
+import emptyheaded_amd as E
+from emptyheaded_amd import graphs as G
+
+
+def check_roundtrip(oracle, s, d, path, **kw):
+    T = oracle.write_refdb(s, d, path, **kw)
+    a, b = T.tuples()
+    cols, annos = E.refdb_read(path)
+    assert annos is None and len(cols) == 2
+    assert np.array_equal(cols[0], a) and np.array_equal(cols[1], b)
+    return len(a)
+
+
+def test_refdb_facebook(tmp_path, oracle, facebook):
+    n = check_roundtrip(oracle, *facebook, str(tmp_path / "Edge_0_1"), nbuffers=4)
+    assert n == 88234
+
+
+@pytest.mark.parametrize("mmapped", [False, True])
+@pytest.mark.parametrize("policy", [0, 1, 2])  # density rule, all uinteger, all range_bitset
+def test_refdb_layouts_and_buffers(tmp_path, oracle, mmapped, policy):
+    """Dense (bitset) and sparse heads and child sets, NextLevel slots by value
+    and by position, one or many thread buffers, ram/ and mmap/ images."""
+    s, d, n = G.rmat_graph(11)
+    for nb in (1, 7):
+        check_roundtrip(oracle, s, d, str(tmp_path / ("r%d" % nb)), nbuffers=nb, mmapped=mmapped, policy=policy)
+    # sparse head (ids spread far apart): NextLevel indexed by position
+    check_roundtrip(oracle, (s.astype(np.uint64) * 1000 + 7).astype(np.uint32), d, str(tmp_path / "sparse"),
+                    nbuffers=3, mmapped=mmapped, policy=policy)
+
+
+def test_refdb_edge_cases(tmp_path, oracle):
+    check_roundtrip(oracle, np.array([5], np.uint32), np.array([1], np.uint32), str(tmp_path / "one"))
+    check_roundtrip(oracle, np.array([0xFFFFFFFE, 0xFFFFFFFE, 3], np.uint32),
+                    np.array([0xFFFFFFFF, 0, 0xFFFFFFFF], np.uint32), str(tmp_path / "top"))
+
+
+def test_refdb_errors(tmp_path, oracle):
+    with pytest.raises(E.EhxError):
+        E.refdb_read(str(tmp_path / "missing"))
+    s, d, n = G.rmat_graph(9)
+    p = str(tmp_path / "cut")
+    oracle.write_refdb(s, d, p, nbuffers=2)
+    f = os.path.join(p, "ram", "data_1.bin")
+    raw = open(f, "rb").read()
+    open(f, "wb").write(raw[: len(raw) // 2])  # truncated buffer image
+    with pytest.raises(E.EhxError):
+        E.refdb_read(p)
