@@ -105,6 +105,8 @@ SIGNATURES = {
     "ehx_refdb_columns": ([_vp, C.POINTER(_u32p), _u64p], C.c_int),
     "ehx_refdb_close": ([_vp], C.c_int),
     "ehx_trie_load_reference": ([_vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
+    "ehx_encoded_open": ([C.c_char_p, C.POINTER(_vp)], C.c_int),
+    "ehx_trie_build_encoded": ([_vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
     "ehx_pagerank": ([_vp, _vp, _u32p, C.POINTER(C.c_float), C.c_uint64, C.c_void_p, C.c_int,
                       C.POINTER(C.c_float), C.POINTER(C.c_double)], C.c_int),
     "ehx_cycle4_count_segments": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
@@ -212,6 +214,16 @@ class Trie:
         self.h = h
         self.ctx = ctx
         self.policy = policy
+
+    @classmethod
+    def from_encoded(cls, ctx, path, first=0, second=1, policy=POLICY_RULE):
+        """The trie of ordering (first, second) of a relation's encoded column
+        store (<path>/encoded.bin), built on the device."""
+        h = _vp()
+        check(lib().ehx_trie_build_encoded(ctx.h, path.encode(), first, second, policy, C.byref(h)))
+        t = cls.__new__(cls)
+        t.h, t.ctx, t.policy = h, ctx, policy
+        return t
 
     @classmethod
     def load_reference(cls, ctx, path, anno_bytes=8, policy=POLICY_RULE):
@@ -386,11 +398,15 @@ def sssp_unweighted(ctx, trie, start, n, iterations=-1):
     return dist, lv.value
 
 
-def refdb_read(path, anno_bytes=8):
+def refdb_read(path, anno_bytes=8, encoded=False):
     """Rows (columns of uint32, lexicographic) and annotations (raw 64-bit
-    values, or None) of a relation the reference engine saved to disk."""
+    values, or None) of a relation the reference engine saved to disk; with
+    encoded=True, the columns of its encoded column store (file order)."""
     h = _vp()
-    check(lib().ehx_refdb_open(path.encode(), anno_bytes, C.byref(h)))
+    if encoded:
+        check(lib().ehx_encoded_open(path.encode(), C.byref(h)))
+    else:
+        check(lib().ehx_refdb_open(path.encode(), anno_bytes, C.byref(h)))
     try:
         nc, rows, ann = C.c_uint32(0), C.c_uint64(0), C.c_int32(0)
         check(lib().ehx_refdb_info(h, C.byref(nc), C.byref(rows), C.byref(ann)))

@@ -239,7 +239,78 @@ int refdb_open(const char *dir, int anno_bytes, ehx_refdb **out) {
   return EHX_OK;
 }
 
+// <dir>/encoded.bin (intermediate/EncodedColumnStore.hpp:88-178): size_t
+// num_rows, num_attributes, num_annotations; per attribute a uint32 max set
+// size and num_rows uint32 keys; per annotation a size_t byte width and
+// num_rows values of that width.  Rows keep the file's order.
+int encoded_open(const char *dir, ehx_refdb **out) {
+  if (!dir || !out) return fail(EHX_ERR_INVALID, "bad argument");
+  const std::string f = std::string(dir) + "/encoded.bin";
+  std::vector<uint8_t> raw;
+  if (!read_file(f, raw)) return fail(EHX_ERR_INVALID, "cannot open %s", f.c_str());
+  uint64_t hdr[3];
+  if (raw.size() < sizeof(hdr)) return fail(EHX_ERR_INVALID, "%s: too short", f.c_str());
+  memcpy(hdr, raw.data(), sizeof(hdr));
+  const uint64_t rows = hdr[0], na = hdr[1], nann = hdr[2];
+  if (na > EHX_MAX_ATTRS || nann > 64) return fail(EHX_ERR_UNSUPPORTED, "%s: %llu attributes, %llu annotations",
+                                                   f.c_str(), (unsigned long long)na, (unsigned long long)nann);
+  size_t p = sizeof(hdr);
+  ehx_refdb *r = new ehx_refdb();
+  r->ncols = (uint32_t)na;
+  r->rows = rows;
+  r->cols.resize(na);
+  for (uint64_t a = 0; a < na; a++) {
+    if (raw.size() - p < 4 || (raw.size() - p - 4) / 4 < rows) {
+      delete r;
+      return fail(EHX_ERR_INVALID, "%s: column %llu truncated", f.c_str(), (unsigned long long)a);
+    }
+    p += 4;  // max set size
+    r->cols[a].resize(rows);
+    if (rows) memcpy(r->cols[a].data(), raw.data() + p, 4 * rows);
+    p += 4 * rows;
+  }
+  for (uint64_t k = 0; k < nann; k++) {
+    uint64_t width = 0;
+    if (raw.size() - p < 8) {
+      delete r;
+      return fail(EHX_ERR_INVALID, "%s: annotation %llu truncated", f.c_str(), (unsigned long long)k);
+    }
+    memcpy(&width, raw.data() + p, 8);
+    p += 8;
+    if (width == 0 || width > 8 || (raw.size() - p) / width < rows) {
+      delete r;
+      return fail(EHX_ERR_INVALID, "%s: annotation %llu of width %llu", f.c_str(), (unsigned long long)k,
+                  (unsigned long long)width);
+    }
+    if (k == 0) {  // the first annotation column is the relation's annotation
+      r->annotated = 1;
+      r->annos.assign(rows, 0);
+      for (uint64_t i = 0; i < rows; i++) memcpy(&r->annos[i], raw.data() + p + i * width, width);
+    }
+    p += width * rows;
+  }
+  *out = r;
+  return EHX_OK;
+}
+
 }  // namespace ehx
+
+int ehx_encoded_open(const char *dir, ehx_refdb **out) { return ehx::encoded_open(dir, out); }
+
+int ehx_trie_build_encoded(ehx_ctx *ctx, const char *dir, int first, int second, int policy, ehx_trie **out) {
+  if (!ctx || !out) return ehx::fail(EHX_ERR_INVALID, "null argument");
+  ehx_refdb *r = nullptr;
+  int rc = ehx::encoded_open(dir, &r);
+  if (rc) return rc;
+  if (first < 0 || second < 0 || (uint32_t)first >= r->ncols || (uint32_t)second >= r->ncols || first == second) {
+    const uint32_t nc = r->ncols;
+    delete r;
+    return ehx::fail(EHX_ERR_INVALID, "ordering (%d,%d) of a relation with %u attributes", first, second, nc);
+  }
+  rc = ehx_trie_build(ctx, r->cols[first].data(), r->cols[second].data(), r->rows, policy, 0, out);
+  delete r;
+  return rc;
+}
 
 int ehx_refdb_open(const char *dir, int anno_bytes, ehx_refdb **out) { return ehx::refdb_open(dir, anno_bytes, out); }
 

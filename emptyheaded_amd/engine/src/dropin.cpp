@@ -113,6 +113,30 @@ HostTrie *load_host_trie(const std::string &path, int anno_bytes) {
     ehx_refdb_close(db);
     return build_host_trie(c0.data(), c1.data(), rows);
   }
+  const size_t slash = path.find_last_of('/');
+  const std::string parent = slash == std::string::npos ? "." : path.substr(0, slash);
+  const std::string leaf = slash == std::string::npos ? path : path.substr(slash + 1);
+  if (!exists(rows_file(path)) && exists(parent + "/encoded.bin")) {
+    // only the relation's encoded column store: build this ordering's trie from
+    // it, as the reference's loader does per ordering (createDB.scala:240-275);
+    // the ordering is the folder name's suffix, <relation>_<i>_<j>
+    const size_t u2 = leaf.find_last_of('_');
+    const size_t u1 = u2 == std::string::npos || u2 == 0 ? std::string::npos : leaf.find_last_of('_', u2 - 1);
+    if (u1 == std::string::npos) not_implemented("relation folder without an _<i>_<j> ordering suffix");
+    const int first = atoi(leaf.substr(u1 + 1, u2 - u1 - 1).c_str()), second = atoi(leaf.substr(u2 + 1).c_str());
+    ehx_refdb *db = nullptr;
+    check(ehx_encoded_open(parent.c_str(), &db), "ehx_encoded_open");
+    uint32_t nc = 0;
+    uint64_t rows = 0;
+    check(ehx_refdb_info(db, &nc, &rows, nullptr), "ehx_refdb_info");
+    if (nc != 2 || first < 0 || second < 0 || first > 1 || second > 1 || first == second)
+      not_implemented("tries over relations of arity other than 2");
+    std::vector<uint32_t> c0(rows), c1(rows);
+    uint32_t *cols[2] = {first == 0 ? c0.data() : c1.data(), first == 0 ? c1.data() : c0.data()};
+    check(ehx_refdb_columns(db, cols, nullptr), "ehx_refdb_columns");
+    ehx_refdb_close(db);
+    return build_host_trie(c0.data(), c1.data(), rows);
+  }
   std::ifstream in(rows_file(path), std::ios::binary);
   if (!in) {
     std::cerr << "ERROR: cannot open " << rows_file(path) << std::endl;

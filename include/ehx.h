@@ -242,6 +242,15 @@ int ehx_refdb_close(ehx_refdb *db);
 /* The same, built into a device trie (binary relations) with layout `policy`:
  * Trie<A,M>::load of a reference database directory. */
 int ehx_trie_load_reference(ehx_ctx *ctx, const char *dir, int anno_bytes, int policy, ehx_trie **out);
+/* A relation's encoded column store, <dir>/encoded.bin
+ * (EncodedColumnStore::from_binary, storage_engine/include/intermediate/
+ * EncodedColumnStore.hpp:116-178): rows in file order, read through the
+ * ehx_refdb calls above (the first annotation column, if any, as annotations). */
+int ehx_encoded_open(const char *dir, ehx_refdb **out);
+/* The trie of ordering (first, second) of that store, built on the device:
+ * what the reference's loader does for every ordering of a relation
+ * (query_compiler/.../createDB.scala:240-275 with Trie.cpp:380-510). */
+int ehx_trie_build_encoded(ehx_ctx *ctx, const char *dir, int first, int second, int policy, ehx_trie **out);
 
 /* ---- recursive annotated SUM (PageRank) -----------------------------------
  * The reference's recursive float-annotated rule

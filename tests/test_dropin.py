@@ -149,3 +149,24 @@ def test_trie_load_reference_device(ctx, tmp_path, oracle):
     ea, eb = T.tuples()
     assert np.array_equal(a, ea) and np.array_equal(b, eb)
     D.close()
+
+
+def test_encoded_store_database(ctx, tmp_path, facebook, fixtures_meta):
+    """A database that holds only the relation's encoded column store
+    (relations/Edge/encoded.bin): the trie of each ordering is built on the
+    device from it, through the C ABI and through the generated code's
+    Trie::load of relations/Edge/Edge_0_1."""
+    import os
+    from test_refdb import write_encoded
+    s, d = facebook
+    db = str(tmp_path / "db")
+    rel = os.path.join(db, "relations", "Edge")
+    write_encoded(rel, [s, d])
+    os.makedirs(os.path.join(db, "encodings", "uint32_t"), exist_ok=True)
+    for (i, j), (a, b) in (((0, 1), (s, d)), ((1, 0), (d, s))):
+        T = E.Trie.from_encoded(ctx, rel, i, j)
+        R = E.Trie(ctx, a, b)
+        ta, tb = T.tuples()
+        ra, rb = R.tuples()
+        assert np.array_equal(ta, ra) and np.array_equal(tb, rb)
+    assert dropin.run("TriangleAgg", db)[0] == fixtures_meta["facebook_pruned"]["TriangleAgg"]

@@ -56,3 +56,37 @@ def test_refdb_errors(tmp_path, oracle):
     open(f, "wb").write(raw[: len(raw) // 2])  # truncated buffer image
     with pytest.raises(E.EhxError):
         E.refdb_read(p)
+
+
+def write_encoded(path, cols, annos=None, width=4):
+    """encoded.bin as EncodedColumnStore::to_binary lays it out
+    (intermediate/EncodedColumnStore.hpp:88-114)."""
+    os.makedirs(path, exist_ok=True)
+    n = len(cols[0])
+    with open(os.path.join(path, "encoded.bin"), "wb") as f:
+        f.write(np.array([n, len(cols), 0 if annos is None else 1], np.uint64).tobytes())
+        for c in cols:
+            f.write(np.uint32(c.max() + 1 if n else 0).tobytes())
+            f.write(np.ascontiguousarray(c, np.uint32).tobytes())
+        if annos is not None:
+            f.write(np.uint64(width).tobytes())
+            f.write(np.ascontiguousarray(annos).tobytes())
+
+
+def test_encoded_column_store(tmp_path):
+    rng = np.random.default_rng(4)
+    s = rng.integers(0, 500, 3000).astype(np.uint32)
+    d = rng.integers(0, 500, 3000).astype(np.uint32)
+    w = rng.uniform(0, 1, 3000).astype(np.float32)
+    p = str(tmp_path / "Edge")
+    write_encoded(p, [s, d], w)
+    cols, annos = E.refdb_read(p, encoded=True)
+    assert np.array_equal(cols[0], s) and np.array_equal(cols[1], d)  # file order, duplicates kept
+    assert np.array_equal(annos.astype(np.uint32).view(np.float32), w)
+    write_encoded(p, [s, d])
+    cols, annos = E.refdb_read(p, encoded=True)
+    assert annos is None and np.array_equal(cols[1], d)
+    with open(os.path.join(p, "encoded.bin"), "r+b") as f:
+        f.truncate(100)
+    with pytest.raises(E.EhxError):
+        E.refdb_read(p, encoded=True)
