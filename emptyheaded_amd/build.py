@@ -80,7 +80,8 @@ def build_dropin():
         os.path.join(ENGINE, "queries", f) for f in os.listdir(os.path.join(ENGINE, "queries")) if f.endswith(".cpp"))
     cxx = shutil.which("g++") or "g++"
     tmp = DROPIN_LIB + ".tmp"
-    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wno-unused-variable",
+    # no fp contraction: the queries' float arithmetic is exactly what their source says
+    cmd = [cxx, "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wno-unused-variable",
            "-I", os.path.join(ENGINE, "include"), "-o", tmp] + srcs + [
         "-L", HERE, "-lehx", "-Wl,-rpath,$ORIGIN/..", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -12,6 +12,20 @@
 #include "layout.hpp"
 #include "utils/ParMemoryBuffer.hpp"
 
+// An annotation value as the 64 bits the runtime keeps (zero-extended), and back.
+template <class A>
+inline uint64_t ehx_anno_bits(const A v) {
+  uint64_t b = 0;
+  memcpy(&b, &v, sizeof(A) < 8 ? sizeof(A) : 8);
+  return b;
+}
+template <class A>
+inline A ehx_anno_value(const uint64_t b) {
+  A a = (A)0;
+  memcpy(&a, &b, sizeof(A) < 8 ? sizeof(A) : 8);
+  return a;
+}
+
 template <class A, class M>
 struct Trie {
   bool annotated;
@@ -33,15 +47,28 @@ struct Trie {
       : annotated(annotated_in), num_rows(0), num_columns(num_columns_in), memoryBuffers(new M(path, 2)),
         annotation((A)0), host(nullptr), out(nullptr) {}
 
-  // Build from encoded columns (Trie.cpp:380-510).  Binary relations.
+  // Build from encoded columns (Trie.cpp:380-510): relations of one or two
+  // attributes, with one annotation per row when annotation_in is not empty.
   Trie(std::string path, std::vector<uint32_t> *max_set_sizes, std::vector<std::vector<uint32_t>> *attr_in,
        std::vector<A> *annotation_in)
       : annotated(annotation_in && !annotation_in->empty()), num_rows(0), num_columns(attr_in->size()),
         memoryBuffers(new M(path, 2)), annotation((A)0), host(nullptr), out(nullptr) {
     (void)max_set_sizes;
-    if (num_columns != 2) ehx_dropin::not_implemented("tries over relations of arity other than 2");
-    assert(attr_in->at(0).size() == attr_in->at(1).size() && !attr_in->at(0).empty());
-    host = ehx_dropin::build_host_trie(attr_in->at(0).data(), attr_in->at(1).data(), attr_in->at(0).size());
+    if (num_columns != 1 && num_columns != 2)
+      ehx_dropin::not_implemented("tries over relations of arity other than 1 or 2");
+    const size_t rows = attr_in->at(0).size();
+    assert(rows > 0 && (num_columns == 1 || attr_in->at(1).size() == rows));
+    if (annotated) assert(annotation_in->size() == rows);
+    if (num_columns == 2 && !annotated) {
+      host = ehx_dropin::build_host_trie(attr_in->at(0).data(), attr_in->at(1).data(), rows);
+    } else {
+      std::vector<uint64_t> bits;
+      if (annotated)
+        for (const A &a : *annotation_in) bits.push_back(ehx_anno_bits<A>(a));
+      host = ehx_dropin::build_annotated_trie((int)num_columns, attr_in->at(0).data(),
+                                              num_columns == 2 ? attr_in->at(1).data() : nullptr,
+                                              annotated ? bits.data() : nullptr, rows);
+    }
     num_rows = ehx_dropin::host_trie_rows(host);
   }
 
@@ -76,12 +103,15 @@ struct Trie {
       return;
     }
     if (!host) return;
-    std::vector<uint32_t> c0, c1, t(2);
+    std::vector<uint32_t> c0, c1;
     ehx_dropin::host_trie_tuples(host, c0, c1);
+    const int nc = ehx_dropin::host_trie_columns(host);
+    const std::vector<uint64_t> *an = ehx_dropin::host_trie_annos(host);
+    std::vector<uint32_t> t(nc);
     for (size_t i = 0; i < c0.size(); i++) {
       t[0] = c0[i];
-      t[1] = c1[i];
-      body(&t, (A)0);
+      if (nc == 2) t[1] = c1[i];
+      body(&t, an ? ehx_anno_value<A>(an->at(i)) : (A)0);
     }
   }
 
@@ -90,14 +120,20 @@ struct Trie {
   }
 
   TrieBlock<layout, M> *getHead() {
-    if (!host && out && out->ncols == 2 && out->num_rows()) {
-      // a materialized binary output read as the input of a later bag
-      std::vector<uint32_t> c0(out->num_rows()), c1(out->num_rows());
-      for (size_t r = 0; r < c0.size(); r++) {
-        c0[r] = out->rows[2 * r];
-        c1[r] = out->rows[2 * r + 1];
+    if (!host && out && (out->ncols == 1 || out->ncols == 2) && out->num_rows()) {
+      // a materialized output (with its annotations) read as the input of a
+      // later bag or recursion step
+      const size_t nr = out->num_rows(), nc = out->ncols;
+      std::vector<uint32_t> c0(nr), c1(nr);
+      for (size_t r = 0; r < nr; r++) {
+        c0[r] = out->rows[nc * r];
+        c1[r] = out->rows[nc * r + nc - 1];
       }
-      host = ehx_dropin::build_host_trie(c0.data(), c1.data(), c0.size());
+      const bool an = out->annotated && out->annos.size() == nr;
+      if (nc == 2 && !an)
+        host = ehx_dropin::build_host_trie(c0.data(), c1.data(), nr);
+      else
+        host = ehx_dropin::build_annotated_trie((int)nc, c0.data(), c1.data(), an ? out->annos.data() : nullptr, nr);
     }
     if (!host) return nullptr;
     return (TrieBlock<layout, M> *)ehx_dropin::block_of(host, ehx_dropin::num_heads(host));
@@ -108,8 +144,9 @@ struct Trie {
   static Trie<A, M> *load(std::string path) {
     Trie<A, M> *t = new Trie<A, M>();
     t->memoryBuffers = new M(path, 2);
-    t->num_columns = 2;
     t->host = ehx_dropin::load_host_trie(path, (int)(sizeof(A) < 8 ? sizeof(A) : 8));
+    t->num_columns = (size_t)ehx_dropin::host_trie_columns(t->host);
+    t->annotated = ehx_dropin::host_trie_annotated(t->host);
     t->num_rows = ehx_dropin::host_trie_rows(t->host);
     return t;
   }

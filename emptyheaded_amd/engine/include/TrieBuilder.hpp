@@ -17,14 +17,6 @@
 #include "Trie.hpp"
 #include "utils/parallel.hpp"
 
-// An annotation value as the 64 bits an output row keeps (zero-extended).
-template <class A>
-inline uint64_t ehx_anno_bits(const A v) {
-  uint64_t b = 0;
-  memcpy(&b, &v, sizeof(A) < 8 ? sizeof(A) : 8);
-  return b;
-}
-
 template <class A, class M>
 struct TrieBuilder {
   Trie<A, M> *trie;

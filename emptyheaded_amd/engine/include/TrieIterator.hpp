@@ -23,10 +23,11 @@ struct TrieIterator {
     const TrieBlock<layout, M> *b = levels.at(level);
     levels.at(level + 1) = b ? b->get_next_block(index, data, trie->memoryBuffers) : nullptr;
   }
+  // TrieIterator::get_annotation (TrieIterator.cpp:44-59): the annotation of
+  // `data` in the set at `level`, (A)0 when absent.
   A get_annotation(const size_t level, const uint32_t data) {
-    (void)level;
-    (void)data;
-    ehx_dropin::not_implemented("annotated relations (TrieIterator::get_annotation)");
+    const TrieBlock<layout, M> *b = levels.at(level);
+    return b ? ehx_anno_value<A>(ehx_dropin::annotation_of(b, data)) : (A)0;
   }
   const TrieBlock<hybrid, M> *get_block(const size_t level) const { return levels.at(level); }
 };

@@ -13,6 +13,15 @@ HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows);
 // From a relation folder: the reference's saved image (ram/ or mmap/
 // trieinfo.bin and buffer files; anno_bytes = sizeof(A)), else ehx_rows.bin.
 HostTrie *load_host_trie(const std::string &path, int anno_bytes);
+// An annotated relation of arity 1 or 2 (annos: one value's bytes per row,
+// zero-extended; NULL: unannotated), rows in any order.
+HostTrie *build_annotated_trie(int ncols, const uint32_t *c0, const uint32_t *c1, const uint64_t *annos,
+                               size_t rows);
+bool host_trie_annotated(const HostTrie *t);
+const std::vector<uint64_t> *host_trie_annos(const HostTrie *t);  // NULL when unannotated
+int host_trie_columns(const HostTrie *t);
+// TrieIterator::get_annotation: the annotation bytes of `data` in the block's set.
+uint64_t annotation_of(const void *block, uint32_t data);
 void save_host_trie(const HostTrie *t, const std::string &path);
 void free_host_trie(HostTrie *t);
 size_t host_trie_rows(const HostTrie *t);

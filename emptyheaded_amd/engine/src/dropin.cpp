@@ -24,9 +24,15 @@ namespace ehx_dropin {
 struct HostTrie {
   ehx_trie *dev = nullptr;
   uint64_t nh = 0, m = 0;
+  int ncols = 2;  // 1: a unary relation, kept as the device trie of (x, x)
   std::vector<uint32_t> head, off;
   std::vector<int32_t> dense;
   std::vector<Block> blocks;  // nh + 1 (the last is the head block)
+  // annotated relations: one annotation per row in trie order (per head for a
+  // unary relation), and the child values on the host to find a row's position
+  bool annotated = false;
+  std::vector<uint64_t> annos;
+  std::vector<uint32_t> child_vals;
 };
 
 static std::mutex g_mu;
@@ -89,6 +95,63 @@ HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
   return t;
 }
 
+// Rows sorted lexicographically and deduplicated with their annotations (the
+// last of equal rows wins); a unary relation is the trie of (x, x).
+HostTrie *build_annotated_trie(int ncols, const uint32_t *c0, const uint32_t *c1, const uint64_t *annos,
+                               size_t rows) {
+  if (ncols != 1 && ncols != 2) not_implemented("tries over relations of arity other than 1 or 2");
+  const uint32_t *second = ncols == 1 ? c0 : c1;
+  std::vector<uint64_t> idx(rows);
+  for (size_t i = 0; i < rows; i++) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) {
+    return c0[a] != c0[b] ? c0[a] < c0[b] : second[a] < second[b];
+  });
+  std::vector<uint32_t> s0, s1;
+  std::vector<uint64_t> an;
+  s0.reserve(rows);
+  s1.reserve(rows);
+  for (size_t k = 0; k < rows; k++) {
+    const uint64_t i = idx[k];
+    if (!s0.empty() && s0.back() == c0[i] && s1.back() == second[i]) {
+      if (annos) an.back() = annos[i];
+      continue;
+    }
+    s0.push_back(c0[i]);
+    s1.push_back(second[i]);
+    if (annos) an.push_back(annos[i]);
+  }
+  HostTrie *t = build_host_trie(s0.data(), s1.data(), s0.size());
+  t->ncols = ncols;
+  if (annos) {
+    t->annotated = true;
+    t->annos = std::move(an);
+    if (ncols == 2) t->child_vals = std::move(s1);
+  }
+  return t;
+}
+
+bool host_trie_annotated(const HostTrie *t) { return t->annotated; }
+const std::vector<uint64_t> *host_trie_annos(const HostTrie *t) { return t->annotated ? &t->annos : nullptr; }
+int host_trie_columns(const HostTrie *t) { return t->ncols; }
+
+// TrieIterator::get_annotation (src/TrieIterator.cpp:44-59): the annotation of
+// `data` in the set of block `b`, 0 when absent.
+uint64_t annotation_of(const void *block, uint32_t data) {
+  const Block *b = (const Block *)block;
+  if (!b || !b->trie) return 0;
+  const HostTrie *t = b->trie;
+  if (!t->annotated) not_implemented("TrieIterator::get_annotation on an unannotated relation");
+  if (b->node == t->nh) {  // the head set
+    if (t->ncols != 1) not_implemented("annotations above a relation's last level");
+    const long h = head_node(t, data);
+    return h < 0 ? 0 : t->annos[h];
+  }
+  const uint32_t lo = t->off[b->node], hi = t->off[b->node + 1];
+  auto it = std::lower_bound(t->child_vals.begin() + lo, t->child_vals.begin() + hi, data);
+  if (it == t->child_vals.begin() + hi || *it != data) return 0;
+  return t->annos[it - t->child_vals.begin()];
+}
+
 static std::string rows_file(const std::string &path) { return path + "/ehx_rows.bin"; }
 
 static bool exists(const std::string &f) {
@@ -105,13 +168,14 @@ HostTrie *load_host_trie(const std::string &path, int anno_bytes) {
     uint64_t rows = 0;
     int32_t annotated = 0;
     check(ehx_refdb_info(db, &nc, &rows, &annotated), "ehx_refdb_info");
-    if (nc != 2) not_implemented("tries over relations of arity other than 2");
-    if (annotated) not_implemented("annotated relations (TrieIterator::get_annotation)");
-    std::vector<uint32_t> c0(rows), c1(rows);
+    if (nc != 1 && nc != 2) not_implemented("tries over relations of arity other than 1 or 2");
+    std::vector<uint32_t> c0(rows), c1(nc == 2 ? rows : 0);
+    std::vector<uint64_t> an(annotated ? rows : 0);
     uint32_t *cols[2] = {c0.data(), c1.data()};
-    check(ehx_refdb_columns(db, cols, nullptr), "ehx_refdb_columns");
+    check(ehx_refdb_columns(db, cols, annotated ? an.data() : nullptr), "ehx_refdb_columns");
     ehx_refdb_close(db);
-    return build_host_trie(c0.data(), c1.data(), rows);
+    if (!annotated && nc == 2) return build_host_trie(c0.data(), c1.data(), rows);
+    return build_annotated_trie((int)nc, c0.data(), c1.data(), annotated ? an.data() : nullptr, rows);
   }
   const size_t slash = path.find_last_of('/');
   const std::string parent = slash == std::string::npos ? "." : path.substr(0, slash);
@@ -120,22 +184,30 @@ HostTrie *load_host_trie(const std::string &path, int anno_bytes) {
     // only the relation's encoded column store: build this ordering's trie from
     // it, as the reference's loader does per ordering (createDB.scala:240-275);
     // the ordering is the folder name's suffix, <relation>_<i>_<j>
-    const size_t u2 = leaf.find_last_of('_');
-    const size_t u1 = u2 == std::string::npos || u2 == 0 ? std::string::npos : leaf.find_last_of('_', u2 - 1);
-    if (u1 == std::string::npos) not_implemented("relation folder without an _<i>_<j> ordering suffix");
-    const int first = atoi(leaf.substr(u1 + 1, u2 - u1 - 1).c_str()), second = atoi(leaf.substr(u2 + 1).c_str());
     ehx_refdb *db = nullptr;
     check(ehx_encoded_open(parent.c_str(), &db), "ehx_encoded_open");
     uint32_t nc = 0;
     uint64_t rows = 0;
-    check(ehx_refdb_info(db, &nc, &rows, nullptr), "ehx_refdb_info");
-    if (nc != 2 || first < 0 || second < 0 || first > 1 || second > 1 || first == second)
-      not_implemented("tries over relations of arity other than 2");
+    int32_t annotated = 0;
+    check(ehx_refdb_info(db, &nc, &rows, &annotated), "ehx_refdb_info");
     std::vector<uint32_t> c0(rows), c1(rows);
-    uint32_t *cols[2] = {first == 0 ? c0.data() : c1.data(), first == 0 ? c1.data() : c0.data()};
-    check(ehx_refdb_columns(db, cols, nullptr), "ehx_refdb_columns");
+    std::vector<uint64_t> an(annotated ? rows : 0);
+    if (nc == 1) {
+      uint32_t *cols[1] = {c0.data()};
+      check(ehx_refdb_columns(db, cols, annotated ? an.data() : nullptr), "ehx_refdb_columns");
+    } else {
+      const size_t u2 = leaf.find_last_of('_');
+      const size_t u1 = u2 == std::string::npos || u2 == 0 ? std::string::npos : leaf.find_last_of('_', u2 - 1);
+      if (u1 == std::string::npos) not_implemented("relation folder without an _<i>_<j> ordering suffix");
+      const int first = atoi(leaf.substr(u1 + 1, u2 - u1 - 1).c_str()), second = atoi(leaf.substr(u2 + 1).c_str());
+      if (nc != 2 || first < 0 || second < 0 || first > 1 || second > 1 || first == second)
+        not_implemented("tries over relations of arity other than 1 or 2");
+      uint32_t *cols[2] = {first == 0 ? c0.data() : c1.data(), first == 0 ? c1.data() : c0.data()};
+      check(ehx_refdb_columns(db, cols, annotated ? an.data() : nullptr), "ehx_refdb_columns");
+    }
     ehx_refdb_close(db);
-    return build_host_trie(c0.data(), c1.data(), rows);
+    if (!annotated && nc == 2) return build_host_trie(c0.data(), c1.data(), rows);
+    return build_annotated_trie((int)nc, c0.data(), c1.data(), annotated ? an.data() : nullptr, rows);
   }
   std::ifstream in(rows_file(path), std::ios::binary);
   if (!in) {
@@ -309,7 +381,12 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
   if (ops.n < 1 || ops.n > 8) not_implemented("build_aggregated_set / build_set arity");
   for (int i = 0; i < ops.n; i++)
     if (!blk(ops, i)) { set_empty(out); return 0; }  // a NULL TrieBlock is empty (TrieBuilder.cpp:249-254)
-  const bool innermost = level + 2 == num_attrs;
+  // The innermost aggregated level yields a cardinality only, unless an operand
+  // is annotated: then the generated code loops over the set to read the
+  // annotations (EHGenerator.scala:435-456), so its values are needed.
+  bool any_annotated = false;
+  for (int i = 0; i < ops.n; i++) any_annotated = any_annotated || blk(ops, i)->trie->annotated;
+  const bool innermost = level + 2 == num_attrs && !any_annotated;
   if (ops.n == 1 && ((innermost && !materialize) || blk(ops, 0)->set.values)) {
     *out = blk(ops, 0)->set;  // the set itself (TrieBuilder.cpp:223-240)
     return out->cardinality;

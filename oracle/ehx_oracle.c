@@ -761,6 +761,7 @@ typedef struct {
   const float *w_vals, *cur;
   float *nxt;
   double add, mul, init;
+  int fused;
   uint64_t next;  /* dynamic chunks of 128 heads (src/utils/parallel.cpp:65-79) */
 } rank_ctx;
 
@@ -807,8 +808,8 @@ static void *rank_worker(void *arg) {
         acc = acc + term;
       }
       /* g++ -O3 -march=native (cython/query/setup.py:10-20) fuses add + mul*acc
-         into one double FMA (C++'s default -ffp-contract=fast) */
-      c->nxt[h] = (float)fma(c->mul, (double)acc, c->add);
+         into one double FMA (C++'s default -ffp-contract=fast); unfused on request */
+      c->nxt[h] = c->fused ? (float)fma(c->mul, (double)acc, c->add) : (float)(c->add + c->mul * (double)acc);
     }
   }
   free(b1);
@@ -818,7 +819,7 @@ static void *rank_worker(void *arg) {
 }
 
 int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_vals, uint64_t n_w, double base,
-                 double add, double mul, double init, int iterations, int nthreads, float *out) {
+                 double add, double mul, double init, int iterations, int nthreads, int fused_finish, float *out) {
   if (nthreads < 1) nthreads = 1;
   for (uint64_t i = 1; i < n_w; i++)
     if (w_keys[i] <= w_keys[i - 1]) return -1;
@@ -831,7 +832,7 @@ int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_va
   for (uint64_t h = 0; h < nh; h++) cur[h] = (float)base;
   pthread_t *tid = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
   for (int it = 0; it < iterations; it++) {
-    rank_ctx c = {edge, &w, w_vals, cur, nxt, add, mul, init, 0};
+    rank_ctx c = {edge, &w, w_vals, cur, nxt, add, mul, init, fused_finish, 0};
     for (int k = 1; k < nthreads; k++) pthread_create(&tid[k], NULL, rank_worker, &c);
     rank_worker(&c);
     for (int k = 1; k < nthreads; k++) pthread_join(tid[k], NULL);

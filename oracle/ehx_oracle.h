@@ -122,11 +122,13 @@ uint64_t orc_join_materialize(const orc_plan *p, orc_trie *const *tries, uint32_
  * index, src/TrieIterator.cpp:44-59), then R'(x) = (float)(add + mul * acc)
  * (query_compiler/.../EHGenerator.scala:398-502) with add + mul*acc one
  * double FMA, as g++ -O3 -march=native -mavx2 compiles the generated code
- * (cython/query/setup.py:10-20; C++'s default -ffp-contract=fast).  Nothing
- * else is contracted (built with -ffp-contract=off).  W: sorted distinct keys
- * with float annotations.  out[h] = R of head node h.  0 on success. */
+ * (cython/query/setup.py:10-20; C++'s default -ffp-contract=fast), or a
+ * multiply then add when fused_finish is 0.  Each term is rounded to float
+ * before it is added (what that -O3 build's vectorized loop does; nothing else
+ * is contracted, built with -ffp-contract=off).  W: sorted distinct keys with
+ * float annotations.  out[h] = R of head node h.  0 on success. */
 int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_vals, uint64_t n_w, double base,
-                 double add, double mul, double init, int iterations, int nthreads, float *out);
+                 double add, double mul, double init, int iterations, int nthreads, int fused_finish, float *out);
 
 /* Writes trie t as the reference engine saves a relation (src/Trie.cpp:15-35
  * trieinfo.bin; the TrieBlock layout of build_head / recursive_build,

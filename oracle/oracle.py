@@ -93,7 +93,7 @@ def lib():
         L.orc_cycle4_count.argtypes = [C.POINTER(C.c_uint64), u32p, C.c_uint32, C.c_int, C.POINTER(C.c_int64)]
         L.orc_cycle4_count.restype = C.c_int64
         L.orc_pagerank.argtypes = [C.c_void_p, u32p, C.POINTER(C.c_float), u64, C.c_double, C.c_double, C.c_double,
-                                   C.c_double, C.c_int, C.c_int, C.POINTER(C.c_float)]
+                                   C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
         L.orc_pagerank.restype = C.c_int
         L.orc_write_refdb.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
         L.orc_write_refdb.restype = C.c_int
@@ -298,7 +298,7 @@ def sssp_unweighted(src, dst, start, n, iterations=-1):
 
 
 def pagerank(src, dst, w_keys, w_vals, base, add=0.15, mul=0.85, init=1.0, iterations=5, nthreads=1,
-             policy=POLICY_RULE, trie=None):
+             policy=POLICY_RULE, trie=None, fused_finish=True):
     """The reference's recursive PageRank rule over Edge(src,dst) with the
     unary float relation W, folded per head in ascending order with float32
     accumulation as the generated code does (orc_pagerank): float32 ranks per
@@ -310,7 +310,8 @@ def pagerank(src, dst, w_keys, w_vals, base, add=0.15, mul=0.85, init=1.0, itera
     out = np.zeros(max(1, nh), dtype=np.float32)
     rc = lib().orc_pagerank(T.h, k.ctypes.data_as(C.POINTER(C.c_uint32)), v.ctypes.data_as(C.POINTER(C.c_float)),
                             len(k), C.c_double(base), C.c_double(add), C.c_double(mul), C.c_double(init),
-                            int(iterations), int(nthreads), out.ctypes.data_as(C.POINTER(C.c_float)))
+                            int(iterations), int(nthreads), 1 if fused_finish else 0,
+                            out.ctypes.data_as(C.POINTER(C.c_float)))
     assert rc == 0
     return out[:nh]
 

@@ -170,3 +170,35 @@ def test_encoded_store_database(ctx, tmp_path, facebook, fixtures_meta):
         ra, rb = R.tuples()
         assert np.array_equal(ta, ra) and np.array_equal(tb, rb)
     assert dropin.run("TriangleAgg", db)[0] == fixtures_meta["facebook_pruned"]["TriangleAgg"]
+
+
+def test_annotated_relations_pagerank(ctx, tmp_path, facebook_dup, fixtures_meta):
+    """Annotated input tries through the drop-in API (PageRankSteps.cpp: the
+    reference's PageRank rule in the generated code's idiom, hand-written):
+    InvDegree as a unary float relation read from its encoded column store,
+    each step's PageRank output read back through get_annotation by the next
+    step.  The drop-in is built without fp contraction, so its ranks equal the
+    oracle's with an unfused finish bit for bit, and the device's plain form
+    (fused finish, as the reference's build) to one float ulp; both meet the
+    reference's golden (travis.py:585-598)."""
+    import os
+    from test_refdb import write_encoded
+    meta = fixtures_meta["inv_degree"]
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inv_degree.npz"))
+    keys, vals = d["key"], d["val"]
+    s, t = facebook_dup
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", s, t)
+    write_encoded(os.path.join(db, "relations", "InvDegree"), [keys], vals, width=4)
+    heads = np.unique(s)
+    R, ann, num_rows, got = dropin.rows("PageRankSteps", db, len(heads) + 1)
+    assert got == len(heads) and np.array_equal(R[:, 0], heads)
+    ranks = ann.astype(np.float32)
+    assert abs(float(ranks[0]) - meta["PageRank_row0"]) < meta["PageRank_tolerance"]
+    from oracle import oracle as O
+    unfused = O.pagerank(s, t, keys, vals, 1.0 / len(heads), iterations=5, nthreads=4, fused_finish=False)
+    assert np.array_equal(ranks.view(np.uint32), unfused.view(np.uint32))
+    T = E.Trie(ctx, s, t)
+    exp, _ = E.pagerank(ctx, T, keys, vals, 1.0 / len(heads), iterations=5, form=E.FORM_PLAIN)
+    ulps = np.abs(ranks.view(np.int32).astype(np.int64) - exp.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1
