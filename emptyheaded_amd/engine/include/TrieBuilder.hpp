@@ -162,6 +162,16 @@ struct ParTrieBuilder {
     return ehx_dropin::aggregate_head(o, &head_result);
   }
 
+  // ParTrieBuilder::build_aggregated_equality_selection_set (TrieBuilder.cpp:
+  // 461-467): the index of `data` in s1's set, as Set::find gives it (-1 absent).
+  // Answered from a head set's host values; a child set's values stay on the GPU.
+  size_t build_aggregated_equality_selection_set(const uint32_t data, const TrieBlock<hybrid, M> *s1) {
+    if (!s1) return (size_t)-1;
+    if (!s1->set.values && s1->set.cardinality)
+      ehx_dropin::not_implemented("equality selection over a child set (values on the GPU)");
+    return (size_t)s1->get_set()->find(data);
+  }
+
   template <typename F>
   void par_foreach_aggregate(F f) {
     for (auto *b : builders) {
