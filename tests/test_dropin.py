@@ -202,3 +202,31 @@ def test_annotated_relations_pagerank(ctx, tmp_path, facebook_dup, fixtures_meta
     exp, _ = E.pagerank(ctx, T, keys, vals, 1.0 / len(heads), iterations=5, form=E.FORM_PLAIN)
     ulps = np.abs(ranks.view(np.int32).astype(np.int64) - exp.view(np.int32).astype(np.int64))
     assert ulps.max() <= 1
+
+
+def test_annotated_binary_relation(tmp_path):
+    """An annotated binary relation (WeightedDegree.cpp, hand-written in the
+    generated code's idiom): per x, the float32 sum of its edges' annotations
+    in ascending z through TrieIterator::get_annotation(1, z), bit for bit;
+    duplicate rows keep the last annotation in file order."""
+    import os
+    from test_refdb import write_encoded
+    rng = np.random.default_rng(9)
+    s = rng.integers(0, 3000, 40000).astype(np.uint32)
+    d = rng.integers(0, 3000, 40000).astype(np.uint32)
+    w = rng.uniform(-1, 1, 40000).astype(np.float32)
+    db = str(tmp_path / "db")
+    write_encoded(os.path.join(db, "relations", "Edge"), [s, d], w, width=4)
+    order = np.lexsort((d, s))
+    last = {}
+    for i in order:  # stable: the last of equal rows in file order wins
+        last[(int(s[i]), int(d[i]))] = w[i]
+    heads = np.unique(s)
+    exp = np.zeros(len(heads), np.float32)
+    keys = sorted(last)
+    hi = {int(h): k for k, h in enumerate(heads)}
+    for (x, z) in keys:  # ascending (x, z)
+        exp[hi[x]] = np.float32(exp[hi[x]] + np.float32(last[(x, z)]))
+    R, ann, num_rows, got = dropin.rows("WeightedDegree", db, len(heads) + 1)
+    assert got == len(heads) and np.array_equal(R[:, 0], heads)
+    assert np.array_equal(ann.astype(np.float32).view(np.uint32), exp.view(np.uint32))
