@@ -9,7 +9,9 @@
 #define _GNU_SOURCE
 #include "ehx_oracle.h"
 
+#include <errno.h>
 #include <math.h>
+#include <sys/stat.h>
 #include <stdio.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -895,8 +897,16 @@ static int write_buffer(const char *path, const bytebuf *b, int mmapped) {
 int orc_write_refdb(const orc_trie *t, const char *dir, int nbuffers, int mmapped) {
   if (nbuffers < 1) return -1;
   char path[4096];
-  snprintf(path, sizeof(path), "mkdir -p '%s/%s'", dir, mmapped ? "mmap" : "ram");
-  if (system(path) != 0) return -1;
+  snprintf(path, sizeof(path), "%s", dir);
+  for (char *p = path + 1; *p; p++)  /* every missing parent of dir, then dir */
+    if (*p == '/') {
+      *p = 0;
+      mkdir(path, 0777);
+      *p = '/';
+    }
+  mkdir(path, 0777);
+  snprintf(path, sizeof(path), "%s/%s", dir, mmapped ? "mmap" : "ram");
+  if (mkdir(path, 0777) != 0 && errno != EEXIST) return -1;
   bytebuf head = {0}, *bufs = (bytebuf *)calloc((size_t)nbuffers, sizeof(bytebuf));
   uint64_t tail;
   put_block(&head, &t->head, &tail);
