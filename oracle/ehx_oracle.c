@@ -894,8 +894,30 @@ static int write_buffer(const char *path, const bytebuf *b, int mmapped) {
   return 0;
 }
 
+int orc_write_refdb_annotated(const orc_trie *t, const char *dir, int nbuffers, int mmapped, int ncols,
+                              const uint8_t *annos, int anno_bytes);
+
+/* A[nextSize] after a set (TrieBlock::set_annotation, slot by position when
+   sparse, by value when dense), from the rows' annotation bytes. */
+static void put_annos(bytebuf *b, const orc_set *s, const uint8_t *annos, uint64_t row0, int anno_bytes) {
+  const int sp = orc_is_sparse(s->cardinality, s->range);
+  const uint64_t next_size = s->cardinality == 0 ? 0 : (sp ? s->cardinality : (uint64_t)s->range + 1);
+  const uint64_t at = bb_put(b, NULL, (uint64_t)anno_bytes * next_size);
+  uint32_t *v = (uint32_t *)malloc(4 * ((uint64_t)s->cardinality + 1));
+  const uint64_t n = orc_set_decode(s, v);
+  for (uint64_t i = 0; i < n; i++)
+    memcpy(b->p + at + (uint64_t)anno_bytes * (sp ? i : v[i]), annos + (row0 + i) * (uint64_t)anno_bytes,
+           (size_t)anno_bytes);
+  free(v);
+}
+
 int orc_write_refdb(const orc_trie *t, const char *dir, int nbuffers, int mmapped) {
-  if (nbuffers < 1) return -1;
+  return orc_write_refdb_annotated(t, dir, nbuffers, mmapped, 2, NULL, 8);
+}
+
+int orc_write_refdb_annotated(const orc_trie *t, const char *dir, int nbuffers, int mmapped, int ncols,
+                              const uint8_t *annos, int anno_bytes) {
+  if (nbuffers < 1 || (ncols != 1 && ncols != 2) || anno_bytes < 1 || anno_bytes > 8) return -1;
   char path[4096];
   snprintf(path, sizeof(path), "%s", dir);
   for (char *p = path + 1; *p; p++)  /* every missing parent of dir, then dir */
@@ -912,31 +934,39 @@ int orc_write_refdb(const orc_trie *t, const char *dir, int nbuffers, int mmappe
   put_block(&head, &t->head, &tail);
   const int sp = orc_is_sparse(t->head.cardinality, t->head.range);
   const uint64_t next_size = t->head.cardinality == 0 ? 0 : (sp ? t->head.cardinality : (uint64_t)t->head.range + 1);
-  const uint64_t next_at = bb_put(&head, NULL, 16 * next_size);
-  for (uint64_t e = 0; e < next_size; e++) {  /* slots of absent values: no child */
-    const int32_t none = -1;
-    memcpy(head.p + next_at + 16 * e, &none, 4);
-  }
-  for (uint64_t h = 0; h < t->nheads; h++) {
-    const int b = (int)((h / 128) % (uint64_t)nbuffers);
-    uint64_t ctail;
-    const uint64_t off = put_block(&bufs[b], &t->child[h], &ctail);
-    const uint64_t slot = sp ? h : t->head_vals[h];
-    const int32_t idx = b;
-    memcpy(head.p + next_at + 16 * slot, &idx, 4);
-    memcpy(head.p + next_at + 16 * slot + 8, &off, 8);
+  if (ncols == 1) {  /* a unary relation: the head set, then its annotations */
+    if (annos) put_annos(&head, &t->head, annos, 0, anno_bytes);
+  } else {
+    const uint64_t next_at = bb_put(&head, NULL, 16 * next_size);
+    for (uint64_t e = 0; e < next_size; e++) {  /* slots of absent values: no child */
+      const int32_t none = -1;
+      memcpy(head.p + next_at + 16 * e, &none, 4);
+    }
+    uint64_t row = 0;
+    for (uint64_t h = 0; h < t->nheads; h++) {
+      const int b = (int)((h / 128) % (uint64_t)nbuffers);
+      uint64_t ctail;
+      const uint64_t off = put_block(&bufs[b], &t->child[h], &ctail);
+      if (annos) put_annos(&bufs[b], &t->child[h], annos, row, anno_bytes);
+      row += t->child[h].cardinality;
+      const uint64_t slot = sp ? h : t->head_vals[h];
+      const int32_t idx = b;
+      memcpy(head.p + next_at + 16 * slot, &idx, 4);
+      memcpy(head.p + next_at + 16 * slot + 8, &off, 8);
+    }
   }
   /* trieinfo.bin: annotated, annotation (void*), num_rows, num_columns,
      num_buffers, head size, per-buffer sizes (Trie.cpp:15-35) */
   snprintf(path, sizeof(path), "%s/%s/trieinfo.bin", dir, mmapped ? "mmap" : "ram");
   FILE *f = fopen(path, "wb");
   if (!f) return -1;
-  const uint8_t annotated = 0;
-  const uint64_t annotation = 0, rows = t->rows, ncols = 2, nb = (uint64_t)nbuffers;
+  const uint8_t annotated = annos != NULL;
+  const uint64_t annotation = 0, rows = ncols == 1 ? t->nheads : t->rows, nc = (uint64_t)ncols,
+                 nb = (uint64_t)nbuffers;
   fwrite(&annotated, 1, 1, f);
-  fwrite(&annotation, 8, 1, f);
+  fwrite(&annotation, (size_t)anno_bytes, 1, f);
   fwrite(&rows, 8, 1, f);
-  fwrite(&ncols, 8, 1, f);
+  fwrite(&nc, 8, 1, f);
   fwrite(&nb, 8, 1, f);
   fwrite(&head.n, 8, 1, f);
   for (int b = 0; b < nbuffers; b++) fwrite(&bufs[b].n, 8, 1, f);

@@ -137,6 +137,13 @@ int orc_pagerank(const orc_trie *edge, const uint32_t *w_keys, const float *w_va
  * Child blocks go to nbuffers thread buffers, chunks of 128 head values dealt
  * round-robin as the dynamic scheduler would.  0 on success. */
 int orc_write_refdb(const orc_trie *t, const char *dir, int nbuffers, int mmapped);
+/* The same with ncols 1 (the head set alone: a unary relation) or 2, and, when
+ * annos is not NULL, one anno_bytes-wide annotation per row in the trie's row
+ * order, written after the last level's sets as the reference places them
+ * (src/Trie.cpp:357-376 for two columns, :490-503 for one); trieinfo.bin's
+ * annotation field is anno_bytes wide (sizeof(A)). */
+int orc_write_refdb_annotated(const orc_trie *t, const char *dir, int nbuffers, int mmapped, int ncols,
+                              const uint8_t *annos, int anno_bytes);
 
 /* 4-cycles (each once) of an undirected graph in symmetric CSR form (off[n+1],
  * adjacency sorted per vertex); per_vertex[a] (may be NULL) receives the cycles

@@ -97,6 +97,9 @@ def lib():
         L.orc_pagerank.restype = C.c_int
         L.orc_write_refdb.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
         L.orc_write_refdb.restype = C.c_int
+        L.orc_write_refdb_annotated.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                                C.c_int]
+        L.orc_write_refdb_annotated.restype = C.c_int
         _lib = L
     return _lib
 
@@ -316,11 +319,21 @@ def pagerank(src, dst, w_keys, w_vals, base, add=0.15, mul=0.85, init=1.0, itera
     return out[:nh]
 
 
-def write_refdb(src, dst, path, nbuffers=4, mmapped=False, policy=POLICY_RULE):
+def write_refdb(src, dst, path, nbuffers=4, mmapped=False, policy=POLICY_RULE, annos=None, unary=False):
     """Saves the relation (src, dst) as the reference engine does
     (orc_write_refdb): <path>/ram/ or <path>/mmap/ with trieinfo.bin and the
-    buffer images."""
+    buffer images.  unary: the relation of src alone (dst ignored).  annos:
+    one value per distinct row in lexicographic row order (its dtype's width is
+    sizeof(A))."""
+    if unary:
+        dst = src
     T = Trie(src, dst, policy)
-    rc = lib().orc_write_refdb(T.h, path.encode(), int(nbuffers), 1 if mmapped else 0)
+    if annos is None and not unary:
+        rc = lib().orc_write_refdb(T.h, path.encode(), int(nbuffers), 1 if mmapped else 0)
+    else:
+        a = None if annos is None else np.ascontiguousarray(annos)
+        rc = lib().orc_write_refdb_annotated(T.h, path.encode(), int(nbuffers), 1 if mmapped else 0,
+                                             1 if unary else 2, None if a is None else a.ctypes.data,
+                                             8 if a is None else a.dtype.itemsize)
     assert rc == 0
     return T

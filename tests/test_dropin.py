@@ -230,3 +230,40 @@ def test_annotated_binary_relation(tmp_path):
     R, ann, num_rows, got = dropin.rows("WeightedDegree", db, len(heads) + 1)
     assert got == len(heads) and np.array_equal(R[:, 0], heads)
     assert np.array_equal(ann.astype(np.float32).view(np.uint32), exp.view(np.uint32))
+
+
+def test_annotated_reference_images(tmp_path, oracle, facebook_dup):
+    """The same annotated programs over relations saved in the reference's
+    on-disk format with their annotations (InvDegree as a unary float image,
+    weighted edges as a binary float image): ranks equal the encoded-store run,
+    edge sums the sequential float32 fold."""
+    import os
+    from test_refdb import write_encoded
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "inv_degree.npz"))
+    keys, vals = d["key"], d["val"]
+    s, t = facebook_dup
+    db1, db2 = str(tmp_path / "enc"), str(tmp_path / "img")
+    for db in (db1, db2):
+        dropin.write_relation(db, "Edge", s, t)
+    write_encoded(os.path.join(db1, "relations", "InvDegree"), [keys], vals, width=4)
+    oracle.write_refdb(keys, None, os.path.join(db2, "relations", "InvDegree", "InvDegree_0"), nbuffers=2,
+                       annos=vals, unary=True)
+    heads = np.unique(s)
+    _, a1, _, _ = dropin.rows("PageRankSteps", db1, len(heads) + 1)
+    _, a2, _, _ = dropin.rows("PageRankSteps", db2, len(heads) + 1)
+    assert np.array_equal(a1.astype(np.float32).view(np.uint32), a2.astype(np.float32).view(np.uint32))
+    # weighted edges as a saved image
+    rng = np.random.default_rng(10)
+    T = oracle.Trie(s, t)
+    ra, rb = T.tuples()
+    w = rng.uniform(-1, 1, len(ra)).astype(np.float32)
+    db3 = str(tmp_path / "w")
+    oracle.write_refdb(s, t, os.path.join(db3, "relations", "Edge", "Edge_0_1"), nbuffers=4, annos=w)
+    hx = np.unique(ra)
+    exp = np.zeros(len(hx), np.float32)
+    idx = np.searchsorted(hx, ra)
+    for i in range(len(ra)):  # ascending (x, z)
+        exp[idx[i]] = np.float32(exp[idx[i]] + w[i])
+    R, ann, _, got = dropin.rows("WeightedDegree", db3, len(hx) + 1)
+    assert got == len(hx) and np.array_equal(R[:, 0], hx)
+    assert np.array_equal(ann.astype(np.float32).view(np.uint32), exp.view(np.uint32))

@@ -90,3 +90,28 @@ def test_encoded_column_store(tmp_path):
         f.truncate(100)
     with pytest.raises(E.EhxError):
         E.refdb_read(p, encoded=True)
+
+
+@pytest.mark.parametrize("mmapped", [False, True])
+@pytest.mark.parametrize("policy", [0, 2])
+def test_refdb_annotated(tmp_path, oracle, mmapped, policy):
+    """Annotated relations: float32 (4-byte A) on a binary relation, int64 on a
+    unary one; annotation slots by position (sparse sets) and by value (dense
+    sets, as the all-bitset policy makes them)."""
+    rng = np.random.default_rng(6)
+    s, d, n = G.rmat_graph(10)
+    T = oracle.Trie(s, d)
+    a, b = T.tuples()
+    w = rng.uniform(-2, 2, len(a)).astype(np.float32)
+    p = str(tmp_path / "bin")
+    oracle.write_refdb(s, d, p, nbuffers=3, mmapped=mmapped, policy=policy, annos=w)
+    cols, annos = E.refdb_read(p, anno_bytes=4)
+    assert np.array_equal(cols[0], a) and np.array_equal(cols[1], b)
+    assert np.array_equal(annos.astype(np.uint32).view(np.float32), w)
+    keys = np.unique(rng.integers(0, 5000, 700)).astype(np.uint32)
+    v = rng.integers(-10**12, 10**12, len(keys)).astype(np.int64)
+    p = str(tmp_path / "unary")
+    oracle.write_refdb(keys, None, p, nbuffers=2, mmapped=mmapped, policy=policy, annos=v, unary=True)
+    cols, annos = E.refdb_read(p, anno_bytes=8)
+    assert len(cols) == 1 and np.array_equal(cols[0], keys)
+    assert np.array_equal(annos.view(np.int64), v)
