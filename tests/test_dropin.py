@@ -267,3 +267,29 @@ def test_annotated_reference_images(tmp_path, oracle, facebook_dup):
     R, ann, _, got = dropin.rows("WeightedDegree", db3, len(hx) + 1)
     assert got == len(hx) and np.array_equal(R[:, 0], hx)
     assert np.array_equal(ann.astype(np.float32).view(np.uint32), exp.view(np.uint32))
+
+
+def test_selection_golden(tmp_path, facebook_dup, fixtures_meta):
+    """FliqueSelAgg (x=0) through the drop-in's selection calls (hand-written in
+    the generated code's idiom): the reference's 3,759,972 (travis.py:385-400),
+    and the same after every vertex is relabelled through a permutation that a
+    reference-format encoding.bin records, so value_to_key.at(0) must map the
+    constant to its key."""
+    import os
+    s, t = facebook_dup
+    exp = fixtures_meta["facebook_duplicated"]["FliqueSelAgg_x0"]
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", s, t)
+    dropin.write_relation(db, "Edge", t, s, ordering="1_0")
+    assert dropin.run("FliqueSelAgg", db)[0] == exp
+    n = int(max(s.max(), t.max())) + 1
+    perm = np.random.default_rng(12).permutation(n).astype(np.uint32)  # key of value v
+    db2 = str(tmp_path / "db2")
+    dropin.write_relation(db2, "Edge", perm[s], perm[t])
+    dropin.write_relation(db2, "Edge", perm[t], perm[s], ordering="1_0")
+    key_to_value = np.empty(n, np.uint32)
+    key_to_value[perm] = np.arange(n, dtype=np.uint32)
+    with open(os.path.join(db2, "encodings", "uint32_t", "encoding.bin"), "wb") as f:
+        f.write(np.uint64(n).tobytes())
+        f.write(key_to_value.tobytes())
+    assert dropin.run("FliqueSelAgg", db2)[0] == exp
