@@ -115,3 +115,18 @@ def test_refdb_annotated(tmp_path, oracle, mmapped, policy):
     cols, annos = E.refdb_read(p, anno_bytes=8)
     assert len(cols) == 1 and np.array_equal(cols[0], keys)
     assert np.array_equal(annos.view(np.int64), v)
+
+
+def test_refdb_argument_errors(tmp_path, oracle):
+    """Bad arguments come back as errors (no GPU involved)."""
+    import ctypes as C
+    s, d, n = G.rmat_graph(8)
+    p = str(tmp_path / "r")
+    oracle.write_refdb(s, d, p)
+    h = C.c_void_p()
+    for ab in (0, 9, -1):  # sizeof(A) must be 1..8
+        assert E.lib().ehx_refdb_open(p.encode(), ab, C.byref(h)) == -1
+    assert E.lib().ehx_refdb_open(None, 8, C.byref(h)) == -1
+    assert E.lib().ehx_refdb_info(None, None, None, None) == -1
+    assert E.lib().ehx_encoded_open(str(tmp_path / "none").encode(), C.byref(h)) == -1
+    assert E.lib().ehx_pagerank(None, None, None, None, 0, None, 0, None, None) == -1
