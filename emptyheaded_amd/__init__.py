@@ -100,6 +100,8 @@ SIGNATURES = {
     "ehx_weighted_pairs": ([_vp, _vp, C.POINTER(C.c_int64), C.c_uint64, C.POINTER(C.c_int64), C.c_uint64,
                             C.POINTER(C.c_int64)], C.c_int),
     "ehx_sssp_unweighted": ([_vp, _vp, C.c_uint32, C.c_int32, _u32p, C.c_uint64, _u32p], C.c_int),
+    "ehx_sssp_unweighted_dir": ([_vp, _vp, _vp, C.c_uint32, C.c_int32, _u32p, C.c_uint64, _u32p, C.c_int32],
+                                C.c_int),
     "ehx_refdb_open": ([C.c_char_p, C.c_int, C.POINTER(_vp)], C.c_int),
     "ehx_refdb_info": ([_vp, _u32p, _u64p, C.POINTER(C.c_int32)], C.c_int),
     "ehx_refdb_columns": ([_vp, C.POINTER(_u32p), _u64p], C.c_int),
@@ -389,12 +391,18 @@ def weighted_pairs(ctx, trie, wa, wb):
 UNREACHED = 0xFFFFFFFF
 
 
-def sssp_unweighted(ctx, trie, start, n, iterations=-1):
+def sssp_unweighted(ctx, trie, start, n, iterations=-1, rev=None, bottom_up=0):
     """BFS levels from `start` over the trie's child sets, for values 0..n-1
-    (UNREACHED where never reached), and the number of levels run."""
+    (UNREACHED where never reached), and the number of levels run.  With `rev`
+    (the reversed relation's trie; the trie itself when symmetric), levels whose
+    frontier holds at least n / bottom_up vertices are found bottom-up."""
     dist = np.empty(n, dtype=np.uint32)
     lv = C.c_uint32(0)
-    check(lib().ehx_sssp_unweighted(ctx.h, trie.h, start, iterations, ptr_u32(dist), n, C.byref(lv)))
+    if rev is None:
+        check(lib().ehx_sssp_unweighted(ctx.h, trie.h, start, iterations, ptr_u32(dist), n, C.byref(lv)))
+    else:
+        check(lib().ehx_sssp_unweighted_dir(ctx.h, trie.h, rev.h, start, iterations, ptr_u32(dist), n, C.byref(lv),
+                                            bottom_up))
     return dist, lv.value
 
 

@@ -49,6 +49,7 @@ int join_count_segments(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, con
 int cycle4_count(ehx_ctx *, const ehx_trie *, uint64_t, uint64_t, const SegOut *, int64_t *, ehx_join_stats *);
 int weighted_pairs(ehx_ctx *, const ehx_trie *, const int64_t *, uint64_t, const int64_t *, uint64_t, int64_t *);
 int sssp_unweighted(ehx_ctx *, const ehx_trie *, uint32_t, int, uint32_t *, uint64_t, uint32_t *);
+int sssp_directed(ehx_ctx *, const ehx_trie *, const ehx_trie *, uint32_t, int, uint32_t *, uint64_t, uint32_t *, int);
 int rank_recursive(ehx_ctx *, const ehx_trie *, const uint32_t *, const float *, uint64_t, const ehx_rank_rule *, int,
                    float *, double *);
 int join_materialize(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint32_t *const *, uint64_t, uint64_t *,
@@ -231,6 +232,15 @@ int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t
   if (!ctx || !t || !dist_out) return fail(EHX_ERR_INVALID, "null argument");
   if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
   return sssp_unweighted(ctx, t, start, iterations, dist_out, nd, levels);
+}
+
+int ehx_sssp_unweighted_dir(ehx_ctx *ctx, const ehx_trie *t, const ehx_trie *rev, uint32_t start, int32_t iterations,
+                            uint32_t *dist_out, uint64_t nd, uint32_t *levels, int32_t bottom_up) {
+  if (!ctx || !t || !rev || !dist_out) return fail(EHX_ERR_INVALID, "null argument");
+  if (t->device != ctx->device || rev->device != ctx->device)
+    return fail(EHX_ERR_INVALID, "trie lives on another device");
+  if (bottom_up < 0) return fail(EHX_ERR_INVALID, "bottom_up %d", bottom_up);
+  return sssp_directed(ctx, t, rev, start, iterations, dist_out, nd, levels, bottom_up);
 }
 
 int ehx_pagerank(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *w_keys, const float *w_vals, uint64_t n_w,

@@ -45,8 +45,43 @@ __global__ __launch_bounds__(256) void k_bfs_level(DTrie t, const uint32_t *__re
   }
 }
 
-int sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int iterations, uint32_t *dist_out, uint64_t nd,
-                    uint32_t *levels_out) {
+// Bottom-up step (direction-optimizing BFS, for large frontiers): one lane per
+// unreached vertex v < nd scans its in-neighbours (v's child set in the reverse
+// trie) for one at the frontier's level and stops at the first; v then takes
+// level + 1 and joins the next frontier.  A vertex claimed in this step holds
+// level + 1, never `level`, so plain stores suffice.
+__global__ __launch_bounds__(256) void k_bfs_bottom_up(DTrie r, uint32_t *__restrict__ dist, uint64_t nd,
+                                                       uint32_t level, uint32_t *__restrict__ nxt,
+                                                       unsigned *__restrict__ ncnt) {
+  const uint32_t lane = __lane_id();
+  for (uint64_t v0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ull; v0 < nd;
+       v0 += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = v0 + lane;
+    bool found = false;
+    if (v < nd && dist[v] == UNREACHED) {
+      const int32_t h = node_of(r, (uint32_t)v);
+      if (h >= 0)
+        for (uint32_t k = r.off[h], e = r.off[h + 1]; k < e; k++) {
+          const uint32_t u = r.vals[k];
+          if (u < nd && dist[u] == level) {
+            found = true;
+            break;
+          }
+        }
+      if (found) dist[v] = level + 1;
+    }
+    const uint64_t m = __ballot(found);
+    unsigned base = 0;
+    if (lane == 0 && m) base = atomicAdd(ncnt, (unsigned)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (found) nxt[base + wave_popc_below(m)] = (uint32_t)v;
+  }
+}
+
+// Top-down steps, and bottom-up ones over the reverse trie `rev` while the
+// frontier holds at least nd / bottom_up vertices (bottom_up > 0).
+int sssp_directed(ehx_ctx *ctx, const ehx_trie *t, const ehx_trie *rev, uint32_t start, int iterations,
+                  uint32_t *dist_out, uint64_t nd, uint32_t *levels_out, int bottom_up) {
   EHX_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   int rc;
@@ -64,10 +99,16 @@ int sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int iterati
   k_set_u32<<<1, 1, 0, st>>>(cur, 0, start);
   EHX_HIP(hipGetLastError());
   const DTrie dv = view(t);
+  const DTrie rv = rev ? view(rev) : dv;
+  const uint64_t switch_at = bottom_up > 0 ? nd / (uint64_t)bottom_up : ~0ull;
   uint32_t fn = 1, level = 0;
   while (fn && (iterations < 0 || (int)level < iterations)) {
     if ((rc = fill_u32(ctx, cnt, 0u, 1))) return rc;
-    k_bfs_level<<<grid_for((uint64_t)fn * 64, 256, 16384), 256, 0, st>>>(dv, cur, fn, dist, nd, level + 1, nxt, cnt);
+    if (rev && fn >= switch_at)
+      k_bfs_bottom_up<<<grid_for(nd, 256, 16384), 256, 0, st>>>(rv, dist, nd, level, nxt, cnt);
+    else
+      k_bfs_level<<<grid_for((uint64_t)fn * 64, 256, 16384), 256, 0, st>>>(dv, cur, fn, dist, nd, level + 1, nxt,
+                                                                             cnt);
     EHX_HIP(hipGetLastError());
     if ((rc = copy_d2h(ctx, &fn, cnt, 4))) return rc;
     uint32_t *tmp = cur;
@@ -77,6 +118,11 @@ int sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int iterati
   }
   if (levels_out) *levels_out = level;
   return copy_d2h(ctx, dist_out, dist, nd * 4);
+}
+
+int sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int iterations, uint32_t *dist_out, uint64_t nd,
+                    uint32_t *levels_out) {
+  return sssp_directed(ctx, t, nullptr, start, iterations, dist_out, nd, levels_out, 0);
 }
 
 }  // namespace ehx

@@ -222,6 +222,14 @@ int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint6
  * (test/graph/travis.py:556-567) run. */
 int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t iterations, uint32_t *dist_out,
                         uint64_t nd, uint32_t *levels);
+/* The same distances, direction-optimizing: `rev` is the trie of the reversed
+ * relation (t itself for a symmetric one); while the frontier holds at least
+ * nd / bottom_up vertices (bottom_up > 0) a level is found bottom-up -- each
+ * unreached vertex looks among its in-neighbours for one in the frontier --
+ * else top-down as ehx_sssp_unweighted; bottom_up = 0: top-down only.  BFS
+ * levels are unique, so the result does not depend on the direction taken. */
+int ehx_sssp_unweighted_dir(ehx_ctx *ctx, const ehx_trie *t, const ehx_trie *rev, uint32_t start, int32_t iterations,
+                            uint32_t *dist_out, uint64_t nd, uint32_t *levels, int32_t bottom_up);
 
 /* ---- the reference's on-disk relations --------------------------------------
  * A relation the reference's loader saved (Trie<A,M>::save, src/Trie.cpp:15-35;

@@ -379,6 +379,8 @@ def test_sssp_golden_and_oracle(ctx, oracle, facebook_dup):
     n = int(max(s.max(), d.max())) + 1
     dist, _ = E.sssp_unweighted(ctx, E.Trie(ctx, s, d), 0, n)
     assert np.array_equal(dist, oracle.sssp_unweighted(s, d, 0, n)[0])
+    Tf = E.Trie(ctx, s, d)
+    assert np.array_equal(E.sssp_unweighted(ctx, Tf, 0, n, rev=Tf, bottom_up=32)[0], dist)
     visited = np.flatnonzero(dist != E.UNREACHED)
     assert dist[visited[1000]] == 2
     for sym in (False, True):
@@ -389,9 +391,13 @@ def test_sssp_golden_and_oracle(ctx, oracle, facebook_dup):
         deg = np.bincount(s.astype(np.int64), minlength=n)
         starts = [int(np.argmax(deg)), int(np.flatnonzero(deg)[len(np.flatnonzero(deg)) // 2])]
         starts += [int(np.flatnonzero(deg == 0)[0])] if (deg == 0).any() else []
+        R = T if sym else E.Trie(ctx, d, s)  # the reversed relation
         for start in starts:
             for it in (-1, 1, 3):
-                got, lv = E.sssp_unweighted(ctx, T, start, n, it)
                 exp, elv = oracle.sssp_unweighted(s, d, start, n, it)
-                assert np.array_equal(got, exp), (sym, start, it)
-                assert lv == elv
+                # top-down; direction-optimizing at the default switch; bottom-up
+                # for every level after the first (n // 2**30 == 0 vertices)
+                for rev, bu in ((None, 0), (R, 32), (R, 2 ** 30)):
+                    got, lv = E.sssp_unweighted(ctx, T, start, n, it, rev=rev, bottom_up=bu)
+                    assert np.array_equal(got, exp), (sym, start, it, bu)
+                    assert lv == elv
