@@ -106,6 +106,7 @@ SIGNATURES = {
     "ehx_refdb_info": ([_vp, _u32p, _u64p, C.POINTER(C.c_int32)], C.c_int),
     "ehx_refdb_columns": ([_vp, C.POINTER(_u32p), _u64p], C.c_int),
     "ehx_refdb_close": ([_vp], C.c_int),
+    "ehx_trie_orient": ([_vp, _vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), _u64p], C.c_int),
     "ehx_trie_load_reference": ([_vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
     "ehx_encoded_open": ([C.c_char_p, C.POINTER(_vp)], C.c_int),
     "ehx_trie_build_encoded": ([_vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
@@ -216,6 +217,21 @@ class Trie:
         self.h = h
         self.ctx = ctx
         self.policy = policy
+
+    def orient(self, policy=POLICY_RULE):
+        """(lower, reverse, counts): the rows x < y as a device trie, its reverse
+        ordering, and the numbers of rows with x < y, x > y, x == y."""
+        lo, rv = _vp(), _vp()
+        cnt = np.zeros(3, dtype=np.uint64)
+        check(lib().ehx_trie_orient(self.ctx.h, self.h, policy, C.byref(lo), C.byref(rv), cnt.ctypes.data_as(_u64p)))
+        out = []
+        for h in (lo, rv):
+            t = None
+            if h.value:
+                t = Trie.__new__(Trie)
+                t.h, t.ctx, t.policy = h, self.ctx, policy
+            out.append(t)
+        return out[0], out[1], [int(x) for x in cnt]
 
     @classmethod
     def from_encoded(cls, ctx, path, first=0, second=1, policy=POLICY_RULE):

@@ -50,6 +50,7 @@ int cycle4_count(ehx_ctx *, const ehx_trie *, uint64_t, uint64_t, const SegOut *
 int weighted_pairs(ehx_ctx *, const ehx_trie *, const int64_t *, uint64_t, const int64_t *, uint64_t, int64_t *);
 int sssp_unweighted(ehx_ctx *, const ehx_trie *, uint32_t, int, uint32_t *, uint64_t, uint32_t *);
 int sssp_directed(ehx_ctx *, const ehx_trie *, const ehx_trie *, uint32_t, int, uint32_t *, uint64_t, uint32_t *, int);
+int trie_orient(ehx_ctx *, const ehx_trie *, int, ehx_trie **, ehx_trie **, uint64_t *);
 int rank_recursive(ehx_ctx *, const ehx_trie *, const uint32_t *, const float *, uint64_t, const ehx_rank_rule *, int,
                    float *, double *);
 int join_materialize(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint32_t *const *, uint64_t, uint64_t *,
@@ -232,6 +233,13 @@ int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t
   if (!ctx || !t || !dist_out) return fail(EHX_ERR_INVALID, "null argument");
   if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
   return sssp_unweighted(ctx, t, start, iterations, dist_out, nd, levels);
+}
+
+int ehx_trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, ehx_trie **reverse,
+                    uint64_t *counts) {
+  if (!ctx || !t || !lower || !reverse) return fail(EHX_ERR_INVALID, "null argument");
+  if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
+  return trie_orient(ctx, t, policy, lower, reverse, counts);
 }
 
 int ehx_sssp_unweighted_dir(ehx_ctx *ctx, const ehx_trie *t, const ehx_trie *rev, uint32_t start, int32_t iterations,

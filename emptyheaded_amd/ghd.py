@@ -16,23 +16,62 @@ import numpy as np
 from . import plans
 
 
-def triangle_by_vertex(ctx, sym, form):
+ORIENT_MIN_ROWS = 1 << 20
+
+
+def triangle_by_vertex(ctx, sym, form, method="auto"):
     """T(v) for every value v of the symmetric edge trie `sym` (int64 array
-    indexed by value), and the total (the ordered triangle count)."""
+    indexed by value), and the total (the ordered triangle count).
+
+    method "direct" runs the bag as written: the triangle plan over `sym` with
+    one segment per vertex.  "oriented" uses that for a symmetric relation
+    without loops T(v) = 2 (t_min(v) + t_mid(v) + t_max(v)), the triangles
+    {a < b < c} in which v is the smallest, middle or largest vertex: three
+    per-segment passes over the relation's oriented half (rows x < y, built on
+    the device) and its reverse (plans.TRIANGLE, TRIANGLE_BY_MIDDLE and
+    TRIANGLE_BY_LAST, segmented by their first attribute), each finding every
+    triangle once.  "auto"
+    takes "oriented" for relations of at least ORIENT_MIN_ROWS rows (below
+    that, orienting and building two tries costs more than it saves) when the
+    rows with x < y and x > y are equally many and no row has x == y, else
+    "direct"."""
     import emptyheaded_amd as E
     nv = int(sym.info()["max_value"]) + 1
     bounds = np.arange(nv + 1, dtype=np.uint32)
+    if method == "auto" and sym.num_rows < ORIENT_MIN_ROWS:
+        method = "direct"
+    if method != "direct":
+        lower, rev, (n_lt, n_gt, n_eq) = sym.orient(getattr(sym, "policy", E.POLICY_RULE))
+        if method == "oriented" or (n_lt == n_gt and n_eq == 0 and lower is not None):
+            if lower is None:
+                return np.zeros(nv, dtype=np.int64), 0
+            from . import tuning as TU
+            t = np.zeros(nv, dtype=np.int64)
+            # each pass under its plan's tuned knobs; the triangle plan's stay applied
+            for knobs, levels, tries in (("triangle_by_middle", plans.TRIANGLE_BY_MIDDLE, [lower, rev]),
+                                         ("triangle_by_middle", plans.TRIANGLE_BY_LAST, [lower, rev]),
+                                         ("triangle", plans.TRIANGLE, [lower])):
+                TU.apply_plan(knobs)
+                c, _ = E.join_count_segments(ctx, levels, tries, bounds, form)
+                t += np.asarray(c, dtype=np.int64)
+            lower.close()
+            rev.close()
+            T = 2 * t
+            return T, int(T.sum())
+        for x in (lower, rev):
+            if x is not None:
+                x.close()
     counts, total = E.join_count_segments(ctx, plans.TRIANGLE, [sym], bounds, form)
     return np.asarray(counts, dtype=np.int64), total
 
 
-def lollipop_count(ctx, sym, form):
+def lollipop_count(ctx, sym, form, method="auto"):
     import emptyheaded_amd as E
-    T, _ = triangle_by_vertex(ctx, sym, form)
+    T, _ = triangle_by_vertex(ctx, sym, form, method)
     return E.weighted_pairs(ctx, sym, T, np.ones(len(T), dtype=np.int64))
 
 
-def barbell_count(ctx, sym, form):
+def barbell_count(ctx, sym, form, method="auto"):
     import emptyheaded_amd as E
-    T, _ = triangle_by_vertex(ctx, sym, form)
+    T, _ = triangle_by_vertex(ctx, sym, form, method)
     return E.weighted_pairs(ctx, sym, T, T)

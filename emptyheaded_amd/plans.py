@@ -32,6 +32,18 @@ TRIANGLE_BY_MIDDLE = [[(0, -1), (1, -1)],
                       [(1, 0), (0, -1)],
                       [(0, 1), (0, 0)]]
 
+# The same triangles found at their largest vertex c, attribute order (c, a, b)
+# over relation 0 = Edge_0_1 and relation 1 = Edge_1_0:
+#   c: head(Edge_1_0)
+#   a: child(Edge_1_0, c)
+#   b: child(Edge_0_1, a) ∩ child(Edge_1_0, c)
+# The last level streams the out-lists of the a's (sum_a d+(a)^2 values, as the
+# middle-vertex order does); per-segment counts by c are the triangles whose
+# largest vertex is c.
+TRIANGLE_BY_LAST = [[(1, -1)],
+                    [(1, 0)],
+                    [(0, 1), (1, 0)]]
+
 # FliqueAgg(;z) :- Edge(a,b),Edge(b,c),Edge(a,c),Edge(a,d),Edge(b,d),Edge(c,d),COUNT(*).
 #   a: heads of Edge_a_b, Edge_a_c, Edge_a_d
 #   b: child(Edge_a_b, a) ∩ head(Edge_b_c) ∩ head(Edge_b_d)

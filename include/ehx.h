@@ -212,6 +212,16 @@ int ehx_join_materialize(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tr
 int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint64_t na, const int64_t *wb,
                        uint64_t nb, int64_t *result);
 
+/* The rows (x, y) of trie t with x < y, as a device trie (*lower) and its
+ * reverse ordering (*reverse, rows (y, x)), both built on the device with
+ * layout `policy` (both NULL when no row has x < y); counts (may be NULL)
+ * receives the numbers of rows with x < y, x > y and x == y.  For a symmetric relation without loops the two
+ * are its oriented halves: per-vertex triangle counts of the symmetric
+ * triangle bag follow from three oriented passes (the two-bag plans' lower
+ * bag, EHGenerator.scala:711-726), see emptyheaded_amd/ghd.py. */
+int ehx_trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, ehx_trie **reverse,
+                    uint64_t *counts);
+
 /* ---- single-source closure --------------------------------------------------
  * Unweighted single-source shortest paths over trie t (x's neighbours = its
  * child set): dist_out[v] (nd host entries) = the BFS level at which v is first
