@@ -137,6 +137,7 @@ struct ParTrieBuilder {
   const TrieBlock<hybrid, M> *tmp_head;
   size_t num_attributes;
   Set<hybrid> head_result;
+  std::vector<uint32_t> head_values;  // head_result's values when computed here
 
   ParTrieBuilder(Trie<A, M> *t_in, const size_t num_attributes_in)
       : trie(t_in), tmp_head(nullptr), num_attributes(num_attributes_in) {
@@ -149,17 +150,17 @@ struct ParTrieBuilder {
 
   size_t build_aggregated_set(const TrieBlock<hybrid, M> *s1) {
     ehx_dropin::Operands o{1, {s1}};
-    return ehx_dropin::aggregate_head(o, &head_result);
+    return ehx_dropin::aggregate_head(o, &head_result, &head_values);
   }
   size_t build_aggregated_set(const TrieBlock<hybrid, M> *s1, const TrieBlock<hybrid, M> *s2) {
     ehx_dropin::Operands o{2, {s1, s2}};
-    return ehx_dropin::aggregate_head(o, &head_result);
+    return ehx_dropin::aggregate_head(o, &head_result, &head_values);
   }
   size_t build_aggregated_set(std::vector<const TrieBlock<hybrid, M> *> *isets) {
     ehx_dropin::Operands o{(int)isets->size(), {}};
     if (o.n > 8) ehx_dropin::not_implemented("more than 8 operand sets");
     for (int i = 0; i < o.n; i++) o.blocks[i] = isets->at(i);
-    return ehx_dropin::aggregate_head(o, &head_result);
+    return ehx_dropin::aggregate_head(o, &head_result, &head_values);
   }
 
   // ParTrieBuilder::build_aggregated_equality_selection_set (TrieBuilder.cpp:
@@ -186,17 +187,17 @@ struct ParTrieBuilder {
   // output trie.  A one-column output is that set.
   size_t build_set(const TrieBlock<hybrid, M> *s1) {
     ehx_dropin::Operands o{1, {s1}};
-    return head_rows(ehx_dropin::aggregate_head(o, &head_result));
+    return head_rows(ehx_dropin::aggregate_head(o, &head_result, &head_values));
   }
   size_t build_set(const TrieBlock<hybrid, M> *s1, const TrieBlock<hybrid, M> *s2) {
     ehx_dropin::Operands o{2, {s1, s2}};
-    return head_rows(ehx_dropin::aggregate_head(o, &head_result));
+    return head_rows(ehx_dropin::aggregate_head(o, &head_result, &head_values));
   }
   size_t build_set(std::vector<const TrieBlock<hybrid, M> *> *isets) {
     ehx_dropin::Operands o{(int)isets->size(), {}};
     if (o.n > 8) ehx_dropin::not_implemented("more than 8 operand sets");
     for (int i = 0; i < o.n; i++) o.blocks[i] = isets->at(i);
-    return head_rows(ehx_dropin::aggregate_head(o, &head_result));
+    return head_rows(ehx_dropin::aggregate_head(o, &head_result, &head_values));
   }
   void allocate_next() {}
   void allocate_annotation() {}

@@ -44,8 +44,9 @@ struct OutRows {
   size_t num_rows() const { return ncols ? rows.size() / ncols : 0; }
 };
 
-// ParTrieBuilder::build_aggregated_set / build_set (level 0): computed at once.
-size_t aggregate_head(const Operands &ops, Set<hybrid> *out);
+// ParTrieBuilder::build_aggregated_set / build_set (level 0): computed at once;
+// a result's values are kept in *storage (the builder's), which out points into.
+size_t aggregate_head(const Operands &ops, Set<hybrid> *out, std::vector<uint32_t> *storage);
 // TrieBuilder::build_aggregated_set / build_set at builder depth `depth`
 // (0 = the level below the head) of a query with `num_attrs` attributes:
 // recorded or replayed by the pass driver.  A materialized call (build_set)

@@ -293,11 +293,8 @@ static type::layout result_type(const Operands &ops) {
   return type::RANGE_BITSET;
 }
 
-// Level-0 results live as long as the builder that asked for them.
-static std::mutex g_head_mu;
-static std::vector<std::vector<uint32_t>> g_head_results;
-
-size_t aggregate_head(const Operands &ops, Set<hybrid> *out) {
+// Level-0 results live in the storage of the builder that asked for them.
+size_t aggregate_head(const Operands &ops, Set<hybrid> *out, std::vector<uint32_t> *storage) {
   for (int i = 0; i < ops.n; i++)
     if (!blk(ops, i)) { set_empty(out); return 0; }
   bool same = true;
@@ -316,11 +313,10 @@ size_t aggregate_head(const Operands &ops, Set<hybrid> *out) {
   std::vector<uint64_t> counts, offsets;
   std::vector<uint32_t> values;
   run_batch(ops.n, tries, nodes, false, counts, offsets, values);
-  std::lock_guard<std::mutex> g(g_head_mu);
-  g_head_results.push_back(std::move(values));
-  *out = Set<hybrid>((uint32_t)counts[0], counts[0] ? g_head_results.back()[counts[0] - 1] : 0, 4 * counts[0],
+  *storage = std::move(values);
+  *out = Set<hybrid>((uint32_t)counts[0], counts[0] ? (*storage)[counts[0] - 1] : 0, 4 * counts[0],
                      result_type(ops));
-  out->values = g_head_results.back().data();
+  out->values = storage->data();
   return counts[0];
 }
 
