@@ -45,6 +45,38 @@ __global__ __launch_bounds__(256) void k_bfs_level(DTrie t, const uint32_t *__re
   }
 }
 
+// Top-down step for a frontier of at most WIDE_FRONTIER vertices (the start,
+// typically a hub): every wave of the grid takes a strided share of each
+// frontier vertex's child set, so a large set is streamed by the whole chip
+// rather than by one wave.  Claims and appends as in k_bfs_level.
+constexpr uint32_t WIDE_FRONTIER = 32;
+__global__ __launch_bounds__(256) void k_bfs_level_wide(DTrie t, const uint32_t *__restrict__ fr, uint32_t fn,
+                                                        uint32_t *__restrict__ dist, uint64_t nd, uint32_t level,
+                                                        uint32_t *__restrict__ nxt, unsigned *__restrict__ ncnt) {
+  const uint32_t lane = __lane_id();
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint32_t i = 0; i < fn; i++) {
+    const int32_t h = node_of(t, fr[i]);
+    if (h < 0) continue;  // no child set (grid-uniform)
+    const uint32_t o = t.off[h], n = t.off[h + 1] - o;
+    for (uint64_t k0 = wave * 64; k0 < n; k0 += nwaves * 64) {
+      const uint64_t k = k0 + lane;
+      bool claim = false;
+      uint32_t x = 0;
+      if (k < n) {
+        x = t.vals[o + k];
+        if (x < nd && dist[x] == UNREACHED) claim = atomicCAS(&dist[x], UNREACHED, level) == UNREACHED;
+      }
+      const uint64_t m = __ballot(claim);
+      unsigned base = 0;
+      if (lane == 0 && m) base = atomicAdd(ncnt, (unsigned)__popcll(m));
+      base = __shfl(base, 0, 64);
+      if (claim) nxt[base + wave_popc_below(m)] = x;
+    }
+  }
+}
+
 // Bottom-up step (direction-optimizing BFS, for large frontiers): one lane per
 // unreached vertex v < nd scans its in-neighbours (v's child set in the reverse
 // trie) for one at the frontier's level and stops at the first; v then takes
@@ -106,6 +138,8 @@ int sssp_directed(ehx_ctx *ctx, const ehx_trie *t, const ehx_trie *rev, uint32_t
     if ((rc = fill_u32(ctx, cnt, 0u, 1))) return rc;
     if (rev && fn >= switch_at)
       k_bfs_bottom_up<<<grid_for(nd, 256, 16384), 256, 0, st>>>(rv, dist, nd, level, nxt, cnt);
+    else if (fn <= WIDE_FRONTIER)
+      k_bfs_level_wide<<<(unsigned)ctx->num_cus * 8, 256, 0, st>>>(dv, cur, fn, dist, nd, level + 1, nxt, cnt);
     else
       k_bfs_level<<<grid_for((uint64_t)fn * 64, 256, 16384), 256, 0, st>>>(dv, cur, fn, dist, nd, level + 1, nxt,
                                                                              cnt);
