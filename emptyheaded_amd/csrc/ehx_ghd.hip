@@ -54,38 +54,76 @@ int weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint64_t 
   return EHX_OK;
 }
 
-// Rows (x, y) of trie t with x < y, compacted into lo0/lo1 (any order); cnt[0]
-// counts them, cnt[1] the rows with x > y, cnt[2] the rows with x == y.  One
-// wavefront per head, its lanes over the child set 64 at a time.
-__global__ __launch_bounds__(256) void k_orient(DTrie t, uint32_t *__restrict__ lo0, uint32_t *__restrict__ lo1,
-                                                unsigned *__restrict__ cnt) {
-  const uint32_t lane = __lane_id();
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+// Orienting trie t: the rows (x, y) with x < y are, for each head x, the
+// suffix of x's sorted child set after the values <= x.  k_orient_count finds
+// that suffix per head by binary search (lcnt[h] = its length) and counts the
+// rows with x > y (cnt[1]) and x == y (cnt[2]), one atomic per block.
+__global__ __launch_bounds__(256) void k_orient_count(DTrie t, uint32_t *__restrict__ lcnt, unsigned *__restrict__ cnt) {
+  __shared__ unsigned red[2][4];
   unsigned n_hi = 0, n_eq = 0;
-  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < t.nh; h += nwaves) {
+  for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < t.nh; h += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t x = t.head[h], o = t.off[h], n = t.off[h + 1] - o;
-    for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-      const uint32_t k = k0 + lane;
-      const uint32_t y = k < n ? t.vals[o + k] : 0u;
-      const bool lo = k < n && x < y;
-      n_hi += (k < n && x > y) ? 1u : 0u;
-      n_eq += (k < n && x == y) ? 1u : 0u;
-      const uint64_t m = __ballot(lo);
-      unsigned base = 0;
-      if (lane == 0 && m) base = atomicAdd(&cnt[0], (unsigned)__popcll(m));
-      base = __shfl(base, 0, 64);
-      if (lo) {
-        const uint32_t p = base + wave_popc_below(m);
-        lo0[p] = x;
-        lo1[p] = y;
-      }
+    uint32_t lo = 0, hi = n;  // first index whose value is >= x
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (t.vals[o + mid] < x) lo = mid + 1;
+      else hi = mid;
     }
+    const uint32_t eq = (lo < n && t.vals[o + lo] == x) ? 1u : 0u;
+    n_hi += lo;
+    n_eq += eq;
+    lcnt[h] = n - lo - eq;
   }
   n_hi = wave_sum(n_hi);
   n_eq = wave_sum(n_eq);
-  if (lane == 0) {
-    if (n_hi) atomicAdd(&cnt[1], n_hi);
-    if (n_eq) atomicAdd(&cnt[2], n_eq);
+  const uint32_t w = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+    red[0][w] = n_hi;
+    red[1][w] = n_eq;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    const unsigned e = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    if (a) atomicAdd(&cnt[1], a);
+    if (e) atomicAdd(&cnt[2], e);
+  }
+}
+
+// The rows with x < y into lo0/lo1 at their place in (x, y) order: pos (nh + 1
+// entries) is the exclusive scan of lcnt.  Each wave fills ORIENT_CHUNK output
+// rows; a lane finds the head of its row by binary search over pos restricted
+// to the heads the chunk spans (none when one head, a hub, fills the chunk).
+constexpr uint32_t ORIENT_CHUNK = 1024;
+// The last index i in [lo, hi] with pos[i] <= r, given pos[lo] <= r and pos
+// non-decreasing.
+__device__ __forceinline__ uint32_t last_at_most(const uint32_t *__restrict__ pos, uint32_t lo, uint32_t hi,
+                                                 uint32_t r) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (pos[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__global__ __launch_bounds__(256) void k_orient_copy(DTrie t, const uint32_t *__restrict__ lcnt,
+                                                     const uint32_t *__restrict__ pos, uint32_t total,
+                                                     uint32_t *__restrict__ lo0, uint32_t *__restrict__ lo1) {
+  const uint32_t lane = __lane_id();
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t nchunks = ((uint64_t)total + ORIENT_CHUNK - 1) / ORIENT_CHUNK;
+  for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nwaves) {
+    const uint32_t r0 = (uint32_t)(c * ORIENT_CHUNK);
+    const uint32_t r1 = (uint32_t)((uint64_t)total < (c + 1) * ORIENT_CHUNK ? total : (c + 1) * ORIENT_CHUNK);
+    // the heads of rows r0 and r1 - 1 (pos[0] = 0 <= r0, pos[nh] = total > r1 - 1)
+    const uint32_t hs = last_at_most(pos, 0, (uint32_t)t.nh, r0);
+    const uint32_t he = last_at_most(pos, hs, (uint32_t)t.nh, r1 - 1);
+    for (uint32_t r = r0 + lane; r < r1; r += 64) {
+      const uint32_t lo = last_at_most(pos, hs, he, r);
+      const uint32_t s = t.off[lo + 1] - lcnt[lo];
+      lo0[r] = t.head[lo];
+      lo1[r] = t.vals[s + (r - pos[lo])];
+    }
   }
 }
 
@@ -99,13 +137,21 @@ int trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, e
   DevBuf &b0 = ctx->scratch[S_F0], &b1 = ctx->scratch[S_F1], &bc = ctx->scratch[S_SEGO];
   if ((rc = b0.ensure(m * 4 + 4)) || (rc = b1.ensure(m * 4 + 4)) || (rc = bc.ensure(64))) return rc;
   if ((rc = fill_u32(ctx, bc.as<uint32_t>(), 0u, 4))) return rc;
-  if (m) {
-    k_orient<<<grid_for(t->nh * 64, 256, 16384), 256, 0, st>>>(view(t), b0.as<uint32_t>(), b1.as<uint32_t>(),
-                                                               bc.as<unsigned>());
-    EHX_HIP(hipGetLastError());
-  }
   uint32_t c[3] = {0, 0, 0};
-  if ((rc = copy_d2h(ctx, c, bc.p, 12))) return rc;
+  if (m) {
+    const uint64_t nh = t->nh;
+    DevBuf &bl = ctx->scratch[S_F2], &bp = ctx->scratch[S_F3];
+    if ((rc = bl.ensure(nh * 4 + 4)) || (rc = bp.ensure(nh * 4 + 8))) return rc;
+    k_orient_count<<<grid_for(nh, 256, 2048), 256, 0, st>>>(view(t), bl.as<uint32_t>(), bc.as<unsigned>());
+    EHX_HIP(hipGetLastError());
+    if ((rc = scan_u32(ctx, bl.as<uint32_t>(), bp.as<uint32_t>(), nh))) return rc;
+    if ((rc = copy_d2h(ctx, &c[0], bp.as<uint32_t>() + nh, 4))) return rc;
+    if (c[0])
+      k_orient_copy<<<grid_for(((uint64_t)c[0] + ORIENT_CHUNK - 1) / ORIENT_CHUNK * 64, 256, 16384), 256, 0, st>>>(
+          view(t), bl.as<uint32_t>(), bp.as<uint32_t>(), c[0], b0.as<uint32_t>(), b1.as<uint32_t>());
+    EHX_HIP(hipGetLastError());
+    if ((rc = copy_d2h(ctx, &c[1], bc.as<unsigned>() + 1, 8))) return rc;
+  }
   if (counts) {
     counts[0] = c[0];
     counts[1] = c[1];
