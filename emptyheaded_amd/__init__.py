@@ -107,6 +107,7 @@ SIGNATURES = {
     "ehx_refdb_columns": ([_vp, C.POINTER(_u32p), _u64p], C.c_int),
     "ehx_refdb_close": ([_vp], C.c_int),
     "ehx_trie_orient": ([_vp, _vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), _u64p], C.c_int),
+    "ehx_trie_link": ([_vp, _vp, _vp, C.POINTER(C.c_int)], C.c_int),
     "ehx_trie_load_reference": ([_vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
     "ehx_encoded_open": ([C.c_char_p, C.POINTER(_vp)], C.c_int),
     "ehx_trie_build_encoded": ([_vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
@@ -218,12 +219,22 @@ class Trie:
         self.ctx = ctx
         self.policy = policy
 
-    def orient(self, policy=POLICY_RULE):
-        """(lower, reverse, counts): the rows x < y as a device trie, its reverse
-        ordering, and the numbers of rows with x < y, x > y, x == y."""
+    def link_to(self, target):
+        """Builds (once) the child pointers from this trie's values into
+        `target`'s child sets; True when the link exists."""
+        ok = C.c_int(0)
+        check(lib().ehx_trie_link(self.ctx.h, self.h, target.h, C.byref(ok)))
+        return bool(ok.value)
+
+    def orient(self, policy=POLICY_RULE, build=True):
+        """(lower, reverse, counts): the rows x < y as a device trie and its
+        reverse ordering (both None unless `build` and the relation is
+        symmetric without loops), and the numbers of rows with x < y, x > y,
+        x == y, and of rows x < y whose reverse row is absent."""
         lo, rv = _vp(), _vp()
-        cnt = np.zeros(3, dtype=np.uint64)
-        check(lib().ehx_trie_orient(self.ctx.h, self.h, policy, C.byref(lo), C.byref(rv), cnt.ctypes.data_as(_u64p)))
+        cnt = np.zeros(4, dtype=np.uint64)
+        check(lib().ehx_trie_orient(self.ctx.h, self.h, policy, C.byref(lo) if build else None,
+                                    C.byref(rv) if build else None, cnt.ctypes.data_as(_u64p)))
         out = []
         for h in (lo, rv):
             t = None

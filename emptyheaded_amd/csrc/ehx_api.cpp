@@ -237,9 +237,20 @@ int ehx_sssp_unweighted(ehx_ctx *ctx, const ehx_trie *t, uint32_t start, int32_t
 
 int ehx_trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, ehx_trie **reverse,
                     uint64_t *counts) {
-  if (!ctx || !t || !lower || !reverse) return fail(EHX_ERR_INVALID, "null argument");
+  if (!ctx || !t) return fail(EHX_ERR_INVALID, "null argument");
   if (t->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
   return trie_orient(ctx, t, policy, lower, reverse, counts);
+}
+
+int ehx_trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int *linked) {
+  if (!ctx || !src || !dst) return fail(EHX_ERR_INVALID, "null argument");
+  if (src->device != ctx->device || dst->device != ctx->device)
+    return fail(EHX_ERR_INVALID, "trie lives on another device");
+  EHX_HIP(hipSetDevice(ctx->device));
+  const uint4 *arr = nullptr;
+  int rc = trie_link(ctx, src, dst, &arr);
+  if (linked) *linked = arr != nullptr;
+  return rc;
 }
 
 int ehx_sssp_unweighted_dir(ehx_ctx *ctx, const ehx_trie *t, const ehx_trie *rev, uint32_t start, int32_t iterations,

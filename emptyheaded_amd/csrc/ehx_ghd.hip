@@ -127,6 +127,36 @@ __global__ __launch_bounds__(256) void k_orient_copy(DTrie t, const uint32_t *__
   }
 }
 
+// Symmetry check over the rows (x, y), x < y, that k_orient_copy wrote: a row
+// whose reverse (y, x) is absent from t (y is no head, or x is not in y's
+// child set) is a miss; cnt[3] += misses, one atomic per block.  With as many
+// rows x > y as x < y and no miss, every row x > y is the reverse of one x < y
+// row, so the relation is symmetric.
+__global__ __launch_bounds__(256) void k_orient_check(DTrie t, const uint32_t *__restrict__ lo0,
+                                                      const uint32_t *__restrict__ lo1, uint32_t total,
+                                                      unsigned *__restrict__ cnt) {
+  __shared__ unsigned red[4];
+  unsigned miss = 0;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < total; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t x = lo0[r], y = lo1[r];
+    const int32_t ny = node_of(t, y);
+    if (ny < 0) {
+      ++miss;
+      continue;
+    }
+    const uint32_t o = t.off[ny], e = t.off[ny + 1];
+    const uint32_t i = lower_bound(t.vals, o, e, x);
+    miss += (i < e && t.vals[i] == x) ? 0u : 1u;
+  }
+  miss = wave_sum(miss);
+  if (__lane_id() == 0) red[threadIdx.x >> 6] = miss;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned a = red[0] + red[1] + red[2] + red[3];
+    if (a) atomicAdd(&cnt[3], a);
+  }
+}
+
 int trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, ehx_trie **reverse,
                 uint64_t *counts) {
   EHX_HIP(hipSetDevice(ctx->device));
@@ -137,7 +167,7 @@ int trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, e
   DevBuf &b0 = ctx->scratch[S_F0], &b1 = ctx->scratch[S_F1], &bc = ctx->scratch[S_SEGO];
   if ((rc = b0.ensure(m * 4 + 4)) || (rc = b1.ensure(m * 4 + 4)) || (rc = bc.ensure(64))) return rc;
   if ((rc = fill_u32(ctx, bc.as<uint32_t>(), 0u, 4))) return rc;
-  uint32_t c[3] = {0, 0, 0};
+  uint32_t c[4] = {0, 0, 0, 0};
   if (m) {
     const uint64_t nh = t->nh;
     DevBuf &bl = ctx->scratch[S_F2], &bp = ctx->scratch[S_F3];
@@ -150,15 +180,19 @@ int trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, e
       k_orient_copy<<<grid_for(((uint64_t)c[0] + ORIENT_CHUNK - 1) / ORIENT_CHUNK * 64, 256, 16384), 256, 0, st>>>(
           view(t), bl.as<uint32_t>(), bp.as<uint32_t>(), c[0], b0.as<uint32_t>(), b1.as<uint32_t>());
     EHX_HIP(hipGetLastError());
-    if ((rc = copy_d2h(ctx, &c[1], bc.as<unsigned>() + 1, 8))) return rc;
+    if (c[0])
+      k_orient_check<<<grid_for(c[0], 256, 16384), 256, 0, st>>>(view(t), b0.as<uint32_t>(), b1.as<uint32_t>(), c[0],
+                                                                bc.as<unsigned>());
+    EHX_HIP(hipGetLastError());
+    if ((rc = copy_d2h(ctx, &c[1], bc.as<unsigned>() + 1, 12))) return rc;
   }
-  if (counts) {
-    counts[0] = c[0];
-    counts[1] = c[1];
-    counts[2] = c[2];
-  }
-  *lower = *reverse = nullptr;
-  if (c[0] == 0) return EHX_OK;  // no rows with x < y: no tries
+  if (counts)
+    for (int i = 0; i < 4; ++i) counts[i] = c[i];
+  if (lower) *lower = nullptr;
+  if (reverse) *reverse = nullptr;
+  // the halves are built only on request and only for a symmetric relation
+  // without loops (the one case the oriented triangle passes are exact for)
+  if (!lower || !reverse || c[0] == 0 || c[1] != c[0] || c[2] != 0 || c[3] != 0) return EHX_OK;
   if ((rc = ehx_trie_build(ctx, b0.as<uint32_t>(), b1.as<uint32_t>(), c[0], policy, 1, lower))) return rc;
   if ((rc = ehx_trie_build(ctx, b1.as<uint32_t>(), b0.as<uint32_t>(), c[0], policy, 1, reverse))) {
     ehx_trie_destroy(*lower);

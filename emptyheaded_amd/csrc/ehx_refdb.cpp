@@ -89,7 +89,8 @@ struct SetView {
 };
 
 int read_set(const std::vector<uint8_t> &b, size_t block, SetView &s) {
-  if (block + 1 + 24 > b.size()) return fail(EHX_ERR_INVALID, "TrieBlock at %zu past its buffer", block);
+  if (block > b.size() || b.size() - block < 1 + 24)
+    return fail(EHX_ERR_INVALID, "TrieBlock at %zu past its buffer", block);
   const uint8_t *h = b.data() + block + 1;
   memcpy(&s.card, h, 4);
   memcpy(&s.range, h + 4, 4);
@@ -146,6 +147,12 @@ int walk(const Image &im, size_t buf, size_t off, uint32_t level, std::vector<ui
   const bool sp = sparse(s.card, s.range);
   const uint64_t next_size = sp ? s.card : (uint64_t)s.range + 1;
   const size_t tail = s.data + s.nbytes;
+  // a dense set's NextLevel and annotation slots are indexed by value: every
+  // value must lie in [0, range]
+  if (!sp)
+    for (size_t i = 0; i < vals.size(); i++)
+      if (vals[i] > s.range)
+        return fail(EHX_ERR_INVALID, "set at %zu: value %u above its range %u", off, vals[i], s.range);
   if (level + 1 == im.ncols) {
     if (im.annotated && next_size * im.anno_bytes > b.size() - tail)
       return fail(EHX_ERR_INVALID, "annotations past their buffer");

@@ -30,37 +30,46 @@ def triangle_by_vertex(ctx, sym, form, method="auto"):
     per-segment passes over the relation's oriented half (rows x < y, built on
     the device) and its reverse (plans.TRIANGLE, TRIANGLE_BY_MIDDLE and
     TRIANGLE_BY_LAST, segmented by their first attribute), each finding every
-    triangle once.  "auto"
-    takes "oriented" for relations of at least ORIENT_MIN_ROWS rows (below
-    that, orienting and building two tries costs more than it saves) when the
-    rows with x < y and x > y are equally many and no row has x == y, else
-    "direct"."""
+    triangle once; it raises ValueError for a relation that is not symmetric
+    or has a loop.  The device checks symmetry exactly: every row x < y must
+    have its reverse row, and the rows x > y must be as many.  "auto" takes
+    "oriented" for symmetric loop-free relations of at least ORIENT_MIN_ROWS
+    rows (below that, orienting and building two tries costs more than it
+    saves), else "direct".  The knobs in force when called are restored."""
     import emptyheaded_amd as E
+    if method not in ("auto", "direct", "oriented"):
+        raise ValueError("method must be 'auto', 'direct' or 'oriented', not %r" % (method,))
     nv = int(sym.info()["max_value"]) + 1
     bounds = np.arange(nv + 1, dtype=np.uint32)
     if method == "auto" and sym.num_rows < ORIENT_MIN_ROWS:
         method = "direct"
     if method != "direct":
-        lower, rev, (n_lt, n_gt, n_eq) = sym.orient(getattr(sym, "policy", E.POLICY_RULE))
-        if method == "oriented" or (n_lt == n_gt and n_eq == 0 and lower is not None):
-            if lower is None:
+        lower, rev, (n_lt, n_gt, n_eq, n_miss) = sym.orient(getattr(sym, "policy", E.POLICY_RULE))
+        symmetric = n_lt == n_gt and n_eq == 0 and n_miss == 0
+        if method == "oriented" and not symmetric:
+            raise ValueError("oriented triangle passes need a symmetric relation without loops "
+                             "(rows x<y %d, x>y %d, x==y %d, x<y rows without their reverse %d)" %
+                             (n_lt, n_gt, n_eq, n_miss))
+        if symmetric:
+            if lower is None:  # no rows at all
                 return np.zeros(nv, dtype=np.int64), 0
             from . import tuning as TU
+            saved = E.tune_get()
             t = np.zeros(nv, dtype=np.int64)
-            # each pass under its plan's tuned knobs; the triangle plan's stay applied
-            for knobs, levels, tries in (("triangle_by_middle", plans.TRIANGLE_BY_MIDDLE, [lower, rev]),
-                                         ("triangle_by_middle", plans.TRIANGLE_BY_LAST, [lower, rev]),
-                                         ("triangle", plans.TRIANGLE, [lower])):
-                TU.apply_plan(knobs)
-                c, _ = E.join_count_segments(ctx, levels, tries, bounds, form)
-                t += np.asarray(c, dtype=np.int64)
-            lower.close()
-            rev.close()
+            try:
+                # each pass under its plan's tuned knobs
+                for knobs, levels, tries in (("triangle_by_middle", plans.TRIANGLE_BY_MIDDLE, [lower, rev]),
+                                             ("triangle_by_middle", plans.TRIANGLE_BY_LAST, [lower, rev]),
+                                             ("triangle", plans.TRIANGLE, [lower])):
+                    E.tune_set(**TU.plan_knobs(knobs))
+                    c, _ = E.join_count_segments(ctx, levels, tries, bounds, form)
+                    t += np.asarray(c, dtype=np.int64)
+            finally:
+                E.tune_set(**saved)
+                lower.close()
+                rev.close()
             T = 2 * t
             return T, int(T.sum())
-        for x in (lower, rev):
-            if x is not None:
-                x.close()
     counts, total = E.join_count_segments(ctx, plans.TRIANGLE, [sym], bounds, form)
     return np.asarray(counts, dtype=np.int64), total
 

@@ -17,15 +17,33 @@ def selector_active():
     return _selector_active
 
 
+_plan_knobs = {}
+_tuning_stamp = None
+
+
+def plan_knobs(plan_name):
+    """The committed knobs with the overrides the autotuner found for one plan
+    (tuning.json "plans"; unknown plans get the global knobs).  The file is
+    parsed once and again only when it changes on disk."""
+    global _tuning_stamp
+    st = os.stat(TUNING_JSON)
+    stamp = (st.st_mtime_ns, st.st_size)
+    if stamp != _tuning_stamp:
+        _plan_knobs.clear()
+        _tuning_stamp = stamp
+    if plan_name not in _plan_knobs:
+        with open(TUNING_JSON) as f:
+            t = json.load(f)
+        params = dict(t["best"])
+        params.update(t.get("plans", {}).get(plan_name, {}).get("best", {}))
+        _plan_knobs[plan_name] = params
+    return dict(_plan_knobs[plan_name])
+
+
 def apply_plan(plan_name):
-    """Apply the committed knobs, then the overrides the autotuner found for
-    one plan (tuning.json "plans"); unknown plans get the global knobs."""
+    """Apply the committed knobs of one plan (plan_knobs) process-wide."""
     import emptyheaded_amd as E
-    with open(TUNING_JSON) as f:
-        t = json.load(f)
-    params = dict(t["best"])
-    params.update(t.get("plans", {}).get(plan_name, {}).get("best", {}))
-    return E.tune_set(**params)
+    return E.tune_set(**plan_knobs(plan_name))
 
 
 def apply_committed(device=0, selector_path=None):

@@ -131,6 +131,19 @@ int ehx_trie_export(ehx_ctx *ctx, const ehx_trie *t, uint32_t *c0, uint32_t *c1)
 /* Per head node: layout of its child set and its cardinality (Set::type and
  * Set::cardinality, storage_engine/include/trie/set/Set.hpp:19-24). */
 int ehx_trie_node_info(ehx_ctx *ctx, const ehx_trie *t, uint8_t *types, uint32_t *cards);
+/* Child pointers from trie src into trie dst: for every value of src's child
+ * sets, its child set in dst (position, length and gap-coded chunks), built
+ * once and cached on src (16 bytes per row of src) until src is destroyed.
+ * The reference keeps a child pointer beside every set element inside one
+ * trie (NextLevel, storage_engine/include/trie/NextLevel.hpp:4-7, filled by
+ * TrieBlock::set_next_block, trie/TrieBlock.hpp:55-74); here the child may
+ * live in another relation's trie (Edge_1_0 -> Edge_0_1 for a plan that
+ * reaches a vertex through one ordering and reads its set in the other).  The
+ * tuned executor builds the link it needs on first use; calling this first
+ * moves that cost out of the first query.  *linked (may be NULL) = 1 when the
+ * link exists, 0 when dst has no value-indexed child table or the device had
+ * no room (queries then look children up by value). */
+int ehx_trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int *linked);
 /* Child lookup for a batch of keys (TrieBlock::get_next_block(data),
  * storage_engine/include/trie/TrieBlock.hpp:86-111): node index or -1. */
 int ehx_trie_lookup(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *keys, uint64_t n,
@@ -212,11 +225,15 @@ int ehx_join_materialize(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tr
 int ehx_weighted_pairs(ehx_ctx *ctx, const ehx_trie *t, const int64_t *wa, uint64_t na, const int64_t *wb,
                        uint64_t nb, int64_t *result);
 
-/* The rows (x, y) of trie t with x < y, as a device trie (*lower) and its
- * reverse ordering (*reverse, rows (y, x)), both built on the device with
- * layout `policy` (both NULL when no row has x < y); counts (may be NULL)
- * receives the numbers of rows with x < y, x > y and x == y.  For a symmetric relation without loops the two
- * are its oriented halves: per-vertex triangle counts of the symmetric
+/* Orienting a symmetric relation.  counts[4] (may be NULL) receives the
+ * numbers of rows (x, y) of trie t with x < y, with x > y, with x == y, and of
+ * rows x < y whose reverse (y, x) is absent from t; the relation is symmetric
+ * without loops exactly when counts[0] == counts[1] and counts[2] ==
+ * counts[3] == 0.  Then, when lower and reverse are both non-NULL, the rows
+ * with x < y are built on the device with layout `policy` as a trie (*lower)
+ * and its reverse ordering (*reverse, rows (y, x)); otherwise (or when there
+ * are no rows) both stay NULL and only the counts are taken.  The two are
+ * the relation's oriented halves: per-vertex triangle counts of the symmetric
  * triangle bag follow from three oriented passes (the two-bag plans' lower
  * bag, EHGenerator.scala:711-726), see emptyheaded_amd/ghd.py. */
 int ehx_trie_orient(ehx_ctx *ctx, const ehx_trie *t, int policy, ehx_trie **lower, ehx_trie **reverse,

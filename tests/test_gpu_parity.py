@@ -369,7 +369,12 @@ def test_triangle_by_vertex_oriented(ctx, oracle, monkeypatch):
                 got, total = ghd.triangle_by_vertex(ctx, T, form, method)
                 assert np.array_equal(got, exp) and total == int(exp.sum()), (scale, form, method)
         lo, rv, cnt = T.orient()
-        assert cnt == [len(s), len(s), 0]
+        assert cnt == [len(s), len(s), 0, 0]
+        assert lo is not None and rv is not None and lo.num_rows == rv.num_rows == len(s)
+        lo.close()
+        rv.close()
+        lo2, rv2, cnt2 = T.orient(build=False)
+        assert lo2 is None and rv2 is None and cnt2 == cnt
         # a loop and a one-way row: the identity does not hold, "auto" goes direct
         # (at any size: the size threshold is lifted here)
         monkeypatch.setattr(ghd, "ORIENT_MIN_ROWS", 0)
@@ -383,6 +388,42 @@ def test_triangle_by_vertex_oriented(ctx, oracle, monkeypatch):
             exp2[np.unique(S2)] = oracle.join_count_per_head(plans.TRIANGLE, [oracle.Trie(S2, D2)], 8)
             got2, _ = ghd.triangle_by_vertex(ctx, E.Trie(ctx, S2, D2), E.FORM_TUNED, "auto")
             assert np.array_equal(got2, exp2)
+
+
+def test_orient_rejects_balanced_asymmetric(ctx, oracle, monkeypatch):
+    """As many one-way rows x < y as x > y, no loop: the relation is not
+    symmetric, so the oriented passes do not apply.  The device's exact check
+    (every x < y row needs its reverse) sends "auto" to the direct bag, whose
+    per-vertex counts equal the oracle's, and "oriented" raises.  Covered on a
+    six-row gadget ({0,1,2} one way up, {4,5,6} one way down: 2 ordered
+    triangles, where the oriented identity would give 6) and on a symmetric
+    R-MAT graph with one one-way row added in each direction."""
+    from emptyheaded_amd import ghd
+    monkeypatch.setattr(ghd, "ORIENT_MIN_ROWS", 0)
+    gad_s = np.array([0, 1, 0, 5, 6, 6], np.uint32)
+    gad_d = np.array([1, 2, 2, 4, 5, 4], np.uint32)
+    s, d, n = G.rmat_graph(12)
+    S, D = G.symmetric(s, d)
+    nv = int(max(S.max(), D.max())) + 1
+    big_s = np.concatenate([S, np.array([1, nv + 7], np.uint32)])
+    big_d = np.concatenate([D, np.array([nv + 3, 2], np.uint32)])
+    for src, dst in ((gad_s, gad_d), (big_s, big_d)):
+        T = E.Trie(ctx, src, dst)
+        lo, rv, cnt = T.orient()
+        assert lo is None and rv is None
+        assert cnt[0] == cnt[1] and cnt[2] == 0 and cnt[3] > 0, cnt
+        nv2 = int(max(src.max(), dst.max())) + 1
+        exp = np.zeros(nv2, np.int64)
+        exp[np.unique(src)] = oracle.join_count_per_head(plans.TRIANGLE, [oracle.Trie(src, dst)], 8)
+        for form in FORMS:
+            got, total = ghd.triangle_by_vertex(ctx, T, form, "auto")
+            assert np.array_equal(got, exp) and total == int(exp.sum()), form
+            with pytest.raises(ValueError):
+                ghd.triangle_by_vertex(ctx, T, form, "oriented")
+        if len(src) == 6:
+            assert int(exp.sum()) == 2
+            assert ghd.lollipop_count(ctx, T, E.FORM_TUNED) == int(exp[src.astype(np.int64)].sum())
+        T.close()
 
 
 def test_weighted_pairs_against_numpy(ctx):

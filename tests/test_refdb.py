@@ -130,3 +130,35 @@ def test_refdb_argument_errors(tmp_path, oracle):
     assert E.lib().ehx_refdb_info(None, None, None, None) == -1
     assert E.lib().ehx_encoded_open(str(tmp_path / "none").encode(), C.byref(h)) == -1
     assert E.lib().ehx_pagerank(None, None, None, None, 0, None, 0, None, None) == -1
+
+
+def _ram_image(path, name):
+    """(header bytes, buffer bytes) of a ram/ buffer image: "<size> <offset> " then the buffer."""
+    raw = open(os.path.join(path, "ram", name), "rb").read()
+    sp1 = raw.index(b" ")
+    sp2 = raw.index(b" ", sp1 + 1)
+    return raw[: sp2 + 1], bytearray(raw[sp2 + 1:])
+
+
+def test_refdb_corrupt_images(tmp_path, oracle):
+    """Malformed images are errors, never reads outside the buffers: a dense
+    set whose header range is below its largest value (its NextLevel slots
+    are indexed by value), and a NextLevel entry whose offset is near 2^64."""
+    s, d, n = G.rmat_graph(9)
+    for case in ("range", "offset"):
+        p = str(tmp_path / case)
+        oracle.write_refdb(s, d, p, nbuffers=1)
+        hdr, buf = _ram_image(p, "data_head.bin")
+        card, rng = np.frombuffer(bytes(buf[1:9]), np.uint32)
+        nbytes = int(np.frombuffer(bytes(buf[9:17]), np.uint64)[0])
+        assert buf[17] == E.RANGE_BITSET and card > 2 and rng <= 128 * card  # a dense head
+        if case == "range":
+            buf[5:9] = np.uint32(rng - 5).tobytes()  # still dense, values up to rng exceed it
+        else:
+            heads = np.unique(s)
+            slot = 1 + 24 + nbytes + 16 * int(heads[0])
+            assert np.frombuffer(bytes(buf[slot:slot + 4]), np.int32)[0] >= 0
+            buf[slot + 8:slot + 16] = np.uint64(2**64 - 5).tobytes()
+        open(os.path.join(p, "ram", "data_head.bin"), "wb").write(hdr + bytes(buf))
+        with pytest.raises(E.EhxError):
+            E.refdb_read(p)
