@@ -29,6 +29,139 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_threads():
+    """Threads for the CPU oracle: `nproc` (which honours the CPU share the
+    host gives this process, and OMP_NUM_THREADS), at most 16 (one GPU's share
+    of a GPU box's host cores)."""
+    import subprocess
+    try:
+        n = int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.strip())
+    except Exception:
+        n = len(os.sched_getaffinity(0))
+    return max(1, min(16, n)), n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def sources_digest():
+    """SHA-256 over what decides the headline kernel's memory traffic: the HIP
+    sources, the committed knobs and the selector weights."""
+    import hashlib
+    h = hashlib.sha256()
+    pkg = os.path.join(ROOT, "emptyheaded_amd")
+    files = sorted(os.path.join(pkg, "csrc", f) for f in os.listdir(os.path.join(pkg, "csrc")))
+    files += [os.path.join(pkg, "tuning.json"), os.path.join(pkg, "selector.json")]
+    for f in files:
+        if os.path.isfile(f):
+            h.update(os.path.basename(f).encode())
+            with open(f, "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()
+
+
+COPY_CALIB_BYTES = 2 << 30
+
+
+def _kernel_base(name):
+    """'void ehx::k_tuned<4096, 1, 1>(ehx::TunedArgs)' -> 'k_tuned'."""
+    n = name.split("(", 1)[0].strip()
+    if n.startswith("void "):
+        n = n[5:]
+    n = n.split("<", 1)[0]
+    return n.rsplit("::", 1)[-1].strip()
+
+
+def _counter_rows(d):
+    """(kernel name, counter name, value) rows of the counter CSVs under d."""
+    import csv
+    rows = []
+    for dirpath, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                with open(os.path.join(dirpath, f), newline="") as fh:
+                    for r in csv.DictReader(fh):
+                        try:
+                            rows.append((r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])))
+                        except (KeyError, ValueError):
+                            continue
+    return rows
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the headline kernel (k_tuned), measured now:
+    two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE; KiB per
+    dispatch) over a child run of this script with the same workload and
+    knobs (headline form only, 1 warm-up + 3 steps, plus one k_copy16 launch
+    moving 2 GiB each way).  Each counter is scaled by the factor that makes
+    it read the copy kernel's known bytes.  Runs before this process touches
+    the GPU.  Returns (result dict or None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not on PATH"
+    got = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="ehx_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc", ctr, "-d", d, "-o", "pmc", "--output-format", "csv",
+               "--", sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "3", "--warmup", "1", "--no-cpu",
+               "--headline-only", "--no-traffic", "--scale", str(args.scale), "--edge-factor", str(args.edge_factor),
+               "--copy-calib", str(COPY_CALIB_BYTES)]
+        if args.selector:
+            cmd += ["--selector", args.selector]
+        log("traffic: %s pass ..." % ctr)
+        env = dict(os.environ, TMPDIR="/tmp")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+        rows = _counter_rows(d)
+        shutil.rmtree(d, ignore_errors=True)
+        tuned = [v for k, c, v in rows if c == ctr and _kernel_base(k) == "k_tuned"]
+        copy = [v for k, c, v in rows if c == ctr and _kernel_base(k) == "k_copy16"]
+        if not tuned or not copy or copy[0] <= 0:
+            names = sorted({_kernel_base(k) for k, c, v in rows})[:12]
+            return None, "%s pass: rc %d, %d counter rows, kernels %s" % (ctr, r.returncode, len(rows), names)
+        # bytes the copy kernel moves on this counter's side (reads for FETCH, writes for WRITE)
+        factor = COPY_CALIB_BYTES / (copy[0] * 1024.0)
+        tuned.sort()
+        got[ctr] = {"kib_per_dispatch": tuned, "factor": factor,
+                    "bytes_per_launch": tuned[len(tuned) // 2] * 1024.0 * factor, "copy_kib": copy[0]}
+        log("traffic: %s k_tuned median %.4g KiB x %.4f = %.4g bytes" %
+            (ctr, tuned[len(tuned) // 2], factor, got[ctr]["bytes_per_launch"]))
+    fetch, write = got["FETCH_SIZE"]["bytes_per_launch"], got["WRITE_SIZE"]["bytes_per_launch"]
+    return {"hbm_bytes_per_launch": fetch + write, "fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
+            "fetch_factor": got["FETCH_SIZE"]["factor"], "write_factor": got["WRITE_SIZE"]["factor"],
+            "fetch_kib_per_dispatch": got["FETCH_SIZE"]["kib_per_dispatch"],
+            "write_kib_per_dispatch": got["WRITE_SIZE"]["kib_per_dispatch"],
+            "source": "measured in this run: rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over a child run of the "
+                      "headline form, each counter calibrated on a 2 GiB k_copy16"}, "ok"
+
+
+def committed_traffic(plan):
+    """The committed counter measurement, only if it was taken on the sources,
+    knobs and weights in this tree (same digest)."""
+    tf = os.path.join(ROOT, "profiles", "traffic_k_tuned_rmat22.json")
+    if not os.path.exists(tf):
+        return None, "no committed measurement"
+    with open(tf) as f:
+        tj = json.load(f)
+    if tj.get("plan") != plan:
+        return None, "committed measurement is of another plan"
+    if tj.get("sources_sha256") != sources_digest():
+        return None, "committed measurement was taken on other sources, knobs or weights"
+    out = {k: tj[k] for k in ("hbm_bytes_per_launch", "fetch_bytes_per_launch", "write_bytes_per_launch")}
+    out["source"] = "profiles/traffic_k_tuned_rmat22.json (same sources_sha256 as this tree)"
+    return out, "ok"
+
+
 def load_tuning(E, selector_path=None):
     """Autotuned knobs and learned selector weights committed with the package."""
     from emptyheaded_amd import tuning
@@ -72,9 +205,10 @@ def bench_small_graphs(E, ctx, steps, cpu=True):
             from oracle import oracle as O
             OT = O.Trie(s, d)
             t = time.perf_counter()
-            oc = O.join_count(plans.TRIANGLE, [OT], min(16, os.cpu_count() or 1))
+            oc = O.join_count(plans.TRIANGLE, [OT], cpu_threads()[0])
             r["cpu_oracle_ms"] = (time.perf_counter() - t) * 1e3
-            r["cpu_threads"] = min(16, os.cpu_count() or 1)
+            r["cpu_threads"] = cpu_threads()[0]
+            r["cpu_pinned"] = True
             assert oc == pc
         out[name] = r
         log("%s: %d triangles; plain %.3f ms, tuned+learned %.3f ms%s" %
@@ -106,7 +240,7 @@ def bench_selections(E, ctx, steps, cpu=True):
         if cpu:
             from oracle import oracle as O
             t = time.perf_counter()
-            assert O.join_count(levels, [O.Trie(s, d), O.Trie(d, s)], min(16, os.cpu_count() or 1)) == meta[key]
+            assert O.join_count(levels, [O.Trie(s, d), O.Trie(d, s)], cpu_threads()[0]) == meta[key]
             r["cpu_oracle_ms_including_trie_build"] = (time.perf_counter() - t) * 1e3
         out[name] = r
         log("%s: %d; plain %.3f ms, tuned %.3f ms, learned %.3f ms" % (name, meta[key], r["plain_ms"], r["tuned_ms"],
@@ -157,7 +291,7 @@ def bench_two_bag(E, ctx, steps, cpu=True):
             OT = O.Trie(s, d)
             nv = int(max(s.max(), d.max())) + 1
             ph = np.zeros(nv, np.int64)
-            ph[np.unique(s)] = O.join_count_per_head(plans.TRIANGLE, [OT], min(16, os.cpu_count() or 1))
+            ph[np.unique(s)] = O.join_count_per_head(plans.TRIANGLE, [OT], cpu_threads()[0])
             s64, d64 = s.astype(np.int64), d.astype(np.int64)
             lol, bar = int(ph[s64].sum()), int((ph[s64] * ph[d64]).sum())
             r["cpu_oracle_ms_including_trie_build"] = (time.perf_counter() - t) * 1e3
@@ -255,9 +389,9 @@ def bench_pagerank(E, ctx, steps, cpu=True, scale=22, iterations=5):
         OT = O.Trie(s, d)
         r1, _ = E.pagerank(ctx, T, keys, vals, base, iterations=1, form=E.FORM_PLAIN)
         t = time.perf_counter()
-        o1 = O.pagerank(s, d, keys, vals, base, iterations=1, nthreads=16, trie=OT)
+        o1 = O.pagerank(s, d, keys, vals, base, iterations=1, nthreads=cpu_threads()[0], trie=OT)
         out["cpu_oracle_ms_per_iteration"] = (time.perf_counter() - t) * 1e3
-        out["cpu_oracle_threads"] = 16
+        out["cpu_oracle_threads"] = cpu_threads()[0]
         out["plain_bit_identical_to_oracle"] = bool(np.array_equal(r1.view(np.uint32), o1.view(np.uint32)))
         del OT
     log("pagerank: plain %.3f ms, tuned %.3f ms for %d iterations (%.3g rows/s, %.0f GB/s)" % (
@@ -454,9 +588,16 @@ def bench_dropin(src, dst, expect, reps=2):
 
 
 def cpu_baseline(src, dst, m, expect, budget_s=20.0):
+    """The oracle (a C port of the reference engine's algorithms: density-rule
+    layouts, merge or ratio-32 gallop per uint pair, probe, AND+popcount;
+    dynamic chunks of 128 level-0 values) on the host cores: `nproc` threads
+    (at most 16), pinned one per CPU as the reference's pool pins them, built
+    -march=native for this host.  Its uint intersections are scalar loops: the
+    reference's SSE shuffle merge and SIMD gallop are not restated, so this
+    understates the reference engine on uint-heavy work."""
     from emptyheaded_amd import plans
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
+    cores, nproc = cpu_threads()
     t = time.perf_counter()
     OT = O.Trie(src, dst)
     build_s = time.perf_counter() - t
@@ -475,6 +616,8 @@ def cpu_baseline(src, dst, m, expect, budget_s=20.0):
         assert full == expect, (full, expect)
     est_full_s = dt * stride
     return {"value": m / est_full_s, "unit": "edges/s", "cores": cores, "kind": "port",
+            "threads": cores, "nproc": nproc, "pinned": True, "cpu_model": cpu_model(),
+            "uint_intersection": "scalar merge / scalar gallop (reference: SSE shuffle merge, SIMD gallop)",
             "sample": ("full workload" if stride == 1 else
                        "every %d-th chunk of 128 level-0 vertices (1/%d of heads), time scaled by %d" %
                        (stride, stride, stride)),
@@ -499,6 +642,8 @@ def main():
                     help="also run one copy kernel of this many bytes (traffic-counter reference)")
     ap.add_argument("--config", default=None, choices=["pagerank", "sssp", "two_bag", "selections"],
                     help="run only this secondary workload and print its JSON (for profiling it alone)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="do not run the two rocprofv3 counter passes that measure the headline kernel's HBM bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the join's multi-GPU form (replicated trie, first attribute "
@@ -506,6 +651,20 @@ def main():
         sys.exit(2)
 
     import numpy as np
+
+    # counter passes in child processes, before this process touches the GPU
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic and not args.headline_only and not args.config:
+        traffic, traffic_note = measure_traffic(args)
+        log("traffic:", traffic_note)
+    # the CPU oracle built for this host (-march=native), threads pinned
+    oracle_build = None
+    if not args.no_cpu and not args.headline_only:
+        import tempfile
+        from oracle import oracle as O
+        oracle_build = O.use_native(tempfile.mkdtemp(prefix="ehx_oracle_", dir="/tmp")) or O.lib_info()
+        O.set_pin_threads(True)
+        log("cpu oracle:", json.dumps(oracle_build))
 
     import emptyheaded_amd as E
     from emptyheaded_amd import graphs as G
@@ -538,12 +697,19 @@ def main():
     # the reverse ordering Edge_1_0, built for every relation as the reference's loader does
     R_learn = E.Trie(ctx, dst, src, learned_pol)
     ctx.synchronize()
+    t_link = time.perf_counter()
+    # the headline plan's child pointers (Edge_1_0 values -> Edge_0_1 child sets), part of the
+    # build: the first query would otherwise build them
+    linked = R_learn.link_to(T_learn)
+    ctx.synchronize()
+    link_ms = (time.perf_counter() - t_link) * 1e3
     build_wall_ms = (time.perf_counter() - t_build) * 1e3
     R_rule = E.Trie(ctx, dst, src, E.POLICY_RULE)
     info = T_learn.info()
-    log("tries built: rule %.2f ms, learned %.2f ms (+ reverse %.2f / %.2f ms); bitsets %d (learned) vs %d (rule)" %
+    log("tries built: rule %.2f ms, learned %.2f ms (+ reverse %.2f / %.2f ms); bitsets %d (learned) vs %d (rule); "
+        "child-pointer link %.2f ms (%s); headline build wall %.2f ms" %
         (T_rule.info()["build_ms"], info["build_ms"], R_rule.info()["build_ms"], R_learn.info()["build_ms"],
-         info["num_bitsets"], T_rule.info()["num_bitsets"]))
+         info["num_bitsets"], T_rule.info()["num_bitsets"], link_ms, "built" if linked else "none", build_wall_ms))
 
     if args.copy_calib:
         import ctypes as C
@@ -591,14 +757,25 @@ def main():
 
     TU.apply_plan(MID)
     headline_knobs = E.tune_get()
-    achieved = balg / (kms * 1e-3)
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_k_tuned_rmat22.json")
-    if os.path.exists(tf) and args.scale == 22:
-        with open(tf) as f:
-            tj = json.load(f)
-        if tj.get("plan") == MID:
-            traffic = tj.get("hbm_bytes_per_launch")
+    if traffic is None and args.scale == 22 and args.edge_factor == 16:
+        traffic, why = committed_traffic(MID)
+        traffic_note += "; " + why
+    # roofline of the headline kernel on HBM bytes it actually moves (counters), which a
+    # memory-bound kernel cannot push past the peak; the plan's algorithmic operand bytes
+    # (both lists of every level-2 intersection at 4 bytes a value) are reported beside it
+    # as work_bytes: the kernel stages one list of each intersection in LDS and streams the
+    # other from a 16-bit gap-coded copy, so it moves far fewer bytes than that
+    tb = traffic["hbm_bytes_per_launch"] if traffic else None
+    achieved = tb / (kms * 1e-3) if tb else None
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": tb,
+                "kernel": "k_tuned", "kernel_ms": kms,
+                "traffic_detail": traffic, "traffic_note": traffic_note,
+                "work_bytes": balg, "work_rate_gbs": balg / (kms * 1e-3) / 1e9,
+                "work_bytes_definition": "4 * sum over oriented edges (a,b) of (d+(a) + d+(b)): both uint32 "
+                                         "operand lists of every level-2 intersection, layout-independent"}
+    if roofline["frac"] is not None:
+        assert 0 < roofline["frac"] <= 1.0, roofline
     out = {
         "metric": METRIC,
         "value": m / (ms * 1e-3),
@@ -621,11 +798,11 @@ def main():
         },
         "results_per_s": count / (ms * 1e-3),
         "algorithmic_bytes": balg,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "k_tuned",
-                     "kernel_ms": kms,
-                     "traffic_frac": (traffic / (kms * 1e-3) / HBM_PEAK) if traffic else None},
+        "roofline": roofline,
         "trie_build_wall_ms": build_wall_ms,
+        "trie_build_includes": "Edge_0_1 and Edge_1_0 from host arrays (PCIe copy, device sort and build, "
+                               "learned layouts) and the child-pointer link the headline plan reads",
+        "link_ms": link_ms,
         "edges_per_s_including_build": m / ((ms + build_wall_ms) * 1e-3),
         "ms_per_step_by_form": {k: r[1] for k, r in res.items()},
         "last_level_ms_by_form": {k: r[2] for k, r in res.items()},
@@ -654,6 +831,7 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(src, dst, m, count)
+        out["cpu_baseline"]["build"] = oracle_build
     else:
         out["cpu_baseline"] = None
     ctx.synchronize()

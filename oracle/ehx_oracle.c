@@ -14,6 +14,7 @@
 #include <sys/stat.h>
 #include <stdio.h>
 #include <pthread.h>
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -426,6 +427,7 @@ typedef struct {
 
 typedef struct {
   join_ctx *jc;
+  int index;             /* worker number (0 = the calling thread) */
   uint8_t *buf[ORC_MAX_ATTRS][2];
   uint32_t *vals[ORC_MAX_ATTRS];
   uint32_t tuple[ORC_MAX_ATTRS];
@@ -497,9 +499,36 @@ static int64_t join_rec(join_thread *th, int level) {
   return acc;
 }
 
+/* Thread pinning for timing runs.  The reference's pool pins worker i to CPU
+ * i (src/utils/thread_pool.cpp:44-49); here worker i runs on the i-th CPU of
+ * the process's affinity mask (so a CPU share given by the host is honoured),
+ * wrapping around when there are more workers than CPUs.  Off by default; the
+ * calling thread's own mask is restored after the join. */
+static int g_pin_threads = 0;
+static cpu_set_t g_allowed;
+static int g_allowed_n = 0;
+
+void orc_set_pin_threads(int on) { g_pin_threads = on; }
+
+static void pin_to(int index) {
+  if (g_allowed_n <= 0) return;
+  int k = index % g_allowed_n, seen = 0;
+  for (int c = 0; c < CPU_SETSIZE; c++) {
+    if (!CPU_ISSET(c, &g_allowed)) continue;
+    if (seen++ == k) {
+      cpu_set_t one;
+      CPU_ZERO(&one);
+      CPU_SET(c, &one);
+      pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
+      return;
+    }
+  }
+}
+
 static void *join_worker(void *arg) {
   join_thread *th = (join_thread *)arg;
   join_ctx *jc = th->jc;
+  if (g_pin_threads) pin_to(th->index);
   th->sum = 0;
   for (;;) {
     /* par::for_range dynamic chunks of 128 (parallel.cpp:65-79). */
@@ -588,8 +617,16 @@ static int64_t run_join(const orc_plan *p, orc_trie *const *tries, int nthreads,
   jc.stride = stride;
   join_thread *th = (join_thread *)calloc((size_t)nthreads, sizeof(join_thread));
   pthread_t *tid = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+  cpu_set_t caller_mask;
+  int restore = 0;
+  if (g_pin_threads && pthread_getaffinity_np(pthread_self(), sizeof(caller_mask), &caller_mask) == 0) {
+    g_allowed = caller_mask;
+    g_allowed_n = CPU_COUNT(&g_allowed);
+    restore = 1;
+  }
   for (int t = 0; t < nthreads; t++) {
     th[t].jc = &jc;
+    th[t].index = t;
     for (int k = 0; k < p->num_attrs; k++) {
       th[t].buf[k][0] = (uint8_t *)malloc(jc.maxbytes + 64);
       th[t].buf[k][1] = (uint8_t *)malloc(jc.maxbytes + 64);
@@ -599,6 +636,7 @@ static int64_t run_join(const orc_plan *p, orc_trie *const *tries, int nthreads,
   for (int t = 1; t < nthreads; t++) pthread_create(&tid[t], NULL, join_worker, &th[t]);
   join_worker(&th[0]);
   for (int t = 1; t < nthreads; t++) pthread_join(tid[t], NULL);
+  if (restore) pthread_setaffinity_np(pthread_self(), sizeof(caller_mask), &caller_mask);
   /* par::reducer::evaluate folds the per-thread slots (parallel.hpp:28-33). */
   int64_t total = 0;
   for (int t = 0; t < nthreads; t++) {

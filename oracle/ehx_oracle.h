@@ -102,6 +102,10 @@ int64_t orc_join_count(const orc_plan *p, orc_trie *const *tries, int nthreads,
 /* Same over every stride-th chunk of 128 level-0 values (a bounded,
  * stratified sample for CPU timing). */
 int64_t orc_join_count_sampled(const orc_plan *p, orc_trie *const *tries, int nthreads, uint64_t stride);
+/* Pin join workers to CPUs (worker i on the i-th CPU of the caller's affinity
+ * mask, as src/utils/thread_pool.cpp:44-49 pins worker i to CPU i); for
+ * timing runs. */
+void orc_set_pin_threads(int on);
 /* Size of the level-0 candidate set of a plan. */
 uint64_t orc_join_level0_size(const orc_plan *p, orc_trie *const *tries);
 /* Per level-0 value counts (out has orc_join_level0_size entries). */

@@ -37,6 +37,31 @@ def build():
 
 
 _lib = None
+_lib_info = {"path": LIB_PATH, "march": "x86-64-v3"}
+
+
+def use_native(outdir):
+    """Compiles the oracle for the host CPU (gcc -O3 -march=native, otherwise
+    the Makefile's flags) into outdir and makes lib() return that build, for
+    the CPU-baseline timing.  Call before anything else loads the oracle.
+    Returns the build description, or None when the compile fails (lib() then
+    stays the committed-flags build)."""
+    global _lib, _lib_info
+    if _lib is not None:
+        raise RuntimeError("the oracle is already loaded")
+    path = os.path.join(outdir, "libehx_oracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fPIC", "-std=gnu11", "-pthread", "-shared",
+           "-o", path, os.path.join(HERE, "ehx_oracle.c"), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return None
+    _lib = _configure(C.CDLL(path))
+    _lib_info = {"path": path, "march": "native", "command": " ".join(cmd[:8])}
+    return dict(_lib_info)
+
+
+def lib_info():
+    return dict(_lib_info)
 
 
 def lib():
@@ -44,64 +69,73 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        u32p, u64 = C.POINTER(C.c_uint32), C.c_uint64
-        L.orc_is_sparse.argtypes = [u64, u64]
-        L.orc_is_sparse.restype = C.c_int
-        L.orc_layout_of.argtypes = [u32p, u64]
-        L.orc_layout_of.restype = C.c_int
-        L.orc_bytes_needed.argtypes = [u64, u32p, C.c_int]
-        L.orc_bytes_needed.restype = u64
-        L.orc_set_build.argtypes = [C.c_void_p, u32p, u64, C.c_int, C.POINTER(OrcSet)]
-        L.orc_set_build.restype = u64
-        L.orc_set_find.argtypes = [C.POINTER(OrcSet), C.c_uint32]
-        L.orc_set_find.restype = C.c_int64
-        L.orc_set_decode.argtypes = [C.POINTER(OrcSet), u32p]
-        L.orc_set_decode.restype = u64
-        L.orc_intersect.argtypes = [C.POINTER(OrcSet), C.c_void_p, C.POINTER(OrcSet), C.POINTER(OrcSet)]
-        L.orc_intersect.restype = u64
-        L.orc_intersect_count.argtypes = [C.POINTER(OrcSet), C.POINTER(OrcSet)]
-        L.orc_intersect_count.restype = u64
-        L.orc_algorithm_of.argtypes = [C.POINTER(OrcSet), C.POINTER(OrcSet)]
-        L.orc_algorithm_of.restype = C.c_int
-        L.orc_uu_merge.argtypes = [u32p, u32p, u64, u32p, u64]
-        L.orc_uu_merge.restype = u64
-        L.orc_uu_gallop.argtypes = [u32p, u32p, u64, u32p, u64]
-        L.orc_uu_gallop.restype = u64
-        L.orc_trie_build.argtypes = [u32p, u32p, u64, C.c_int]
-        L.orc_trie_build.restype = C.c_void_p
-        L.orc_trie_free.argtypes = [C.c_void_p]
-        L.orc_trie_num_rows.argtypes = [C.c_void_p]
-        L.orc_trie_num_rows.restype = u64
-        L.orc_trie_head.argtypes = [C.c_void_p, C.POINTER(OrcSet)]
-        L.orc_trie_child.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(OrcSet)]
-        L.orc_trie_child.restype = C.c_int
-        L.orc_trie_tuples.argtypes = [C.c_void_p, u32p, u32p]
-        L.orc_trie_tuples.restype = u64
-        L.orc_trie_node_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), u32p]
-        L.orc_join_count.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int, u64, u64]
-        L.orc_join_count.restype = C.c_int64
-        L.orc_join_count_sampled.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int, u64]
-        L.orc_join_count_sampled.restype = C.c_int64
-        L.orc_join_level0_size.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p)]
-        L.orc_join_level0_size.restype = u64
-        L.orc_join_count_per_head.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int,
-                                              C.POINTER(C.c_int64)]
-        L.orc_join_count_per_head.restype = C.c_int
-        L.orc_join_materialize.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), u32p, u64]
-        L.orc_join_materialize.restype = u64
-        L.orc_cycle4_count.argtypes = [C.POINTER(C.c_uint64), u32p, C.c_uint32, C.c_int, C.POINTER(C.c_int64)]
-        L.orc_cycle4_count.restype = C.c_int64
-        L.orc_pagerank.argtypes = [C.c_void_p, u32p, C.POINTER(C.c_float), u64, C.c_double, C.c_double, C.c_double,
-                                   C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
-        L.orc_pagerank.restype = C.c_int
-        L.orc_write_refdb.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
-        L.orc_write_refdb.restype = C.c_int
-        L.orc_write_refdb_annotated.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
-                                                C.c_int]
-        L.orc_write_refdb_annotated.restype = C.c_int
-        _lib = L
+        _lib = _configure(C.CDLL(LIB_PATH))
     return _lib
+
+
+def _configure(L):
+    u32p, u64 = C.POINTER(C.c_uint32), C.c_uint64
+    L.orc_is_sparse.argtypes = [u64, u64]
+    L.orc_is_sparse.restype = C.c_int
+    L.orc_layout_of.argtypes = [u32p, u64]
+    L.orc_layout_of.restype = C.c_int
+    L.orc_bytes_needed.argtypes = [u64, u32p, C.c_int]
+    L.orc_bytes_needed.restype = u64
+    L.orc_set_build.argtypes = [C.c_void_p, u32p, u64, C.c_int, C.POINTER(OrcSet)]
+    L.orc_set_build.restype = u64
+    L.orc_set_find.argtypes = [C.POINTER(OrcSet), C.c_uint32]
+    L.orc_set_find.restype = C.c_int64
+    L.orc_set_decode.argtypes = [C.POINTER(OrcSet), u32p]
+    L.orc_set_decode.restype = u64
+    L.orc_intersect.argtypes = [C.POINTER(OrcSet), C.c_void_p, C.POINTER(OrcSet), C.POINTER(OrcSet)]
+    L.orc_intersect.restype = u64
+    L.orc_intersect_count.argtypes = [C.POINTER(OrcSet), C.POINTER(OrcSet)]
+    L.orc_intersect_count.restype = u64
+    L.orc_algorithm_of.argtypes = [C.POINTER(OrcSet), C.POINTER(OrcSet)]
+    L.orc_algorithm_of.restype = C.c_int
+    L.orc_uu_merge.argtypes = [u32p, u32p, u64, u32p, u64]
+    L.orc_uu_merge.restype = u64
+    L.orc_uu_gallop.argtypes = [u32p, u32p, u64, u32p, u64]
+    L.orc_uu_gallop.restype = u64
+    L.orc_trie_build.argtypes = [u32p, u32p, u64, C.c_int]
+    L.orc_trie_build.restype = C.c_void_p
+    L.orc_trie_free.argtypes = [C.c_void_p]
+    L.orc_trie_num_rows.argtypes = [C.c_void_p]
+    L.orc_trie_num_rows.restype = u64
+    L.orc_trie_head.argtypes = [C.c_void_p, C.POINTER(OrcSet)]
+    L.orc_trie_child.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(OrcSet)]
+    L.orc_trie_child.restype = C.c_int
+    L.orc_trie_tuples.argtypes = [C.c_void_p, u32p, u32p]
+    L.orc_trie_tuples.restype = u64
+    L.orc_trie_node_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), u32p]
+    L.orc_join_count.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int, u64, u64]
+    L.orc_join_count.restype = C.c_int64
+    L.orc_join_count_sampled.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int, u64]
+    L.orc_join_count_sampled.restype = C.c_int64
+    L.orc_join_level0_size.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p)]
+    L.orc_join_level0_size.restype = u64
+    L.orc_join_count_per_head.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), C.c_int,
+                                          C.POINTER(C.c_int64)]
+    L.orc_join_count_per_head.restype = C.c_int
+    L.orc_join_materialize.argtypes = [C.POINTER(OrcPlan), C.POINTER(C.c_void_p), u32p, u64]
+    L.orc_join_materialize.restype = u64
+    L.orc_cycle4_count.argtypes = [C.POINTER(C.c_uint64), u32p, C.c_uint32, C.c_int, C.POINTER(C.c_int64)]
+    L.orc_cycle4_count.restype = C.c_int64
+    L.orc_pagerank.argtypes = [C.c_void_p, u32p, C.POINTER(C.c_float), u64, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.orc_pagerank.restype = C.c_int
+    L.orc_write_refdb.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
+    L.orc_write_refdb.restype = C.c_int
+    L.orc_write_refdb_annotated.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                            C.c_int]
+    L.orc_write_refdb_annotated.restype = C.c_int
+    L.orc_set_pin_threads.argtypes = [C.c_int]
+    L.orc_set_pin_threads.restype = None
+    return L
+
+
+def set_pin_threads(on):
+    lib().orc_set_pin_threads(1 if on else 0)
 
 
 def _u32(a):

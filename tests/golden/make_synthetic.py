@@ -4,7 +4,7 @@ their committed seeds).  These pin the generators and guard regressions; the
 oracle itself is pinned by the reference's golden counts and by brute force
 (tests/test_oracle.py, tests/test_batch.py).
 
-    python tests/golden/make_synthetic.py
+    python tests/golden/make_synthetic.py [--only-triples]
 """
 import hashlib
 import json
@@ -30,7 +30,28 @@ def digest(*arrays):
     return h.hexdigest()
 
 
+def triples_entry():
+    """The triple table's two most frequent predicates (distinct rows) and the
+    closed forms of their 2-hop star and path joins (no join engine involved)."""
+    from emptyheaded_amd import triples as TR
+    s, p, o = TR.triple_table()
+    rels = TR.by_predicate(s, p, o, 64)
+    del s, p, o
+    R = [TR.distinct(*rels[0]), TR.distinct(*rels[1])]
+    return {"triples": 50_000_000, "predicates": 64, "zipf_s": 1.1, "rows": [len(R[0][0]), len(R[1][0])],
+            "sha256": digest(R[0][0], R[0][1], R[1][0], R[1][1]),
+            "star": TR.star_count(R[0], R[1]), "path": TR.path_count(R[0], R[1])}
+
+
 def main():
+    if "--only-triples" in sys.argv:  # refresh one entry of the existing file
+        with open(os.path.join(HERE, "synthetic.json")) as f:
+            meta = json.load(f)
+        meta["triples_50m_top2"] = triples_entry()
+        with open(os.path.join(HERE, "synthetic.json"), "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        print(json.dumps(meta["triples_50m_top2"], indent=1))
+        return
     meta = {}
     s, d, n = G.er_graph()
     meta["er_65536"] = {"vertices": n, "edges": len(s), "triangles": O.join_count(plans.TRIANGLE, [O.Trie(s, d)], THREADS)}
@@ -54,6 +75,7 @@ def main():
     meta["egonets_rmat22"] = {"nets": len(base) - 1, "edges": len(es), "vertices": int(base[-1]),
                               "sha256": digest(es, ed, base, off), "triangles": tri, "cycles4": cyc,
                               "seed": G.EGO_SEED, "min_edges": 64, "max_edges": 65536}
+    meta["triples_50m_top2"] = triples_entry()
     with open(os.path.join(HERE, "synthetic.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
     print(json.dumps(meta, indent=1))
