@@ -9,10 +9,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DROPIN_LIB = os.path.join(HERE, "engine", "libehx_dropin.so")
 QUERIES = ("TriangleAgg", "FliqueAgg", "Triangle", "TriangleCounts", "PageRankSteps", "WeightedDegree",
-           "FliqueSelAgg")
+           "FliqueSelAgg", "SSSP", "SSSPIter2")
 # Materializing queries: output columns and whether rows carry an annotation.
 MATERIALIZED = {"Triangle": (3, False), "TriangleCounts": (1, True), "PageRankSteps": (1, True),
-                "WeightedDegree": (1, True)}
+                "WeightedDegree": (1, True), "SSSP": (1, True), "SSSPIter2": (1, True)}
 
 _lib = None
 

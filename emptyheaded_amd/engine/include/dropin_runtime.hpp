@@ -65,6 +65,15 @@ bool replaying();
 void emit_rows(const uint32_t *prefix, int plen, const Set<hybrid> &leaf);
 void emit_row(const uint32_t *row, int ncols, uint64_t anno_bits);
 
+// tc::unweighted_single_source: the BFS level of every value from `start`
+// over the relation's child sets, computed on the GPU (direction-optimizing
+// when the relation is symmetric: the trie is then its own reverse).
+// dist[v] = level or 0xFFFFFFFF (unreached), for v below max(num_distinct,
+// largest value + 1); at most `iterations` levels (< 0: until the frontier
+// empties).  Returns the number of levels run.
+uint32_t single_source_levels(HostTrie *t, uint32_t start, int iterations, size_t num_distinct,
+                              std::vector<uint32_t> &dist);
+
 [[noreturn]] void not_implemented(const char *what);
 
 }  // namespace ehx_dropin

@@ -247,6 +247,34 @@ void free_host_trie(HostTrie *t) {
   delete t;
 }
 
+uint32_t single_source_levels(HostTrie *t, uint32_t start, int iterations, size_t num_distinct,
+                              std::vector<uint32_t> &dist) {
+  ehx_trie_meta meta;
+  size_t nd = num_distinct;
+  if (t && t->dev) {
+    check(ehx_trie_info(t->dev, &meta), "ehx_trie_info");
+    if (t->m && (size_t)meta.max_value + 1 > nd) nd = (size_t)meta.max_value + 1;
+  }
+  if ((size_t)start + 1 > nd) nd = (size_t)start + 1;
+  dist.assign(nd, 0xFFFFFFFFu);
+  if (!t || !t->dev || !t->m) {
+    dist[start] = 0;
+    return 0;
+  }
+  uint32_t levels = 0;
+  uint64_t c[4] = {0, 0, 0, 0};
+  check(ehx_trie_orient(ctx(), t->dev, EHX_POLICY_RULE, nullptr, nullptr, c), "ehx_trie_orient");
+  if (c[0] == c[1] && c[3] == 0)  // symmetric (loops do not matter here): the trie is its own reverse
+    check(ehx_sssp_unweighted_dir(ctx(), t->dev, t->dev, start, iterations, dist.data(), nd, &levels, 32),
+          "ehx_sssp_unweighted_dir");
+  else
+    check(ehx_sssp_unweighted(ctx(), t->dev, start, iterations, dist.data(), nd, &levels), "ehx_sssp_unweighted");
+  uint32_t deepest = 0;
+  for (uint32_t d : dist)
+    if (d != 0xFFFFFFFFu && d > deepest) deepest = d;
+  return deepest;
+}
+
 size_t host_trie_rows(const HostTrie *t) { return t->m; }
 uint32_t num_heads(const HostTrie *t) { return (uint32_t)t->nh; }
 void *block_of(const HostTrie *t, uint32_t node) { return (void *)&t->blocks[node]; }

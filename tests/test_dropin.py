@@ -293,3 +293,34 @@ def test_selection_golden(tmp_path, facebook_dup, fixtures_meta):
         f.write(np.uint64(n).tobytes())
         f.write(key_to_value.tobytes())
     assert dropin.run("FliqueSelAgg", db2)[0] == exp
+
+
+def test_sssp_generated_program(tmp_path, oracle, facebook_dup):
+    """The generated SSSP program (travis.py:556-567: a MIN recursion from the
+    selected source 0, compiled to tc::unweighted_single_source) through the
+    drop-in: the reference's check (the output's row 1000, rows in value order,
+    has annotation 2) and every row against the oracle's BFS levels (the start
+    keeps annotation 0, every visited value is a row, num_rows counts them);
+    the *[i=2] form stops after two levels.  On the symmetric facebook graph
+    (direction-optimizing on the device) and on a directed random graph."""
+    rng = np.random.default_rng(12)
+    rs = rng.integers(0, 6000, 30000).astype(np.uint32)
+    rd = rng.integers(0, 6000, 30000).astype(np.uint32)
+    rs[:3], rd[:3] = 0, [5, 17, 4000]
+    for name, (s, d) in (("facebook", facebook_dup), ("directed", (rs, rd))):
+        db = str(tmp_path / name)
+        dropin.write_relation(db, "Edge", s, d)
+        n = int(max(s.max(), d.max())) + 1
+        exp, _ = oracle.sssp_unweighted(s, d, 0, n)
+        for query, limit in (("SSSP", None), ("SSSPIter2", 2)):
+            keep = exp != E.UNREACHED if limit is None else exp <= limit
+            reached = np.flatnonzero(keep)
+            rows, ann, num_rows, got = dropin.rows(query, db, len(reached) + 16)
+            assert num_rows == got == len(reached), (name, query)
+            assert np.array_equal(rows[:, 0], reached), (name, query)
+            assert np.array_equal(ann, exp[reached].astype(np.float64)), (name, query)
+            v, nr = dropin.run(query, db)
+            assert nr == num_rows
+        if name == "facebook":  # the reference's own check
+            _, full_ann, _, _ = dropin.rows("SSSP", db, n)
+            assert full_ann[1000] == 2
