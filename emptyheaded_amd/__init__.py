@@ -82,6 +82,7 @@ SIGNATURES = {
     "ehx_trie_lookup": ([_vp, _vp, _u32p, C.c_uint64, C.POINTER(C.c_int64)], C.c_int),
     "ehx_trie_device_arrays": ([_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)], C.c_int),
     "ehx_intersect_count_batch": ([_vp, _vp, _u32p, _vp, _u32p, C.c_uint64, _u64p], C.c_int),
+    "ehx_intersect_count_batch_noalloc": ([_vp, _vp, _u32p, _vp, _u32p, C.c_uint64, _u64p], C.c_int),
     "ehx_intersect_batch": ([_vp, _vp, _u32p, _vp, _u32p, C.c_uint64, _u64p, _u32p, C.c_uint64, _u64p],
                             C.c_int),
     "ehx_intersect_nodes_count": ([_vp, C.c_int, C.POINTER(_vp), C.POINTER(_u32p), C.c_uint64, _u64p], C.c_int),
@@ -311,6 +312,19 @@ def intersect_count(ctx, ta, keys_a, tb, keys_b):
     out = np.zeros(len(ka), dtype=np.uint64)
     check(lib().ehx_intersect_count_batch(ctx.h, ta.h, ptr_u32(ka), tb.h, ptr_u32(kb), len(ka),
                                           out.ctypes.data_as(_u64p)))
+    return out
+
+
+NOALLOC_MAX_PAIRS = 1 << 20
+
+
+def intersect_count_noalloc(ctx, ta, keys_a, tb, keys_b):
+    """intersect_count through the entry point that allocates nothing (at most
+    NOALLOC_MAX_PAIRS pairs a call; calls on one GPU must not overlap)."""
+    ka, kb = u32(keys_a), u32(keys_b)
+    out = np.zeros(len(ka), dtype=np.uint64)
+    check(lib().ehx_intersect_count_batch_noalloc(ctx.h, ta.h, ptr_u32(ka), tb.h, ptr_u32(kb), len(ka),
+                                                  out.ctypes.data_as(_u64p)))
     return out
 
 

@@ -57,6 +57,8 @@ int join_materialize(ehx_ctx *, const ehx_plan *, ehx_trie *const *, int, uint32
                      ehx_join_stats *);
 int cycle4_count_segments(ehx_ctx *, const ehx_trie *, const uint32_t *, uint64_t, int64_t *, int64_t *,
                           ehx_join_stats *);
+int intersect_count_noalloc(ehx_ctx *, const ehx_trie *, const uint32_t *, const ehx_trie *, const uint32_t *, uint64_t,
+                            uint64_t *);
 int intersect_pairs(ehx_ctx *, const ehx_trie *, const uint32_t *, const ehx_trie *, const uint32_t *, uint64_t,
                     uint64_t *, uint64_t *, uint32_t *, uint64_t, uint64_t *);
 int intersect_nodes(ehx_ctx *, int, ehx_trie *const *, const uint32_t *const *, uint64_t, uint64_t *, uint64_t *,
@@ -162,6 +164,11 @@ int ehx_trie_device_arrays(const ehx_trie *t, const uint32_t **head_vals, const 
   if (offsets) *offsets = t->off;
   if (values) *values = t->vals;
   return EHX_OK;
+}
+
+int ehx_intersect_count_batch_noalloc(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, const ehx_trie *tb,
+                                      const uint32_t *keys_b, uint64_t n, uint64_t *counts) {
+  return intersect_count_noalloc(ctx, ta, keys_a, tb, keys_b, n, counts);
 }
 
 int ehx_intersect_count_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, const ehx_trie *tb,

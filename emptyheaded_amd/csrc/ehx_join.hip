@@ -15,6 +15,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "ehx_join_common.h"
 #include "ehx_tuned.h"
 
@@ -560,6 +562,62 @@ int intersect_pairs(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, co
   EHX_HIP(hipGetLastError());
   if ((rc = copy_d2h(ctx, offsets, o.p, (n + 1) * 8))) return rc;
   return copy_d2h(ctx, values, v.p, tot * 4);
+}
+
+// ---- pair intersection counts without allocating ---------------------------------
+// The scratch of ehx_intersect_count_batch_noalloc lives in this file-scope
+// device array: the staged keys of both sides (u32 each) and the counts (u64),
+// for at most EHX_NOALLOC_MAX_PAIRS pairs (16 MiB).  Each device holds its own
+// copy until the process ends.  Host code never passes `scratch` itself (its
+// name is a host-side stand-in there): the device address of each device's
+// copy is taken once with hipGetSymbolAddress, with that device current, and
+// kept in g_noalloc_ptr.
+constexpr uint64_t NOALLOC_KEYS_B = EHX_NOALLOC_MAX_PAIRS * 4;
+constexpr uint64_t NOALLOC_BYTES = EHX_NOALLOC_MAX_PAIRS * 16;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static uint8_t *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// The device address of `device`'s copy of scratch (that device current).
+static int noalloc_scratch(int device, uint8_t **out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(EHX_ERR_UNSUPPORTED, "device %d", device);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return fail(EHX_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    }
+    g_noalloc_ptr[device] = (uint8_t *)ptr;
+  }
+  *out = g_noalloc_ptr[device];
+  return EHX_OK;
+}
+
+int intersect_count_noalloc(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, const ehx_trie *tb,
+                            const uint32_t *keys_b, uint64_t n, uint64_t *counts) {
+  if (!ctx || !ta || !tb || (n && (!keys_a || !keys_b || !counts))) return fail(EHX_ERR_INVALID, "null argument");
+  if (n > EHX_NOALLOC_MAX_PAIRS)
+    return fail(EHX_ERR_INVALID, "%llu pairs: the no-allocation batch takes at most %llu", (unsigned long long)n,
+                (unsigned long long)EHX_NOALLOC_MAX_PAIRS);
+  if (ta->device != ctx->device || tb->device != ctx->device)
+    return fail(EHX_ERR_INVALID, "trie lives on another device");
+  if (n == 0) return EHX_OK;
+  EHX_HIP(hipSetDevice(ctx->device));
+  uint8_t *base = nullptr;
+  int rc;
+  if ((rc = noalloc_scratch(ctx->device, &base))) return rc;
+  uint32_t *ka = (uint32_t *)base, *kb = (uint32_t *)(base + NOALLOC_KEYS_B);
+  unsigned long long *cnt = (unsigned long long *)(base + 2 * NOALLOC_KEYS_B);
+  hipStream_t st = ctx->stream;
+  EHX_HIP(hipMemcpyAsync(ka, keys_a, n * 4, hipMemcpyHostToDevice, st));
+  EHX_HIP(hipMemcpyAsync(kb, keys_b, n * 4, hipMemcpyHostToDevice, st));
+  // every count is written (lane 0 of the pair's wave), so nothing is cleared first
+  k_pairs<false><<<grid_for(n * 64, 256, 65536), 256, 0, st>>>(view(ta), view(tb), ka, kb, n, cnt, nullptr, nullptr);
+  EHX_HIP(hipGetLastError());
+  return copy_d2h(ctx, counts, cnt, n * 8);
 }
 
 // ---- batched N-way intersections of trie nodes ------------------------------------

@@ -160,6 +160,18 @@ int ehx_trie_device_arrays(const ehx_trie *t, const uint32_t **head_vals, const 
 int ehx_intersect_count_batch(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a,
                               const ehx_trie *tb, const uint32_t *keys_b, uint64_t n,
                               uint64_t *counts);
+/* The same counts without allocating: no reference counterpart.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): no hipMalloc, hipFree or
+ * allocating library call; the staged keys and the counts use a fixed device
+ * array of the library's, one per GPU, held until the process ends.  At most
+ * EHX_NOALLOC_MAX_PAIRS pairs per call (EHX_ERR_INVALID beyond).  Calls that
+ * overlap on one GPU (from several threads or contexts) share that array and
+ * race; serialize them.  The context and tries are made beforehand. */
+#define EHX_NOALLOC_MAX_PAIRS (1ull << 20)
+int ehx_intersect_count_batch_noalloc(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a,
+                                      const ehx_trie *tb, const uint32_t *keys_b, uint64_t n,
+                                      uint64_t *counts);
 /* Materializing batch: the sorted intersections, concatenated into `values`
  * (capacity `cap`), with offsets[n+1]; ops::set_intersect(C,A,B) (set.hpp:53-91).
  * *total receives the number of values even when it exceeds cap (then nothing
