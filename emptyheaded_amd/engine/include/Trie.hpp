@@ -47,28 +47,25 @@ struct Trie {
       : annotated(annotated_in), num_rows(0), num_columns(num_columns_in), memoryBuffers(new M(path, 2)),
         annotation((A)0), host(nullptr), out(nullptr) {}
 
-  // Build from encoded columns (Trie.cpp:380-510): relations of one or two
+  // Build from encoded columns (Trie.cpp:380-510): relations of one to eight
   // attributes, with one annotation per row when annotation_in is not empty.
   Trie(std::string path, std::vector<uint32_t> *max_set_sizes, std::vector<std::vector<uint32_t>> *attr_in,
        std::vector<A> *annotation_in)
       : annotated(annotation_in && !annotation_in->empty()), num_rows(0), num_columns(attr_in->size()),
         memoryBuffers(new M(path, 2)), annotation((A)0), host(nullptr), out(nullptr) {
     (void)max_set_sizes;
-    if (num_columns != 1 && num_columns != 2)
-      ehx_dropin::not_implemented("tries over relations of arity other than 1 or 2");
     const size_t rows = attr_in->at(0).size();
-    assert(rows > 0 && (num_columns == 1 || attr_in->at(1).size() == rows));
-    if (annotated) assert(annotation_in->size() == rows);
-    if (num_columns == 2 && !annotated) {
-      host = ehx_dropin::build_host_trie(attr_in->at(0).data(), attr_in->at(1).data(), rows);
-    } else {
-      std::vector<uint64_t> bits;
-      if (annotated)
-        for (const A &a : *annotation_in) bits.push_back(ehx_anno_bits<A>(a));
-      host = ehx_dropin::build_annotated_trie((int)num_columns, attr_in->at(0).data(),
-                                              num_columns == 2 ? attr_in->at(1).data() : nullptr,
-                                              annotated ? bits.data() : nullptr, rows);
+    std::vector<const uint32_t *> cols;
+    for (size_t c = 0; c < num_columns; c++) {
+      assert(attr_in->at(c).size() == rows);
+      cols.push_back(attr_in->at(c).data());
     }
+    std::vector<uint64_t> bits;
+    if (annotated) {
+      assert(annotation_in->size() == rows);
+      for (const A &a : *annotation_in) bits.push_back(ehx_anno_bits<A>(a));
+    }
+    host = ehx_dropin::build_rows_trie((int)num_columns, cols.data(), rows, annotated ? bits.data() : nullptr);
     num_rows = ehx_dropin::host_trie_rows(host);
   }
 
@@ -103,15 +100,14 @@ struct Trie {
       return;
     }
     if (!host) return;
-    std::vector<uint32_t> c0, c1;
-    ehx_dropin::host_trie_tuples(host, c0, c1);
-    const int nc = ehx_dropin::host_trie_columns(host);
-    const std::vector<uint64_t> *an = ehx_dropin::host_trie_annos(host);
+    std::vector<uint32_t> rows;
+    std::vector<uint64_t> an;
+    ehx_dropin::host_trie_all_rows(host, rows, &an);
+    const size_t nc = (size_t)ehx_dropin::host_trie_columns(host);
     std::vector<uint32_t> t(nc);
-    for (size_t i = 0; i < c0.size(); i++) {
-      t[0] = c0[i];
-      if (nc == 2) t[1] = c1[i];
-      body(&t, an ? ehx_anno_value<A>(an->at(i)) : (A)0);
+    for (size_t i = 0; i < an.size(); i++) {
+      for (size_t c = 0; c < nc; c++) t[c] = rows[i * nc + c];
+      body(&t, ehx_anno_value<A>(an[i]));
     }
   }
 
@@ -120,20 +116,18 @@ struct Trie {
   }
 
   TrieBlock<layout, M> *getHead() {
-    if (!host && out && (out->ncols == 1 || out->ncols == 2) && out->num_rows()) {
+    if (!host && out && out->ncols >= 1 && out->num_rows()) {
       // a materialized output (with its annotations) read as the input of a
       // later bag or recursion step
       const size_t nr = out->num_rows(), nc = out->ncols;
-      std::vector<uint32_t> c0(nr), c1(nr);
-      for (size_t r = 0; r < nr; r++) {
-        c0[r] = out->rows[nc * r];
-        c1[r] = out->rows[nc * r + nc - 1];
+      std::vector<std::vector<uint32_t>> c(nc, std::vector<uint32_t>(nr));
+      std::vector<const uint32_t *> cols(nc);
+      for (size_t k = 0; k < nc; k++) {
+        for (size_t r = 0; r < nr; r++) c[k][r] = out->rows[nc * r + k];
+        cols[k] = c[k].data();
       }
       const bool an = out->annotated && out->annos.size() == nr;
-      if (nc == 2 && !an)
-        host = ehx_dropin::build_host_trie(c0.data(), c1.data(), nr);
-      else
-        host = ehx_dropin::build_annotated_trie((int)nc, c0.data(), c1.data(), an ? out->annos.data() : nullptr, nr);
+      host = ehx_dropin::build_rows_trie((int)nc, cols.data(), nr, an ? out->annos.data() : nullptr);
     }
     if (!host) return nullptr;
     return (TrieBlock<layout, M> *)ehx_dropin::block_of(host, ehx_dropin::num_heads(host));

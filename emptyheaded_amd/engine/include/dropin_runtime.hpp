@@ -17,6 +17,11 @@ HostTrie *load_host_trie(const std::string &path, int anno_bytes);
 // zero-extended; NULL: unannotated), rows in any order.
 HostTrie *build_annotated_trie(int ncols, const uint32_t *c0, const uint32_t *c1, const uint64_t *annos,
                                size_t rows);
+// A relation of any arity (1..8) from its columns, with one annotation per row
+// (value bytes zero-extended; NULL: unannotated); rows in any order.
+HostTrie *build_rows_trie(int ncols, const uint32_t *const *cols, size_t rows, const uint64_t *annos);
+// Every row (row-major, lexicographic) and, when annos is not NULL, its annotation.
+void host_trie_all_rows(const HostTrie *t, std::vector<uint32_t> &rows, std::vector<uint64_t> *annos);
 bool host_trie_annotated(const HostTrie *t);
 const std::vector<uint64_t> *host_trie_annos(const HostTrie *t);  // NULL when unannotated
 int host_trie_columns(const HostTrie *t);

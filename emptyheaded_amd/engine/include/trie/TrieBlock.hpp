@@ -15,6 +15,9 @@ long head_node(const HostTrie *t, uint32_t data);
 // The block of head node `node` (node == num_heads: the head block).
 void *block_of(const HostTrie *t, uint32_t node);
 uint32_t num_heads(const HostTrie *t);
+// The block one level below `data` in the non-head block (t, node), or NULL.
+void *child_of(const HostTrie *t, uint32_t node, uint32_t data);
+bool block_contains(const HostTrie *t, uint32_t node, uint32_t data);
 }  // namespace ehx_dropin
 
 template <class T, class M>
@@ -29,10 +32,12 @@ struct TrieBlock {
   inline Set<T> *get_set() const { return const_cast<Set<T> *>(&set); }
   inline bool is_head() const { return trie && node == ehx_dropin::num_heads(trie); }
 
-  // TrieBlock::get_next_block(data): the child block of `data`, NULL if absent.
+  // TrieBlock::get_next_block(data): the child block of `data`, NULL if absent
+  // (or when this block is the relation's last level).
   inline TrieBlock<T, M> *get_next_block(const uint32_t data, M *buffer) const {
     (void)buffer;
-    if (!is_head()) return nullptr;  // binary relations: child sets are leaves
+    if (!trie) return nullptr;
+    if (!is_head()) return (TrieBlock<T, M> *)ehx_dropin::child_of(trie, node, data);
     long h = ehx_dropin::head_node(trie, data);
     return h < 0 ? nullptr : (TrieBlock<T, M> *)ehx_dropin::block_of(trie, (uint32_t)h);
   }
@@ -41,7 +46,7 @@ struct TrieBlock {
     (void)index;
     return get_next_block(data, buffer);
   }
-  inline bool contains(const uint32_t data) const { return ehx_dropin::head_node(trie, data) >= 0 && is_head(); }
+  inline bool contains(const uint32_t data) const { return trie && ehx_dropin::block_contains(trie, node, data); }
 };
 
 #endif

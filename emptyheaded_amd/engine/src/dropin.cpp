@@ -10,6 +10,7 @@
 #include <stdlib.h>
 
 #include <atomic>
+#include <functional>
 #include <fstream>
 #include <mutex>
 #include <thread>
@@ -33,6 +34,11 @@ struct HostTrie {
   bool annotated = false;
   std::vector<uint64_t> annos;
   std::vector<uint32_t> child_vals;
+  // A relation of three or more attributes is a chain of binary tries: this
+  // one holds its first two levels (or, further down, one level keyed by the
+  // row position of the prefix in the trie above), `next` the level below,
+  // keyed by the position of each row of this trie (its index in `vals`).
+  HostTrie *next = nullptr;
 };
 
 static std::mutex g_mu;
@@ -130,9 +136,138 @@ HostTrie *build_annotated_trie(int ncols, const uint32_t *c0, const uint32_t *c1
   return t;
 }
 
-bool host_trie_annotated(const HostTrie *t) { return t->annotated; }
-const std::vector<uint64_t> *host_trie_annos(const HostTrie *t) { return t->annotated ? &t->annos : nullptr; }
+static const HostTrie *last_level(const HostTrie *t) {
+  while (t && t->next) t = t->next;
+  return t;
+}
+
+// Rows of any arity from columns: sorted lexicographically, equal rows merged
+// (the last annotation wins).  Arity 1 and 2 are one device trie; arity k >= 3
+// a chain of k-1 device tries: level 0 over the distinct (c0, c1) pairs, level
+// j over (position of the (c0..cj) prefix in level j-1, c(j+1)).  Every level
+// keeps its child values on the host (navigation, and sets the generated code
+// iterates), the last one the annotations.
+HostTrie *build_rows_trie(int ncols, const uint32_t *const *cols, size_t rows, const uint64_t *annos) {
+  if (ncols < 1 || ncols > 8) not_implemented("tries over relations of arity above 8");
+  if (ncols <= 2) {
+    if (ncols == 2 && !annos) return build_host_trie(cols[0], cols[1], rows);
+    return build_annotated_trie(ncols, cols[0], ncols == 2 ? cols[1] : nullptr, annos, rows);
+  }
+  std::vector<uint64_t> idx(rows);
+  for (size_t i = 0; i < rows; i++) idx[i] = i;
+  auto less = [&](uint64_t a, uint64_t b) {
+    for (int c = 0; c < ncols; c++)
+      if (cols[c][a] != cols[c][b]) return cols[c][a] < cols[c][b];
+    return false;
+  };
+  std::stable_sort(idx.begin(), idx.end(), less);
+  std::vector<std::vector<uint32_t>> C(ncols);
+  std::vector<uint64_t> an;
+  for (size_t k = 0; k < rows; k++) {
+    const uint64_t i = idx[k];
+    if (k && !less(idx[k - 1], i)) {  // equal to the previous row
+      if (annos) an.back() = annos[i];
+      continue;
+    }
+    for (int c = 0; c < ncols; c++) C[c].push_back(cols[c][i]);
+    if (annos) an.push_back(annos[i]);
+  }
+  const size_t n = C[0].size();
+  std::vector<uint32_t> pos(n), npos(n);
+  HostTrie *first = nullptr, *prev = nullptr;
+  for (int lv = 0; lv + 1 < ncols; lv++) {
+    std::vector<uint32_t> hk, vk;
+    hk.reserve(n);
+    vk.reserve(n);
+    for (size_t r = 0; r < n; r++) {
+      const uint32_t key = lv == 0 ? C[0][r] : pos[r], v = C[lv + 1][r];
+      if (hk.empty() || hk.back() != key || vk.back() != v) {
+        hk.push_back(key);
+        vk.push_back(v);
+      }
+      npos[r] = (uint32_t)(hk.size() - 1);
+    }
+    HostTrie *t = build_host_trie(hk.data(), vk.data(), hk.size());
+    t->ncols = ncols;
+    t->child_vals = std::move(vk);
+    for (uint64_t h = 0; h < t->nh; h++) {
+      Block &b = t->blocks[h];
+      b.set.values = t->child_vals.data() + t->off[h];
+      b.set.range = b.set.cardinality ? b.set.values[b.set.cardinality - 1] : 0;
+    }
+    if (!first) first = t;
+    else prev->next = t;
+    prev = t;
+    pos.swap(npos);
+  }
+  if (annos) {
+    prev->annotated = true;
+    prev->annos = std::move(an);
+  }
+  return first;
+}
+
+bool host_trie_annotated(const HostTrie *t) { return last_level(t)->annotated; }
+const std::vector<uint64_t> *host_trie_annos(const HostTrie *t) {
+  const HostTrie *l = last_level(t);
+  return l->annotated ? &l->annos : nullptr;
+}
 int host_trie_columns(const HostTrie *t) { return t->ncols; }
+
+// The block of `data` in the set of non-head block (t, node): the set one level
+// down, found through the row position of `data` (NULL when absent or when
+// the set is a relation's last level).
+void *child_of(const HostTrie *t, uint32_t node, uint32_t data) {
+  if (!t->next) return nullptr;
+  const uint32_t lo = t->off[node], hi = t->off[node + 1];
+  auto it = std::lower_bound(t->child_vals.begin() + lo, t->child_vals.begin() + hi, data);
+  if (it == t->child_vals.begin() + hi || *it != data) return nullptr;
+  const long h = head_node(t->next, (uint32_t)(it - t->child_vals.begin()));
+  return h < 0 ? nullptr : block_of(t->next, (uint32_t)h);
+}
+
+bool block_contains(const HostTrie *t, uint32_t node, uint32_t data) {
+  if (node == t->nh) return head_node(t, data) >= 0;
+  if (t->child_vals.empty()) not_implemented("membership test on a child set whose values are on the GPU");
+  const uint32_t lo = t->off[node], hi = t->off[node + 1];
+  return std::binary_search(t->child_vals.begin() + lo, t->child_vals.begin() + hi, data);
+}
+
+// Every row in lexicographic order (row-major), with its annotation.
+void host_trie_all_rows(const HostTrie *t, std::vector<uint32_t> &rows, std::vector<uint64_t> *annos) {
+  rows.clear();
+  if (annos) annos->clear();
+  if (t->ncols <= 2) {
+    std::vector<uint32_t> c0, c1;
+    host_trie_tuples(t, c0, c1);
+    const std::vector<uint64_t> *an = host_trie_annos(t);
+    for (size_t i = 0; i < c0.size(); i++) {
+      rows.push_back(c0[i]);
+      if (t->ncols == 2) rows.push_back(c1[i]);
+      if (annos) annos->push_back(an ? an->at(i) : 0);
+    }
+    return;
+  }
+  const int nc = t->ncols;
+  std::vector<uint32_t> prefix(nc);
+  // depth-first over the chain: (trie, node) at level lv covers prefix[0..lv]
+  std::function<void(const HostTrie *, uint32_t, int)> walk = [&](const HostTrie *L, uint32_t node, int lv) {
+    for (uint32_t p = L->off[node]; p < L->off[node + 1]; p++) {
+      prefix[lv + 1] = L->child_vals[p];
+      if (!L->next) {
+        rows.insert(rows.end(), prefix.begin(), prefix.end());
+        if (annos) annos->push_back(L->annotated ? L->annos[p] : 0);
+        continue;
+      }
+      const long h = head_node(L->next, p);
+      if (h >= 0) walk(L->next, (uint32_t)h, lv + 1);
+    }
+  };
+  for (uint32_t h = 0; h < t->nh; h++) {
+    prefix[0] = t->head[h];
+    walk(t, h, 0);
+  }
+}
 
 // TrieIterator::get_annotation (src/TrieIterator.cpp:44-59): the annotation of
 // `data` in the set of block `b`, 0 when absent.
@@ -140,6 +275,7 @@ uint64_t annotation_of(const void *block, uint32_t data) {
   const Block *b = (const Block *)block;
   if (!b || !b->trie) return 0;
   const HostTrie *t = b->trie;
+  if (t->next) not_implemented("annotations above a relation's last level");
   if (!t->annotated) not_implemented("TrieIterator::get_annotation on an unannotated relation");
   if (b->node == t->nh) {  // the head set
     if (t->ncols != 1) not_implemented("annotations above a relation's last level");
@@ -168,14 +304,15 @@ HostTrie *load_host_trie(const std::string &path, int anno_bytes) {
     uint64_t rows = 0;
     int32_t annotated = 0;
     check(ehx_refdb_info(db, &nc, &rows, &annotated), "ehx_refdb_info");
-    if (nc != 1 && nc != 2) not_implemented("tries over relations of arity other than 1 or 2");
-    std::vector<uint32_t> c0(rows), c1(nc == 2 ? rows : 0);
+    if (nc < 1 || nc > 8) not_implemented("tries over relations of arity above 8");
+    std::vector<std::vector<uint32_t>> c(nc, std::vector<uint32_t>(rows));
     std::vector<uint64_t> an(annotated ? rows : 0);
-    uint32_t *cols[2] = {c0.data(), c1.data()};
-    check(ehx_refdb_columns(db, cols, annotated ? an.data() : nullptr), "ehx_refdb_columns");
+    std::vector<uint32_t *> cols(nc);
+    std::vector<const uint32_t *> ccols(nc);
+    for (uint32_t k = 0; k < nc; k++) ccols[k] = cols[k] = c[k].data();
+    check(ehx_refdb_columns(db, cols.data(), annotated ? an.data() : nullptr), "ehx_refdb_columns");
     ehx_refdb_close(db);
-    if (!annotated && nc == 2) return build_host_trie(c0.data(), c1.data(), rows);
-    return build_annotated_trie((int)nc, c0.data(), c1.data(), annotated ? an.data() : nullptr, rows);
+    return build_rows_trie((int)nc, ccols.data(), rows, annotated ? an.data() : nullptr);
   }
   const size_t slash = path.find_last_of('/');
   const std::string parent = slash == std::string::npos ? "." : path.substr(0, slash);
@@ -190,24 +327,35 @@ HostTrie *load_host_trie(const std::string &path, int anno_bytes) {
     uint64_t rows = 0;
     int32_t annotated = 0;
     check(ehx_refdb_info(db, &nc, &rows, &annotated), "ehx_refdb_info");
-    std::vector<uint32_t> c0(rows), c1(rows);
+    if (nc < 1 || nc > 8) not_implemented("tries over relations of arity above 8");
+    std::vector<std::vector<uint32_t>> c(nc, std::vector<uint32_t>(rows));
     std::vector<uint64_t> an(annotated ? rows : 0);
-    if (nc == 1) {
-      uint32_t *cols[1] = {c0.data()};
-      check(ehx_refdb_columns(db, cols, annotated ? an.data() : nullptr), "ehx_refdb_columns");
-    } else {
-      const size_t u2 = leaf.find_last_of('_');
-      const size_t u1 = u2 == std::string::npos || u2 == 0 ? std::string::npos : leaf.find_last_of('_', u2 - 1);
-      if (u1 == std::string::npos) not_implemented("relation folder without an _<i>_<j> ordering suffix");
-      const int first = atoi(leaf.substr(u1 + 1, u2 - u1 - 1).c_str()), second = atoi(leaf.substr(u2 + 1).c_str());
-      if (nc != 2 || first < 0 || second < 0 || first > 1 || second > 1 || first == second)
-        not_implemented("tries over relations of arity other than 1 or 2");
-      uint32_t *cols[2] = {first == 0 ? c0.data() : c1.data(), first == 0 ? c1.data() : c0.data()};
-      check(ehx_refdb_columns(db, cols, annotated ? an.data() : nullptr), "ehx_refdb_columns");
+    // the ordering: the folder name's suffix _<i0>_<i1>..., attribute i_k on level k
+    std::vector<int> order(nc, 0);
+    if (nc > 1) {
+      size_t end = leaf.size();
+      for (int k = (int)nc - 1; k >= 0; k--) {
+        const size_t u = end == 0 ? std::string::npos : leaf.find_last_of('_', end - 1);
+        if (u == std::string::npos) not_implemented("relation folder without an _<i>_<j>... ordering suffix");
+        order[k] = atoi(leaf.substr(u + 1, end - u - 1).c_str());
+        end = u;
+      }
+      std::vector<int> seen(nc, 0);
+      for (int k = 0; k < (int)nc; k++) {
+        if (order[k] < 0 || order[k] >= (int)nc || seen[order[k]]++)
+          not_implemented("relation folder whose ordering suffix is not a permutation of its attributes");
+      }
     }
+    // column j of the store lands on the level that holds attribute j
+    std::vector<uint32_t *> cols(nc);
+    std::vector<const uint32_t *> levels(nc);
+    for (uint32_t k = 0; k < nc; k++) {
+      cols[order[k]] = c[k].data();
+      levels[k] = c[k].data();
+    }
+    check(ehx_refdb_columns(db, cols.data(), annotated ? an.data() : nullptr), "ehx_refdb_columns");
     ehx_refdb_close(db);
-    if (!annotated && nc == 2) return build_host_trie(c0.data(), c1.data(), rows);
-    return build_annotated_trie((int)nc, c0.data(), c1.data(), annotated ? an.data() : nullptr, rows);
+    return build_rows_trie((int)nc, levels.data(), rows, annotated ? an.data() : nullptr);
   }
   std::ifstream in(rows_file(path), std::ios::binary);
   if (!in) {
@@ -233,6 +381,7 @@ void host_trie_tuples(const HostTrie *t, std::vector<uint32_t> &c0, std::vector<
 }
 
 void save_host_trie(const HostTrie *t, const std::string &path) {
+  if (t->ncols > 2) not_implemented("Trie::save of a relation of arity above 2");
   std::vector<uint32_t> c0, c1;
   host_trie_tuples(t, c0, c1);
   std::ofstream out(rows_file(path), std::ios::binary);
@@ -243,8 +392,12 @@ void save_host_trie(const HostTrie *t, const std::string &path) {
 }
 
 void free_host_trie(HostTrie *t) {
-  if (t->dev) ehx_trie_destroy(t->dev);
-  delete t;
+  while (t) {
+    HostTrie *n = t->next;
+    if (t->dev) ehx_trie_destroy(t->dev);
+    delete t;
+    t = n;
+  }
 }
 
 uint32_t single_source_levels(HostTrie *t, uint32_t start, int iterations, size_t num_distinct,
@@ -275,7 +428,7 @@ uint32_t single_source_levels(HostTrie *t, uint32_t start, int iterations, size_
   return deepest;
 }
 
-size_t host_trie_rows(const HostTrie *t) { return t->m; }
+size_t host_trie_rows(const HostTrie *t) { return last_level(t)->m; }
 uint32_t num_heads(const HostTrie *t) { return (uint32_t)t->nh; }
 void *block_of(const HostTrie *t, uint32_t node) { return (void *)&t->blocks[node]; }
 

@@ -15,6 +15,13 @@ and the selections test_duplicated runs (travis.py:647-693):
   BarbellSelAgg, p=6 = 26,936,100 (travis.py:473-489; BarbellSel rows, :519-530)
   LollipopAgg        = 1,426,911,480      (travis.py:14-30)
   BarbellAgg         = 20,371,831,447,136 (travis.py:55-70)
+and the materialized multi-bag queries over test/graph/data/simple.tsv (two
+triangles joined by an edge, both directions), read back through the
+reference's DataFrame (its columns the output trie's levels permuted by the
+relation's ordering, codegen/trie/Trie.scala:219-230):
+  Lollipop(a,b,c,x)     = 28 rows, df.iloc[27] = (5,3,4,4)      (travis.py:157-172)
+  Barbell(a,b,c,x,y,z)  = 56 rows, df.iloc[55] = (5,3,4,3,4,5)  (travis.py:109-125,
+                          checked there under check_big_out)
 and PageRank over facebook_duplicated with test/graph/data/inv_degree.tsv
 (4,039 float32 inverse degrees): the rank of the first vertex after the base
 case and 5 iterations is 15.227079960463206, checked to 0.0001
@@ -99,6 +106,11 @@ def main(ref_root):
             "source": "test/graph/data/simple.tsv",
             "sha256": hashlib.sha256(rawt).hexdigest(),
             "rows": int(simple.shape[0]),
+            "Lollipop_rows": 28,
+            "Lollipop_row27": [5, 3, 4, 4],
+            "Barbell_rows": 56,
+            "Barbell_row55": [5, 3, 4, 3, 4, 5],
+            "cite": "test/graph/travis.py:109-125,157-172",
         },
     }
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:

@@ -324,3 +324,119 @@ def test_sssp_generated_program(tmp_path, oracle, facebook_dup):
         if name == "facebook":  # the reference's own check
             _, full_ann, _, _ = dropin.rows("SSSP", db, n)
             assert full_ann[1000] == 2
+
+
+# ---- multi-bag plans (LollipopAgg / BarbellAgg and their materialized forms) -----
+
+def _sym_sets(s, d):
+    n = int(max(s.max(), d.max())) + 1
+    N = [set() for _ in range(n)]
+    for a, b in zip(s.tolist(), d.tolist()):
+        N[a].add(b)
+    return N
+
+
+def _lollipop_rows(N):
+    """Lollipop(a,b,c,x) by brute force, as (a, x, b, c) tuples sorted: the
+    output trie's level order."""
+    out = []
+    for a in range(len(N)):
+        for b in N[a]:
+            for c in N[b] & N[a]:
+                for x in N[a]:
+                    out.append((a, x, b, c))
+    return np.array(sorted(out), dtype=np.uint32).reshape(-1, 4)
+
+
+def _barbell_rows(N):
+    """Barbell(a,b,c,x,y,z) by brute force, as (a, x, y, z, b, c) sorted."""
+    tri = [[(b, c) for b in N[v] for c in (N[b] & N[v])] for v in range(len(N))]
+    out = []
+    for a in range(len(N)):
+        for x in N[a]:
+            for (y, z) in tri[x]:
+                for (b, c) in tri[a]:
+                    out.append((a, x, y, z, b, c))
+    return np.array(sorted(out), dtype=np.uint32).reshape(-1, 6)
+
+
+def reference_dataframe(trie_rows, ordering):
+    """The reference's DataFrame of an output trie (codegen/trie/Trie.scala:
+    219-230): column i holds the trie's level ordering[i], where ordering is
+    the output relation's attribute permutation (QueryPlan.scala:156-170)."""
+    return trie_rows[:, list(ordering)]
+
+
+def test_multibag_generated_aggregates_golden(tmp_path, facebook_dup, fixtures_meta):
+    """The generated LollipopAgg and BarbellAgg programs (a triangle bag
+    annotated by its COUNT per vertex, then the root bag over Edge(a,x) folding
+    those annotations) give the reference's 1,426,911,480 and
+    20,371,831,447,136 on the symmetric facebook graph (travis.py:14-70)."""
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", *facebook_dup)
+    meta = fixtures_meta["facebook_duplicated"]
+    assert dropin.run("LollipopAgg", db) == (meta["LollipopAgg"], 0)
+    assert dropin.run("BarbellAgg", db) == (meta["BarbellAgg"], 0)
+
+
+@pytest.mark.parametrize("scale", [9, 12])
+def test_multibag_generated_aggregates_oracle(tmp_path, oracle, scale):
+    """The same programs on symmetric R-MAT graphs against the oracle's
+    per-vertex triangle counts folded over the edges in numpy."""
+    s, d, n = G.rmat_graph(scale)
+    s, d = G.symmetric(s, d)
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", s, d)
+    nv = int(max(s.max(), d.max())) + 1
+    T = np.zeros(nv, np.int64)
+    T[np.unique(s)] = oracle.join_count_per_head(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
+    s64, d64 = s.astype(np.int64), d.astype(np.int64)
+    with np.errstate(over="ignore"):
+        assert dropin.run("LollipopAgg", db)[0] == int(T[s64].sum())
+        assert dropin.run("BarbellAgg", db)[0] == int((T[s64] * T[d64]).sum())
+
+
+def test_multibag_materialized_simple_golden(tmp_path, fixtures_meta):
+    """The generated Lollipop and Barbell programs (triangle bag, root bag,
+    top-down pass reading the three-level triangle trie) on the reference's
+    simple.tsv: 28 and 56 rows, every row equal to brute force in the output
+    trie's level order, and the reference's DataFrame rows travis.py checks:
+    df.iloc[27] = (5,3,4,4) and df.iloc[55] = (5,3,4,3,4,5)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "simple.npz"))
+    s, d = z["src"], z["dst"]
+    meta = fixtures_meta["simple"]
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", s, d)
+    N = _sym_sets(s, d)
+    rows, _, nr, got = dropin.rows("Lollipop", db, 1000)
+    assert nr == got == meta["Lollipop_rows"] == 28
+    assert np.array_equal(rows, _lollipop_rows(N))
+    assert tuple(reference_dataframe(rows, (0, 3, 1, 2))[27]) == tuple(meta["Lollipop_row27"])
+    rows, _, nr, got = dropin.rows("Barbell", db, 1000)
+    assert nr == got == meta["Barbell_rows"] == 56
+    assert np.array_equal(rows, _barbell_rows(N))
+    assert tuple(reference_dataframe(rows, (0, 3, 4, 5, 1, 2))[55]) == tuple(meta["Barbell_row55"])
+    assert dropin.run("Lollipop", db)[1] == 28 and dropin.run("Barbell", db)[1] == 56
+
+
+def test_multibag_materialized_random(tmp_path):
+    """Materialized Lollipop and Barbell on random symmetric graphs with hubs
+    and isolated values, every row against brute force."""
+    rng = np.random.default_rng(31)
+    for n, m in ((40, 120), (90, 300)):
+        a = rng.integers(0, n, m)
+        b = (a + rng.integers(1, 6, m)) % n  # short-range edges make triangles
+        a = np.concatenate([a, np.zeros(12, np.int64)])
+        b = np.concatenate([b, rng.integers(1, n, 12)])  # vertex 0 a hub
+        keep = a != b
+        s = np.concatenate([a[keep], b[keep]]).astype(np.uint32)
+        d = np.concatenate([b[keep], a[keep]]).astype(np.uint32)
+        db = str(tmp_path / ("g%d" % n))
+        dropin.write_relation(db, "Edge", s, d)
+        N = _sym_sets(s, d)
+        for q, brute in (("Lollipop", _lollipop_rows), ("Barbell", _barbell_rows)):
+            exp = brute(N)
+            rows, _, nr, got = dropin.rows(q, db, len(exp) + 10)
+            assert nr == got == len(exp), (q, n)
+            assert np.array_equal(rows, exp), (q, n)
