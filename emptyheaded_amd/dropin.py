@@ -9,11 +9,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DROPIN_LIB = os.path.join(HERE, "engine", "libehx_dropin.so")
 QUERIES = ("TriangleAgg", "FliqueAgg", "Triangle", "TriangleCounts", "PageRankSteps", "WeightedDegree",
-           "FliqueSelAgg", "SSSP", "SSSPIter2", "LollipopAgg", "BarbellAgg", "Lollipop", "Barbell")
+           "FliqueSelAgg", "SSSP", "SSSPIter2", "LollipopAgg", "BarbellAgg", "Lollipop", "Barbell", "TriangleProj")
 # Materializing queries: output columns and whether rows carry an annotation.
 MATERIALIZED = {"Triangle": (3, False), "TriangleCounts": (1, True), "PageRankSteps": (1, True),
                 "WeightedDegree": (1, True), "SSSP": (1, True), "SSSPIter2": (1, True),
-                "Lollipop": (4, False), "Barbell": (6, False)}
+                "Lollipop": (4, False), "Barbell": (6, False), "TriangleProj": (2, False)}
 
 _lib = None
 

@@ -10,6 +10,7 @@
 #include <stdlib.h>
 
 #include <atomic>
+#include <chrono>
 #include <functional>
 #include <fstream>
 #include <mutex>
@@ -620,6 +621,16 @@ static void run_chunks(const Set<hybrid> *level0, size_t nthreads,
   for (auto &x : th) x.join();
 }
 
+// EHX_DROPIN_PROFILE=1: per pass, the wall time of the host lambdas, of
+// gathering the recorded calls and of the batched GPU launch, on stderr.
+static bool profiling() {
+  static const bool on = getenv("EHX_DROPIN_PROFILE") && atoi(getenv("EHX_DROPIN_PROFILE")) > 0;
+  return on;
+}
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
                 const std::function<void(size_t, uint32_t, uint32_t)> &body, OutRows *out) {
   if (g_sess) not_implemented("nested par_foreach_aggregate / par_foreach_builder");
@@ -637,10 +648,12 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
     S.annos.resize(nchunks);
   }
   g_sess = &S;
+  const bool prof = profiling();
   for (size_t p = 0; p < levels; p++) {
     S.record_level = (int)p;
     for (auto &cl : S.logs)
       for (auto &l : cl) l.cursor = 0;
+    const double t0 = prof ? now_ms() : 0;
     run_chunks(level0, nthreads, body);
     // one batch for the level, calls in chunk order
     LevelBatch &B = S.batches[p];
@@ -649,7 +662,11 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
       cl[p].base = total;
       total += cl[p].ncalls;
     }
-    if (total == 0) continue;
+    const double t1 = prof ? now_ms() : 0;
+    if (total == 0) {
+      if (prof) fprintf(stderr, "[dropin] pass %zu: lambdas %.2f ms, no recorded calls\n", p, t1 - t0);
+      continue;
+    }
     std::vector<std::vector<uint32_t>> nodes(B.nops, std::vector<uint32_t>(total));
     for (auto &cl : S.logs) {
       LevelLog &L = cl[p];
@@ -657,12 +674,18 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
         for (int s = 0; s < B.nops; s++) nodes[s][L.base + i] = L.nodes[i * B.nops + s];
       std::vector<uint32_t>().swap(L.nodes);
     }
+    const double t2 = prof ? now_ms() : 0;
     run_batch(B.nops, B.tries, nodes, !B.materialized, B.counts, B.offsets, B.values);
+    if (prof)
+      fprintf(stderr, "[dropin] pass %zu: lambdas %.2f ms, gather %.2f ms, GPU batch of %llu calls (%s) %.2f ms\n", p,
+              t1 - t0, t2 - t1, (unsigned long long)total, B.materialized ? "materialized" : "counts", now_ms() - t2);
   }
   S.record_level = -1;
   for (auto &cl : S.logs)
     for (auto &l : cl) l.cursor = 0;
+  const double t3 = prof ? now_ms() : 0;
   run_chunks(level0, nthreads, body);
+  if (prof) fprintf(stderr, "[dropin] replay: lambdas %.2f ms\n", now_ms() - t3);
   g_sess = nullptr;
   if (out) {  // chunks follow the level-0 set, so their concatenation is lexicographic
     size_t nr = 0, na = 0;

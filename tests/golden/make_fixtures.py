@@ -8,6 +8,7 @@ the constants those tests assert:
   FliqueAgg    = 30,004,668  (travis.py:238-254)
   Triangle     = 1,612,010 rows, first row (6,5,2) (travis.py:302-318)
   Flique       = 30,004,668 rows, first row (9,8,7,0) (travis.py:342-354)
+  TriangleProj = 85,658 rows, first row (6,5)  (travis.py:282-300)
 and, for the graph with both edge directions (facebook_duplicated.tsv),
 9,672,060 triangle tuples (docs/notebooks/Getting Started.ipynb, cell 22),
 and the selections test_duplicated runs (travis.py:647-693):
@@ -78,7 +79,9 @@ def main(ref_root):
             "Triangle_row0": [6, 5, 2],
             "Flique_rows": 30004668,
             "Flique_row0": [9, 8, 7, 0],
-            "cite": "test/graph/travis.py:197-213,238-254,302-318,342-354",
+            "TriangleProj_rows": 85658,
+            "TriangleProj_row0": [6, 5],
+            "cite": "test/graph/travis.py:197-213,238-254,282-300,302-318,342-354",
         },
         "facebook_duplicated": {
             "source": "test/graph/data/facebook_duplicated.tsv",

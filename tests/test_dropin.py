@@ -440,3 +440,16 @@ def test_multibag_materialized_random(tmp_path):
             rows, _, nr, got = dropin.rows(q, db, len(exp) + 10)
             assert nr == got == len(exp), (q, n)
             assert np.array_equal(rows, exp), (q, n)
+
+
+def test_triangle_projection_generated(tmp_path, oracle, facebook, fixtures_meta):
+    """The generated TriangleProj program (a child bag that keeps (a, b) with c
+    dropped, then the root over Edge(a,b)) through the drop-in: the
+    reference's 85,658 rows, first row (6,5), every row equal to the oracle's
+    one-bag plan (travis.py:282-300)."""
+    meta = fixtures_meta["facebook_pruned"]
+    db = str(tmp_path / "db")
+    dropin.write_relation(db, "Edge", *facebook)
+    rows, _, nr, got = dropin.rows("TriangleProj", db, 100000)
+    assert nr == got == meta["TriangleProj_rows"] and list(rows[0]) == meta["TriangleProj_row0"]
+    assert np.array_equal(rows, oracle.join_materialize(plans.TRIANGLE_PROJ, [oracle.Trie(*facebook)]))

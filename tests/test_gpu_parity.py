@@ -477,3 +477,19 @@ def test_sssp_golden_and_oracle(ctx, oracle, facebook_dup):
                     got, lv = E.sssp_unweighted(ctx, T, start, n, it, rev=rev, bottom_up=bu)
                     assert np.array_equal(got, exp), (sym, start, it, bu)
                     assert lv == elv
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_triangle_projection_golden(ctx, oracle, facebook, fixtures_meta, form):
+    """TriangleProj(a,b) (travis.py:282-300) through ehx_join_materialize: the
+    reference's 85,658 rows and first row (6,5), every row equal to the
+    oracle's, under the density rule and all-bitset layouts."""
+    meta = fixtures_meta["facebook_pruned"]
+    exp = oracle.join_materialize(plans.TRIANGLE_PROJ, [oracle.Trie(*facebook)])
+    for policy in (E.POLICY_RULE, E.POLICY_ALL_BITSET):
+        T = E.Trie(ctx, *facebook, policy=policy)
+        R = E.join_materialize(ctx, plans.TRIANGLE_PROJ, [T], form)
+        assert len(R) == meta["TriangleProj_rows"] and list(R[0]) == meta["TriangleProj_row0"]
+        assert np.array_equal(R, exp)
+        assert E.join_count(ctx, plans.TRIANGLE_PROJ, [T], form) == len(exp)
+        T.close()

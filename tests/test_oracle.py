@@ -470,3 +470,23 @@ def test_triangles_by_vertex_identity(oracle):
             A[S, D] = True
             A3 = (A.astype(np.int64) @ A.astype(np.int64)) * A
             assert np.array_equal(T, A3.sum(1))
+
+
+def test_triangle_projection_golden(oracle, facebook, fixtures_meta):
+    """TriangleProj(a,b) as the reference evaluates it (the existential c is
+    dropped by its generator): 85,658 rows with first row (6,5) on
+    facebook_pruned (travis.py:282-300) -- exactly the edges whose second
+    vertex has a child set, computed here in numpy; the edges that close a
+    triangle (scipy: (A A^T) masked by A) are fewer, 84,185."""
+    import scipy.sparse as sp
+    from emptyheaded_amd import plans
+    s, d = facebook
+    meta = fixtures_meta["facebook_pruned"]
+    R = oracle.join_materialize(plans.TRIANGLE_PROJ, [oracle.Trie(s, d)])
+    assert len(R) == meta["TriangleProj_rows"] == 85658 and list(R[0]) == meta["TriangleProj_row0"]
+    keep = np.isin(d, np.unique(s))
+    exp = np.array(sorted(zip(s[keep].tolist(), d[keep].tolist())), dtype=np.uint32)
+    assert np.array_equal(R, exp)
+    n = int(max(s.max(), d.max())) + 1
+    A = sp.csr_matrix((np.ones(len(s)), (s.astype(np.int64), d.astype(np.int64))), shape=(n, n))
+    assert (A.multiply(A @ A.T) > 0).sum() == 84185
