@@ -133,6 +133,8 @@ SIGNATURES = {
     "ehx_selector_clear": ([], C.c_int),
     "ehx_trie_set_features": ([_vp, _vp, C.POINTER(C.c_float)], C.c_int),
     "ehx_copy_calibrate": ([_vp, C.c_uint64, C.POINTER(C.c_double)], C.c_int),
+    "ehx_host_alloc": ([C.c_uint64, C.POINTER(_vp)], C.c_int),
+    "ehx_host_free": ([_vp], C.c_int),
     "ehx_join_record": ([_vp, C.POINTER(Plan), C.POINTER(_vp), C.c_int, _u64p, C.c_uint64, _u64p,
                          C.POINTER(C.c_int64)], C.c_int),
     "ehx_mlp_create": ([_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint64,
@@ -312,6 +314,18 @@ def intersect_count(ctx, ta, keys_a, tb, keys_b):
     out = np.zeros(len(ka), dtype=np.uint64)
     check(lib().ehx_intersect_count_batch(ctx.h, ta.h, ptr_u32(ka), tb.h, ptr_u32(kb), len(ka),
                                           out.ctypes.data_as(_u64p)))
+    return out
+
+
+def intersect_nodes_count(ctx, tries, nodes):
+    """|set(tries[0], nodes[0][i]) ∩ ... | for every call i: node == num_heads
+    names a trie's head set (TrieBuilder::build_aggregated_set over blocks)."""
+    ns = [u32(x) for x in nodes]
+    n = len(ns[0])
+    out = np.zeros(n, dtype=np.uint64)
+    tarr = (_vp * len(tries))(*[t.h.value for t in tries])
+    narr = (_u32p * len(ns))(*[ptr_u32(x) for x in ns])
+    check(lib().ehx_intersect_nodes_count(ctx.h, len(tries), tarr, narr, n, out.ctypes.data_as(_u64p)))
     return out
 
 

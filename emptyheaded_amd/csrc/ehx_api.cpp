@@ -166,6 +166,25 @@ int ehx_trie_device_arrays(const ehx_trie *t, const uint32_t **head_vals, const 
   return EHX_OK;
 }
 
+int ehx_host_alloc(uint64_t bytes, void **out) {
+  if (!out) return fail(EHX_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (!bytes) return EHX_OK;
+  void *p = nullptr;
+  const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(EHX_ERR_NOMEM, "hipHostMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+  }
+  *out = p;
+  return EHX_OK;
+}
+
+int ehx_host_free(void *p) {
+  if (p) EHX_HIP(hipHostFree(p));
+  return EHX_OK;
+}
+
 int ehx_intersect_count_batch_noalloc(ehx_ctx *ctx, const ehx_trie *ta, const uint32_t *keys_a, const ehx_trie *tb,
                                       const uint32_t *keys_b, uint64_t n, uint64_t *counts) {
   return intersect_count_noalloc(ctx, ta, keys_a, tb, keys_b, n, counts);

@@ -640,6 +640,56 @@ __global__ __launch_bounds__(256) void k_nodes(NodeOps O, uint64_t n, unsigned l
   }
 }
 
+// Operand node lists checked on the device before any set is read: bad[0] =
+// 1 when some node of operand s exceeds lim[s] (node == num_heads names the
+// head set).
+struct NodeLimits {
+  const uint32_t *nodes[EHX_MAX_OPS];
+  uint32_t lim[EHX_MAX_OPS];
+  int32_t nsets;
+};
+__global__ __launch_bounds__(256) void k_check_nodes(NodeLimits L, uint64_t n, unsigned *__restrict__ bad) {
+  bool over = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    for (int s = 0; s < L.nsets; s++) over |= L.nodes[s][i] > L.lim[s];
+  if (__any(over) && __lane_id() == 0) atomicOr(bad, 1u);
+}
+
+// Count-only pairs whose smaller set holds at most SMALL_PAIR values, 16
+// lanes per pair (four pairs per wavefront): each lane probes its share of the
+// smaller set's values in the larger set (binary search in a uint set, a bit
+// test in a bitset), and the group sums the hits.  A wavefront per pair would
+// leave most of its lanes idle on such sets.  Pairs above the bound are left
+// to k_nodes, which skips the small ones.
+constexpr uint32_t SMALL_PAIR = 256;
+__device__ __forceinline__ bool small_pair(const DSet &A, const DSet &B) { return min(A.n, B.n) <= SMALL_PAIR; }
+__global__ __launch_bounds__(256) void k_pairs_small(NodeOps O, uint64_t n, unsigned long long *__restrict__ counts) {
+  const uint32_t g = __lane_id() & 15;
+  const uint64_t ngroups = ((uint64_t)gridDim.x * blockDim.x) >> 4;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; t < n; t += ngroups) {
+    const DSet A = set_of(O.t[0], O.nodes[0][t]), B = set_of(O.t[1], O.nodes[1][t]);
+    if (!small_pair(A, B)) continue;  // group-uniform
+    const DSet &r = A.n <= B.n ? A : B, &f = A.n <= B.n ? B : A;
+    uint32_t hits = 0;
+    for (uint32_t i = g; i < r.n; i += 16) hits += set_contains(f, r.v[i]) ? 1u : 0u;
+#pragma unroll
+    for (int d = 8; d >= 1; d >>= 1) hits += __shfl_xor(hits, d, 16);
+    if (g == 0) counts[t] = hits;
+  }
+}
+
+template <bool EMIT, bool BIG_ONLY>
+__global__ __launch_bounds__(256) void k_nodes_big(NodeOps O, uint64_t n, unsigned long long *__restrict__ counts) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nwaves) {
+    DSet S[2] = {set_of(O.t[0], O.nodes[0][t]), set_of(O.t[1], O.nodes[1][t])};
+    if (BIG_ONLY && small_pair(S[0], S[1])) continue;  // wave-uniform
+    int alg;
+    const uint32_t c = wave_intersect<false>(S, 2, nullptr, &alg);
+    if (__lane_id() == 0) counts[t] = c;
+  }
+}
+
 int intersect_nodes(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint32_t *const *nodes, uint64_t n,
                     uint64_t *counts, uint64_t *offsets, uint32_t *values, uint64_t cap, uint64_t *total) {
   if (!ctx || !tries || !nodes || nsets < 1 || nsets > EHX_MAX_OPS) return fail(EHX_ERR_INVALID, "bad node batch");
@@ -654,20 +704,38 @@ int intersect_nodes(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint3
   memset(&O, 0, sizeof(O));
   O.nsets = nsets;
   DevBuf &nb = ctx->scratch[S_IN0], &c = ctx->scratch[S_COUNTS], &o = ctx->scratch[S_OFFS];
+  DevBuf &fl = ctx->scratch[S_FLAGS];
   int rc;
-  if ((rc = nb.ensure((uint64_t)nsets * n * 4)) || (rc = c.ensure(n * 8 + 8)) || (rc = o.ensure(n * 8 + 8))) return rc;
+  if ((rc = nb.ensure((uint64_t)nsets * n * 4)) || (rc = c.ensure(n * 8 + 8)) || (rc = o.ensure(n * 8 + 8)) ||
+      (rc = fl.ensure(64)))
+    return rc;
+  NodeLimits NL;
+  memset(&NL, 0, sizeof(NL));
+  NL.nsets = nsets;
   for (int s = 0; s < nsets; s++) {
     if (!tries[s] || !nodes[s]) return fail(EHX_ERR_INVALID, "operand %d: null trie or nodes", s);
     if (tries[s]->device != ctx->device) return fail(EHX_ERR_INVALID, "operand %d lives on another device", s);
-    for (uint64_t i = 0; i < n; i++)
-      if (nodes[s][i] > tries[s]->nh) return fail(EHX_ERR_INVALID, "operand %d call %llu: node out of range", s,
-                                                  (unsigned long long)i);
     O.t[s] = view(tries[s]);
     O.nodes[s] = nb.as<uint32_t>() + (uint64_t)s * n;
+    NL.nodes[s] = O.nodes[s];
+    NL.lim[s] = (uint32_t)tries[s]->nh;
     EHX_HIP(hipMemcpyAsync((void *)O.nodes[s], nodes[s], n * 4, hipMemcpyHostToDevice, st));
   }
+  // every node in range before a set is read
+  if ((rc = fill_u32(ctx, fl.as<uint32_t>(), 0u, 1))) return rc;
+  k_check_nodes<<<grid_for(n, 256, 8192), 256, 0, st>>>(NL, n, fl.as<unsigned>());
+  EHX_HIP(hipGetLastError());
+  uint32_t bad = 0;
+  if ((rc = copy_d2h(ctx, &bad, fl.p, 4))) return rc;
+  if (bad) return fail(EHX_ERR_INVALID, "an operand node is out of range (above its trie's head count)");
   unsigned G = grid_for(n * 64, 256, 65536);
-  k_nodes<false><<<G, 256, 0, st>>>(O, n, c.as<unsigned long long>(), nullptr, nullptr);
+  if (counts && nsets == 2) {  // count-only pairs: small ones 16 lanes each, the rest a wavefront each
+    k_pairs_small<<<grid_for(n * 16, 256, 65536), 256, 0, st>>>(O, n, c.as<unsigned long long>());
+    EHX_HIP(hipGetLastError());
+    k_nodes_big<false, true><<<G, 256, 0, st>>>(O, n, c.as<unsigned long long>());
+  } else {
+    k_nodes<false><<<G, 256, 0, st>>>(O, n, c.as<unsigned long long>(), nullptr, nullptr);
+  }
   EHX_HIP(hipGetLastError());
   if (counts) return copy_d2h(ctx, counts, c.p, n * 8);
   if ((rc = scan_u64(ctx, c.as<uint64_t>(), o.as<uint64_t>(), n))) return rc;

@@ -4,6 +4,7 @@
 #define EHX_DROPIN_RUNTIME_H
 
 #include "utils/common.hpp"
+#include "utils/parallel.hpp"
 #include "trie/TrieBlock.hpp"
 
 namespace ehx_dropin {
@@ -63,8 +64,6 @@ size_t aggregate(size_t num_attrs, uint32_t depth, const Operands &ops, Set<hybr
 // final replay; rows emitted during the replay are gathered into `out`.
 void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
                 const std::function<void(size_t, uint32_t, uint32_t)> &body, OutRows *out = nullptr);
-// True during the final replay, the only run whose effects count.
-bool replaying();
 // Output rows of the current level-0 chunk (replay only): `prefix` followed by
 // each value of `leaf`, or one row with an annotation.
 void emit_rows(const uint32_t *prefix, int plen, const Set<hybrid> &leaf);

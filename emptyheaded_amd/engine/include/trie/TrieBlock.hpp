@@ -9,6 +9,17 @@
 #include "set/Set.hpp"
 
 namespace ehx_dropin {
+// What inline head navigation reads: the first base of every HostTrie (the
+// runtime's host mirror of a device trie), so a block reaches it from its
+// trie pointer without a call.
+struct HostNav {
+  const int32_t *dense = nullptr;  // value -> head node, when the head is dense
+  uint64_t dense_len = 0;
+  const uint32_t *head = nullptr;  // sorted head values
+  uint32_t nh = 0;
+  char *blocks = nullptr;   // nh + 1 blocks by node (the last one is the head block)
+  char *vblocks = nullptr;  // with a dense head: dense_len blocks by value (absent values: no trie)
+};
 struct HostTrie;
 // Node index of `data` in the trie's head set, or -1.
 long head_node(const HostTrie *t, uint32_t data);
@@ -30,16 +41,25 @@ struct TrieBlock {
 
   inline const Set<T> *get_const_set() const { return &set; }
   inline Set<T> *get_set() const { return const_cast<Set<T> *>(&set); }
-  inline bool is_head() const { return trie && node == ehx_dropin::num_heads(trie); }
+  inline const ehx_dropin::HostNav *nav() const { return reinterpret_cast<const ehx_dropin::HostNav *>(trie); }
+  inline bool is_head() const { return trie && node == nav()->nh; }
 
   // TrieBlock::get_next_block(data): the child block of `data`, NULL if absent
-  // (or when this block is the relation's last level).
+  // (or when this block is the relation's last level).  The head's NextLevel
+  // lookup (TrieBlock.hpp:86-111) runs inline on the host mirror.
   inline TrieBlock<T, M> *get_next_block(const uint32_t data, M *buffer) const {
     (void)buffer;
     if (!trie) return nullptr;
-    if (!is_head()) return (TrieBlock<T, M> *)ehx_dropin::child_of(trie, node, data);
-    long h = ehx_dropin::head_node(trie, data);
-    return h < 0 ? nullptr : (TrieBlock<T, M> *)ehx_dropin::block_of(trie, (uint32_t)h);
+    const ehx_dropin::HostNav *nv = nav();
+    if (node != nv->nh) return (TrieBlock<T, M> *)ehx_dropin::child_of(trie, node, data);
+    if (nv->vblocks) {  // one load: the block stored at the value's slot
+      if (data >= nv->dense_len) return nullptr;
+      TrieBlock<T, M> *b = reinterpret_cast<TrieBlock<T, M> *>(nv->vblocks) + data;
+      return b->trie ? b : nullptr;
+    }
+    const uint32_t *p = std::lower_bound(nv->head, nv->head + nv->nh, data);
+    const long h = (p != nv->head + nv->nh && *p == data) ? (long)(p - nv->head) : -1;
+    return h < 0 ? nullptr : reinterpret_cast<TrieBlock<T, M> *>(nv->blocks) + h;
   }
   // get_next_block(index, data): dense heads index by data, sparse by index.
   inline TrieBlock<T, M> *get_next_block(const uint32_t index, const uint32_t data, M *buffer) const {

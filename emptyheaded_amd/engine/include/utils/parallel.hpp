@@ -7,9 +7,12 @@
 #include "common.hpp"
 
 namespace ehx_dropin {
-// True while the builders record calls instead of computing them; reducer
-// updates are dropped then, so only the final replay folds values.
-bool recording();
+// 0: no pass running, 1: the builders record calls instead of computing them,
+// 2: the final replay.  Reducer updates are dropped while recording, so only
+// the final replay folds values.
+extern int g_phase;
+inline bool recording() { return g_phase == 1; }
+inline bool replaying() { return g_phase == 2; }
 }  // namespace ehx_dropin
 
 namespace par {

@@ -17,19 +17,21 @@
 #include <thread>
 
 #include "../../../include/ehx.h"
+#include "../../../include/ehx_ext.h"
 #include "../include/emptyheaded.hpp"
 
 typedef TrieBlock<hybrid, ParMemoryBuffer> Block;
 
 namespace ehx_dropin {
 
-struct HostTrie {
+struct HostTrie : HostNav {
   ehx_trie *dev = nullptr;
   uint64_t nh = 0, m = 0;
   int ncols = 2;  // 1: a unary relation, kept as the device trie of (x, x)
   std::vector<uint32_t> head, off;
   std::vector<int32_t> dense;
-  std::vector<Block> blocks;  // nh + 1 (the last is the head block)
+  std::vector<Block> blocks;
+  std::vector<Block> value_blocks;  // dense heads: a copy of each child block at its value's slot  // nh + 1 (the last is the head block)
   // annotated relations: one annotation per row in trie order (per head for a
   // unary relation), and the child values on the host to find a row's position
   bool annotated = false;
@@ -63,6 +65,19 @@ static ehx_ctx *ctx() {
   return g_ctx;
 }
 
+// With a dense head, a copy of every child block at its value's slot, so a
+// child lookup reads one block instead of the value -> node array and then
+// the block (a random host access each, per call of the generated code).
+// Rebuilt whenever the node-indexed blocks change.
+static void index_blocks_by_value(HostTrie *t) {
+  std::vector<Block>().swap(t->value_blocks);
+  t->vblocks = nullptr;
+  if (t->dense.empty()) return;
+  t->value_blocks.resize(t->dense.size());  // default blocks: no trie (absent values)
+  for (uint64_t h = 0; h < t->nh; h++) t->value_blocks[t->head[h]] = t->blocks[h];
+  t->vblocks = (char *)t->value_blocks.data();
+}
+
 HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
   HostTrie *t = new HostTrie();
   check(ehx_trie_build(ctx(), c0, c1, rows, EHX_POLICY_RULE, 0, &t->dev), "ehx_trie_build");
@@ -83,6 +98,13 @@ HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
     for (uint64_t h = 0; h < t->nh; h++) t->dense[t->head[h]] = (int32_t)h;
   }
   t->blocks.resize(t->nh + 1);
+  HostNav &nv = *t;
+  nv.dense = t->dense.empty() ? nullptr : t->dense.data();
+  nv.dense_len = t->dense.size();
+  nv.head = t->head.data();
+  nv.nh = (uint32_t)t->nh;
+  nv.blocks = (char *)t->blocks.data();
+  if ((const void *)static_cast<const HostNav *>(t) != (const void *)t) not_implemented("host trie layout");
   for (uint64_t h = 0; h <= t->nh; h++) {
     Block &b = t->blocks[h];
     b.trie = t;
@@ -99,6 +121,7 @@ HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
       b.set.number_of_bytes = 4ull * t->nh;
     }
   }
+  index_blocks_by_value(t);
   return t;
 }
 
@@ -196,6 +219,7 @@ HostTrie *build_rows_trie(int ncols, const uint32_t *const *cols, size_t rows, c
       b.set.values = t->child_vals.data() + t->off[h];
       b.set.range = b.set.cardinality ? b.set.values[b.set.cardinality - 1] : 0;
     }
+    index_blocks_by_value(t);
     if (!first) first = t;
     else prev->next = t;
     prev = t;
@@ -462,6 +486,38 @@ static void run_batch(int nops, const HostTrie *const *tries, std::vector<std::v
                "ehx_intersect_nodes");
 }
 
+// Page-locked host memory, grow-only (ehx_host_alloc): the batches of a level
+// cross PCIe at the full rate.  Pool buffers are kept for the life of the
+// process (one query at a time runs its passes, so one set per builder level).
+struct Pinned {
+  void *p = nullptr;
+  size_t cap = 0;
+  template <class T> T *as() const { return (T *)p; }
+  void ensure(size_t bytes) {
+    if (bytes <= cap) return;
+    if (p) check(ehx_host_free(p), "ehx_host_free");
+    p = nullptr;
+    cap = 0;
+    const size_t want = bytes + bytes / 4;  // headroom: the next query is often a little larger
+    check(ehx_host_alloc(want, &p), "ehx_host_alloc");
+    cap = want;
+  }
+};
+struct LevelBuffers {
+  Pinned nodes, counts, offsets, values;
+};
+static std::vector<LevelBuffers *> g_level_pool;
+static LevelBuffers &level_buffers(size_t level) {
+  while (g_level_pool.size() <= level) g_level_pool.push_back(new LevelBuffers());
+  return *g_level_pool[level];
+}
+
+// Cardinality of a set named by (trie, node): the bound of any intersection
+// that takes it as an operand.
+static inline uint64_t card_of(const HostTrie *t, uint32_t node) {
+  return node == t->nh ? t->nh : (uint64_t)(t->off[node + 1] - t->off[node]);
+}
+
 static inline void set_empty(Set<hybrid> *out) {
   *out = Set<hybrid>(0, 0, 0, type::UINTEGER);
 }
@@ -511,8 +567,10 @@ struct LevelBatch {
   int nops = 0;
   const HostTrie *tries[8] = {};
   bool materialized = false;
-  std::vector<uint64_t> counts, offsets;
-  std::vector<uint32_t> values;
+  // results in the level's pinned buffers: counts (count-only levels), or
+  // offsets[n+1] and the values (materialized levels)
+  const uint64_t *counts = nullptr, *offsets = nullptr;
+  const uint32_t *values = nullptr;
 };
 
 struct Session {
@@ -527,11 +585,14 @@ struct Session {
 };
 
 static Session *g_sess = nullptr;
-static thread_local uint64_t t_chunk = 0;
+// the chunk of level-0 values a worker is running (initial-exec: one load, no call)
+static thread_local uint64_t t_chunk __attribute__((tls_model("initial-exec"))) = 0;
+// 0: no pass running, 1: recording pass, 2: final replay.  Written only by the
+// thread driving the passes while no worker runs (workers start after the
+// write and are joined before the next), read inline by reducers and builders.
+int g_phase = 0;
 static const uint64_t CHUNK = 128;  // par::for_range dynamic chunk (parallel.cpp:65-79)
 
-bool recording() { return g_sess && g_sess->record_level >= 0; }
-bool replaying() { return g_sess && g_sess->record_level < 0; }
 
 void emit_rows(const uint32_t *prefix, int plen, const Set<hybrid> &leaf) {
   Session *S = g_sess;
@@ -594,7 +655,7 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
   if (B.materialized) {
     const uint64_t a = B.offsets[idx], b = B.offsets[idx + 1];
     *out = Set<hybrid>((uint32_t)(b - a), b > a ? B.values[b - 1] : 0, 4 * (b - a), result_type(ops));
-    out->values = B.values.data() + a;
+    out->values = B.values + a;
   } else {
     *out = Set<hybrid>((uint32_t)B.counts[idx], 0, 4 * B.counts[idx], result_type(ops));
   }
@@ -651,6 +712,7 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
   const bool prof = profiling();
   for (size_t p = 0; p < levels; p++) {
     S.record_level = (int)p;
+    g_phase = 1;
     for (auto &cl : S.logs)
       for (auto &l : cl) l.cursor = 0;
     const double t0 = prof ? now_ms() : 0;
@@ -667,26 +729,74 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
       if (prof) fprintf(stderr, "[dropin] pass %zu: lambdas %.2f ms, no recorded calls\n", p, t1 - t0);
       continue;
     }
-    std::vector<std::vector<uint32_t>> nodes(B.nops, std::vector<uint32_t>(total));
-    for (auto &cl : S.logs) {
-      LevelLog &L = cl[p];
-      for (uint64_t i = 0; i < L.ncalls; i++)
-        for (int s = 0; s < B.nops; s++) nodes[s][L.base + i] = L.nodes[i * B.nops + s];
-      std::vector<uint32_t>().swap(L.nodes);
+    // operand nodes, operand-major, gathered from the chunk logs in parallel
+    // (each chunk fills its own range), with the bound on the results' size
+    LevelBuffers &LB = level_buffers(p);
+    const int nops = B.nops;
+    LB.nodes.ensure((size_t)nops * total * 4);
+    uint32_t *nodes = LB.nodes.as<uint32_t>();
+    std::atomic<uint64_t> next_chunk(0), bound(0);
+    auto gather = [&]() {
+      uint64_t local = 0;
+      for (;;) {
+        const uint64_t c = next_chunk.fetch_add(1);
+        if (c >= S.logs.size()) break;
+        LevelLog &L = S.logs[c][p];
+        for (uint64_t i = 0; i < L.ncalls; i++) {
+          uint64_t mn = ~0ull;
+          for (int s = 0; s < nops; s++) {
+            const uint32_t nd = L.nodes[i * nops + s];
+            nodes[(uint64_t)s * total + L.base + i] = nd;
+            if (B.materialized) mn = std::min(mn, card_of(B.tries[s], nd));
+          }
+          if (B.materialized) local += mn;
+        }
+        std::vector<uint32_t>().swap(L.nodes);
+      }
+      bound += local;
+    };
+    {
+      std::vector<std::thread> th;
+      for (size_t t = 1; t < nthreads; t++) th.emplace_back(gather);
+      gather();
+      for (auto &x : th) x.join();
     }
     const double t2 = prof ? now_ms() : 0;
-    run_batch(B.nops, B.tries, nodes, !B.materialized, B.counts, B.offsets, B.values);
+    ehx_trie *dt[8];
+    const uint32_t *np[8];
+    for (int s = 0; s < nops; s++) {
+      dt[s] = B.tries[s]->dev;
+      np[s] = nodes + (uint64_t)s * total;
+    }
+    if (B.materialized) {
+      const uint64_t cap = bound.load();
+      LB.offsets.ensure((total + 1) * 8);
+      LB.values.ensure((cap ? cap : 1) * 4);
+      uint64_t got = 0;
+      check(ehx_intersect_nodes(ctx(), nops, dt, np, total, LB.offsets.as<uint64_t>(), LB.values.as<uint32_t>(), cap,
+                                &got),
+            "ehx_intersect_nodes");
+      B.offsets = LB.offsets.as<uint64_t>();
+      B.values = LB.values.as<uint32_t>();
+    } else {
+      LB.counts.ensure(total * 8);
+      check(ehx_intersect_nodes_count(ctx(), nops, dt, np, total, LB.counts.as<uint64_t>()),
+            "ehx_intersect_nodes_count");
+      B.counts = LB.counts.as<uint64_t>();
+    }
     if (prof)
       fprintf(stderr, "[dropin] pass %zu: lambdas %.2f ms, gather %.2f ms, GPU batch of %llu calls (%s) %.2f ms\n", p,
               t1 - t0, t2 - t1, (unsigned long long)total, B.materialized ? "materialized" : "counts", now_ms() - t2);
   }
   S.record_level = -1;
+  g_phase = 2;
   for (auto &cl : S.logs)
     for (auto &l : cl) l.cursor = 0;
   const double t3 = prof ? now_ms() : 0;
   run_chunks(level0, nthreads, body);
   if (prof) fprintf(stderr, "[dropin] replay: lambdas %.2f ms\n", now_ms() - t3);
   g_sess = nullptr;
+  g_phase = 0;
   if (out) {  // chunks follow the level-0 set, so their concatenation is lexicographic
     size_t nr = 0, na = 0;
     for (auto &r : S.rows) nr += r.size();

@@ -62,6 +62,13 @@ int ehx_mlp_destroy(ehx_mlp *h);
 int ehx_mlp_grad(ehx_mlp *h, const float *params, const float *norm, float *grad, float *loss);
 int ehx_mlp_predict(ehx_ctx *ctx, const float *params, const float *norm, const float *x, uint64_t n, float *out);
 
+/* Page-locked host memory (hipHostMalloc), for callers that stage large
+ * batches of host arrays through the entry points above (operand node lists,
+ * counts, materialized values): copies from and to it run at the full PCIe
+ * rate.  Free with ehx_host_free. */
+int ehx_host_alloc(uint64_t bytes, void **out);
+int ehx_host_free(void *p);
+
 /* Measurement aid: one launch of a plain 16-byte-per-thread copy kernel that
  * reads and writes `bytes` bytes (the reference for scaling HBM traffic
  * counters); *ms receives its event-timed duration. */

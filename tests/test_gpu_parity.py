@@ -493,3 +493,34 @@ def test_triangle_projection_golden(ctx, oracle, facebook, fixtures_meta, form):
         assert np.array_equal(R, exp)
         assert E.join_count(ctx, plans.TRIANGLE_PROJ, [T], form) == len(exp)
         T.close()
+
+
+def test_batched_node_counts_every_layout_pair(ctx):
+    """ehx_intersect_nodes_count (the drop-in's batched build_aggregated_set):
+    pairs of small sets (16 lanes a pair), pairs above that bound (a wavefront
+    a pair), head sets, empty sets and three-way calls, against numpy, for
+    every layout pair."""
+    rng = np.random.default_rng(23)
+    sizes = np.concatenate([rng.integers(1, 60, 300), rng.integers(200, 600, 60), rng.integers(2000, 9000, 8)])
+    src, dst = set_collection(rng, sizes, 400000, dense_frac=0.4)
+    nsets = int(src.max()) + 1
+    sets = [dst[src == i] for i in range(nsets)]
+    for pa, pb in ((E.POLICY_ALL_UINT, E.POLICY_ALL_UINT), (E.POLICY_ALL_UINT, E.POLICY_ALL_BITSET),
+                   (E.POLICY_ALL_BITSET, E.POLICY_ALL_BITSET), (E.POLICY_RULE, E.POLICY_RULE)):
+        TA, TB = E.Trie(ctx, src, dst, pa), E.Trie(ctx, src, dst, pb)
+        heads_a = TA.info()["num_heads"]
+        na = rng.integers(0, nsets, 5000).astype(np.uint32)
+        nb = rng.integers(0, nsets, 5000).astype(np.uint32)
+        na[:3] = heads_a  # the head set
+        got = E.intersect_nodes_count(ctx, [TA, TB], [na, nb])
+        head = np.unique(src).astype(np.uint32)
+        A = [head if x == heads_a else sets[x] for x in na]
+        exp = np.array([len(np.intersect1d(a, sets[y])) for a, y in zip(A, nb)], np.uint64)
+        assert np.array_equal(got, exp), (pa, pb)
+        nc = rng.integers(0, nsets, 5000).astype(np.uint32)
+        got3 = E.intersect_nodes_count(ctx, [TA, TB, TA], [na, nb, nc])
+        exp3 = np.array([len(np.intersect1d(np.intersect1d(a, sets[y]), sets[z])) for a, y, z in zip(A, nb, nc)],
+                        np.uint64)
+        assert np.array_equal(got3, exp3), (pa, pb)
+        TA.close()
+        TB.close()
