@@ -706,10 +706,12 @@ def main():
     build_wall_ms = (time.perf_counter() - t_build) * 1e3
     R_rule = E.Trie(ctx, dst, src, E.POLICY_RULE)
     info = T_learn.info()
-    log("tries built: rule %.2f ms, learned %.2f ms (+ reverse %.2f / %.2f ms); bitsets %d (learned) vs %d (rule); "
+    log("tries built: rule %.2f ms, learned %.2f ms (+ reverse %.2f / %.2f ms); sets staged as bitmaps %d (learned) "
+        "vs %d (rule); "
         "child-pointer link %.2f ms (%s); headline build wall %.2f ms" %
         (T_rule.info()["build_ms"], info["build_ms"], R_rule.info()["build_ms"], R_learn.info()["build_ms"],
-         info["num_bitsets"], T_rule.info()["num_bitsets"], link_ms, "built" if linked else "none", build_wall_ms))
+         info["num_staged_bitmaps"], T_rule.info()["num_staged_bitmaps"], link_ms, "built" if linked else "none",
+         build_wall_ms))
 
     if args.copy_calib:
         import ctypes as C

@@ -39,7 +39,7 @@ class TrieMeta(C.Structure):
     _fields_ = [("num_rows", C.c_uint64), ("num_heads", C.c_uint64), ("num_bitsets", C.c_uint64),
                 ("bitset_words", C.c_uint64), ("bitset_elems", C.c_uint64), ("max_card", C.c_uint32),
                 ("head_layout", C.c_uint32), ("dense_nav", C.c_uint32), ("max_value", C.c_uint32),
-                ("build_ms", C.c_double)]
+                ("build_ms", C.c_double), ("num_staged_bitmaps", C.c_uint64)]
 
 
 class JoinStats(C.Structure):

@@ -65,6 +65,10 @@ struct ehx_trie {
   uint64_t m = 0, nh = 0;
   uint32_t *head = nullptr, *off = nullptr, *vals = nullptr;
   uint8_t *layout = nullptr;
+  // learned tries: the selector's layout per set, used only to stage a set as
+  // the staged operand of the tuned level (bitmap when EHX_RANGE_BITSET);
+  // `layout` keeps the density rule for every other use (null otherwise)
+  uint8_t *stage = nullptr;
   uint32_t *bs_off = nullptr, *bs_start = nullptr, *rank = nullptr;
   uint64_t *words = nullptr;
   int32_t *dense = nullptr;
@@ -118,6 +122,7 @@ constexpr uint32_t PK_MIN = 16;
 struct DTrie {
   const uint32_t *head, *off, *vals;
   const uint8_t *layout;
+  const uint8_t *stage;  // learned tries: the selector's staging layout (else null)
   const uint32_t *bs_off, *bs_start, *rank;
   const uint64_t *words;
   const int32_t *dense;
@@ -130,7 +135,7 @@ struct DTrie {
 
 inline DTrie view(const ehx_trie *t) {
   DTrie d;
-  d.head = t->head; d.off = t->off; d.vals = t->vals; d.layout = t->layout;
+  d.head = t->head; d.off = t->off; d.vals = t->vals; d.layout = t->layout; d.stage = t->stage;
   d.bs_off = t->bs_off; d.bs_start = t->bs_start; d.rank = t->rank; d.words = t->words;
   d.dense = t->dense; d.dense_len = t->dense_len; d.nh = (uint32_t)t->nh;
   d.pk_off = t->pk_off; d.pk = t->pk;

@@ -59,8 +59,12 @@ __global__ void k_split(const uint64_t *__restrict__ keys, const uint32_t *__res
 }
 
 // Layout and bitset word count per set; set nh is the head set.
+// With the learned policy the density rule stays in `layout` and the
+// selector's choice goes to `stage` (the staging layout of the tuned level);
+// a set gets bitset words when either says range_bitset.
 __global__ void k_layouts(DTrie t, int policy, SelectorWeights sel, uint8_t *__restrict__ layout,
-                          uint32_t *__restrict__ bs_start, uint32_t *__restrict__ wcount) {
+                          uint8_t *__restrict__ stage, uint32_t *__restrict__ bs_start,
+                          uint32_t *__restrict__ wcount) {
   const uint32_t nh = t.nh;
   for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h <= nh; h += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t *v;
@@ -69,7 +73,7 @@ __global__ void k_layouts(DTrie t, int policy, SelectorWeights sel, uint8_t *__r
     else { v = t.vals + t.off[h]; n = t.off[h + 1] - t.off[h]; }
     uint32_t first = n ? v[0] : 0, last = n ? v[n - 1] : 0;
     uint64_t span = (n > 1) ? (uint64_t)(last - first) : 0;
-    int lay;
+    int lay, sl;
     if (n == 0) lay = EHX_UINTEGER;
     else if (policy == EHX_POLICY_ALL_UINT) lay = EHX_UINTEGER;
     else if (policy == EHX_POLICY_ALL_BITSET) lay = EHX_RANGE_BITSET;
@@ -77,12 +81,13 @@ __global__ void k_layouts(DTrie t, int policy, SelectorWeights sel, uint8_t *__r
       // common::is_sparse (common.hpp:56-67): sparse iff len <= 2 or span/len > 128;
       // for 32-bit values the double test equals span > 128*len.
       lay = (n <= 2 || span > 128ull * n) ? EHX_UINTEGER : EHX_RANGE_BITSET;
-      if (policy == EHX_POLICY_LEARNED && sel.valid && h < nh)
-        lay = selector_layout(sel, n, span, first, last, set_work(t, v, n), lay);
     }
+    sl = lay;
+    if (stage && n && h < nh) sl = selector_layout(sel, n, span, first, last, set_work(t, v, n), lay);
     layout[h] = (uint8_t)lay;
+    if (stage) stage[h] = (uint8_t)sl;
     bs_start[h] = first >> 6;
-    wcount[h] = (lay == EHX_RANGE_BITSET && n) ? ((last >> 6) - (first >> 6) + 1) : 0;
+    wcount[h] = ((lay == EHX_RANGE_BITSET || sl == EHX_RANGE_BITSET) && n) ? ((last >> 6) - (first >> 6) + 1) : 0;
   }
 }
 
@@ -176,21 +181,24 @@ __global__ void k_dense(const uint32_t *__restrict__ head, uint64_t nh, int32_t 
 
 // Per-set statistics for ehx_trie_info.
 __global__ void k_stats(DTrie t, unsigned long long *__restrict__ acc) {
-  unsigned long long nb = 0, el = 0;
+  unsigned long long nb = 0, el = 0, ns = 0;
   uint32_t mx = 0;
   for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < t.nh; h += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t n = t.off[h + 1] - t.off[h];
     mx = max(mx, n);
     if (t.layout[h] == EHX_RANGE_BITSET) { nb++; el += n; }
+    if ((t.stage ? t.stage[h] : t.layout[h]) == EHX_RANGE_BITSET) ns++;
   }
   nb = wave_sum(nb);
   el = wave_sum(el);
+  ns = wave_sum(ns);
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, 64));
   if (__lane_id() == 0) {
     atomicAdd(&acc[0], nb);
     atomicAdd(&acc[1], el);
     atomicMax(&acc[2], (unsigned long long)mx);
+    atomicAdd(&acc[3], ns);
   }
 }
 
@@ -274,7 +282,8 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   size_t o_lay = o_vals + ((m * 4 + 255) & ~255ull);
   size_t o_bsoff = o_lay + (((nh + 1) + 255) & ~255ull);
   size_t o_bsst = o_bsoff + (((nh + 2) * 4 + 255) & ~255ull);
-  size_t total = o_bsst + (((nh + 1) * 4 + 255) & ~255ull);
+  size_t o_stage = o_bsst + (((nh + 1) * 4 + 255) & ~255ull);
+  size_t total = o_stage + (((nh + 1) + 255) & ~255ull);
   hipError_t e = hipMalloc(&t->block, total);
   if (e != hipSuccess) {
     delete t;
@@ -287,6 +296,7 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   t->layout = B + o_lay;
   t->bs_off = (uint32_t *)(B + o_bsoff);
   t->bs_start = (uint32_t *)(B + o_bsst);
+  t->stage = nullptr;  // set below for a learned build with selector weights loaded
   k_split<<<grid_for(m, 256, 16384), 256, 0, st>>>(tmp, flags, pos, m, b1, t->head, t->off, t->vals, nh);
   e = hipGetLastError();
   if (e != hipSuccess) { free_trie_arrays(t); delete t; return fail(EHX_ERR_DEVICE, "split: %s", hipGetErrorString(e)); }
@@ -296,9 +306,10 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   DevBuf &wc = ctx->scratch[S_COUNTS];
   if ((rc = wc.ensure((nh + 2) * 4))) { free_trie_arrays(t); delete t; return rc; }
   SelectorWeights sel = selector_current();
+  if (policy == EHX_POLICY_LEARNED && sel.valid) t->stage = B + o_stage;
   DTrie dv = view(t);
   dv.layout = t->layout;
-  k_layouts<<<grid_for(nh + 1, 256, 16384), 256, 0, st>>>(dv, policy, sel, t->layout, t->bs_start,
+  k_layouts<<<grid_for(nh + 1, 256, 16384), 256, 0, st>>>(dv, policy, sel, t->layout, t->stage, t->bs_start,
                                                           wc.as<uint32_t>());
   if ((rc = scan_u32(ctx, wc.as<uint32_t>(), t->bs_off, nh + 1))) { free_trie_arrays(t); delete t; return rc; }
   uint32_t W = 0;
@@ -384,6 +395,7 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   I.dense_nav = dense ? 1u : 0u;
   I.max_value = max0 > max1 ? max0 : max1;
   I.build_ms = ms;
+  I.num_staged_bitmaps = acc[3];
   *out = t;
   return EHX_OK;
 }

@@ -90,6 +90,9 @@ void tuned_bind(const TunedPlan &P, const Tries &T, TunedArgs &A) {
     A.ts[i] = T.t[P.s_rel[i]];
     A.sp[i] = A.prefix[P.s_parent[i]];
   }
+  // a single staged set takes its staging layout from the learned selector
+  // when its trie has one; every other use keeps the density rule's layout
+  if (P.ns == 1 && A.ts[0].stage) A.ts[0].layout = A.ts[0].stage;
   A.tx = T.t[P.x_rel];
   if (P.nd) {
     A.td = T.t[P.d_rel];

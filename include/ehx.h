@@ -90,6 +90,8 @@ typedef struct {
   uint32_t dense_nav;     /* 1 if child lookup is a dense array (NextLevel[data]) */
   uint32_t max_value;     /* largest value in either column */
   double build_ms;        /* device time of the build */
+  uint64_t num_staged_bitmaps; /* child sets the tuned level stages as a bitmap: the learned
+                                  selector's choice (EHX_POLICY_LEARNED), else num_bitsets */
 } ehx_trie_meta;
 
 typedef struct {

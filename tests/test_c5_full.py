@@ -54,12 +54,17 @@ def test_rmat22_triangle_every_form_and_layout(ctx, synthetic_meta, committed_tu
     expect = meta["triangles"]
     assert expect == 2110480353
     bitsets = {}
+    rule_bitsets = None
     for pname, policy in (("rule", E.POLICY_RULE), ("learned", E.POLICY_LEARNED)):
         T = E.Trie(ctx, s, d, policy)
         R = E.Trie(ctx, d, s, policy)
-        bitsets[pname] = T.info()["num_bitsets"]
-        if policy == E.POLICY_LEARNED:  # the network, not the rule, chose these layouts
-            assert bitsets["learned"] != bitsets["rule"], bitsets
+        info = T.info()
+        if policy == E.POLICY_RULE:
+            rule_bitsets = info["num_bitsets"]
+        bitsets[pname] = info["num_staged_bitmaps"]
+        if policy == E.POLICY_LEARNED:  # the network, not the rule, chose the staging layouts;
+            assert bitsets["learned"] != bitsets["rule"], bitsets  # the rule's stay for the rest
+            assert info["num_bitsets"] == rule_bitsets
         for plan_name, plan, tries in (("triangle", plans.TRIANGLE, [T]),
                                        ("triangle_by_middle", plans.TRIANGLE_BY_MIDDLE, [T, R])):
             TU.apply_plan(plan_name)
