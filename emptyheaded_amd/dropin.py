@@ -9,11 +9,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DROPIN_LIB = os.path.join(HERE, "engine", "libehx_dropin.so")
 QUERIES = ("TriangleAgg", "FliqueAgg", "Triangle", "TriangleCounts", "PageRankSteps", "WeightedDegree",
-           "FliqueSelAgg", "SSSP", "SSSPIter2", "LollipopAgg", "BarbellAgg", "Lollipop", "Barbell", "TriangleProj")
+           "FliqueSelAgg", "SSSP", "SSSPIter2", "LollipopAgg", "BarbellAgg", "Lollipop", "Barbell", "TriangleProj", "lubm1", "lubm6")
 # Materializing queries: output columns and whether rows carry an annotation.
 MATERIALIZED = {"Triangle": (3, False), "TriangleCounts": (1, True), "PageRankSteps": (1, True),
                 "WeightedDegree": (1, True), "SSSP": (1, True), "SSSPIter2": (1, True),
-                "Lollipop": (4, False), "Barbell": (6, False), "TriangleProj": (2, False)}
+                "Lollipop": (4, False), "Barbell": (6, False), "TriangleProj": (2, False), "lubm1": (1, False),
+                "lubm6": (1, False)}
 
 _lib = None
 
@@ -54,6 +55,17 @@ def write_relation(db, name, src, dst, ordering="0_1"):
     with open(os.path.join(db, "encodings", "uint32_t", "ehx_encoding.bin"), "wb") as f:
         f.write(np.uint64(n).tobytes())
     return d
+
+
+def write_string_database(db, encoding, relations):
+    """A database of string-typed relations as the generated code reads it:
+    relations/<R>/<R>_0_1 and <R>_1_0 (key rows) for every relation, and the
+    type's encoding in the reference's layout under encodings/string/.
+    relations: {name: (key col0, key col1)}; encoding: rdf.StringEncoding."""
+    for name, (a, b) in relations.items():
+        write_relation(db, name, a, b, "0_1")
+        write_relation(db, name, b, a, "1_0")
+    encoding.to_binary(os.path.join(db, "encodings", "string"))
 
 
 def run(query, db):

@@ -23,6 +23,11 @@ relation's ordering, codegen/trie/Trie.scala:219-230):
   Lollipop(a,b,c,x)     = 28 rows, df.iloc[27] = (5,3,4,4)      (travis.py:157-172)
   Barbell(a,b,c,x,y,z)  = 56 rows, df.iloc[55] = (5,3,4,3,4,5)  (travis.py:109-125,
                           checked there under check_big_out)
+the LUBM relations of test/rdf/travis.py:245-300 (ten string-typed relations
+from test/rdf/data/, dictionary-encoded into one string encoding) with the row
+counts its rules assert: lubm1 4, lubm2 0, lubm4 14, lubm6 5,916, lubm7 59,
+lubm8 5,916, lubm12 125 (travis.py:8-223), stored as the sorted dictionary
+and each relation's key columns (tests/golden/lubm.npz);
 and PageRank over facebook_duplicated with test/graph/data/inv_degree.tsv
 (4,039 float32 inverse degrees): the rank of the first vertex after the base
 case and 5 iterations is 15.227079960463206, checked to 0.0001
@@ -66,6 +71,25 @@ def main(ref_root):
     assert inv.shape == (4039, 2)
     np.savez_compressed(os.path.join(HERE, "inv_degree.npz"), key=inv[:, 0].astype(np.uint32),
                         val=inv[:, 1].astype(np.float32))
+    # LUBM: one string encoding over the ten relations
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from emptyheaded_amd import rdf
+    rdir = os.path.join(ref_root, "test", "rdf", "data")
+    rels = {n: rdf.read_tsv_strings(os.path.join(rdir, f)) for n, f in rdf.LUBM_RELATIONS}
+    enc, coded = rdf.encode_database(rels)
+    np.savez_compressed(os.path.join(HERE, "lubm.npz"),
+                        strings=np.frombuffer("\n".join(enc.key_to_value).encode("utf-8"), dtype=np.uint8),
+                        **{n: np.stack(coded[n], 1).astype(np.uint32) for n, _ in rdf.LUBM_RELATIONS})
+    lubm_meta = {
+        "source": "test/rdf/data/*.tsv (relations of test/rdf/travis.py:245-300)",
+        "sha256": {f: hashlib.sha256(open(os.path.join(rdir, f), "rb").read()).hexdigest()
+                   for _, f in rdf.LUBM_RELATIONS},
+        "rows": {n: len(rels[n][0]) for n, _ in rdf.LUBM_RELATIONS},
+        "distinct_strings": enc.num_distinct,
+        "rows_expected": {"lubm1": 4, "lubm2": 0, "lubm4": 14, "lubm6": 5916, "lubm7": 59, "lubm8": 5916,
+                          "lubm12": 125},
+        "cite": "test/rdf/travis.py:8-223",
+    }
     pairs = {tuple(r) for r in dup.tolist()}
     dup_is_sym = all((y, x) in pairs for (x, y) in pairs)
     meta = {
@@ -105,6 +129,7 @@ def main(ref_root):
             "PageRank_tolerance": 0.0001,
             "cite": "test/graph/travis.py:585-598,659-667",
         },
+        "lubm": lubm_meta,
         "simple": {
             "source": "test/graph/data/simple.tsv",
             "sha256": hashlib.sha256(rawt).hexdigest(),
