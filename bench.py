@@ -640,7 +640,8 @@ def main():
                     help="time only the headline form (for profiling its kernel alone)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="also run one copy kernel of this many bytes (traffic-counter reference)")
-    ap.add_argument("--config", default=None, choices=["pagerank", "sssp", "two_bag", "selections"],
+    ap.add_argument("--config", default=None,
+                    choices=["pagerank", "sssp", "two_bag", "selections", "four_clique", "egonets"],
                     help="run only this secondary workload and print its JSON (for profiling it alone)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="do not run the two rocprofv3 counter passes that measure the headline kernel's HBM bytes")
@@ -670,11 +671,25 @@ def main():
     from emptyheaded_amd import graphs as G
     from emptyheaded_amd import plans
 
+    def copy_calibration(ctx):
+        import ctypes as C
+        cms = C.c_double(0)
+        E.check(E.lib().ehx_copy_calibrate(ctx.h, args.copy_calib, C.byref(cms)))
+        log("copy kernel: %d bytes in %.3f ms (%.1f GB/s read+write)" %
+            (args.copy_calib, cms.value, 2 * args.copy_calib / cms.value / 1e6))
+
     if args.config:
         ctx = E.Context(0)
+        steps = max(2, args.steps // 2)
+        if args.config in ("four_clique", "egonets"):
+            load_tuning(E, args.selector)
         fn = {"pagerank": bench_pagerank, "sssp": bench_sssp, "two_bag": bench_two_bag,
-              "selections": bench_selections}[args.config]
-        print(json.dumps(fn(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu)))
+              "selections": bench_selections,
+              "four_clique": lambda E, ctx, steps, cpu: bench_four_clique(E, ctx, steps),
+              "egonets": lambda E, ctx, steps, cpu: bench_egonets(E, ctx, steps)}[args.config]
+        print(json.dumps(fn(E, ctx, steps, cpu=not args.no_cpu)))
+        if args.copy_calib:
+            copy_calibration(ctx)
         ctx.close()
         return
 
@@ -714,11 +729,7 @@ def main():
          build_wall_ms))
 
     if args.copy_calib:
-        import ctypes as C
-        cms = C.c_double(0)
-        E.check(E.lib().ehx_copy_calibrate(ctx.h, args.copy_calib, C.byref(cms)))
-        log("copy kernel: %d bytes in %.3f ms (%.1f GB/s read+write)" %
-            (args.copy_calib, cms.value, 2 * args.copy_calib / cms.value / 1e6))
+        copy_calibration(ctx)
 
     MID, REF = "triangle_by_middle", "triangle"
     PLAN = {MID: (plans.TRIANGLE_BY_MIDDLE, [T_learn, R_learn], [T_rule, R_rule]),

@@ -31,46 +31,77 @@ struct OutFrontier {
 
 enum { MODE_COUNT = 0, MODE_WRITE = 1, MODE_SUM = 2 };
 
+// One wavefront per tuple.  The tuple and its operand sets are wave-uniform and
+// indexed at run time (by the plan's parent and operand numbers), so they live
+// in the wave's LDS rather than in per-lane private arrays, which the compiler
+// would place in scratch memory and re-read on every step.  Lane i < nops
+// resolves operand i (its child lookup runs beside the others).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Frontier in, int k, uint64_t n,
                                                      uint32_t *__restrict__ counts, const uint64_t *__restrict__ offs,
                                                      OutFrontier outf, unsigned long long *__restrict__ acc,
                                                      SegOut seg, int pick) {
   __shared__ unsigned long long s_acc[6];
+  __shared__ DSet s_sets[4][EHX_MAX_OPS];
+  __shared__ uint32_t s_tuple[4][EHX_MAX_ATTRS];
   if (MODE == MODE_SUM) {
     if (threadIdx.x < 6) s_acc[threadIdx.x] = 0;
     __syncthreads();
   }
+  const uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
+  DSet *S = s_sets[wv];
+  uint32_t *tuple = s_tuple[wv];
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  unsigned long long sum = 0, algc[5] = {0, 0, 0, 0, 0};
+  unsigned long long sum = 0, n_merge = 0, n_gallop = 0, n_probe = 0, n_and = 0, n_empty = 0;
   for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nwaves) {
-    uint32_t tuple[EHX_MAX_ATTRS];
-    for (int j = 0; j < k; j++) tuple[j] = in.col[j][t];
-    DSet S[EHX_MAX_OPS];
+    if (lane < (uint32_t)k) tuple[lane] = in.col[lane][t];
+    wave_sync();
+    bool ok = true;
+    if (lane < (uint32_t)L.nops) {
+      const DTrie &tr = T.t[L.rel[lane]];
+      const int par = L.parent[lane];
+      if (par == EHX_PARENT_HEAD) {
+        S[lane] = set_of(tr, tr.nh);
+      } else {
+        const int32_t h = node_of(tr, par >= 0 ? tuple[par] : L.value[lane]);  // else EHX_PARENT_CONST
+        if (h < 0) ok = false;  // NULL TrieBlock: empty (TrieBuilder.cpp:249-254)
+        else S[lane] = set_of(tr, (uint32_t)h);
+      }
+    }
+    const bool resolved = __ballot(!ok) == 0;
+    wave_sync();
     uint32_t c = 0;
     int alg = ALG_EMPTY;
-    if (resolve(T, L, tuple, S)) {
+    if (resolved) {
       if (MODE == MODE_WRITE) {
         uint64_t o = offs[t];
         c = wave_intersect<true>(S, L.nops, outf.col[k] + o, &alg, pick != 0);
-        for (int j = 0; j < k; j++)
-          for (uint32_t i = __lane_id(); i < c; i += 64) outf.col[j][o + i] = tuple[j];
+        for (int j = 0; j < k; j++) {
+          const uint32_t v = tuple[j];
+          for (uint32_t i = lane; i < c; i += 64) outf.col[j][o + i] = v;
+        }
       } else {
         c = wave_intersect<false>(S, L.nops, nullptr, &alg, pick != 0);
       }
     }
-    if (MODE == MODE_COUNT && __lane_id() == 0) counts[t] = c;
+    if (MODE == MODE_COUNT && lane == 0) counts[t] = c;
     if (MODE == MODE_SUM) {
       sum += c;
-      algc[alg]++;
-      if (seg.out && __lane_id() == 0) seg_add(seg, tuple[0], c);
+      n_merge += alg == ALG_MERGE;
+      n_gallop += alg == ALG_GALLOP;
+      n_probe += alg == ALG_PROBE;
+      n_and += alg == ALG_AND;
+      n_empty += alg == ALG_EMPTY;
+      if (seg.out && lane == 0) seg_add(seg, tuple[0], c);
     }
+    wave_sync();  // every lane has read this tuple's sets before the next overwrites them
   }
   if (MODE == MODE_SUM) {
-    if (__lane_id() == 0) {
-      atomicAdd(&s_acc[0], sum);
-      for (int a = 0; a < 5; a++)
-        if (algc[a]) atomicAdd(&s_acc[1 + a], algc[a]);
+    if (lane == 0) {
+      const unsigned long long per[6] = {sum, n_merge, n_gallop, n_probe, n_and, n_empty};
+#pragma unroll
+      for (int a = 0; a < 6; a++)
+        if (per[a]) atomicAdd(&s_acc[a], per[a]);
     }
     __syncthreads();
     if (threadIdx.x < 6 && s_acc[threadIdx.x]) atomicAdd(&acc[threadIdx.x], s_acc[threadIdx.x]);
@@ -627,16 +658,22 @@ struct NodeOps {
   int32_t nsets;
 };
 
+// One wavefront per call; lane s < nsets resolves operand s into the wave's LDS
+// (runtime-indexed per-lane arrays would live in scratch memory).
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_nodes(NodeOps O, uint64_t n, unsigned long long *__restrict__ counts,
                                                const unsigned long long *__restrict__ offs, uint32_t *__restrict__ out) {
+  __shared__ DSet s_sets[4][EHX_MAX_OPS];
+  const uint32_t lane = __lane_id();
+  DSet *S = s_sets[threadIdx.x >> 6];
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nwaves) {
-    DSet S[EHX_MAX_OPS];
-    for (int s = 0; s < O.nsets; s++) S[s] = set_of(O.t[s], O.nodes[s][t]);
+    if (lane < (uint32_t)O.nsets) S[lane] = set_of(O.t[lane], O.nodes[lane][t]);
+    wave_sync();
     int alg;
     uint32_t c = wave_intersect<EMIT>(S, O.nsets, EMIT ? out + offs[t] : nullptr, &alg);
-    if (!EMIT && __lane_id() == 0) counts[t] = c;
+    if (!EMIT && lane == 0) counts[t] = c;
+    wave_sync();  // every lane has read these sets before the next call overwrites them
   }
 }
 
