@@ -89,9 +89,13 @@ struct ehx_trie {
   // target trie u, arr[e] = u's nav entry of vals[e] (the reference's tries keep
   // a child pointer beside every set element; here the child may live in a
   // different relation's trie)
+  // hint[e]: how many values of that child set are below the head value of
+  // the set holding element e of this trie (the parent of e); null when the
+  // device had no room
   struct Link {
     uint64_t target;
     uint4 *arr;
+    uint32_t *hint;
   };
   std::vector<Link> links;
   uint64_t uid = 0;  // unique per process, never reused
@@ -228,7 +232,8 @@ int reduce_max_u32(ehx_ctx *ctx, const uint32_t *in, uint64_t n, uint32_t *host_
 int fill_u32(ehx_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
 // src's child pointers into dst (built and cached on src on first use); null in
 // *out when dst has no child lookup by value
-int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out);
+int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out,
+              const uint32_t **hint_out = nullptr);
 int fill_u64(ehx_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
 int copy_d2h(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int copy_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);

@@ -394,7 +394,8 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       for (int j = 0; j < EHX_MAX_ATTRS; j++) ta.prefix[j] = in.col[j];
       tuned_bind(tp, T, ta);
       if (tp.ns == 1) {  // child pointers from E(p)'s trie into x's
-        if ((rc = trie_link(ctx, tries[tp.nd ? tp.d_rel : tp.s_rel[0]], tries[tp.x_rel], &ta.xlink))) return rc;
+        if ((rc = trie_link(ctx, tries[tp.nd ? tp.d_rel : tp.s_rel[0]], tries[tp.x_rel], &ta.xlink, &ta.xhint)))
+          return rc;
       }
       ta.nprefix = n;
       ta.force_mode = -1;

@@ -174,6 +174,30 @@ __global__ void k_link(const uint32_t *__restrict__ vals, uint64_t m, DTrie dst,
   }
 }
 
+// hint[e] = number of values of element e's child set in dst (arr[e]) below
+// e's parent value in src: a binary search per element, a wavefront per src set.
+// any[0] = 1 when some hint is nonzero.
+__global__ void k_link_hint(DTrie src, const uint32_t *__restrict__ dvals, const uint4 *__restrict__ arr,
+                            uint32_t *__restrict__ hint, unsigned *__restrict__ any) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  bool some = false;
+  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < src.nh; h += nwaves) {
+    const uint32_t y = src.head[h], a1 = src.off[h + 1];
+    for (uint32_t e = src.off[h] + __lane_id(); e < a1; e += 64) {
+      const uint4 c = arr[e];
+      uint32_t lo = 0, hi = c.y;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (dvals[c.x + mid] < y) lo = mid + 1;
+        else hi = mid;
+      }
+      hint[e] = lo;
+      some |= lo != 0;
+    }
+  }
+  if (__any(some) && __lane_id() == 0) atomicOr(any, 1u);
+}
+
 __global__ void k_dense(const uint32_t *__restrict__ head, uint64_t nh, int32_t *__restrict__ dense) {
   for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += (uint64_t)gridDim.x * blockDim.x)
     dense[head[h]] = (int32_t)h;
@@ -400,13 +424,15 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   return EHX_OK;
 }
 
-int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out) {
+int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out, const uint32_t **hint_out) {
   *out = nullptr;
+  if (hint_out) *hint_out = nullptr;
   if (!dst->nav || !src->m) return EHX_OK;
   std::lock_guard<std::mutex> lk(g_link_mu);
   for (const auto &l : src->links)
     if (l.target == dst->uid) {
       *out = l.arr;
+      if (hint_out) *hint_out = l.hint;
       return EHX_OK;
     }
   uint4 *arr = nullptr;
@@ -417,16 +443,52 @@ int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **ou
   }
   k_link<<<grid_for(src->m, 256, 16384), 256, 0, ctx->stream>>>(src->vals, src->m, view(dst), arr);
   EHX_HIP(hipGetLastError());
+  // hints, kept only when some child set starts below its element's parent
+  // value (for a relation linked to itself in ascending orientation none does)
+  uint32_t *hint = nullptr;
+  DevBuf &fb = ctx->scratch[S_FLAGS];
+  int rc = fb.ensure(64);
+  if (rc) {
+    (void)hipFree(arr);
+    return rc;
+  }
+  if (hipMalloc((void **)&hint, src->m * 4) == hipSuccess) {
+    if ((rc = fill_u32(ctx, fb.as<uint32_t>(), 0u, 1))) {
+      (void)hipFree(arr);
+      (void)hipFree(hint);
+      return rc;
+    }
+    k_link_hint<<<grid_for(src->nh * 64, 256, 16384), 256, 0, ctx->stream>>>(view(src), dst->vals, arr, hint,
+                                                                            fb.as<unsigned>());
+    EHX_HIP(hipGetLastError());
+    unsigned any = 0;
+    if ((rc = copy_d2h(ctx, &any, fb.p, 4))) {
+      (void)hipFree(arr);
+      (void)hipFree(hint);
+      return rc;
+    }
+    if (!any) {
+      EHX_HIP(hipFree(hint));
+      hint = nullptr;
+    }
+  } else {
+    (void)hipGetLastError();
+    hint = nullptr;  // no room: child sets are streamed whole
+  }
   EHX_HIP(hipStreamSynchronize(ctx->stream));  // other contexts may use it next
-  src->links.push_back({dst->uid, arr});
+  src->links.push_back({dst->uid, arr, hint});
   *out = arr;
+  if (hint_out) *hint_out = hint;
   return EHX_OK;
 }
 
 int trie_destroy(ehx_trie *t) {
   if (!t) return EHX_OK;
   EHX_HIP(hipSetDevice(t->device));
-  for (const auto &l : t->links) EHX_HIP(hipFree(l.arr));
+  for (const auto &l : t->links) {
+    EHX_HIP(hipFree(l.arr));
+    if (l.hint) EHX_HIP(hipFree(l.hint));
+  }
   if (t->block) EHX_HIP(hipFree(t->block));
   if (t->words) EHX_HIP(hipFree(t->words));
   if (t->pk_off) EHX_HIP(hipFree(t->pk_off));

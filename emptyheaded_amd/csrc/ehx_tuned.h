@@ -57,6 +57,11 @@ struct TunedArgs {
   // with desc: xlink[e] = x's child list {off, length, first chunk, chunks} in
   // the x trie for the value at position e of E(p)'s trie; null: looked up by value
   const uint4 *xlink;
+  // with xlink: xhint[e] = values of that child list below the parent value
+  // of E(p) (the head of the set holding e); a child list whose values start
+  // below S(p)'s first value is streamed from there when S(p)'s first value is
+  // at least that parent value (null: streamed whole)
+  const uint32_t *xhint;
 };
 
 // Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.

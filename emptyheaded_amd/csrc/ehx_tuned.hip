@@ -335,11 +335,14 @@ template <bool BLOCK>
 __device__ __forceinline__ uint32_t x_width() { return BLOCK ? 16u : 64u; }
 
 // The lane's x at position base + lane of E(p), and its child list; xl: E(p)'s
-// child pointers (A.xlink at E(p)'s offset), or null.
+// child pointers (A.xlink at E(p)'s offset), or null; xh: their hints
+// (A.xhint at E(p)'s offset) when the values below the parent value cannot be
+// in S(p), or null.  A trimmed list starts at its first value not below the
+// parent value (at that value's chunk when streamed gap-coded).
 template <bool BLOCK, int MAXS>
 __device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, const DSet *S, const uint32_t *xsrc,
                                         uint32_t xn, uint32_t base, bool filter_s, int mi, int big_deg, int gap,
-                                        const uint4 *xl) {
+                                        const uint4 *xl, const uint32_t *xh) {
   const DTrie &tx = A.tx;
   const uint32_t lane = __lane_id(), idx = base + lane;
   bool ok = lane < x_width<BLOCK>() && idx < xn;
@@ -347,12 +350,17 @@ __device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, 
   if (xl && !filter_s && A.nh == 0) {  // one 16-byte load, the value itself not needed
     if (ok) {
       const uint4 nv = xl[idx];
+      uint32_t cut = xh ? xh[idx] : 0u;
       r.o = nv.x;
       r.d = nv.y;
-      if (gap && r.d >= (uint32_t)big_deg) {
-        r.ko = nv.z;
-        r.kn = nv.w;
+      if (gap && r.d >= (uint32_t)big_deg && nv.w) {
+        const uint32_t k = cut / PK_VALS;  // whole chunks below the parent value
+        r.ko = nv.z + k;
+        r.kn = nv.w - k;
+        cut = k * PK_VALS;
       }
+      r.o += cut;
+      r.d -= cut;
     }
     return r;
   }
@@ -396,7 +404,7 @@ template <int MODE, bool BLOCK, int MAXS, int DEEP>
 __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const TunedArgs &A, const Stage &st,
                                                     const DSet *S, const uint32_t *xsrc, uint32_t xn, bool filter_s,
                                                     int mi, uint32_t *own, int big_deg, uint32_t slg, int gap,
-                                                    const XList first, const uint4 *xl) {
+                                                    const XList first, const uint4 *xl, const uint32_t *xh) {
   const DTrie &tx = A.tx;
   const uint32_t lane = __lane_id();
   const uint32_t wave = BLOCK ? (threadIdx.x >> 6) : 0u;
@@ -408,7 +416,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
   constexpr bool UNROLL = MODE != 2;
   for (uint32_t base = wave * XW; base < xn; base += nwaves * XW) {
     XList xi = first;
-    if (base != wave * XW) xi = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, filter_s, mi, big_deg, gap, xl);
+    if (base != wave * XW) xi = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, filter_s, mi, big_deg, gap, xl, xh);
     const uint32_t o = xi.o, d = xi.d, ko = xi.ko, kn = xi.kn;
     // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
     // streamed with 128-bit loads, two quads per lane in flight per step (the
@@ -573,10 +581,17 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   const uint32_t wave_x0 = wave * x_width<BLOCK>();
   const bool x_in_d = P.nd || use_desc;  // E(p) is Dd
   const uint4 *xl = (use_desc && A.xlink) ? A.xlink + dsc0.z : nullptr;
+  const uint32_t *xh = nullptr;
+  if (xl && A.xhint) {
+    // E(p)'s parent value y (its set is child(y)); child values below y are
+    // below S(p)'s first value when that is at least y
+    const uint32_t y = __builtin_amdgcn_readfirstlane((P.nd ? A.dp : A.sp[0])[p]);
+    if (dsc1.x >= y) xh = A.xhint + dsc0.z;
+  }
   if (MAXS == 1) {
     const uint32_t *xs = x_in_d ? Dd.v : src;
     const uint32_t xn1 = x_in_d ? Dd.n : sn;
-    first = x_list<BLOCK, MAXS>(P, A, S, xs, xn1, wave_x0, false, 0, big_deg, gap, xl);
+    first = x_list<BLOCK, MAXS>(P, A, S, xs, xn1, wave_x0, false, 0, big_deg, gap, xl, xh);
   }
   // staging decision (group-uniform)
   Stage st;
@@ -646,18 +661,18 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   const uint32_t *xsrc = x_in_d ? Dd.v : (src ? src : Smi.v);
   const uint32_t xn = x_in_d ? Dd.n : (src ? sn : Smi.n);
   const bool filter_s = !P.nd && !src;  // x runs over one S set: test it against the others
-  if (MAXS > 1) first = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, wave_x0, filter_s, mi, big_deg, gap, nullptr);
+  if (MAXS > 1) first = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, wave_x0, filter_s, mi, big_deg, gap, nullptr, nullptr);
   uint64_t hits;
   // one streaming loop per staging mode: no per-probe branch on the mode
   if (st.mode == 0)
     hits = stream_children<0, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
-                                                       first, xl);
+                                                       first, xl, xh);
   else if (st.mode == 1)
     hits = stream_children<1, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
-                                                       first, xl);
+                                                       first, xl, xh);
   else
     hits = stream_children<2, BLOCK, MAXS, DEEP>(P, A, st, S, xsrc, xn, filter_s, mi, own, big_deg, slg, gap,
-                                                       first, xl);
+                                                       first, xl, xh);
   gsync();
   return hits;
 }
@@ -713,13 +728,15 @@ __device__ __forceinline__ void tiny_owners(uint32_t *own, uint32_t r0, const ui
   out[3] = max(v.w, carry) - 1u;
 }
 
-// x's child list {off, length}: from E(p)'s child pointers when there are any,
-// else by value.
+// x's child list {off, length}: from E(p)'s child pointers when there are any
+// (past the values below E(p)'s parent value when `trim`, as in x_list), else
+// by value.
 __device__ __forceinline__ uint2 tiny_child(const TunedPlan &P, const TunedArgs &A, const uint32_t *xvals,
-                                            uint32_t pos) {
+                                            uint32_t pos, bool trim) {
   if (A.xlink && A.nh == 0) {
     const uint4 nv = A.xlink[pos];
-    return make_uint2(nv.x, nv.y);
+    const uint32_t cut = trim ? A.xhint[pos] : 0u;
+    return make_uint2(nv.x + cut, nv.y - cut);
   }
   const uint32_t x = xvals[pos];
   if (!head_ok(P, A, x)) return make_uint2(0, 0);
@@ -764,10 +781,14 @@ __global__ __launch_bounds__(256) void k_tiny(TunedPlan P, TunedArgs A, const ui
     const uint32_t q0 = (uint32_t)q064;
     const uint32_t nb = min(TINY_BATCH, qlen - q0);
     uint4 d0 = make_uint4(0, 0, 0, 0);
-    uint32_t p = 0;
+    uint32_t p = 0, trim = 0;
     if (lane < nb) {
       d0 = A.desc[2ull * (q0 + lane)];  // {S offset, |S|, E offset, |E|}
       p = queue[q0 + lane];
+      // child lists skip their values below E(p)'s parent value when S(p)'s
+      // first value is not below it
+      if (A.xhint && A.xlink && A.nh == 0 && d0.y)
+        trim = A.desc[2ull * (q0 + lane) + 1].x >= (P.nd ? A.dp : A.sp[0])[p] ? 1u : 0u;
     }
     uint32_t sz = 0;  // this prefix's table: a power of two >= 4/3 |S| (load <= 3/4)
     if (lane < nb) {
@@ -787,7 +808,7 @@ __global__ __launch_bounds__(256) void k_tiny(TunedPlan P, TunedArgs A, const ui
       mSo[lane] = d0.x;
       mSs[lane] = sincl - sn;
       mTb[lane] = tincl - mine;
-      mMk[lane] = sz - 1;
+      mMk[lane] = (sz - 1) | (trim << 31);  // bit 31: the prefix's child lists are trimmed
       mEo[lane] = d0.z;
       mXs[lane] = xincl - xn;
       mHit[lane] = 0;
@@ -807,7 +828,7 @@ __global__ __launch_bounds__(256) void k_tiny(TunedPlan P, TunedArgs A, const ui
         for (int u = 0; u < 4; u++) {
           const uint32_t e = r0 + 4 * lane + u;
           if (e >= stot) continue;
-          const uint32_t base = mTb[k[u]], mask = mMk[k[u]];
+          const uint32_t base = mTb[k[u]], mask = mMk[k[u]] & 0x7FFFFFFFu;
           uint32_t h = tiny_hash(c[u]) & mask;
           while (atomicCAS(&tab[base + h], EMPTY, c[u]) != EMPTY) h = (h + 1) & mask;
         }
@@ -824,7 +845,7 @@ __global__ __launch_bounds__(256) void k_tiny(TunedPlan P, TunedArgs A, const ui
           const uint32_t t = x0 + 4 * lane + u;
           o[u] = dl[u] = 0;
           if (t < xtot) {
-            const uint2 cl = tiny_child(P, A, xvals, mEo[k[u]] + t - mXs[k[u]]);
+            const uint2 cl = tiny_child(P, A, xvals, mEo[k[u]] + t - mXs[k[u]], (mMk[k[u]] >> 31) != 0u);
             o[u] = cl.x;
             dl[u] = cl.y;
           }
@@ -858,7 +879,7 @@ __global__ __launch_bounds__(256) void k_tiny(TunedPlan P, TunedArgs A, const ui
           for (int u = 0; u < 4; u++) {
             const uint32_t e = e0 + 4 * lane + u;
             if (e >= ctot) continue;
-            const uint32_t kk = cK[r[u]], base = mTb[kk], mask = mMk[kk];
+            const uint32_t kk = cK[r[u]], base = mTb[kk], mask = mMk[kk] & 0x7FFFFFFFu;
             uint32_t h = tiny_hash(c[u]) & mask, hit = 0;
             for (;;) {  // EMPTY first: 0xFFFFFFFF is never staged
               const uint32_t v = tab[base + h];

@@ -288,6 +288,42 @@ def test_triangle_by_middle_vertex(ctx, oracle, scale):
                 assert st["form_used"] == form  # the driver-set shape runs in the tuned kernel
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_trimmed_child_lists_any_orientation(ctx, oracle, seed):
+    """The tuned level streams a child list of x from its first value not below
+    E(p)'s parent value (link hints) only when S(p)'s first value is not below
+    that parent value.  On directed graphs with edges both ways, prefixes of
+    both kinds occur; counts equal the oracle's with the hints used by the
+    persistent kernel and by the tiny bin, streamed plain and gap-coded."""
+    rng = np.random.default_rng(seed)
+    n, m = 3000, 60000
+    hubs = rng.integers(0, 40, m // 3)  # a few long child lists (gap-coded, streamed by lane groups)
+    src = np.concatenate([rng.integers(0, n, m), hubs]).astype(np.uint32)
+    dst = np.concatenate([rng.integers(0, n, m), rng.integers(0, n, m // 3)]).astype(np.uint32)
+    keep = src != dst
+    e = np.unique(np.stack([src[keep], dst[keep]], 1), axis=0)
+    s, d = e[:, 0].copy(), e[:, 1].copy()
+    exp = oracle.join_count(plans.TRIANGLE_BY_MIDDLE, [oracle.Trie(s, d), oracle.Trie(d, s)], 8)
+    assert exp == oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
+    saved = E.tune_get()
+    try:
+        for policy in (E.POLICY_RULE, E.POLICY_ALL_UINT):
+            T01, T10 = E.Trie(ctx, s, d, policy), E.Trie(ctx, d, s, policy)
+            assert T10.link_to(T01)
+            for knobs in ({"tiny_work": 0, "gap_coded": 1, "big_deg": 16}, {"tiny_work": 4096, "gap_coded": 1},
+                          {"tiny_work": 0, "gap_coded": 0, "big_deg": 8}, {"tiny_work": 256, "gap_coded": 1,
+                                                                           "stream_lanes": 4, "big_work": 64}):
+                E.tune_set(**knobs)
+                for form in (E.FORM_TUNED, E.FORM_LEARNED):
+                    c, st = E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T01, T10], form, stats=True)
+                    assert st["form_used"] == form
+                    assert c == exp, (policy, knobs, form)
+            T01.close()
+            T10.close()
+    finally:
+        E.tune_set(**saved)
+
+
 # ---- equality selections ------------------------------------------------------
 
 @pytest.mark.parametrize("form", FORMS)
