@@ -23,7 +23,7 @@ GRID = {
     "stream_lanes": [2, 4, 8, 16, 32, 64],
     "gap_coded": [0, 1],
     "split_work": [0, 1 << 19, 1 << 20, 1 << 21, 1 << 22],
-    "tiny_work": [0, 256, 1024, 4096, 16384],
+    "tiny_work": [0, 64, 128, 256, 1024, 4096, 16384],
     "bitmap_all": [0, 1],
     "use_graph": [0, 1],
 }
