@@ -131,6 +131,9 @@ SIGNATURES = {
     "ehx_selector_load": ([C.c_int, C.POINTER(C.c_uint16), C.c_uint64, C.POINTER(C.c_float), C.c_float],
                           C.c_int),
     "ehx_selector_clear": ([], C.c_int),
+    "ehx_pair_record": ([_vp, _vp, _vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint64, C.c_int,
+                         C.POINTER(C.c_uint64)], C.c_int),
+    "ehx_pair_table_load": ([C.c_int, C.POINTER(C.c_uint8)], C.c_int),
     "ehx_trie_set_features": ([_vp, _vp, C.POINTER(C.c_float)], C.c_int),
     "ehx_copy_calibrate": ([_vp, C.c_uint64, C.POINTER(C.c_double)], C.c_int),
     "ehx_host_alloc": ([C.c_uint64, C.POINTER(_vp)], C.c_int),
@@ -398,8 +401,9 @@ def join_count_segments(ctx, levels, tries, bounds, form=FORM_PLAIN, stats=False
     return (counts, res.value, st.as_dict()) if stats else (counts, res.value)
 
 
-def join_materialize(ctx, levels, tries, form=FORM_PLAIN):
-    """Every result tuple in lexicographic order: an (rows, num_attrs) uint32 array."""
+def join_materialize(ctx, levels, tries, form=FORM_PLAIN, stats=False):
+    """Every result tuple in lexicographic order: an (rows, num_attrs) uint32
+    array (and the second pass's stats when `stats`)."""
     p = make_plan(levels) if not isinstance(levels, Plan) else levels
     arr = (_vp * len(tries))(*[t.h.value for t in tries])
     rows = C.c_uint64(0)
@@ -411,8 +415,10 @@ def join_materialize(ctx, levels, tries, form=FORM_PLAIN):
     out = np.zeros((p.num_attrs, max(n, 1)), dtype=np.uint32)
     for j in range(p.num_attrs):
         cols[j] = ptr_u32(out[j])
-    check(lib().ehx_join_materialize(ctx.h, C.byref(p), arr, form, cols, n, C.byref(rows), None))
-    return out[:, :n].T.copy()
+    st = JoinStats()
+    check(lib().ehx_join_materialize(ctx.h, C.byref(p), arr, form, cols, n, C.byref(rows), C.byref(st)))
+    res = out[:, :n].T.copy()
+    return (res, st.as_dict()) if stats else res
 
 
 def cycle4_count(ctx, sym, begin=None, end=None, bounds=None, stats=False):

@@ -67,6 +67,8 @@ void tuned_forget_graphs();
 int selector_load(int device, const uint16_t *params, const float *norm, float max_words);
 int selector_clear();
 int copy_calibrate(ehx_ctx *ctx, uint64_t bytes, double *ms);
+int pair_record(ehx_ctx *ctx, const ehx_trie *a, const ehx_trie *b, const uint32_t *nodes0, const uint32_t *nodes1,
+                uint64_t n, int reps, uint64_t *cycles);
 
 }  // namespace ehx
 
@@ -360,7 +362,18 @@ int ehx_selector_load(int device, const uint16_t *params, uint64_t nparams, cons
   return selector_load(device, params, norm, max_words);
 }
 
-int ehx_selector_clear(void) { return selector_clear(); }
+int ehx_selector_clear(void) {
+  int rc = selector_clear();
+  int rc2 = pair_table_load(0, nullptr);
+  return rc ? rc : rc2;
+}
+
+int ehx_pair_record(ehx_ctx *ctx, const ehx_trie *a, const ehx_trie *b, const uint32_t *nodes0, const uint32_t *nodes1,
+                    uint64_t n, int reps, uint64_t *cycles) {
+  return pair_record(ctx, a, b, nodes0, nodes1, n, reps, cycles);
+}
+
+int ehx_pair_table_load(int device, const uint8_t *table) { return pair_table_load(device, table); }
 
 int ehx_copy_calibrate(ehx_ctx *ctx, uint64_t bytes, double *ms) {
   if (!ctx || bytes < 16) return fail(EHX_ERR_INVALID, "bad copy size");

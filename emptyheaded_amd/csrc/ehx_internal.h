@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/ehx.h"
+#include "../../include/ehx_ext.h"
 
 #define EHX_MAX_RELS 8
 
@@ -234,6 +235,9 @@ int fill_u32(ehx_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
 // *out when dst has no child lookup by value
 int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out,
               const uint32_t **hint_out = nullptr);
+// The loaded per-pair table on the current device (null: none).
+const uint8_t *pair_table_current();
+int pair_table_load(int device, const uint8_t *table);
 int fill_u64(ehx_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
 int copy_d2h(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int copy_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);

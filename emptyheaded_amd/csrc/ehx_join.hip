@@ -40,7 +40,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Frontier in, int k, uint64_t n,
                                                      uint32_t *__restrict__ counts, const uint64_t *__restrict__ offs,
                                                      OutFrontier outf, unsigned long long *__restrict__ acc,
-                                                     SegOut seg, int pick) {
+                                                     SegOut seg, int pick, const uint8_t *__restrict__ ptab) {
   __shared__ unsigned long long s_acc[6];
   __shared__ DSet s_sets[4][EHX_MAX_OPS];
   __shared__ uint32_t s_tuple[4][EHX_MAX_ATTRS];
@@ -75,13 +75,13 @@ __global__ __launch_bounds__(256) void k_level_plain(Tries T, LevelDesc L, Front
     if (resolved) {
       if (MODE == MODE_WRITE) {
         uint64_t o = offs[t];
-        c = wave_intersect<true>(S, L.nops, outf.col[k] + o, &alg, pick != 0);
+        c = wave_intersect<true>(S, L.nops, outf.col[k] + o, &alg, pick != 0, ptab);
         for (int j = 0; j < k; j++) {
           const uint32_t v = tuple[j];
           for (uint32_t i = lane; i < c; i += 64) outf.col[j][o + i] = v;
         }
       } else {
-        c = wave_intersect<false>(S, L.nops, nullptr, &alg, pick != 0);
+        c = wave_intersect<false>(S, L.nops, nullptr, &alg, pick != 0, ptab);
       }
     }
     if (MODE == MODE_COUNT && lane == 0) counts[t] = c;
@@ -286,7 +286,7 @@ static int level0(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, const
 // stays lexicographic, the order foreach_aggregate visits).  The k+1 output
 // columns live in scratch slot S_F0 + k.
 static int expand_level(ehx_ctx *ctx, const Tries &T, const LevelDesc &L, Frontier &in, int k, uint64_t n, int pick,
-                        uint64_t *nn_out) {
+                        const uint8_t *ptab, uint64_t *nn_out) {
   hipStream_t st = ctx->stream;
   int rc;
   DevBuf &cb = ctx->scratch[S_COUNTS], &ob = ctx->scratch[S_OFFS];
@@ -296,7 +296,7 @@ static int expand_level(ehx_ctx *ctx, const Tries &T, const LevelDesc &L, Fronti
   memset(&none, 0, sizeof(none));
   if (n) {
     k_level_plain<MODE_COUNT><<<G, 256, 0, st>>>(T, L, in, k, n, cb.as<uint32_t>(), nullptr, none, nullptr,
-                                                 SegOut{}, pick);
+                                                 SegOut{}, pick, ptab);
     EHX_HIP(hipGetLastError());
   }
   if ((rc = scan_u32_to_u64(ctx, cb.as<uint32_t>(), ob.as<uint64_t>(), n))) return rc;
@@ -312,7 +312,7 @@ static int expand_level(ehx_ctx *ctx, const Tries &T, const LevelDesc &L, Fronti
   for (int j = 0; j <= k; j++) out.col[j] = fb.as<uint32_t>() + (uint64_t)j * nn;
   if (n && nn) {
     k_level_plain<MODE_WRITE><<<G, 256, 0, st>>>(T, L, in, k, n, nullptr, ob.as<uint64_t>(), out, nullptr, SegOut{},
-                                                 pick);
+                                                 pick, ptab);
     EHX_HIP(hipGetLastError());
   }
   for (int j = 0; j <= k; j++) in.col[j] = out.col[j];
@@ -338,6 +338,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
   }
   const int K = p->num_attrs;
   const int pick = form != EHX_FORM_PLAIN;  // per-pair algorithm by cost in the tuned and learned forms
+  const uint8_t *ptab = form == EHX_FORM_LEARNED ? pair_table_current() : nullptr;  // learned uint x uint choice
   EHX_HIP(hipEventRecord(ctx->ev[0], st));
 
   // ---- level 0 -------------------------------------------------------------
@@ -380,7 +381,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
     for (int k = 1; k <= last_mat; k++) {
       LevelDesc L = level_desc(p, k);
       uint64_t nn = 0;
-      if ((rc = expand_level(ctx, T, L, in, k, n, pick, &nn))) return rc;
+      if ((rc = expand_level(ctx, T, L, in, k, n, pick, ptab, &nn))) return rc;
       n = nn;
       S.frontier[k] = nn;
       EHX_HIP(hipEventRecord(ctx->ev[1 + k], st));
@@ -424,7 +425,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
         OutFrontier none;
         memset(&none, 0, sizeof(none));
         k_level_plain<MODE_SUM><<<G, 256, 0, st>>>(T, L, in, K - 1, n, nullptr, nullptr, none,
-                                                   ac.as<unsigned long long>(), seg ? *seg : SegOut{}, pick);
+                                                   ac.as<unsigned long long>(), seg ? *seg : SegOut{}, pick, ptab);
         EHX_HIP(hipGetLastError());
       }
       EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));
@@ -484,9 +485,10 @@ int join_materialize(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, in
   Frontier in;
   memset(&in, 0, sizeof(in));
   in.col[0] = f0;
+  const uint8_t *ptab = form == EHX_FORM_LEARNED ? pair_table_current() : nullptr;
   for (int k = 1; k < K; k++) {
     uint64_t nn = 0;
-    if ((rc = expand_level(ctx, T, level_desc(p, k), in, k, n, form != EHX_FORM_PLAIN, &nn))) return rc;
+    if ((rc = expand_level(ctx, T, level_desc(p, k), in, k, n, form != EHX_FORM_PLAIN, ptab, &nn))) return rc;
     n = nn;
     S.frontier[k] = nn;
   }
@@ -500,7 +502,7 @@ int join_materialize(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, in
   }
   float ms = 0;
   if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) S.total_ms = ms;
-  S.form_used = EHX_FORM_PLAIN;
+  S.form_used = form;  // the per-pair choices differ by form (cost rule, learned table)
   if (stats) *stats = S;
   return EHX_OK;
 }
@@ -678,6 +680,42 @@ __global__ __launch_bounds__(256) void k_nodes(NodeOps O, uint64_t n, unsigned l
   }
 }
 
+// Engine-recorded timings of the two uint x uint algorithms: for pair t, one
+// wavefront counts A ∩ B by merge path, then by galloping the smaller set into
+// the larger, `reps` times, and keeps the fewest shader-clock cycles of each
+// (cyc[2t], cyc[2t + 1]); both counts must agree (bad[0] = 1 otherwise).  A
+// pair that is not uint x uint, or has an empty set, records zeros.
+__global__ __launch_bounds__(256) void k_pair_record(NodeOps O, uint64_t n, int reps,
+                                                     unsigned long long *__restrict__ cyc, unsigned *__restrict__ bad) {
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nwaves) {
+    const DSet A = set_of(O.t[0], O.nodes[0][t]), B = set_of(O.t[1], O.nodes[1][t]);
+    unsigned long long best0 = ~0ull, best1 = ~0ull;
+    if (A.lay != EHX_UINTEGER || B.lay != EHX_UINTEGER || A.n == 0 || B.n == 0) {
+      best0 = best1 = 0;
+    } else {
+      const DSet &r = (A.n > B.n) ? B : A;
+      const DSet &f = (A.n > B.n) ? A : B;
+      for (int k = 0; k < reps; k++) {
+        const unsigned long long t0 = __builtin_amdgcn_s_memtime();
+        const uint32_t c0 = __builtin_amdgcn_readfirstlane(wave_merge<false>(r.v, r.n, f.v, f.n, nullptr));
+        asm volatile("" ::"s"(c0));  // the clock is read after the count is known
+        const unsigned long long t1 = __builtin_amdgcn_s_memtime();
+        const uint32_t c1 = __builtin_amdgcn_readfirstlane(wave_gallop<false>(r.v, r.n, f.v, f.n, nullptr));
+        asm volatile("" ::"s"(c1));
+        const unsigned long long t2 = __builtin_amdgcn_s_memtime();
+        best0 = min(best0, t1 - t0);
+        best1 = min(best1, t2 - t1);
+        if (c0 != c1 && __lane_id() == 0) atomicOr(bad, 1u);
+      }
+    }
+    if (__lane_id() == 0) {
+      cyc[2 * t] = best0;
+      cyc[2 * t + 1] = best1;
+    }
+  }
+}
+
 // Operand node lists checked on the device before any set is read: bad[0] =
 // 1 when some node of operand s exceeds lim[s] (node == num_heads names the
 // head set).
@@ -726,6 +764,39 @@ __global__ __launch_bounds__(256) void k_nodes_big(NodeOps O, uint64_t n, unsign
     const uint32_t c = wave_intersect<false>(S, 2, nullptr, &alg);
     if (__lane_id() == 0) counts[t] = c;
   }
+}
+
+int pair_record(ehx_ctx *ctx, const ehx_trie *a, const ehx_trie *b, const uint32_t *nodes0, const uint32_t *nodes1,
+                uint64_t n, int reps, uint64_t *cycles) {
+  if (!ctx || !a || !b || !nodes0 || !nodes1 || !cycles) return fail(EHX_ERR_INVALID, "null argument");
+  if (reps < 1 || reps > 16) return fail(EHX_ERR_INVALID, "reps %d (1..16)", reps);
+  if (a->device != ctx->device || b->device != ctx->device) return fail(EHX_ERR_INVALID, "trie lives on another device");
+  for (uint64_t i = 0; i < n; i++)
+    if (nodes0[i] > a->nh || nodes1[i] > b->nh)
+      return fail(EHX_ERR_INVALID, "pair %llu: node out of range", (unsigned long long)i);
+  if (n == 0) return EHX_OK;
+  EHX_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  NodeOps O;
+  memset(&O, 0, sizeof(O));
+  O.nsets = 2;
+  DevBuf &nb = ctx->scratch[S_IN0], &c = ctx->scratch[S_COUNTS], &fl = ctx->scratch[S_FLAGS];
+  int rc;
+  if ((rc = nb.ensure(n * 8)) || (rc = c.ensure(n * 16)) || (rc = fl.ensure(64))) return rc;
+  O.t[0] = view(a);
+  O.t[1] = view(b);
+  O.nodes[0] = nb.as<uint32_t>();
+  O.nodes[1] = nb.as<uint32_t>() + n;
+  if ((rc = copy_h2d(ctx, (void *)O.nodes[0], nodes0, n * 4)) || (rc = copy_h2d(ctx, (void *)O.nodes[1], nodes1, n * 4)) ||
+      (rc = fill_u32(ctx, fl.as<uint32_t>(), 0u, 1)))
+    return rc;
+  k_pair_record<<<grid_for(n * 64, 256, 65536), 256, 0, st>>>(O, n, reps, c.as<unsigned long long>(),
+                                                                fl.as<unsigned>());
+  EHX_HIP(hipGetLastError());
+  uint32_t bad = 0;
+  if ((rc = copy_d2h(ctx, &bad, fl.p, 4))) return rc;
+  if (bad) return fail(EHX_ERR_DEVICE, "merge and gallop counts disagree");
+  return copy_d2h(ctx, cycles, c.p, n * 16);
 }
 
 int intersect_nodes(ehx_ctx *ctx, int nsets, ehx_trie *const *tries, const uint32_t *const *nodes, uint64_t n,

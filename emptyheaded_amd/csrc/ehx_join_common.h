@@ -140,13 +140,30 @@ __device__ uint32_t wave_nway(const DSet *S, int n, uint32_t *out) {
   return cnt;
 }
 
+// Learned uint x uint choice: ptab[s * PAIR_RATIO_BINS + r] for a smaller set
+// of 2^s .. 2^(s+1) - 1 values and a larger/smaller ratio of 2^r .. 2^(r+1) - 1
+// (bins past the last are clamped into it): 0 merge, 1 gallop, 2 the
+// reference's ratio rule.
+constexpr uint32_t PAIR_SMALL_BINS = EHX_PAIR_SMALL_BINS, PAIR_RATIO_BINS = EHX_PAIR_RATIO_BINS;
+
+__device__ __forceinline__ bool pair_gallop(const uint8_t *ptab, uint32_t rn, uint32_t fn) {
+  const bool rule = (uint64_t)fn > 32ull * rn;  // uinteger.hpp:1209-1227
+  if (!ptab) return rule;
+  const uint32_t sb = min(31u - (uint32_t)__builtin_clz(rn), PAIR_SMALL_BINS - 1);
+  const uint32_t rb = min(31u - (uint32_t)__builtin_clz(fn / rn), PAIR_RATIO_BINS - 1);
+  const uint32_t c = ptab[sb * PAIR_RATIO_BINS + rb];
+  return c < 2u ? c == 1u : rule;
+}
+
 // Candidate set of one tuple: the intersection of n resolved sets.  With
 // `pick`, a bitset pair whose smaller set holds fewer values than the pair's
 // overlapping words is probed value by value instead of AND-ed word by word
 // (same result, less traffic for sparse windows); without it the algorithms are
-// exactly the reference's per-pair choice.
+// exactly the reference's per-pair choice.  ptab (learned form, when a pair
+// table is loaded): the uint x uint choice of merge or gallop (pair_gallop).
 template <bool EMIT>
-__device__ __forceinline__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg, bool pick = false) {
+__device__ __forceinline__ uint32_t wave_intersect(const DSet *S, int n, uint32_t *out, int *alg, bool pick = false,
+                                                   const uint8_t *ptab = nullptr) {
   for (int i = 0; i < n; i++)
     if (S[i].n == 0) { *alg = ALG_EMPTY; return 0; }
   if (n == 1) {
@@ -173,7 +190,7 @@ __device__ __forceinline__ uint32_t wave_intersect(const DSet *S, int n, uint32_
     if (B.lay == EHX_RANGE_BITSET) { *alg = ALG_PROBE; return wave_probe<EMIT>(A, B, out); }
     const DSet &r = (A.n > B.n) ? B : A;
     const DSet &f = (A.n > B.n) ? A : B;
-    if ((uint64_t)f.n > 32ull * r.n) { *alg = ALG_GALLOP; return wave_gallop<EMIT>(r.v, r.n, f.v, f.n, out); }
+    if (pair_gallop(ptab, r.n, f.n)) { *alg = ALG_GALLOP; return wave_gallop<EMIT>(r.v, r.n, f.v, f.n, out); }
     *alg = ALG_MERGE;
     return wave_merge<EMIT>(r.v, r.n, f.v, f.n, out);
   }

@@ -48,6 +48,38 @@ int selector_load(int device, const uint16_t *params, const float *norm, float m
   return EHX_OK;
 }
 
+// The learned per-pair table (uint x uint: merge or gallop), one device.
+static uint8_t *g_pair = nullptr;
+static int g_pair_device = -1;
+
+const uint8_t *pair_table_current() {
+  int dev = -1;
+  if (!g_pair || hipGetDevice(&dev) != hipSuccess || dev != g_pair_device) return nullptr;
+  return g_pair;
+}
+
+int pair_table_load(int device, const uint8_t *table) {
+  if (g_pair) {
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(g_pair_device);
+    (void)hipFree(g_pair);
+    if (cur >= 0) (void)hipSetDevice(cur);
+    g_pair = nullptr;
+    g_pair_device = -1;
+  }
+  if (!table) return EHX_OK;
+  for (int i = 0; i < EHX_PAIR_SMALL_BINS * EHX_PAIR_RATIO_BINS; i++)
+    if (table[i] > 2) return fail(EHX_ERR_INVALID, "pair table entry %d is %d (0 merge, 1 gallop, 2 rule)", i, table[i]);
+  EHX_HIP(hipSetDevice(device));
+  uint8_t *m = nullptr;
+  EHX_HIP(hipMalloc((void **)&m, EHX_PAIR_SMALL_BINS * EHX_PAIR_RATIO_BINS));
+  EHX_HIP(hipMemcpy(m, table, EHX_PAIR_SMALL_BINS * EHX_PAIR_RATIO_BINS, hipMemcpyHostToDevice));
+  g_pair = m;
+  g_pair_device = device;
+  return EHX_OK;
+}
+
 }  // namespace ehx
 
 extern "C" int ehx_selector_num_params(void) { return ehx::SEL_NPARAM; }

@@ -61,4 +61,7 @@ def apply_committed(device=0, selector_path=None):
         from . import selector
         applied["selector"] = selector.load_committed(device, sp)
         _selector_active = bool(applied["selector"])
+    from . import pairsel
+    if os.path.exists(pairsel.PAIR_JSON):
+        applied["pair_selector"] = pairsel.load_committed(device)
     return applied

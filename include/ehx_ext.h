@@ -52,6 +52,28 @@ int ehx_trie_set_features(ehx_ctx *ctx, const ehx_trie *t, float *out);
 int ehx_join_record(ehx_ctx *ctx, const ehx_plan *plan, ehx_trie *const *tries, int force_mode, uint64_t *cycles,
                     uint64_t cap, uint64_t *nprefix, int64_t *result);
 
+/* Engine-recorded timings of the two uint x uint algorithms, for the learned
+ * per-pair choice: pair i is child set nodes0[i] of trie a and child set
+ * nodes1[i] of trie b (a node equal to the trie's num_heads names its head
+ * set).  cycles[2i] receives the shader-clock cycles of counting their
+ * intersection by merge path, cycles[2i+1] by galloping the smaller set into
+ * the larger, each the fewest of `reps` runs (1..16); both are 0 when the pair
+ * is not uint x uint or a set is empty.  Fails when the two counts differ. */
+int ehx_pair_record(ehx_ctx *ctx, const ehx_trie *a, const ehx_trie *b, const uint32_t *nodes0, const uint32_t *nodes1,
+                    uint64_t n, int reps, uint64_t *cycles);
+
+/* Learned per-pair algorithm of the learned form's uint x uint intersections
+ * (the levels it does not run in the persistent kernel): entry
+ * s * EHX_PAIR_RATIO_BINS + r is the choice for a smaller set of 2^s to
+ * 2^(s+1) - 1 values and a larger/smaller ratio of 2^r to 2^(r+1) - 1 (the last
+ * bins take everything beyond): 0 merge path, 1 gallop, 2 the reference's rule
+ * (gallop when the ratio exceeds 32, uinteger.hpp:1209-1227).  The table is
+ * the selector network's decision, tabulated by its trainer; table NULL
+ * unloads it (ehx_selector_clear does too). */
+#define EHX_PAIR_SMALL_BINS 24
+#define EHX_PAIR_RATIO_BINS 24
+int ehx_pair_table_load(int device, const uint8_t *table);
+
 /* Training the selector on the GPU: data (n samples of 8 features, 2 targets,
  * 2 mask weights) lives on the device; ehx_mlp_grad returns the masked mean
  * squared error and its gradient with respect to the FP16-rounded weights
