@@ -428,6 +428,33 @@ def bench_four_clique(E, ctx, steps, scale=18):
     return out
 
 
+def bench_listing(E, ctx, steps, scale=18):
+    """Triangle listing (materialized rows) on R-MAT scale 18 in uint layout:
+    one uint x uint intersection per edge at the last level, picked by the
+    reference's ratio rule (tuned form) or the learned pair table (learned)."""
+    from emptyheaded_amd import graphs as G
+    from emptyheaded_amd import plans
+    s, d, n = G.rmat_graph(scale)
+    T = E.Trie(ctx, s, d, E.POLICY_ALL_UINT)
+    out = {"workload": "Triangle(a,b,c) rows on R-MAT scale %d, edge factor 16, pruned src<dst, uint layout" % scale}
+    rows = set()
+    for name, form in (("rule", E.FORM_TUNED), ("learned", E.FORM_LEARNED)):
+        E.join_materialize(ctx, plans.TRIANGLE, [T], form)
+        ts = []
+        for _ in range(steps):
+            R, st = E.join_materialize(ctx, plans.TRIANGLE, [T], form, stats=True)
+            ts.append(st["total_ms"])
+            rows.add(len(R))
+        out[name + "_ms"] = sorted(ts)[len(ts) // 2]
+    assert len(rows) == 1, "pair choices changed the rows"
+    out["rows"] = rows.pop()
+    out["speedup_learned_vs_rule"] = out["rule_ms"] / out["learned_ms"]
+    log("triangle listing R-MAT-%d: rule %.3f ms, learned pair table %.3f ms, %d rows" %
+        (scale, out["rule_ms"], out["learned_ms"], out["rows"]))
+    T.close()
+    return out
+
+
 def _selector_loaded(E):
     from emptyheaded_amd import tuning
     return tuning.selector_active()
@@ -834,6 +861,7 @@ def main():
     if not args.no_configs and not args.headline_only:
         out["configs"] = {"small_graphs": bench_small_graphs(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
                           "rmat18_four_clique": bench_four_clique(E, ctx, max(2, args.steps // 2)),
+                          "rmat18_listing": bench_listing(E, ctx, max(3, args.steps // 2)),
                           "egonet_batch": bench_egonets(E, ctx, max(2, args.steps // 2)),
                           "triple_star_path": bench_triples(E, ctx, max(2, args.steps // 2)),
                           "selections": bench_selections(E, ctx, max(2, args.steps // 2), cpu=not args.no_cpu),
