@@ -186,6 +186,8 @@ __global__ void k_link_hint(DTrie src, const uint32_t *__restrict__ dvals, const
     for (uint32_t e = src.off[h] + __lane_id(); e < a1; e += 64) {
       const uint4 c = arr[e];
       uint32_t lo = 0, hi = c.y;
+      if (hi && dvals[c.x] >= y) hi = 0;  // the whole set lies at or above y (the common case of
+                                          // a relation linked to itself in ascending orientation)
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         if (dvals[c.x + mid] < y) lo = mid + 1;
