@@ -296,9 +296,31 @@ __device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int n
 // Probes the values of one gap-coded chunk (a, b): the first value, then 14
 // values reached by the 16-bit gaps; only values below `lim` count (the last
 // chunk of a list is zero-padded).
+// Bitmap probe of a value held relative to the bitmap's first value
+// (r = c - 32 * bm_base): one subtraction fewer per probe than probe_m<0>.
+// Bit c & 31 is bit r & 31, since the base is a multiple of 32; a value below
+// the window wraps to a large r and, like one above it, reads the zero word.
+__device__ __forceinline__ uint32_t probe_rebased(const Stage &s, uint32_t r) {
+  const uint32_t w = min(r >> 5, s.bm_words);
+  return __builtin_amdgcn_ubfe(s.tab[w], r, 1u);
+}
+
 template <int MODE, int MAXS, bool TAIL>
 __device__ __forceinline__ uint32_t probe_chunk(const Stage &st, const DSet *S, int ns, const uint4 a, const uint4 b,
                                                 uint32_t lim) {
+  if (MODE == 0) {  // the running sum of the gaps is kept relative to the bitmap
+    uint32_t r = a.x - (st.bm_base << 5);
+    uint32_t h = probe_rebased(st, r);
+    const uint32_t w[7] = {a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+      r += w[j] & 0xFFFFu;
+      if (!TAIL || 2u * j + 1u < lim) h += probe_rebased(st, r);
+      r += w[j] >> 16;
+      if (!TAIL || 2u * j + 2u < lim) h += probe_rebased(st, r);
+    }
+    return h;
+  }
   uint32_t v = a.x;
   uint32_t h = probe_m<MODE, MAXS>(st, S, ns, v);
   if (MODE != 2) {
