@@ -64,6 +64,7 @@ int intersect_pairs(ehx_ctx *, const ehx_trie *, const uint32_t *, const ehx_tri
 int intersect_nodes(ehx_ctx *, int, ehx_trie *const *, const uint32_t *const *, uint64_t, uint64_t *, uint64_t *,
                     uint32_t *, uint64_t, uint64_t *);
 void tuned_forget_graphs();
+void query_graph_forget();
 int selector_load(int device, const uint16_t *params, const float *norm, float max_words);
 int selector_clear();
 int copy_calibrate(ehx_ctx *ctx, uint64_t bytes, double *ms);
@@ -115,6 +116,7 @@ int ehx_ctx_destroy(ehx_ctx *ctx) {
   EHX_HIP(hipSetDevice(ctx->device));
   EHX_HIP(hipStreamSynchronize(ctx->stream));
   tuned_forget_graphs();
+  query_graph_forget();
   for (auto &b : ctx->scratch) b.release();
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);

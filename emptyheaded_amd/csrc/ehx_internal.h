@@ -46,6 +46,8 @@ struct DevBuf {
   void release();
   template <class T> T *as() const { return reinterpret_cast<T *>(p); }
 };
+// Changes whenever any context buffer is freed or reallocated.
+uint64_t devbuf_generation();
 
 }  // namespace ehx
 
@@ -224,6 +226,9 @@ template <class T> __device__ __forceinline__ T wave_sum(T v) {
 // Exclusive scan of n counts into out[0..n]; out[n] = total.  `level` picks the
 // scratch slots used for recursion.
 int scan_u32_to_u64(ehx_ctx *ctx, const uint32_t *in, uint64_t *out, uint64_t n);
+// Sizes the scans' block-sum buffers for n elements (a later scan of at most n
+// allocates nothing).
+int scan_reserve(ehx_ctx *ctx, uint64_t n);
 int scan_u64(ehx_ctx *ctx, const uint64_t *in, uint64_t *out, uint64_t n);
 int scan_u32(ehx_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
 // Stable LSD radix sort of n 64-bit keys on their low `bits` bits; keys[] holds

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <vector>
 
 #include "ehx_join_common.h"
 #include "ehx_tuned.h"
@@ -149,6 +150,8 @@ __global__ __launch_bounds__(256) void k_level_card(Tries T, LevelDesc L, Fronti
   }
 }
 
+__global__ void k_copy_u64(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst) { *dst = *src; }
+
 // Level 0: flags over the driver's set dv[0, dn), membership in the other
 // operands' sets (a head set, or the child set of a selection constant).
 __global__ void k_level0_flags(Tries T, LevelDesc L, int driver, const uint32_t *__restrict__ dv, uint64_t dn,
@@ -228,8 +231,12 @@ __global__ void k_seg_level0(const uint32_t *__restrict__ f0, uint64_t n, SegOut
 // (TrieBuilder.cpp:478-578).  One relation: its head set; several: the driver
 // (a selection's child set if there is one, else the smallest head) filtered by
 // membership in the others, then compacted.
+// n0_dev (may be null): when level 0 is a filtered head set, leave its count on
+// the device instead (*n0_dev points at it) and return its upper bound in
+// *n0_out, with no host synchronization.
 static int level0(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, const Tries &T, const uint32_t **f0_out,
-                  uint64_t *n0_out) {
+                  uint64_t *n0_out, const uint64_t **n0_dev = nullptr) {
+  if (n0_dev) *n0_dev = nullptr;
   hipStream_t st = ctx->stream;
   int rc;
   LevelDesc L0 = level_desc(p, 0);
@@ -277,8 +284,138 @@ static int level0(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, const
   k_compact_u32<<<grid_for(dn, 256, 16384), 256, 0, st>>>(dv, fl.as<uint32_t>(), ps.as<uint64_t>(), dn,
                                                           fb.as<uint32_t>());
   EHX_HIP(hipGetLastError());
-  if ((rc = copy_d2h(ctx, n0_out, ps.as<uint64_t>() + dn, 8))) return rc;
   *f0_out = fb.as<uint32_t>();
+  if (n0_dev) {
+    *n0_dev = ps.as<uint64_t>() + dn;
+    *n0_out = dn;
+    return EHX_OK;
+  }
+  return copy_d2h(ctx, n0_out, ps.as<uint64_t>() + dn, 8);
+}
+
+// ---- a whole count query in one graph -----------------------------------------
+// A plan whose level 0 intersects head sets and whose remaining levels the
+// tuned kernel runs (three attributes) has no host synchronization left once
+// level 0's count stays on the device: its launches are captured once and the
+// graph is replayed.  The key holds every value the launches bake in; context
+// buffer addresses are covered by the buffers' generation count.
+struct QueryGraph {
+  bool valid = false;
+  std::vector<unsigned char> key;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int64_t n0_bound = 0;
+};
+static QueryGraph g_qgraph;
+static std::mutex g_qgraph_mu;
+
+static void query_graph_drop() {
+  if (g_qgraph.exec) (void)hipGraphExecDestroy(g_qgraph.exec);
+  if (g_qgraph.graph) (void)hipGraphDestroy(g_qgraph.graph);
+  g_qgraph = QueryGraph();
+}
+
+void query_graph_forget() {
+  std::lock_guard<std::mutex> lk(g_qgraph_mu);
+  query_graph_drop();
+}
+
+template <class V>
+static void key_add(std::vector<unsigned char> &k, const V &v) {
+  const unsigned char *b = reinterpret_cast<const unsigned char *>(&v);
+  k.insert(k.end(), b, b + sizeof(V));
+}
+
+// Level 0 of p reads head sets only (no selection constant) and is not one
+// trie's head alone.
+static bool level0_filtered_heads(const ehx_plan *p, ehx_trie *const *tries) {
+  const LevelDesc L0 = level_desc(p, 0);
+  for (int i = 0; i < L0.nops; i++)
+    if (L0.parent[i] != EHX_PARENT_HEAD) return false;
+  for (int i = 1; i < L0.nops; i++)
+    if (L0.rel[i] != L0.rel[0]) return true;
+  return false;
+}
+
+static int query_graph_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int nrels, const Tries &T,
+                             const TunedPlan &tp, int form, DevBuf &ac, ehx_join_stats &S) {
+  hipStream_t st = ctx->stream;
+  int rc;
+  // everything that allocates or synchronizes happens before the capture
+  TunedArgs ta;
+  memset(&ta, 0, sizeof(ta));
+  tuned_bind(tp, T, ta);
+  if (tp.ns == 1 &&
+      (rc = trie_link(ctx, tries[tp.nd ? tp.d_rel : tp.s_rel[0]], tries[tp.x_rel], &ta.xlink, &ta.xhint)))
+    return rc;
+  const LevelDesc L0 = level_desc(p, 0);
+  int driver = 0;
+  for (int i = 1; i < L0.nops; i++)
+    if (tries[L0.rel[i]]->nh <= tries[L0.rel[driver]]->nh) driver = i;
+  const uint64_t dn = tries[L0.rel[driver]]->nh;
+  DevBuf &fl = ctx->scratch[S_FLAGS], &ps = ctx->scratch[S_POS], &fb = ctx->scratch[S_F0];
+  if ((rc = fl.ensure(dn * 4 + 4)) || (rc = ps.ensure(dn * 8 + 8)) || (rc = fb.ensure(dn * 4 + 4)) ||
+      (rc = scan_reserve(ctx, dn)) || (rc = tuned_reserve(ctx, tp, dn)))
+    return rc;
+  std::vector<unsigned char> key;
+  key_add(key, *p);
+  key_add(key, form);
+  key_add(key, ctx->device);
+  key_add(key, st);
+  key_add(key, ac.p);
+  key_add(key, tune_params());
+  key_add(key, pair_table_current());
+  key_add(key, devbuf_generation());
+  for (int r = 0; r < nrels; r++) {
+    key_add(key, tries[r]->uid);
+    key_add(key, T.t[r]);
+  }
+  key_add(key, ta.xlink);
+  key_add(key, ta.xhint);
+  std::lock_guard<std::mutex> lk(g_qgraph_mu);
+  if (!(g_qgraph.valid && g_qgraph.key == key)) {
+    query_graph_drop();
+    EHX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const uint32_t *f0 = nullptr;
+    uint64_t n0 = 0;
+    const uint64_t *n0_dev = nullptr;
+    rc = fill_u64(ctx, ac.as<uint64_t>(), 0, 8);
+    if (!rc) rc = level0(ctx, p, tries, T, &f0, &n0, &n0_dev);
+    if (!rc && (!n0_dev || n0 != dn)) rc = fail(EHX_ERR_DEVICE, "level 0 of a captured query");
+    if (!rc) {
+      for (int j = 0; j < EHX_MAX_ATTRS; j++) ta.prefix[j] = nullptr;
+      ta.prefix[0] = f0;
+      tuned_bind(tp, T, ta);
+      ta.nprefix = dn;
+      ta.nprefix_dev = n0_dev;
+      ta.force_mode = -1;
+      rc = tuned_run(ctx, tp, ta, form, ac.as<unsigned long long>(), false);
+    }
+    // the level-0 count beside the result, read back with it
+    if (!rc) {
+      k_copy_u64<<<1, 1, 0, st>>>(n0_dev, ac.as<uint64_t>() + 7);
+      if (hipGetLastError() != hipSuccess) rc = fail(EHX_ERR_DEVICE, "launch in a captured query");
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return rc ? rc : fail(EHX_ERR_DEVICE, "query capture: %s", hipGetErrorString(e));
+    }
+    const hipError_t ie = hipGraphInstantiate(&g_qgraph.exec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return fail(EHX_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    }
+    g_qgraph.graph = g;
+    g_qgraph.key = key;
+    g_qgraph.valid = true;
+  }
+  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
+  EHX_HIP(hipGraphLaunch(g_qgraph.exec, st));
+  EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K1], st));
+  S.frontier[0] = dn;  // replaced by the count read back with the result
   return EHX_OK;
 }
 
@@ -340,6 +477,29 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
   const int pick = form != EHX_FORM_PLAIN;  // per-pair algorithm by cost in the tuned and learned forms
   const uint8_t *ptab = form == EHX_FORM_LEARNED ? pair_table_current() : nullptr;  // learned uint x uint choice
   EHX_HIP(hipEventRecord(ctx->ev[0], st));
+
+  // ---- the whole query as one replayed graph ------------------------------------
+  TunedPlan qtp;
+  if (tune_params().use_graph && form != EHX_FORM_PLAIN && K == 3 && !rec && !seg && begin == 0 &&
+      end == ~0ull && tuned_applicable(p, &qtp) && level0_filtered_heads(p, tries)) {
+    DevBuf &ac = ctx->scratch[S_ACC];
+    if ((rc = ac.ensure(8 * 8))) return rc;
+    if ((rc = query_graph_count(ctx, p, tries, nrels, T, qtp, form, ac, S))) return rc;
+    unsigned long long accv[8];
+    if ((rc = copy_d2h(ctx, accv, ac.p, 64))) return rc;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev[EHX_EV_K0], ctx->ev[EHX_EV_K1]) == hipSuccess) {
+      S.main_kernel_ms = ms;
+      S.level_ms[K - 1] = ms;
+    }
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[EHX_EV_K1]) == hipSuccess) S.total_ms = ms;
+    S.frontier[0] = accv[7];
+    for (int a = 0; a < 5; a++) S.pairs_by_alg[a] = accv[1 + a];
+    S.form_used = form;
+    *result = (int64_t)accv[0];
+    if (stats) *stats = S;
+    return EHX_OK;
+  }
 
   // ---- level 0 -------------------------------------------------------------
   const uint32_t *f0;

@@ -9,6 +9,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <atomic>
+
 #include "ehx_internal.h"
 
 namespace ehx {
@@ -93,6 +95,17 @@ static int scan_impl(ehx_ctx *ctx, const Tin *in, Tout *out, uint64_t n, int lev
   if (rc) return rc;
   k_scan_apply<Tin, Tout><<<(unsigned)nb, SCAN_T, 0, ctx->stream>>>(in, out, sums, n);
   EHX_HIP(hipGetLastError());
+  return EHX_OK;
+}
+
+int scan_reserve(ehx_ctx *ctx, uint64_t n) {
+  for (int level = 0; level < 4; level++) {
+    const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb <= 1) return EHX_OK;
+    int rc = ctx->scratch[S_SCAN0 + level].ensure(8 * (nb + 1));
+    if (rc) return rc;
+    n = nb;
+  }
   return EHX_OK;
 }
 
@@ -277,8 +290,15 @@ int copy_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes) {
   return EHX_OK;
 }
 
+// Bumped whenever a context buffer is freed or moves: a captured graph that
+// names buffer addresses is replayed only while the count it was captured at
+// holds.
+static std::atomic<uint64_t> g_devbuf_gen{0};
+uint64_t devbuf_generation() { return g_devbuf_gen.load(); }
+
 int DevBuf::ensure(size_t bytes) {
   if (bytes <= cap) return EHX_OK;
+  g_devbuf_gen.fetch_add(1);
   if (p) {
     hipError_t e = hipFree(p);
     if (e != hipSuccess) return fail(EHX_ERR_DEVICE, "hipFree: %s", hipGetErrorString(e));
@@ -296,6 +316,7 @@ int DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
+  if (p) g_devbuf_gen.fetch_add(1);
   if (p) (void)hipFree(p);
   p = nullptr;
   cap = 0;

@@ -62,6 +62,10 @@ struct TunedArgs {
   // below S(p)'s first value is streamed from there when S(p)'s first value is
   // at least that parent value (null: streamed whole)
   const uint32_t *xhint;
+  // with a whole query captured in one graph: the prefix count on the device
+  // (level 0's output); nprefix is then its upper bound, which sizes the grids
+  // and buffers.  Null: nprefix is the count.
+  const uint64_t *nprefix_dev;
 };
 
 // Knobs picked by the autotuner (ehx_tune_set); defaults are the measured best.
@@ -90,7 +94,13 @@ struct RecordReq {
 bool tuned_applicable(const ehx_plan *p, TunedPlan *tp);
 // Fills A's operand slots from the plan, the tries and A.prefix.
 void tuned_bind(const TunedPlan &P, const Tries &T, TunedArgs &A);
-int tuned_run(ehx_ctx *ctx, const TunedPlan &tp, const TunedArgs &a, int form, unsigned long long *acc);
+// allow_graph = false: enqueue the level's launches on the stream as they are
+// (the caller may be capturing them itself).
+int tuned_run(ehx_ctx *ctx, const TunedPlan &tp, const TunedArgs &a, int form, unsigned long long *acc,
+              bool allow_graph = true);
+// Sizes the context buffers tuned_run uses for up to nprefix prefixes, so that a
+// later tuned_run of at most that many allocates nothing.
+int tuned_reserve(ehx_ctx *ctx, const TunedPlan &tp, uint64_t nprefix);
 TuneParams &tune_params();
 
 }  // namespace ehx

@@ -152,8 +152,8 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
   if (threadIdx.x < NBUCKETS) h[threadIdx.x] = 0;
   __syncthreads();
   const DTrie &tx = A.tx;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.nprefix;
-       p += (uint64_t)gridDim.x * blockDim.x) {
+  const uint64_t np = A.nprefix_dev ? *A.nprefix_dev : A.nprefix;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (uint64_t)gridDim.x * blockDim.x) {
     DSet S[TUNED_MAX_S];
     DSet Dd;
     uint32_t b = BUCKET_NONE, parts = 1;
@@ -224,10 +224,12 @@ __global__ void k_bucket_scan(unsigned *__restrict__ ctl) {
   if (lane == 63) { ctl[128] = incl; ctl[129] = 0; ctl[130] = 0; }
 }
 
-__global__ __launch_bounds__(256) void k_enqueue(const uint32_t *__restrict__ bucket, uint64_t n,
-                                                 unsigned *__restrict__ ctl, uint32_t *__restrict__ queue,
-                                                 const uint4 *__restrict__ pdesc, uint4 *__restrict__ qdesc) {
+__global__ __launch_bounds__(256) void k_enqueue(const uint32_t *__restrict__ bucket, uint64_t n_host,
+                                                 const uint64_t *__restrict__ n_dev, unsigned *__restrict__ ctl,
+                                                 uint32_t *__restrict__ queue, const uint4 *__restrict__ pdesc,
+                                                 uint4 *__restrict__ qdesc) {
   __shared__ unsigned cnt[NBUCKETS], base[NBUCKETS];
+  const uint64_t n = n_dev ? *n_dev : n_host;
   for (uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x; t0 < n; t0 += (uint64_t)gridDim.x * blockDim.x) {
     if (threadIdx.x < NBUCKETS) cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -1077,7 +1079,7 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
   EHX_HIP(hipGetLastError());
   k_bucket_scan<<<1, 64, 0, st>>>(ctl);
   EHX_HIP(hipGetLastError());
-  k_enqueue<<<grid_for(A.nprefix, 256, 8192), 256, 0, st>>>(bucket, A.nprefix, ctl, queue, pdesc,
+  k_enqueue<<<grid_for(A.nprefix, 256, 8192), 256, 0, st>>>(bucket, A.nprefix, A.nprefix_dev, ctl, queue, pdesc,
                                                             const_cast<uint4 *>(A.desc));
   EHX_HIP(hipGetLastError());
   const unsigned per_cu = (unsigned)(tp.waves_per_cu > 4 ? tp.waves_per_cu / 4 : 1);
@@ -1102,24 +1104,30 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
   return EHX_OK;
 }
 
-int tuned_run(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A_in, int form, unsigned long long *acc) {
+int tuned_reserve(ehx_ctx *ctx, const TunedPlan &P, uint64_t nprefix) {
+  DevBuf &bb = ctx->scratch[S_QUEUE], &cb = ctx->scratch[S_MISC], &db = ctx->scratch[S_DESC];
+  int rc;
+  // bucket[nprefix], then the queue (split parts add up to SPLIT_EXTRA entries)
+  if ((rc = bb.ensure((nprefix * 2 + SPLIT_EXTRA) * 4 + 64)) || (rc = cb.ensure(256 * 4))) return rc;
+  // descriptors: queue order, then by prefix (32 bytes each)
+  if (P.ns == 1 && (rc = db.ensure((nprefix * 2 + SPLIT_EXTRA) * 32))) return rc;
+  return EHX_OK;
+}
+
+int tuned_run(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A_in, int form, unsigned long long *acc,
+              bool allow_graph) {
   (void)form;
   if (A_in.nprefix == 0) return EHX_OK;
   if (A_in.nprefix >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "tuned level: more than 2^32 prefixes");
   const TuneParams tp = g_params;
-  DevBuf &bb = ctx->scratch[S_QUEUE], &cb = ctx->scratch[S_MISC], &db = ctx->scratch[S_DESC];
   int rc;
-  // bucket[nprefix], then the queue (split parts add up to SPLIT_EXTRA entries)
-  if ((rc = bb.ensure((A_in.nprefix * 2 + SPLIT_EXTRA) * 4 + 64)) || (rc = cb.ensure(256 * 4))) return rc;
+  if ((rc = tuned_reserve(ctx, P, A_in.nprefix))) return rc;
+  DevBuf &bb = ctx->scratch[S_QUEUE], &cb = ctx->scratch[S_MISC], &db = ctx->scratch[S_DESC];
   TunedArgs A = A_in;
-  A.desc = nullptr;
-  if (P.ns == 1) {  // descriptors: queue order, then by prefix (32 bytes each)
-    if ((rc = db.ensure((A.nprefix * 2 + SPLIT_EXTRA) * 32))) return rc;
-    A.desc = db.as<uint4>();
-  }
+  A.desc = P.ns == 1 ? db.as<uint4>() : nullptr;
   uint32_t *bucket = bb.as<uint32_t>(), *queue = bucket + A.nprefix;
   unsigned *ctl = cb.as<unsigned>();
-  if (!tp.use_graph) return enqueue_level(ctx, P, A, acc, bucket, ctl, queue, tp);
+  if (!tp.use_graph || !allow_graph) return enqueue_level(ctx, P, A, acc, bucket, ctl, queue, tp, allow_graph);
 
   unsigned char key[sizeof(g_cache.key)];
   memset(key, 0, sizeof(key));

@@ -189,6 +189,47 @@ def test_graph_replay_matches_eager(ctx):
         E.tune_set(**saved)
 
 
+def test_query_graph_replay_matches_eager(ctx, oracle):
+    """The middle-vertex plan (level 0 intersects two tries' heads, then the
+    tuned level) captured whole in one graph: every replay equals the eager
+    count and reports the level-0 count, with eager work between replays, after
+    switching to other tries and back (a new capture each time), and after a
+    larger query has grown the context's buffers (their addresses change)."""
+    s, d, n = G.rmat_graph(14)
+    s2, d2, n2 = G.rmat_graph(12)
+    T, R = E.Trie(ctx, s, d), E.Trie(ctx, d, s)
+    T2, R2 = E.Trie(ctx, s2, d2), E.Trie(ctx, d2, s2)
+    exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 8)
+    exp2 = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s2, d2)], 8)
+    n0 = len(np.intersect1d(np.unique(s), np.unique(d)))
+    saved = E.tune_get()
+    try:
+        E.tune_set(use_graph=0)
+        c, st = E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], E.FORM_LEARNED, stats=True)
+        assert c == exp and st["frontier"][0] == n0
+        E.tune_set(use_graph=1)
+        for form in (E.FORM_TUNED, E.FORM_LEARNED):
+            for i in range(3):
+                c, st = E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], form, stats=True)
+                assert c == exp and st["frontier"][0] == n0 and st["form_used"] == form
+                assert E.join_count(ctx, plans.FOUR_CLIQUE, [T], E.FORM_PLAIN) >= 0  # eager work between replays
+                ctx.synchronize()
+            assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T2, R2], form) == exp2
+            assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], form) == exp
+        big_s, big_d, _ = G.rmat_graph(17)
+        B, BR = E.Trie(ctx, big_s, big_d), E.Trie(ctx, big_d, big_s)
+        assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [B, BR], E.FORM_LEARNED) == \
+            E.join_count(ctx, plans.TRIANGLE, [B], E.FORM_PLAIN)
+        for i in range(3):
+            assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], E.FORM_LEARNED) == exp
+        B.close()
+        BR.close()
+    finally:
+        E.tune_set(**saved)
+    for t in (T, R, T2, R2):
+        t.close()
+
+
 @pytest.mark.parametrize("lds,big,bitmap_all,lanes,big_work,split,tiny", [
     (2048, 16, 0, 4, 16384, 0, 0), (8192, 256, 1, 64, 256, 1024, 1 << 20), (4096, 64, 1, 16, 4096, 1 << 21, 1024),
     (4096, 4, 0, 8, 64, 64, 64), (2048, 5, 1, 32, 512, 4096, 16384), (4096, 16, 0, 2, 4096, 1 << 21, 256)])
