@@ -82,7 +82,7 @@ static constexpr uint32_t SPLIT_MAX = 64, SPLIT_EXTRA = 1u << 16;
 // queue), which k_tiny runs up to 64 prefixes per wavefront at a time.
 static constexpr uint32_t TINY_BUCKET = 62, TINY_S = 128;
 static constexpr uint32_t TINY_U = 4;       // elements per lane per step in k_tiny
-static constexpr uint32_t TINY_BATCH = 32;  // queue entries a wavefront takes at once (<= 64)
+static constexpr uint32_t TINY_BATCH = 16;  // queue entries a wavefront takes at once (<= 64)
 static constexpr int OWN = 256;  // owner window per wave
 
 void tuned_bind(const TunedPlan &P, const Tries &T, TunedArgs &A) {
