@@ -49,6 +49,30 @@ def test_empty_and_single_row_relations(ctx):
     assert E.cycle4_count(ctx, S) == 0
 
 
+def test_query_graph_edge_cases(ctx, oracle):
+    """The middle-vertex plan replayed as one graph: no common heads (level 0
+    empty, its count known only on the device), one row, and ids at the top of
+    the uint32 range, each replayed twice."""
+    top = 2**32 - 1
+    edges = [(top - i, top - j) for i in range(6) for j in range(i)] + [(1, 2), (2, 3), (1, 3)]
+    cases = [(np.array([3], np.uint32), np.array([7], np.uint32)),
+             (np.array([1, 2], np.uint32), np.array([5, 6], np.uint32)),
+             oriented(edges)]
+    saved = E.tune_get()
+    try:
+        for s, d in cases:
+            exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(s, d)], 2)
+            T, R = E.Trie(ctx, s, d), E.Trie(ctx, d, s)
+            for ug in (0, 1, 1):
+                E.tune_set(use_graph=ug)
+                for form in (E.FORM_TUNED, E.FORM_LEARNED):
+                    assert E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T, R], form) == exp
+            T.close()
+            R.close()
+    finally:
+        E.tune_set(**saved)
+
+
 def test_ids_at_the_top_of_the_range(ctx, oracle):
     """Cliques on the largest uint32 ids: a K6 and a K5 sharing a vertex, plus
     noise; ids 0xFFFFFFFF and 0xFFFFFFFE are in sets that get staged."""
