@@ -95,10 +95,13 @@ struct ehx_trie {
   // hint[e]: how many values of that child set are below the head value of
   // the set holding element e of this trie (the parent of e); null when the
   // device had no room
+  // work[h]: the sum over set h of this trie of its elements' child-set
+  // lengths: the tuned level's binning reads it instead of sampling
   struct Link {
     uint64_t target;
     uint4 *arr;
     uint32_t *hint;
+    uint64_t *work;
   };
   std::vector<Link> links;
   uint64_t uid = 0;  // unique per process, never reused
@@ -239,7 +242,7 @@ int fill_u32(ehx_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
 // src's child pointers into dst (built and cached on src on first use); null in
 // *out when dst has no child lookup by value
 int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out,
-              const uint32_t **hint_out = nullptr);
+              const uint32_t **hint_out = nullptr, const uint64_t **work_out = nullptr);
 // The loaded per-pair table on the current device (null: none).
 const uint8_t *pair_table_current();
 int pair_table_load(int device, const uint8_t *table);

@@ -346,7 +346,7 @@ static int query_graph_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *t
   memset(&ta, 0, sizeof(ta));
   tuned_bind(tp, T, ta);
   if (tp.ns == 1 &&
-      (rc = trie_link(ctx, tries[tp.nd ? tp.d_rel : tp.s_rel[0]], tries[tp.x_rel], &ta.xlink, &ta.xhint)))
+      (rc = trie_link(ctx, tries[tp.nd ? tp.d_rel : tp.s_rel[0]], tries[tp.x_rel], &ta.xlink, &ta.xhint, &ta.xwork)))
     return rc;
   const LevelDesc L0 = level_desc(p, 0);
   int driver = 0;
@@ -372,6 +372,7 @@ static int query_graph_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *t
   }
   key_add(key, ta.xlink);
   key_add(key, ta.xhint);
+  key_add(key, ta.xwork);
   std::lock_guard<std::mutex> lk(g_qgraph_mu);
   if (!(g_qgraph.valid && g_qgraph.key == key)) {
     query_graph_drop();
@@ -555,7 +556,7 @@ int join_count(ehx_ctx *ctx, const ehx_plan *p, ehx_trie *const *tries, int form
       for (int j = 0; j < EHX_MAX_ATTRS; j++) ta.prefix[j] = in.col[j];
       tuned_bind(tp, T, ta);
       if (tp.ns == 1) {  // child pointers from E(p)'s trie into x's
-        if ((rc = trie_link(ctx, tries[tp.nd ? tp.d_rel : tp.s_rel[0]], tries[tp.x_rel], &ta.xlink, &ta.xhint)))
+        if ((rc = trie_link(ctx, tries[tp.nd ? tp.d_rel : tp.s_rel[0]], tries[tp.x_rel], &ta.xlink, &ta.xhint, &ta.xwork)))
           return rc;
       }
       ta.nprefix = n;

@@ -62,6 +62,9 @@ struct TunedArgs {
   // below S(p)'s first value is streamed from there when S(p)'s first value is
   // at least that parent value (null: streamed whole)
   const uint32_t *xhint;
+  // with xlink: xwork[h] = the sum of the child-list lengths of the elements
+  // of set h of E(p)'s trie, the binning's work estimate
+  const uint64_t *xwork;
   // with a whole query captured in one graph: the prefix count on the device
   // (level 0's output); nprefix is then its upper bound, which sizes the grids
   // and buffers.  Null: nprefix is the count.

@@ -145,6 +145,8 @@ __device__ __forceinline__ bool head_ok(const TunedPlan &P, const TunedArgs &A, 
 __host__ __device__ constexpr uint32_t wave_hash_cap(int slots) { return (uint32_t)slots / 8; }
 
 // bucket[p] = bin | parts << 6; hist[b] counts queue entries (parts)
+// MAXS: S sets resolved per prefix (1 for single-S plans: no runtime-indexed array)
+template <int MAXS>
 __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ bucket, unsigned *__restrict__ hist,
                            int slots, int big_work, uint4 *__restrict__ pdesc, uint32_t split_work,
                            uint32_t tiny_work) {
@@ -154,30 +156,35 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
   const DTrie &tx = A.tx;
   const uint64_t np = A.nprefix_dev ? *A.nprefix_dev : A.nprefix;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (uint64_t)gridDim.x * blockDim.x) {
-    DSet S[TUNED_MAX_S];
+    DSet S[MAXS];
     DSet Dd;
     uint32_t b = BUCKET_NONE, parts = 1;
-    if (resolve_s<TUNED_MAX_S>(P, A, p, S) && resolve_d(P, A, p, Dd)) {
+    if (resolve_s<MAXS>(P, A, p, S) && resolve_d(P, A, p, Dd)) {
       int mi = 0;
-      for (int i = 1; i < P.ns; i++)
+      for (int i = 1; i < P.ns && i < MAXS; i++)
         if (S[i].n < S[mi].n) mi = i;
-      const DSet D = S[mi];
+      const DSet D = MAXS == 1 ? S[0] : S[mi];
       const DSet X = P.nd ? Dd : D;  // the set x runs over
-      const uint32_t ns = X.n < 32 ? X.n : 32;  // sample up to 32 of its elements
       uint64_t w = 0;
-      if (pdesc && A.xlink) {  // child pointers: one independent load per sample
-        const uint4 *xl = A.xlink + (X.v - (P.nd ? A.td.vals : A.ts[0].vals));
-#pragma unroll 8
-        for (uint32_t k = 0; k < ns; k++) w += xl[(uint64_t)k * X.n / ns].y;
+      if (pdesc && A.xlink && A.xwork) {  // the set's child-list lengths, summed when it was linked
+        const int32_t hx = P.nd ? node_of(A.td, A.dp[p]) : node_of(A.ts[0], A.sp[0][p]);
+        w = (hx >= 0 ? A.xwork[hx] : 0ull) + X.n + (P.nd ? D.n : 0u);
       } else {
+        const uint32_t ns = X.n < 32 ? X.n : 32;  // sample up to 32 of its elements
+        if (pdesc && A.xlink) {  // child pointers: one independent load per sample
+          const uint4 *xl = A.xlink + (X.v - (P.nd ? A.td.vals : A.ts[0].vals));
 #pragma unroll 8
-        for (uint32_t k = 0; k < ns; k++) {
-          uint32_t x = X.v[(uint64_t)k * X.n / ns];
-          int32_t nx = node_of(tx, x);
-          if (nx >= 0) w += tx.off[nx + 1] - tx.off[nx];
+          for (uint32_t k = 0; k < ns; k++) w += xl[(uint64_t)k * X.n / ns].y;
+        } else {
+#pragma unroll 8
+          for (uint32_t k = 0; k < ns; k++) {
+            uint32_t x = X.v[(uint64_t)k * X.n / ns];
+            int32_t nx = node_of(tx, x);
+            if (nx >= 0) w += tx.off[nx + 1] - tx.off[nx];
+          }
         }
+        w = w * X.n / (ns ? ns : 1) + X.n + (P.nd ? D.n : 0u);
       }
-      w = w * X.n / (ns ? ns : 1) + X.n + (P.nd ? D.n : 0u);
       if (P.ns > 1) w = w / 2 + 1;
       // the wave bin needs S(p) to fit a quarter of the LDS, as a hash table or as a bitmap
       bool fits_wave = D.n <= wave_hash_cap(slots);
@@ -1074,8 +1081,12 @@ static int enqueue_level(ehx_ctx *ctx, const TunedPlan &P, const TunedArgs &A, u
   if ((rc = fill_u32(ctx, ctl, 0u, 134))) return rc;
   // descriptors by prefix after the queue-ordered ones (A.desc)
   uint4 *pdesc = A.desc ? const_cast<uint4 *>(A.desc) + 2 * (A.nprefix + SPLIT_EXTRA) : nullptr;
-  k_classify<<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(P, A, bucket, ctl, tp.lds_slots, tp.big_work, pdesc,
-                                                               (uint32_t)tp.split_work, (uint32_t)tp.tiny_work);
+  if (P.ns == 1)
+    k_classify<1><<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(P, A, bucket, ctl, tp.lds_slots, tp.big_work, pdesc,
+                                                                    (uint32_t)tp.split_work, (uint32_t)tp.tiny_work);
+  else
+    k_classify<TUNED_MAX_S><<<grid_for(A.nprefix, 256, 16384), 256, 0, st>>>(
+        P, A, bucket, ctl, tp.lds_slots, tp.big_work, pdesc, (uint32_t)tp.split_work, (uint32_t)tp.tiny_work);
   EHX_HIP(hipGetLastError());
   k_bucket_scan<<<1, 64, 0, st>>>(ctl);
   EHX_HIP(hipGetLastError());
