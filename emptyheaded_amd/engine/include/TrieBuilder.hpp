@@ -17,18 +17,19 @@
 #include "Trie.hpp"
 #include "utils/parallel.hpp"
 
+// One per thread, written on every build call: its own cache lines.
 template <class A, class M>
-struct TrieBuilder {
+struct alignas(128) TrieBuilder {
   Trie<A, M> *trie;
-  std::vector<const Set<hybrid> *> aggregate_sets;
-  std::vector<Set<hybrid>> results;
+  ehx_dropin::line_vector<const Set<hybrid> *> aggregate_sets;
+  ehx_dropin::line_vector<Set<hybrid>> results;
   uint32_t cur_level;
   uint32_t tmp_level;
   size_t num_attributes;
   // Materialized sets by output level (cur_level) and the value the enclosing
   // foreach is at on each level: a row of the output trie is a path of them.
-  std::vector<Set<hybrid>> level_sets;
-  std::vector<uint32_t> prefix;
+  ehx_dropin::line_vector<Set<hybrid>> level_sets;
+  ehx_dropin::line_vector<uint32_t> prefix;
 
   TrieBuilder(Trie<A, M> *t_in, const size_t num_attributes_in)
       : trie(t_in), cur_level(1), tmp_level(0), num_attributes(num_attributes_in) {

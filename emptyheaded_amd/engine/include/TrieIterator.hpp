@@ -5,10 +5,11 @@
 
 #include "Trie.hpp"
 
+// One per thread, written on every child lookup: its own cache lines.
 template <class A, class M>
-struct TrieIterator {
+struct alignas(128) TrieIterator {
   Trie<A, M> *trie;
-  std::vector<const TrieBlock<layout, M> *> levels;
+  ehx_dropin::line_vector<const TrieBlock<layout, M> *> levels;
   size_t num_rows;
 
   TrieIterator(Trie<A, M> *t_in) : trie(t_in), num_rows(0) {

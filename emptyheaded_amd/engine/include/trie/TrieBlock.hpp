@@ -17,8 +17,7 @@ struct HostNav {
   uint64_t dense_len = 0;
   const uint32_t *head = nullptr;  // sorted head values
   uint32_t nh = 0;
-  char *blocks = nullptr;   // nh + 1 blocks by node (the last one is the head block)
-  char *vblocks = nullptr;  // with a dense head: dense_len blocks by value (absent values: no trie)
+  char *blocks = nullptr;  // nh + 1 blocks by node (the last one is the head block)
 };
 struct HostTrie;
 // Node index of `data` in the trie's head set, or -1.
@@ -52,10 +51,10 @@ struct TrieBlock {
     if (!trie) return nullptr;
     const ehx_dropin::HostNav *nv = nav();
     if (node != nv->nh) return (TrieBlock<T, M> *)ehx_dropin::child_of(trie, node, data);
-    if (nv->vblocks) {  // one load: the block stored at the value's slot
+    if (nv->dense) {  // one 4-byte load; the block itself is not read here
       if (data >= nv->dense_len) return nullptr;
-      TrieBlock<T, M> *b = reinterpret_cast<TrieBlock<T, M> *>(nv->vblocks) + data;
-      return b->trie ? b : nullptr;
+      const int32_t h = nv->dense[data];
+      return h < 0 ? nullptr : reinterpret_cast<TrieBlock<T, M> *>(nv->blocks) + h;
     }
     const uint32_t *p = std::lower_bound(nv->head, nv->head + nv->nh, data);
     const long h = (p != nv->head + nv->nh && *p == data) ? (long)(p - nv->head) : -1;

@@ -5,12 +5,14 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <cassert>
 #include <functional>
 #include <iostream>
+#include <new>
 #include <string>
 #include <tuple>
 #include <unordered_map>
@@ -25,6 +27,32 @@ static size_t NUM_THREADS = NUM_THREADS_IN;
 
 // Padding between per-thread reducer slots (common.hpp:38).
 #define PADDING 300
+
+namespace ehx_dropin {
+// Allocator of per-thread state: each allocation starts on a 128-byte boundary
+// and is padded to a multiple of 128 bytes, so vectors written by different
+// threads never share a cache line (or an adjacent-line prefetch pair).
+template <class T>
+struct LineAlloc {
+  typedef T value_type;
+  LineAlloc() = default;
+  template <class U>
+  LineAlloc(const LineAlloc<U> &) {}
+  T *allocate(size_t n) {
+    const size_t bytes = (n * sizeof(T) + 127) / 128 * 128;
+    void *p = aligned_alloc(128, bytes ? bytes : 128);
+    if (!p) throw std::bad_alloc();
+    return static_cast<T *>(p);
+  }
+  void deallocate(T *p, size_t) { free(p); }
+  template <class U>
+  bool operator==(const LineAlloc<U> &) const { return true; }
+  template <class U>
+  bool operator!=(const LineAlloc<U> &) const { return false; }
+};
+template <class T>
+using line_vector = std::vector<T, LineAlloc<T>>;
+}  // namespace ehx_dropin
 
 namespace common {
 // Density rule (common.hpp:56-67): bitset iff more than 2 elements and

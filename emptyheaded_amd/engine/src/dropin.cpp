@@ -30,8 +30,8 @@ struct HostTrie : HostNav {
   int ncols = 2;  // 1: a unary relation, kept as the device trie of (x, x)
   std::vector<uint32_t> head, off;
   std::vector<int32_t> dense;
-  std::vector<Block> blocks;
-  std::vector<Block> value_blocks;  // dense heads: a copy of each child block at its value's slot  // nh + 1 (the last is the head block)
+  std::vector<Block> blocks;         // nh + 1 (the last is the head block)
+  std::vector<uint8_t> node_layout;  // nh + 1: each block's set layout (type::layout), read without the block
   // annotated relations: one annotation per row in trie order (per head for a
   // unary relation), and the child values on the host to find a row's position
   bool annotated = false;
@@ -63,19 +63,6 @@ static ehx_ctx *ctx() {
   std::lock_guard<std::mutex> g(g_mu);
   if (!g_ctx) check(ehx_ctx_create(0, &g_ctx), "ehx_ctx_create");
   return g_ctx;
-}
-
-// With a dense head, a copy of every child block at its value's slot, so a
-// child lookup reads one block instead of the value -> node array and then
-// the block (a random host access each, per call of the generated code).
-// Rebuilt whenever the node-indexed blocks change.
-static void index_blocks_by_value(HostTrie *t) {
-  std::vector<Block>().swap(t->value_blocks);
-  t->vblocks = nullptr;
-  if (t->dense.empty()) return;
-  t->value_blocks.resize(t->dense.size());  // default blocks: no trie (absent values)
-  for (uint64_t h = 0; h < t->nh; h++) t->value_blocks[t->head[h]] = t->blocks[h];
-  t->vblocks = (char *)t->value_blocks.data();
 }
 
 HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
@@ -121,7 +108,8 @@ HostTrie *build_host_trie(const uint32_t *c0, const uint32_t *c1, size_t rows) {
       b.set.number_of_bytes = 4ull * t->nh;
     }
   }
-  index_blocks_by_value(t);
+  t->node_layout.resize(t->nh + 1);
+  for (uint64_t h = 0; h <= t->nh; h++) t->node_layout[h] = (uint8_t)t->blocks[h].set.type;
   return t;
 }
 
@@ -219,7 +207,6 @@ HostTrie *build_rows_trie(int ncols, const uint32_t *const *cols, size_t rows, c
       b.set.values = t->child_vals.data() + t->off[h];
       b.set.range = b.set.cardinality ? b.set.values[b.set.cardinality - 1] : 0;
     }
-    index_blocks_by_value(t);
     if (!first) first = t;
     else prev->next = t;
     prev = t;
@@ -558,7 +545,9 @@ size_t aggregate_head(const Operands &ops, Set<hybrid> *out, std::vector<uint32_
   return counts[0];
 }
 
-struct LevelLog {  // calls one chunk of level-0 values made at one builder level
+// Calls one chunk of level-0 values made at one builder level.  Chunks run on
+// different threads, so each log has its own cache lines.
+struct alignas(128) LevelLog {
   std::vector<uint32_t> nodes;  // call-major, nops per call
   uint64_t ncalls = 0, base = 0, cursor = 0;
 };
@@ -567,6 +556,7 @@ struct LevelBatch {
   int nops = 0;
   const HostTrie *tries[8] = {};
   bool materialized = false;
+  bool innermost = false;  // innermost aggregated level with no annotated operand
   // results in the level's pinned buffers: counts (count-only levels), or
   // offsets[n+1] and the values (materialized levels)
   const uint64_t *counts = nullptr, *offsets = nullptr;
@@ -614,12 +604,75 @@ void emit_row(const uint32_t *row, int ncols, uint64_t anno_bits) {
   S->annos[t_chunk].push_back(anno_bits);
 }
 
+// Node of block p when p is one of trie t's node blocks (the block itself is
+// not read), else -1.
+static inline long node_in(const HostTrie *t, const void *p) {
+  const uintptr_t base = (uintptr_t)t->blocks.data(), a = (uintptr_t)p;
+  if (a < base) return -1;
+  const uintptr_t k = (a - base) / sizeof(Block);
+  return (k < t->blocks.size() && a == base + k * sizeof(Block)) ? (long)k : -1;
+}
+
+// The level's call signature, from its first recorded call.
+static void sign(LevelBatch &B, size_t num_attrs, uint32_t level, const Operands &ops, bool materialize) {
+  bool any_annotated = false;
+  for (int i = 0; i < ops.n; i++) any_annotated = any_annotated || blk(ops, i)->trie->annotated;
+  B.nops = ops.n;
+  for (int i = 0; i < ops.n; i++) B.tries[i] = blk(ops, i)->trie;
+  B.innermost = level + 2 == num_attrs && !any_annotated;
+  B.materialized = materialize || !B.innermost;
+}
+
+// A recorded call's result in the replay: the values of a materialized level,
+// the cardinality of a count-only one.
+static inline size_t replay_result(const LevelBatch &B, uint64_t idx, type::layout t, Set<hybrid> *out) {
+  if (B.materialized) {
+    const uint64_t a = B.offsets[idx], b = B.offsets[idx + 1];
+    *out = Set<hybrid>((uint32_t)(b - a), b > a ? B.values[b - 1] : 0, 4 * (b - a), t);
+    out->values = B.values + a;
+  } else {
+    *out = Set<hybrid>((uint32_t)B.counts[idx], 0, 4 * B.counts[idx], t);
+  }
+  return out->cardinality;
+}
+
 size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybrid> *out, bool materialize) {
   Session *S = g_sess;
   if (!S) not_implemented("TrieBuilder::build_aggregated_set / build_set outside par_foreach_aggregate / builder");
   if (ops.n < 1 || ops.n > 8) not_implemented("build_aggregated_set / build_set arity");
   for (int i = 0; i < ops.n; i++)
     if (!blk(ops, i)) { set_empty(out); return 0; }  // a NULL TrieBlock is empty (TrieBuilder.cpp:249-254)
+  const bool known = level < S->logs[t_chunk].size();
+  const bool recording = S->record_level == (int)level;
+  if (known && ops.n >= 2 && (recording || S->record_level < 0 || (int)level < S->record_level)) {
+    // Every operand a node block of the trie the level's signature names: the
+    // nodes come from the block addresses and the layouts from node_layout, so
+    // no block is read (one random host access saved per call).  Anything
+    // else takes the general path below.
+    LevelLog &L = S->logs[t_chunk][level];
+    LevelBatch &B = S->batches[level];
+    if (recording) std::call_once(S->sig_once[level], [&]() { sign(B, num_attrs, level, ops, materialize); });
+    uint32_t nodes[8];
+    bool fast = B.nops == ops.n && B.materialized == (materialize || !B.innermost);
+    for (int i = 0; fast && i < ops.n; i++) {
+      const long h = node_in(B.tries[i], ops.blocks[i]);
+      fast = h >= 0;
+      nodes[i] = (uint32_t)h;
+    }
+    if (fast) {
+      if (recording) {
+        L.nodes.insert(L.nodes.end(), nodes, nodes + ops.n);
+        L.ncalls++;
+        set_empty(out);
+        return 0;
+      }
+      // U x U -> U, U x R -> U, R x R -> R (ops/intersection/set.hpp:53-91)
+      type::layout t = type::RANGE_BITSET;
+      for (int i = 0; i < ops.n; i++)
+        if (B.tries[i]->node_layout[nodes[i]] != (uint8_t)type::RANGE_BITSET) t = type::UINTEGER;
+      return replay_result(B, L.base + L.cursor++, t, out);
+    }
+  }
   // The innermost aggregated level yields a cardinality only, unless an operand
   // is annotated: then the generated code loops over the set to read the
   // annotations (EHGenerator.scala:435-456), so its values are needed.
@@ -630,15 +683,11 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
     *out = blk(ops, 0)->set;  // the set itself (TrieBuilder.cpp:223-240)
     return out->cardinality;
   }
-  if (level >= S->logs[t_chunk].size()) not_implemented("builder level beyond num_attributes");
+  if (!known) not_implemented("builder level beyond num_attributes");
   LevelLog &L = S->logs[t_chunk][level];
   LevelBatch &B = S->batches[level];
-  if (S->record_level == (int)level) {
-    std::call_once(S->sig_once[level], [&]() {
-      B.nops = ops.n;
-      for (int i = 0; i < ops.n; i++) B.tries[i] = blk(ops, i)->trie;
-      B.materialized = materialize || !innermost;
-    });
+  if (recording) {
+    std::call_once(S->sig_once[level], [&]() { sign(B, num_attrs, level, ops, materialize); });
     if (B.nops != ops.n) not_implemented("two call sites with different arity at one builder level");
     if (B.materialized != (materialize || !innermost))
       not_implemented("build_set and build_aggregated_set at one builder level");
@@ -651,15 +700,7 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
     return 0;
   }
   if (S->record_level >= 0 && (int)level > S->record_level) not_implemented("call below the level being recorded");
-  const uint64_t idx = L.base + L.cursor++;
-  if (B.materialized) {
-    const uint64_t a = B.offsets[idx], b = B.offsets[idx + 1];
-    *out = Set<hybrid>((uint32_t)(b - a), b > a ? B.values[b - 1] : 0, 4 * (b - a), result_type(ops));
-    out->values = B.values + a;
-  } else {
-    *out = Set<hybrid>((uint32_t)B.counts[idx], 0, 4 * B.counts[idx], result_type(ops));
-  }
-  return out->cardinality;
+  return replay_result(B, L.base + L.cursor++, result_type(ops), out);
 }
 
 static void run_chunks(const Set<hybrid> *level0, size_t nthreads,
