@@ -12,6 +12,7 @@
 #include <atomic>
 #include <chrono>
 #include <functional>
+#include <memory>
 #include <fstream>
 #include <mutex>
 #include <thread>
@@ -568,7 +569,8 @@ struct Session {
   int record_level = -1;
   std::vector<std::vector<LevelLog>> logs;  // [chunk][level]
   std::vector<LevelBatch> batches;          // [level]
-  std::vector<std::once_flag> sig_once;
+  std::unique_ptr<std::atomic<bool>[]> signed_levels;  // [level]: its LevelBatch signature is set
+  std::mutex sign_mu;
   OutRows *out = nullptr;                   // materialized output (par_foreach_builder)
   std::vector<std::vector<uint32_t>> rows;  // [chunk] rows emitted by the replay
   std::vector<std::vector<uint64_t>> annos; // [chunk]
@@ -623,6 +625,17 @@ static void sign(LevelBatch &B, size_t num_attrs, uint32_t level, const Operands
   B.materialized = materialize || !B.innermost;
 }
 
+// The level's signature from the first call that records it (double-checked:
+// one acquire load per later call).
+static inline void sign_once(Session *S, LevelBatch &B, size_t num_attrs, uint32_t level, const Operands &ops,
+                             bool materialize) {
+  if (S->signed_levels[level].load(std::memory_order_acquire)) return;
+  std::lock_guard<std::mutex> g(S->sign_mu);
+  if (S->signed_levels[level].load(std::memory_order_relaxed)) return;
+  sign(B, num_attrs, level, ops, materialize);
+  S->signed_levels[level].store(true, std::memory_order_release);
+}
+
 // A recorded call's result in the replay: the values of a materialized level,
 // the cardinality of a count-only one.
 static inline size_t replay_result(const LevelBatch &B, uint64_t idx, type::layout t, Set<hybrid> *out) {
@@ -651,7 +664,7 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
     // else takes the general path below.
     LevelLog &L = S->logs[t_chunk][level];
     LevelBatch &B = S->batches[level];
-    if (recording) std::call_once(S->sig_once[level], [&]() { sign(B, num_attrs, level, ops, materialize); });
+    if (recording) sign_once(S, B, num_attrs, level, ops, materialize);
     uint32_t nodes[8];
     bool fast = B.nops == ops.n && B.materialized == (materialize || !B.innermost);
     for (int i = 0; fast && i < ops.n; i++) {
@@ -666,9 +679,11 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
         set_empty(out);
         return 0;
       }
-      // U x U -> U, U x R -> U, R x R -> R (ops/intersection/set.hpp:53-91)
-      type::layout t = type::RANGE_BITSET;
-      for (int i = 0; i < ops.n; i++)
+      // U x U -> U, U x R -> U, R x R -> R (ops/intersection/set.hpp:53-91).  A
+      // count-only result is never iterated (it holds no values), so its
+      // layout is not looked up.
+      type::layout t = B.materialized ? type::RANGE_BITSET : type::UINTEGER;
+      for (int i = 0; B.materialized && i < ops.n; i++)
         if (B.tries[i]->node_layout[nodes[i]] != (uint8_t)type::RANGE_BITSET) t = type::UINTEGER;
       return replay_result(B, L.base + L.cursor++, t, out);
     }
@@ -687,7 +702,7 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
   LevelLog &L = S->logs[t_chunk][level];
   LevelBatch &B = S->batches[level];
   if (recording) {
-    std::call_once(S->sig_once[level], [&]() { sign(B, num_attrs, level, ops, materialize); });
+    sign_once(S, B, num_attrs, level, ops, materialize);
     if (B.nops != ops.n) not_implemented("two call sites with different arity at one builder level");
     if (B.materialized != (materialize || !innermost))
       not_implemented("build_set and build_aggregated_set at one builder level");
@@ -703,12 +718,19 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
   return replay_result(B, L.base + L.cursor++, result_type(ops), out);
 }
 
+static bool profiling();
+static double now_ms();
+
 static void run_chunks(const Set<hybrid> *level0, size_t nthreads,
                        const std::function<void(size_t, uint32_t, uint32_t)> &body) {
   const uint64_t n0 = level0->cardinality;
   const uint64_t nchunks = (n0 + CHUNK - 1) / CHUNK;
   std::atomic<uint64_t> next(0);
+  const bool prof = profiling();
+  std::vector<double> first(nthreads, 0.0), last(nthreads, 0.0);
+  const double t0 = prof ? now_ms() : 0.0;
   auto worker = [&](size_t tid) {
+    if (prof) first[tid] = now_ms() - t0;
     for (;;) {
       const uint64_t c = next.fetch_add(1);
       if (c >= nchunks) break;
@@ -716,11 +738,23 @@ static void run_chunks(const Set<hybrid> *level0, size_t nthreads,
       const uint64_t e = std::min(n0, (c + 1) * CHUNK);
       for (uint64_t i = c * CHUNK; i < e; i++) body(tid, (uint32_t)i, level0->values[i]);
     }
+    if (prof) last[tid] = now_ms() - t0;
   };
   std::vector<std::thread> th;
   for (size_t t = 1; t < nthreads; t++) th.emplace_back(worker, t);
   worker(0);
   for (auto &x : th) x.join();
+  if (prof) {
+    double busy = 0, f_max = 0, l_min = 1e30, l_max = 0;
+    for (size_t t = 0; t < nthreads; t++) {
+      busy += last[t] - first[t];
+      f_max = std::max(f_max, first[t]);
+      l_min = std::min(l_min, last[t]);
+      l_max = std::max(l_max, last[t]);
+    }
+    fprintf(stderr, "[dropin]   %zu workers: last start %.2f ms, first done %.2f ms, last done %.2f ms, busy %.1f ms\n",
+            nthreads, f_max, l_min, l_max, busy);
+  }
 }
 
 // EHX_DROPIN_PROFILE=1: per pass, the wall time of the host lambdas, of
@@ -743,7 +777,8 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
   S.num_attrs = num_attrs;
   S.logs.assign(nchunks, std::vector<LevelLog>(levels));
   S.batches.resize(levels);
-  S.sig_once = std::vector<std::once_flag>(levels);
+  S.signed_levels.reset(new std::atomic<bool>[levels ? levels : 1]);
+  for (size_t l = 0; l < levels; l++) S.signed_levels[l].store(false);
   S.out = out;
   if (out) {
     S.rows.resize(nchunks);
