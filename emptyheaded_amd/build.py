@@ -81,7 +81,7 @@ def build_dropin():
     cxx = shutil.which("g++") or "g++"
     tmp = DROPIN_LIB + ".tmp"
     # no fp contraction: the queries' float arithmetic is exactly what their source says
-    cmd = [cxx, "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wno-unused-variable",
+    cmd = [cxx, "-O3", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wno-unused-variable",
            "-I", os.path.join(ENGINE, "include"), "-o", tmp] + srcs + [
         "-L", HERE, "-lehx", "-Wl,-rpath,$ORIGIN/..", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)

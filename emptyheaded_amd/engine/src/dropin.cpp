@@ -11,6 +11,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <functional>
 #include <memory>
 #include <fstream>
@@ -550,7 +551,11 @@ size_t aggregate_head(const Operands &ops, Set<hybrid> *out, std::vector<uint32_
 // different threads, so each log has its own cache lines.
 struct alignas(128) LevelLog {
   std::vector<uint32_t> nodes;  // call-major, nops per call
-  uint64_t ncalls = 0, base = 0, cursor = 0;
+  uint64_t ncalls = 0, cursor = 0;
+  // the chunk's results in its batch's pinned buffers: a count per call
+  // (count-only levels), or offsets[ncalls + 1] into `vals` (materialized)
+  const uint64_t *res = nullptr;
+  const uint32_t *vals = nullptr;
 };
 
 struct LevelBatch {
@@ -558,10 +563,65 @@ struct LevelBatch {
   const HostTrie *tries[8] = {};
   bool materialized = false;
   bool innermost = false;  // innermost aggregated level with no annotated operand
-  // results in the level's pinned buffers: counts (count-only levels), or
-  // offsets[n+1] and the values (materialized levels)
-  const uint64_t *counts = nullptr, *offsets = nullptr;
-  const uint32_t *values = nullptr;
+};
+
+// Worker threads kept for one run of passes: run(fn) calls fn(tid) on every
+// worker (tid 0 on the calling thread) and returns once all have finished, so
+// the many short parallel phases of a run (chunk ranges, gathers) start no
+// threads of their own.
+class Workers {
+ public:
+  explicit Workers(size_t n) : n_(n ? n : 1) {
+    for (size_t t = 1; t < n_; t++) th_.emplace_back([this, t]() { loop(t); });
+  }
+  ~Workers() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto &x : th_) x.join();
+  }
+  size_t size() const { return n_; }
+  void run(const std::function<void(size_t)> &fn) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &fn;
+      pending_ = n_ - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    fn(0);
+    std::unique_lock<std::mutex> l(mu_);
+    done_.wait(l, [this]() { return pending_ == 0; });
+  }
+
+ private:
+  void loop(size_t tid) {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(size_t)> *f;
+      {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [&]() { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+        f = fn_;
+      }
+      (*f)(tid);
+      std::lock_guard<std::mutex> g(mu_);
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  size_t n_;
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(size_t)> *fn_ = nullptr;
+  size_t pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
 };
 
 struct Session {
@@ -638,13 +698,14 @@ static inline void sign_once(Session *S, LevelBatch &B, size_t num_attrs, uint32
 
 // A recorded call's result in the replay: the values of a materialized level,
 // the cardinality of a count-only one.
-static inline size_t replay_result(const LevelBatch &B, uint64_t idx, type::layout t, Set<hybrid> *out) {
+static inline size_t replay_result(const LevelBatch &B, LevelLog &L, type::layout t, Set<hybrid> *out) {
+  const uint64_t idx = L.cursor++;
   if (B.materialized) {
-    const uint64_t a = B.offsets[idx], b = B.offsets[idx + 1];
-    *out = Set<hybrid>((uint32_t)(b - a), b > a ? B.values[b - 1] : 0, 4 * (b - a), t);
-    out->values = B.values + a;
+    const uint64_t a = L.res[idx], b = L.res[idx + 1];
+    *out = Set<hybrid>((uint32_t)(b - a), b > a ? L.vals[b - 1] : 0, 4 * (b - a), t);
+    out->values = L.vals + a;
   } else {
-    *out = Set<hybrid>((uint32_t)B.counts[idx], 0, 4 * B.counts[idx], t);
+    *out = Set<hybrid>((uint32_t)L.res[idx], 0, 4 * L.res[idx], t);
   }
   return out->cardinality;
 }
@@ -685,7 +746,7 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
       type::layout t = B.materialized ? type::RANGE_BITSET : type::UINTEGER;
       for (int i = 0; B.materialized && i < ops.n; i++)
         if (B.tries[i]->node_layout[nodes[i]] != (uint8_t)type::RANGE_BITSET) t = type::UINTEGER;
-      return replay_result(B, L.base + L.cursor++, t, out);
+      return replay_result(B, L, t, out);
     }
   }
   // The innermost aggregated level yields a cardinality only, unless an operand
@@ -715,17 +776,21 @@ size_t aggregate(size_t num_attrs, uint32_t level, const Operands &ops, Set<hybr
     return 0;
   }
   if (S->record_level >= 0 && (int)level > S->record_level) not_implemented("call below the level being recorded");
-  return replay_result(B, L.base + L.cursor++, result_type(ops), out);
+  return replay_result(B, L, result_type(ops), out);
 }
 
 static bool profiling();
 static double now_ms();
 
-static void run_chunks(const Set<hybrid> *level0, size_t nthreads,
-                       const std::function<void(size_t, uint32_t, uint32_t)> &body) {
+// The body over the level-0 values of chunks [c_begin, c_end), chunks handed
+// out dynamically to the pool's workers.
+static void run_chunks(const Set<hybrid> *level0, Workers &pool,
+                       const std::function<void(size_t, uint32_t, uint32_t)> &body, uint64_t c_begin,
+                       uint64_t c_end) {
+  const size_t nthreads = pool.size();
   const uint64_t n0 = level0->cardinality;
-  const uint64_t nchunks = (n0 + CHUNK - 1) / CHUNK;
-  std::atomic<uint64_t> next(0);
+  const uint64_t nchunks = c_end;
+  std::atomic<uint64_t> next(c_begin);
   const bool prof = profiling();
   std::vector<double> first(nthreads, 0.0), last(nthreads, 0.0);
   const double t0 = prof ? now_ms() : 0.0;
@@ -740,10 +805,7 @@ static void run_chunks(const Set<hybrid> *level0, size_t nthreads,
     }
     if (prof) last[tid] = now_ms() - t0;
   };
-  std::vector<std::thread> th;
-  for (size_t t = 1; t < nthreads; t++) th.emplace_back(worker, t);
-  worker(0);
-  for (auto &x : th) x.join();
+  pool.run(worker);
   if (prof) {
     double busy = 0, f_max = 0, l_min = 1e30, l_max = 0;
     for (size_t t = 0; t < nthreads; t++) {
@@ -767,6 +829,125 @@ static double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Calls of chunks [c0, c1) of level p, gathered for one GPU batch.
+struct Gathered {
+  uint64_t c0 = 0, c1 = 0, total = 0, bound = 0;
+  LevelBuffers *LB = nullptr;
+};
+
+// Operand nodes of the recorded calls of chunks [c0, c1), operand-major, in
+// chunk order, gathered from the chunk logs by the pool's threads (each chunk
+// fills its own range), with the bound on the results' size when the level is
+// materialized.
+static Gathered gather_calls(Session &S, size_t p, uint64_t c0, uint64_t c1, LevelBuffers &LB, Workers &pool) {
+  LevelBatch &B = S.batches[p];
+  Gathered g;
+  g.c0 = c0;
+  g.c1 = c1;
+  g.LB = &LB;
+  std::vector<uint64_t> base(c1 - c0);
+  for (uint64_t c = c0; c < c1; c++) {
+    base[c - c0] = g.total;
+    g.total += S.logs[c][p].ncalls;
+  }
+  if (g.total == 0) return g;
+  const int nops = B.nops;
+  LB.nodes.ensure((size_t)nops * g.total * 4);
+  uint32_t *nodes = LB.nodes.as<uint32_t>();
+  std::atomic<uint64_t> next_chunk(c0), bound(0);
+  auto worker = [&](size_t) {
+    uint64_t local = 0;
+    for (;;) {
+      const uint64_t c = next_chunk.fetch_add(1);
+      if (c >= c1) break;
+      LevelLog &L = S.logs[c][p];
+      const uint64_t b0 = base[c - c0];
+      for (uint64_t i = 0; i < L.ncalls; i++) {
+        uint64_t mn = ~0ull;
+        for (int s = 0; s < nops; s++) {
+          const uint32_t nd = L.nodes[i * nops + s];
+          nodes[(uint64_t)s * g.total + b0 + i] = nd;
+          if (B.materialized) mn = std::min(mn, card_of(B.tries[s], nd));
+        }
+        if (B.materialized) local += mn;
+      }
+      std::vector<uint32_t>().swap(L.nodes);
+    }
+    bound += local;
+  };
+  pool.run(worker);
+  g.bound = bound.load();
+  return g;
+}
+
+// The GPU batch of gathered calls; each chunk's log then points at its results.
+static void run_batch_gathered(Session &S, size_t p, const Gathered &g) {
+  if (g.total == 0) return;
+  LevelBatch &B = S.batches[p];
+  LevelBuffers &LB = *g.LB;
+  const int nops = B.nops;
+  ehx_trie *dt[8];
+  const uint32_t *np[8];
+  for (int s = 0; s < nops; s++) {
+    dt[s] = B.tries[s]->dev;
+    np[s] = LB.nodes.as<uint32_t>() + (uint64_t)s * g.total;
+  }
+  const uint64_t *res;
+  const uint32_t *vals = nullptr;
+  if (B.materialized) {
+    LB.offsets.ensure((g.total + 1) * 8);
+    LB.values.ensure((g.bound ? g.bound : 1) * 4);
+    uint64_t got = 0;
+    check(ehx_intersect_nodes(ctx(), nops, dt, np, g.total, LB.offsets.as<uint64_t>(), LB.values.as<uint32_t>(),
+                              g.bound, &got),
+          "ehx_intersect_nodes");
+    res = LB.offsets.as<uint64_t>();
+    vals = LB.values.as<uint32_t>();
+  } else {
+    LB.counts.ensure(g.total * 8);
+    check(ehx_intersect_nodes_count(ctx(), nops, dt, np, g.total, LB.counts.as<uint64_t>()),
+          "ehx_intersect_nodes_count");
+    res = LB.counts.as<uint64_t>();
+  }
+  uint64_t b = 0;
+  for (uint64_t c = g.c0; c < g.c1; c++) {
+    LevelLog &L = S.logs[c][p];
+    L.res = res + b;
+    L.vals = vals;
+    b += L.ncalls;
+  }
+}
+
+// A count-only level is recorded in up to PIECES ranges of chunks: once a
+// range is recorded, its calls are gathered and its GPU batch runs on a helper
+// thread while the workers record the next range (batches run one at a time:
+// the helper of the previous range is joined first).  Materialized levels
+// need every offset of one batch, so they record all chunks first.
+static const uint64_t PIECES = 8, MIN_CHUNKS_PER_PIECE = 32;
+
+// Chunk bounds of the ranges of pass p, cut at equal shares of an estimate of
+// each chunk's calls at level p: the values its calls at level p-1 returned
+// (each value runs the loop body that makes level p's calls once), one per
+// chunk besides.
+static std::vector<uint64_t> piece_bounds(const Session &S, size_t p, uint64_t nchunks, uint64_t pieces) {
+  std::vector<uint64_t> w(nchunks, 1);
+  if (p > 0 && S.batches[p - 1].materialized)
+    for (uint64_t c = 0; c < nchunks; c++) {
+      const LevelLog &L = S.logs[c][p - 1];
+      if (L.ncalls && L.res) w[c] += L.res[L.ncalls] - L.res[0];
+    }
+  uint64_t total = 0;
+  for (uint64_t x : w) total += x;
+  std::vector<uint64_t> b(1, 0);
+  uint64_t acc = 0;
+  for (uint64_t c = 0; c < nchunks && b.size() < pieces; c++) {
+    acc += w[c];
+    if (acc * pieces >= total * b.size() && c + 1 - b.back() >= MIN_CHUNKS_PER_PIECE) b.push_back(c + 1);
+  }
+  if (b.back() != nchunks) b.push_back(nchunks);
+  return b;
+}
+
 void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
                 const std::function<void(size_t, uint32_t, uint32_t)> &body, OutRows *out) {
   if (g_sess) not_implemented("nested par_foreach_aggregate / par_foreach_builder");
@@ -786,90 +967,69 @@ void run_passes(size_t num_attrs, const Set<hybrid> *level0, size_t nthreads,
   }
   g_sess = &S;
   const bool prof = profiling();
+  Workers pool(nthreads);
   for (size_t p = 0; p < levels; p++) {
     S.record_level = (int)p;
     g_phase = 1;
     for (auto &cl : S.logs)
       for (auto &l : cl) l.cursor = 0;
-    const double t0 = prof ? now_ms() : 0;
-    run_chunks(level0, nthreads, body);
-    // one batch for the level, calls in chunk order
     LevelBatch &B = S.batches[p];
-    uint64_t total = 0;
-    for (auto &cl : S.logs) {
-      cl[p].base = total;
-      total += cl[p].ncalls;
-    }
-    const double t1 = prof ? now_ms() : 0;
-    if (total == 0) {
-      if (prof) fprintf(stderr, "[dropin] pass %zu: lambdas %.2f ms, no recorded calls\n", p, t1 - t0);
-      continue;
-    }
-    // operand nodes, operand-major, gathered from the chunk logs in parallel
-    // (each chunk fills its own range), with the bound on the results' size
-    LevelBuffers &LB = level_buffers(p);
-    const int nops = B.nops;
-    LB.nodes.ensure((size_t)nops * total * 4);
-    uint32_t *nodes = LB.nodes.as<uint32_t>();
-    std::atomic<uint64_t> next_chunk(0), bound(0);
-    auto gather = [&]() {
-      uint64_t local = 0;
-      for (;;) {
-        const uint64_t c = next_chunk.fetch_add(1);
-        if (c >= S.logs.size()) break;
-        LevelLog &L = S.logs[c][p];
-        for (uint64_t i = 0; i < L.ncalls; i++) {
-          uint64_t mn = ~0ull;
-          for (int s = 0; s < nops; s++) {
-            const uint32_t nd = L.nodes[i * nops + s];
-            nodes[(uint64_t)s * total + L.base + i] = nd;
-            if (B.materialized) mn = std::min(mn, card_of(B.tries[s], nd));
-          }
-          if (B.materialized) local += mn;
-        }
-        std::vector<uint32_t>().swap(L.nodes);
+    double t_lambdas = 0, t_gather = 0, t_wait = 0;
+    uint64_t handed = 0, calls = 0;  // chunks whose calls went to a batch
+    std::thread gpu;
+    // only the innermost builder level can be count-only (LevelBatch::innermost)
+    const std::vector<uint64_t> bounds = piece_bounds(S, p, nchunks, p + 1 == levels ? PIECES : 1);
+    const uint64_t pieces = bounds.size() - 1;
+    for (uint64_t k = 0; k < pieces; k++) {
+      const uint64_t c0 = bounds[k], c1 = bounds[k + 1];
+      double t = prof ? now_ms() : 0;
+      run_chunks(level0, pool, body, c0, c1);
+      if (prof) t_lambdas += now_ms() - t;
+      // the signature's first call has been recorded by now if the range had one
+      const bool count_only = S.signed_levels[p].load() && !B.materialized;
+      if (!count_only) continue;
+      t = prof ? now_ms() : 0;
+      // each range's results stay in its own buffers until the final replay
+      LevelBuffers &LB = level_buffers(p * PIECES + k);
+      const Gathered g = gather_calls(S, p, handed, c1, LB, pool);
+      if (prof) t_gather += now_ms() - t;
+      if (gpu.joinable()) {
+        const double tw = prof ? now_ms() : 0;
+        gpu.join();
+        if (prof) t_wait += now_ms() - tw;
       }
-      bound += local;
-    };
-    {
-      std::vector<std::thread> th;
-      for (size_t t = 1; t < nthreads; t++) th.emplace_back(gather);
-      gather();
-      for (auto &x : th) x.join();
+      calls += g.total;
+      handed = c1;
+      gpu = std::thread([&S, p, g]() { run_batch_gathered(S, p, g); });
     }
-    const double t2 = prof ? now_ms() : 0;
-    ehx_trie *dt[8];
-    const uint32_t *np[8];
-    for (int s = 0; s < nops; s++) {
-      dt[s] = B.tries[s]->dev;
-      np[s] = nodes + (uint64_t)s * total;
+    if (gpu.joinable()) {
+      const double tw = prof ? now_ms() : 0;
+      gpu.join();
+      if (prof) t_wait += now_ms() - tw;
     }
-    if (B.materialized) {
-      const uint64_t cap = bound.load();
-      LB.offsets.ensure((total + 1) * 8);
-      LB.values.ensure((cap ? cap : 1) * 4);
-      uint64_t got = 0;
-      check(ehx_intersect_nodes(ctx(), nops, dt, np, total, LB.offsets.as<uint64_t>(), LB.values.as<uint32_t>(), cap,
-                                &got),
-            "ehx_intersect_nodes");
-      B.offsets = LB.offsets.as<uint64_t>();
-      B.values = LB.values.as<uint32_t>();
-    } else {
-      LB.counts.ensure(total * 8);
-      check(ehx_intersect_nodes_count(ctx(), nops, dt, np, total, LB.counts.as<uint64_t>()),
-            "ehx_intersect_nodes_count");
-      B.counts = LB.counts.as<uint64_t>();
+    if (handed < nchunks) {  // a materialized level, or calls not handed out above: one batch
+      const double t = prof ? now_ms() : 0;
+      const Gathered g = gather_calls(S, p, handed, nchunks, level_buffers(p * PIECES), pool);
+      const double t2 = prof ? now_ms() : 0;
+      run_batch_gathered(S, p, g);
+      calls += g.total;
+      if (prof) {
+        t_gather += t2 - t;
+        t_wait += now_ms() - t2;
+      }
     }
     if (prof)
-      fprintf(stderr, "[dropin] pass %zu: lambdas %.2f ms, gather %.2f ms, GPU batch of %llu calls (%s) %.2f ms\n", p,
-              t1 - t0, t2 - t1, (unsigned long long)total, B.materialized ? "materialized" : "counts", now_ms() - t2);
+      fprintf(stderr,
+              "[dropin] pass %zu: lambdas %.2f ms, gather %.2f ms, GPU batches of %llu calls (%s) %.2f ms not "
+              "overlapped\n",
+              p, t_lambdas, t_gather, (unsigned long long)calls, B.materialized ? "materialized" : "counts", t_wait);
   }
   S.record_level = -1;
   g_phase = 2;
   for (auto &cl : S.logs)
     for (auto &l : cl) l.cursor = 0;
   const double t3 = prof ? now_ms() : 0;
-  run_chunks(level0, nthreads, body);
+  run_chunks(level0, pool, body, 0, nchunks);
   if (prof) fprintf(stderr, "[dropin] replay: lambdas %.2f ms\n", now_ms() - t3);
   g_sess = nullptr;
   g_phase = 0;
