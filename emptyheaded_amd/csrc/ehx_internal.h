@@ -85,7 +85,8 @@ struct ehx_trie {
   uint64_t npk = 0;
   // child lookup by head value (dense tries with a compact head domain):
   // nav[x] = {off, length, first chunk, chunks} of x's child list, zeros when x
-  // is not a head
+  // is not a head; for a list read as bitset words (NAV_WORDS set in .w),
+  // {off, length, word base, NAV_WORDS | first word}
   uint4 *nav = nullptr;
   uint32_t nav_len = 0;
   // child pointers into other tries (or this one), built on first use: for
@@ -128,6 +129,14 @@ static_assert(S_NUM_SLOTS <= (int)(sizeof(ehx_ctx::scratch) / sizeof(ehx_ctx::sc
 constexpr uint32_t PK_VALS = 15;
 // child lists shorter than this are not gap-coded
 constexpr uint32_t PK_MIN = 16;
+// A child list of at least PK_MIN values whose bitset window holds at most one
+// 64-bit word per NAV_WORDS_DENSITY values is read by the tuned level as those
+// words (AND + popcount against a staged bitmap, 64 values per word) instead
+// of value by value.  Its nav entry then carries NAV_WORDS in .w, the first
+// word index (first value >> 6) in .w's low bits and, in .z, the pool offset
+// of its words less that index (word w of the list is words[.z + w], mod 2^32).
+constexpr uint32_t NAV_WORDS = 0x80000000u;
+constexpr uint32_t NAV_WORDS_DENSITY = 4;
 
 struct DTrie {
   const uint32_t *head, *off, *vals;

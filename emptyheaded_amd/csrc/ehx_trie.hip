@@ -162,8 +162,13 @@ __global__ void k_pk_fill(DTrie t, uint64_t m, const uint64_t *__restrict__ pos,
 
 __global__ void k_nav(DTrie t, uint4 *__restrict__ nav) {
   for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h < t.nh; h += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t a = t.off[h], c = t.pk_off[h];
-    nav[t.head[h]] = make_uint4(a, t.off[h + 1] - a, c, t.pk_off[h + 1] - c);
+    const uint32_t a = t.off[h], n = t.off[h + 1] - a, wb = t.bs_off[h], nw = t.bs_off[h + 1] - wb;
+    if (nw && n >= PK_MIN && NAV_WORDS_DENSITY * nw <= n) {  // read as bitset words
+      nav[t.head[h]] = make_uint4(a, n, wb - t.bs_start[h], NAV_WORDS | t.bs_start[h]);
+    } else {
+      const uint32_t c = t.pk_off[h];
+      nav[t.head[h]] = make_uint4(a, n, c, t.pk_off[h + 1] - c);
+    }
   }
 }
 

@@ -144,6 +144,14 @@ __device__ __forceinline__ bool head_ok(const TunedPlan &P, const TunedArgs &A, 
 // Wave capacity of the hash table for a given LDS slot budget.
 __host__ __device__ constexpr uint32_t wave_hash_cap(int slots) { return (uint32_t)slots / 8; }
 
+// 32-bit words of the bitmap staging of a set with values in [first, last]:
+// it starts at the 64-bit word holding `first` (an even 32-bit word, so a
+// 64-bit bitset word maps onto one aligned 64-bit LDS word) and has an even
+// length.  The staging needs one more, zero, word after it.
+__host__ __device__ __forceinline__ uint32_t bitmap_words(uint32_t first, uint32_t last) {
+  return (((last >> 5) - ((first >> 6) << 1)) | 1u) + 1u;
+}
+
 // bucket[p] = bin | parts << 6; hist[b] counts queue entries (parts)
 // MAXS: S sets resolved per prefix (1 for single-S plans: no runtime-indexed array)
 template <int MAXS>
@@ -189,7 +197,7 @@ __global__ void k_classify(TunedPlan P, TunedArgs A, uint32_t *__restrict__ buck
       // the wave bin needs S(p) to fit a quarter of the LDS, as a hash table or as a bitmap
       bool fits_wave = D.n <= wave_hash_cap(slots);
       if (!fits_wave && P.ns == 1 && D.n)  // a wave can stage it as a bitmap
-        fits_wave = ((D.v[D.n - 1] >> 5) - (D.v[0] >> 5) + 1) < (uint32_t)slots / 4;
+        fits_wave = bitmap_words(D.v[0], D.v[D.n - 1]) < (uint32_t)slots / 4;
       const bool big = w >= (uint64_t)big_work || !fits_wave;
       uint32_t lg = 63 - __clzll(w | 1);
       b = (big ? 0u : WAVE_BIN) + (31 - (lg > 31 ? 31 : lg));
@@ -355,7 +363,9 @@ __device__ __forceinline__ uint32_t probe_chunk(const Stage &st, const DSet *S, 
 }
 
 // The child list of one x of E(p) held by a lane: values vals[o, o + d), and
-// its gap-coded chunks pk[ko, ko + kn) when streamed from that copy.
+// its gap-coded chunks pk[ko, ko + kn) when streamed from that copy; or, when
+// kn carries NAV_WORDS, its bitset words (ko: word base, kn's low bits: first
+// word index), which a bitmap staging reads instead of the values.
 struct XList {
   uint32_t o, d, ko, kn;
 };
@@ -384,7 +394,10 @@ __device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, 
       uint32_t cut = xh ? xh[idx] : 0u;
       r.o = nv.x;
       r.d = nv.y;
-      if (gap && r.d >= (uint32_t)big_deg && nv.w) {
+      if (nv.w & NAV_WORDS) {  // bitset words: the values, if read, are trimmed exactly
+        r.ko = nv.z;
+        r.kn = nv.w;
+      } else if (gap && r.d >= (uint32_t)big_deg && nv.w) {
         const uint32_t k = cut / PK_VALS;  // whole chunks below the parent value
         r.ko = nv.z + k;
         r.kn = nv.w - k;
@@ -407,7 +420,7 @@ __device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, 
       const uint4 nv = tx.nav[x];
       r.o = nv.x;
       r.d = nv.y;
-      if (gap && r.d >= (uint32_t)big_deg) {
+      if ((nv.w & NAV_WORDS) || (gap && r.d >= (uint32_t)big_deg)) {
         r.ko = nv.z;
         r.kn = nv.w;
       }
@@ -449,14 +462,65 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
     XList xi = first;
     if (base != wave * XW) xi = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, filter_s, mi, big_deg, gap, xl, xh);
     const uint32_t o = xi.o, d = xi.d, ko = xi.ko, kn = xi.kn;
+    const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
+    // long child sets held as bitset words, against a bitmap staging: 64 / G
+    // at a time, one per group of G lanes, each lane ANDing 64-bit words of the
+    // list with the staged bitmap's words over their common window (64 values
+    // per word, popcount of the AND)
+    const bool wl = MODE == 0 && MAXS == 1 && d >= (uint32_t)big_deg && d && (kn & NAV_WORDS);
+    if (MODE == 0 && MAXS == 1) {
+      const uint64_t wm = __ballot(wl);
+      if (wm) {
+        const uint32_t nwl = (uint32_t)__popcll(wm);
+        if (wl) own[wave_popc_below(wm)] = lane;
+        wave_sync();
+        const uint32_t s_lo = st.bm_base >> 1, s_hi = s_lo + (st.bm_words >> 1) - 1u;  // in 64-bit words
+        const uint2 *tab2 = reinterpret_cast<const uint2 *>(st.tab);                   // bm_base is even
+        for (uint32_t r0 = 0; r0 < nwl; r0 += lists) {
+          const int myl = (r0 + qt < nwl) ? (int)own[r0 + qt] : -1;
+          const int src = myl < 0 ? 0 : myl;
+          const uint32_t so = __shfl(o, src, 64), sd = __shfl(d, src, 64);
+          const uint32_t sko = __shfl(ko, src, 64), skn = __shfl(kn, src, 64);
+          uint32_t h32 = 0;
+          if (myl >= 0) {
+            // the list's words cover [first value >> 6, last value >> 6]
+            const uint32_t lo = max(skn & ~NAV_WORDS, s_lo), hi = min(tx.vals[so + sd - 1] >> 6, s_hi);
+            const uint32_t nk = hi >= lo ? hi - lo + 1u : 0u;
+            const uint64_t *wp = tx.words + (uint32_t)(sko + lo);
+            const uint2 *tp = tab2 + (lo - s_lo);
+            uint32_t k = ql;
+            for (; DEEP && k + 3 * G < nk; k += 4 * G) {  // four words in flight
+              const uint64_t g0 = wp[k], g1 = wp[k + G], g2 = wp[k + 2 * G], g3 = wp[k + 3 * G];
+              const uint2 t0 = tp[k], t1 = tp[k + G], t2 = tp[k + 2 * G], t3 = tp[k + 3 * G];
+              h32 += __popc((uint32_t)g0 & t0.x) + __popc((uint32_t)(g0 >> 32) & t0.y) +
+                     __popc((uint32_t)g1 & t1.x) + __popc((uint32_t)(g1 >> 32) & t1.y) +
+                     __popc((uint32_t)g2 & t2.x) + __popc((uint32_t)(g2 >> 32) & t2.y) +
+                     __popc((uint32_t)g3 & t3.x) + __popc((uint32_t)(g3 >> 32) & t3.y);
+            }
+            for (; k + G < nk; k += 2 * G) {  // two words in flight
+              const uint64_t g0 = wp[k], g1 = wp[k + G];
+              const uint2 t0 = tp[k], t1 = tp[k + G];
+              h32 += __popc((uint32_t)g0 & t0.x) + __popc((uint32_t)(g0 >> 32) & t0.y) +
+                     __popc((uint32_t)g1 & t1.x) + __popc((uint32_t)(g1 >> 32) & t1.y);
+            }
+            if (k < nk) {
+              const uint64_t g0 = wp[k];
+              const uint2 t0 = tp[k];
+              h32 += __popc((uint32_t)g0 & t0.x) + __popc((uint32_t)(g0 >> 32) & t0.y);
+            }
+          }
+          hits += h32;
+        }
+        wave_sync();  // own[] is reused below
+      }
+    }
     // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
     // streamed with 128-bit loads, two quads per lane in flight per step (the
     // x's of one prefix are adjacent ids, so the lengths in a round are alike)
-    const uint64_t big = __ballot(d >= (uint32_t)big_deg);
+    const uint64_t big = __ballot(d >= (uint32_t)big_deg && !wl);
     const uint32_t nbig = (uint32_t)__popcll(big);
-    const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
     // own[k] = the lane holding the k-th long set (the owner window is free until below)
-    if (d >= (uint32_t)big_deg) own[wave_popc_below(big)] = lane;
+    if (d >= (uint32_t)big_deg && !wl) own[wave_popc_below(big)] = lane;
     wave_sync();
     for (uint32_t r0 = 0; r0 < nbig; r0 += lists) {
       const int myl = (r0 + qt < nbig) ? (int)own[r0 + qt] : -1;
@@ -467,7 +531,7 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       uint32_t h32 = 0;  // this round's hits of the lane (< 2^32)
       // gap-coded: chunks of PK_VALS values, one per lane per step (single-S plans only:
       // the 4-clique's kernel keeps its registers for the materialized S(p))
-      if (MAXS == 1 && myl >= 0 && skn) {
+      if (MAXS == 1 && myl >= 0 && skn && !(skn & NAV_WORDS)) {
         const uint4 *q = tx.pk + 2ull * sko;
         const uint32_t nfull = bd / PK_VALS;
         uint32_t c = ql;
@@ -633,7 +697,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   st.bm_base = st.bm_words = st.hshift = st.hmask = st.fshift = 0;
   if (src) {
     const uint32_t first = use_desc ? dsc1.x : src[0], last = use_desc ? dsc1.y : src[sn - 1];
-    const uint32_t w32 = (last >> 5) - (first >> 5) + 1;
+    const uint32_t w32 = bitmap_words(first, last);
     const bool is_bitset = (P.ns == 1) ? (S[0].lay == EHX_RANGE_BITSET)
                                        : (sn > 2 && (uint64_t)(last - first) <= 128ull * sn);
     const bool bm_fits = w32 < tab_slots, hash_fits = sn <= tab_slots / 2 && last != EMPTY;  // bitmap + zero word
@@ -643,7 +707,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
     else if (hash_fits) st.mode = 1;
     else if (bm_fits) st.mode = 0;  // a bitmap still beats probing the sets in HBM
     if (st.mode == 0) {
-      st.bm_base = first >> 5;
+      st.bm_base = (first >> 6) << 1;
       st.bm_words = w32;
       for (uint32_t i = gtid; i <= w32; i += gsize) tab[i] = 0;  // tab[w32]: the zero word
       gsync();
@@ -651,7 +715,7 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
         // copy range_bitset words; 32-bit word i of the bitmap covers values [(bm_base + i) * 32, +32)
         for (uint32_t k = gtid; k < S[0].W; k += gsize) {
           const uint64_t wv = S[0].w[k];
-          const uint32_t i0 = (S[0].start + k) * 2u - st.bm_base;  // wraps when the low half precedes `first`
+          const uint32_t i0 = (S[0].start + k) * 2u - st.bm_base;  // the base is S's first 64-bit word
           if (i0 < w32) tab[i0] = (uint32_t)wv;
           if (i0 + 1u < w32) tab[i0 + 1u] = (uint32_t)(wv >> 32);
         }
@@ -952,7 +1016,7 @@ __global__ __launch_bounds__(256, MAXS == 1 ? 6 : (MAXS == 2 ? 5 : 2)) void k_tu
   constexpr uint32_t WS = SLOTS / 4;                 // per-wave table slots
   constexpr uint32_t FWB = SLOTS / 4, FWW = FWB / 4;  // filter words per block / wave
   constexpr uint32_t VC = (MAXS > 1) ? WS : 1u;      // materialized S capacity per wave
-  __shared__ uint32_t tab[SLOTS];
+  __shared__ __align__(16) uint32_t tab[SLOTS];  // 64-bit aligned bitmap words
   __shared__ uint32_t flt[FWB];
   __shared__ uint32_t own[4][OWN];
   __shared__ uint32_t sv[4 * VC];
