@@ -365,6 +365,70 @@ def test_trimmed_child_lists_any_orientation(ctx, oracle, seed):
         E.tune_set(**saved)
 
 
+def dense_block_graph(rng, blocks=24, width=300, p=0.3, extra=3000):
+    """Dense communities of `width` consecutive ids (edge probability p) plus
+    sparse random edges between them, both directions kept, no loops."""
+    src, dst = [], []
+    for j in range(blocks):
+        a = rng.integers(0, width, int(p * width * width)) + j * width
+        b = rng.integers(0, width, int(p * width * width)) + j * width
+        src.append(a)
+        dst.append(b)
+    n = blocks * width
+    src.append(rng.integers(0, n, extra))
+    dst.append(rng.integers(0, n, extra))
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    keep = src != dst
+    e = np.unique(np.stack([src[keep], dst[keep]], 1).astype(np.uint32), axis=0)
+    return e[:, 0].copy(), e[:, 1].copy()
+
+
+def lists_read_as_words(s, d):
+    """Child lists the tuned level reads as bitset words: at least 16 values,
+    a range_bitset by the density rule, at least 4 values per 64-bit word."""
+    heads, start = np.unique(s, return_index=True)
+    n = np.diff(np.append(start, len(s)))
+    first, last = d[start].astype(np.int64), d[start + n - 1].astype(np.int64)
+    words = (last >> 6) - (first >> 6) + 1
+    dense = (n > 2) & (last - first <= 128 * n)
+    return int(((n >= 16) & dense & (4 * words <= n)).sum())
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_dense_lists_read_as_words(ctx, oracle, seed):
+    """Dense child lists are read as bitset words ANDed with a bitmap staging
+    of S(p): lists inside one community, lists reaching into others (their
+    words overlap the staged window only in part) and staged windows starting
+    at any 32-bit word.  Counts equal the oracle's in both attribute orders,
+    both layouts that give words, every stream width, with and without gap
+    coding and with every set staged as a bitmap where it fits."""
+    rng = np.random.default_rng(seed)
+    s, d = dense_block_graph(rng)
+    assert lists_read_as_words(s, d) > 1000
+    o, r = s < d, s > d
+    so, do_ = s[o], d[o]  # oriented half for the (a, b, c) order
+    exp = oracle.join_count(plans.TRIANGLE, [oracle.Trie(so, do_)], 8)
+    exp_mid = oracle.join_count(plans.TRIANGLE_BY_MIDDLE, [oracle.Trie(s, d), oracle.Trie(d, s)], 8)
+    say("dense blocks", len(s), "rows; triangles", exp, exp_mid, "; lists read as words", lists_read_as_words(s, d))
+    saved = E.tune_get()
+    try:
+        for policy in (E.POLICY_RULE, E.POLICY_ALL_BITSET):
+            T = E.Trie(ctx, so, do_, policy)
+            T01, T10 = E.Trie(ctx, s, d, policy), E.Trie(ctx, d, s, policy)
+            for lanes in (2, 4, 64):
+                for gap, bitmap_all in ((1, 0), (0, 1)):
+                    E.tune_set(stream_lanes=lanes, gap_coded=gap, bitmap_all=bitmap_all, tiny_work=0, big_deg=16)
+                    for form in (E.FORM_TUNED, E.FORM_LEARNED):
+                        assert E.join_count(ctx, plans.TRIANGLE, [T], form) == exp, (policy, lanes, gap, form)
+                        c, st = E.join_count(ctx, plans.TRIANGLE_BY_MIDDLE, [T01, T10], form, stats=True)
+                        assert st["form_used"] == form
+                        assert c == exp_mid, (policy, lanes, gap, form)
+            for t in (T, T01, T10):
+                t.close()
+    finally:
+        E.tune_set(**saved)
+
+
 # ---- equality selections ------------------------------------------------------
 
 @pytest.mark.parametrize("form", FORMS)
