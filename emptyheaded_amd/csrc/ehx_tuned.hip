@@ -310,9 +310,6 @@ __device__ __forceinline__ uint32_t probe_m(const Stage &s, const DSet *S, int n
   return 1u;
 }
 
-// Probes the values of one gap-coded chunk (a, b): the first value, then 14
-// values reached by the 16-bit gaps; only values below `lim` count (the last
-// chunk of a list is zero-padded).
 // Bitmap probe of a value held relative to the bitmap's first value
 // (r = c - 32 * bm_base): one subtraction fewer per probe than probe_m<0>.
 // Bit c & 31 is bit r & 31, since the base is a multiple of 32; a value below
@@ -322,43 +319,53 @@ __device__ __forceinline__ uint32_t probe_rebased(const Stage &s, uint32_t r) {
   return __builtin_amdgcn_ubfe(s.tab[w], r, 1u);
 }
 
-template <int MODE, int MAXS, bool TAIL>
+// Probes the 15 values of one gap-coded chunk (a, b): the first value, then
+// the 14 reached by its 16-bit gaps; `last` receives the probe of the 15th.
+// The last chunk of a list is padded with zero gaps, so its padding repeats
+// the list's last value: the caller takes `last` once per padding slot off that
+// chunk's count.
+template <int MODE, int MAXS>
 __device__ __forceinline__ uint32_t probe_chunk(const Stage &st, const DSet *S, int ns, const uint4 a, const uint4 b,
-                                                uint32_t lim) {
+                                                uint32_t &last) {
   if (MODE == 0) {  // the running sum of the gaps is kept relative to the bitmap
     uint32_t r = a.x - (st.bm_base << 5);
-    uint32_t h = probe_rebased(st, r);
+    uint32_t h = probe_rebased(st, r), p = 0;
     const uint32_t w[7] = {a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int j = 0; j < 7; j++) {
       r += w[j] & 0xFFFFu;
-      if (!TAIL || 2u * j + 1u < lim) h += probe_rebased(st, r);
+      h += probe_rebased(st, r);
       r += w[j] >> 16;
-      if (!TAIL || 2u * j + 2u < lim) h += probe_rebased(st, r);
+      p = probe_rebased(st, r);
+      h += p;
     }
+    last = p;
     return h;
   }
   uint32_t v = a.x;
-  uint32_t h = probe_m<MODE, MAXS>(st, S, ns, v);
+  uint32_t h = probe_m<MODE, MAXS>(st, S, ns, v), p = 0;
   if (MODE != 2) {
     const uint32_t w[7] = {a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int j = 0; j < 7; j++) {
       v += w[j] & 0xFFFFu;
-      if (!TAIL || 2u * j + 1u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+      h += probe_m<MODE, MAXS>(st, S, ns, v);
       v += w[j] >> 16;
-      if (!TAIL || 2u * j + 2u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+      p = probe_m<MODE, MAXS>(st, S, ns, v);
+      h += p;
     }
   } else {  // probes of sets in HBM (rare): a rolled loop keeps the code small
 #pragma unroll 1
     for (uint32_t j = 0; j < 7; j++) {
       const uint32_t wv = j == 0 ? a.y : j == 1 ? a.z : j == 2 ? a.w : j == 3 ? b.x : j == 4 ? b.y : j == 5 ? b.z : b.w;
       v += wv & 0xFFFFu;
-      if (!TAIL || 2u * j + 1u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+      h += probe_m<MODE, MAXS>(st, S, ns, v);
       v += wv >> 16;
-      if (!TAIL || 2u * j + 2u < lim) h += probe_m<MODE, MAXS>(st, S, ns, v);
+      p = probe_m<MODE, MAXS>(st, S, ns, v);
+      h += p;
     }
   }
+  last = p;
   return h;
 }
 
@@ -533,18 +540,22 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
       // the 4-clique's kernel keeps its registers for the materialized S(p))
       if (MAXS == 1 && myl >= 0 && skn && !(skn & NAV_WORDS)) {
         const uint4 *q = tx.pk + 2ull * sko;
-        const uint32_t nfull = bd / PK_VALS;
-        uint32_t c = ql;
-        for (; UNROLL && c + G < nfull; c += 2 * G) {  // two chunks (30 values) in flight
+        // every chunk, the zero-padded last one too: its padding repeats the
+        // list's last value, probed `pad` times more than it occurs
+        const uint32_t nk = (bd + PK_VALS - 1) / PK_VALS, klast = nk - 1u, pad = nk * PK_VALS - bd;
+        uint32_t c = ql, corr = 0;
+        for (; UNROLL && c + G < nk; c += 2 * G) {  // two chunks (30 values) in flight
           const uint4 a0 = q[2 * c], b0 = q[2 * c + 1], a1 = q[2 * (c + G)], b1 = q[2 * (c + G) + 1];
-          h32 += probe_chunk<MODE, MAXS, false>(st, S, P.ns, a0, b0, 0) +
-                 probe_chunk<MODE, MAXS, false>(st, S, P.ns, a1, b1, 0);
+          uint32_t l0, l1;
+          h32 += probe_chunk<MODE, MAXS>(st, S, P.ns, a0, b0, l0) + probe_chunk<MODE, MAXS>(st, S, P.ns, a1, b1, l1);
+          if (c + G == klast) corr = l1;
         }
-        for (; c < nfull; c += G) h32 += probe_chunk<MODE, MAXS, false>(st, S, P.ns, q[2 * c], q[2 * c + 1], 0);
-        // exactly one lane of the group reaches c == nfull: the partial last chunk
-        const uint32_t rem = bd - nfull * PK_VALS;
-        if (c == nfull && rem) h32 += probe_chunk<MODE, MAXS, true>(st, S, P.ns, q[2 * c], q[2 * c + 1], rem);
-        hits += h32;
+        for (; c < nk; c += G) {
+          uint32_t l0;
+          h32 += probe_chunk<MODE, MAXS>(st, S, P.ns, q[2 * c], q[2 * c + 1], l0);
+          if (c == klast) corr = l0;
+        }
+        hits += h32 - pad * corr;  // only the lane that took the last chunk has corr != 0
         continue;
       }
       const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
