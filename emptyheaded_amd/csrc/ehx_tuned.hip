@@ -521,43 +521,59 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
         wave_sync();  // own[] is reused below
       }
     }
-    // long child sets: 64 / G at a time, one per group of G = 2^slg lanes,
-    // streamed with 128-bit loads, two quads per lane in flight per step (the
-    // x's of one prefix are adjacent ids, so the lengths in a round are alike)
-    const uint64_t big = __ballot(d >= (uint32_t)big_deg && !wl);
+    // long gap-coded child sets (single-S plans only: the 4-clique's kernel keeps
+    // its registers for the materialized S(p)): 64 / G at a time, one per group
+    // of G = 2^slg lanes, one chunk of PK_VALS values per lane per step, two in
+    // flight (the x's of one prefix are adjacent ids, so the lengths in a round
+    // are alike).  Rounds of coded lists only, so no group waits on another
+    // group's other kind of list.  Every chunk is probed whole, the zero-padded
+    // last one too: its padding repeats the list's last value, which is probed
+    // `pad` times more than it occurs and taken off again by the lane that took
+    // that chunk.
+    const bool cl = MAXS == 1 && d >= (uint32_t)big_deg && !wl && kn && !(kn & NAV_WORDS);
+    if (MAXS == 1) {
+      const uint64_t cm = __ballot(cl);
+      if (cm) {
+        const uint32_t ncl = (uint32_t)__popcll(cm);
+        if (cl) own[wave_popc_below(cm)] = lane;
+        wave_sync();
+        for (uint32_t r0 = 0; r0 < ncl; r0 += lists) {
+          const int myl = (r0 + qt < ncl) ? (int)own[r0 + qt] : -1;
+          const int src = myl < 0 ? 0 : myl;
+          const uint32_t sd = __shfl(d, src, 64), sko = __shfl(ko, src, 64);
+          const uint32_t bd = myl < 0 ? 0u : sd;
+          const uint4 *q = tx.pk + 2ull * sko;
+          const uint32_t nk = (bd + PK_VALS - 1) / PK_VALS, klast = nk - 1u, pad = nk * PK_VALS - bd;
+          uint32_t h32 = 0, corr = 0, c = ql;
+          for (; UNROLL && c + G < nk; c += 2 * G) {  // two chunks (30 values) in flight
+            const uint4 a0 = q[2 * c], b0 = q[2 * c + 1], a1 = q[2 * (c + G)], b1 = q[2 * (c + G) + 1];
+            uint32_t l0, l1;
+            h32 += probe_chunk<MODE, MAXS>(st, S, P.ns, a0, b0, l0) + probe_chunk<MODE, MAXS>(st, S, P.ns, a1, b1, l1);
+            if (c + G == klast) corr = l1;
+          }
+          for (; c < nk; c += G) {
+            uint32_t l0;
+            h32 += probe_chunk<MODE, MAXS>(st, S, P.ns, q[2 * c], q[2 * c + 1], l0);
+            if (c == klast) corr = l0;
+          }
+          hits += h32 - pad * corr;  // only the lane that took the last chunk has corr != 0
+        }
+        wave_sync();  // own[] is reused below
+      }
+    }
+    // other long child sets (not gap-coded): 64 / G at a time, one per group,
+    // streamed with 128-bit loads, two quads per lane in flight per step
+    const uint64_t big = __ballot(d >= (uint32_t)big_deg && !wl && !cl);
     const uint32_t nbig = (uint32_t)__popcll(big);
     // own[k] = the lane holding the k-th long set (the owner window is free until below)
-    if (d >= (uint32_t)big_deg && !wl) own[wave_popc_below(big)] = lane;
+    if (d >= (uint32_t)big_deg && !wl && !cl) own[wave_popc_below(big)] = lane;
     wave_sync();
     for (uint32_t r0 = 0; r0 < nbig; r0 += lists) {
       const int myl = (r0 + qt < nbig) ? (int)own[r0 + qt] : -1;
       const int src = myl < 0 ? 0 : myl;
       const uint32_t so = __shfl(o, src, 64), sd = __shfl(d, src, 64);
-      const uint32_t sko = __shfl(ko, src, 64), skn = __shfl(kn, src, 64);
       const uint32_t bo = so, bd = myl < 0 ? 0u : sd;
       uint32_t h32 = 0;  // this round's hits of the lane (< 2^32)
-      // gap-coded: chunks of PK_VALS values, one per lane per step (single-S plans only:
-      // the 4-clique's kernel keeps its registers for the materialized S(p))
-      if (MAXS == 1 && myl >= 0 && skn && !(skn & NAV_WORDS)) {
-        const uint4 *q = tx.pk + 2ull * sko;
-        // every chunk, the zero-padded last one too: its padding repeats the
-        // list's last value, probed `pad` times more than it occurs
-        const uint32_t nk = (bd + PK_VALS - 1) / PK_VALS, klast = nk - 1u, pad = nk * PK_VALS - bd;
-        uint32_t c = ql, corr = 0;
-        for (; UNROLL && c + G < nk; c += 2 * G) {  // two chunks (30 values) in flight
-          const uint4 a0 = q[2 * c], b0 = q[2 * c + 1], a1 = q[2 * (c + G)], b1 = q[2 * (c + G) + 1];
-          uint32_t l0, l1;
-          h32 += probe_chunk<MODE, MAXS>(st, S, P.ns, a0, b0, l0) + probe_chunk<MODE, MAXS>(st, S, P.ns, a1, b1, l1);
-          if (c + G == klast) corr = l1;
-        }
-        for (; c < nk; c += G) {
-          uint32_t l0;
-          h32 += probe_chunk<MODE, MAXS>(st, S, P.ns, q[2 * c], q[2 * c + 1], l0);
-          if (c == klast) corr = l0;
-        }
-        hits += h32 - pad * corr;  // only the lane that took the last chunk has corr != 0
-        continue;
-      }
       const uint32_t head = min((4u - (bo & 3u)) & 3u, bd);
       for (uint32_t j = ql; j < head; j += G) h32 += probe_m<MODE, MAXS>(st, S, P.ns, tx.vals[bo + j]);
       const uint4 *q = reinterpret_cast<const uint4 *>(tx.vals + bo + head);  // 16-byte aligned
