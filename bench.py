@@ -229,10 +229,13 @@ def bench_selections(E, ctx, steps, cpu=True):
         meta = json.load(f)["facebook_duplicated"]
     s, d = z["src"], z["dst"]
     T, R = E.Trie(ctx, s, d), E.Trie(ctx, d, s)
+    from emptyheaded_amd import tuning as TU
     out = {"workload": "facebook_duplicated (176,468 rows, both directions): FliqueSelAgg x=0, BarbellSelAgg p=6"}
-    for name, levels, key in (("flique_sel_x0", plans.flique_sel(0), "FliqueSelAgg_x0"),
-                              ("barbell_sel_p6", plans.barbell_sel(6), "BarbellSelAgg_p6")):
-        r = {"expected": meta[key]}
+    # each under the knobs of the plan its tuned levels have the shape of
+    for name, levels, key, knobs in (("flique_sel_x0", plans.flique_sel(0), "FliqueSelAgg_x0", "four_clique"),
+                                     ("barbell_sel_p6", plans.barbell_sel(6), "BarbellSelAgg_p6", "triangle")):
+        TU.apply_plan(knobs)
+        r = {"expected": meta[key], "knobs": knobs}
         for fname, form in (("plain", E.FORM_PLAIN), ("tuned", E.FORM_TUNED), ("learned", E.FORM_LEARNED)):
             c, ms, _, _ = time_form(E, ctx, T, levels, form, steps, 1, tries=[T, R])
             assert c == meta[key], "%s %s: %d" % (name, fname, c)
