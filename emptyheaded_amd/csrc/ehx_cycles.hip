@@ -42,12 +42,29 @@ __device__ __forceinline__ uint32_t c4_hash(uint32_t x) {
 
 __device__ __forceinline__ uint64_t pairs_of(uint32_t k) { return (uint64_t)k * (k - (k ? 1u : 0u)) / 2; }
 
-// Wedge bound of a: sum of deg(b) over b = vals[b0..b1), summed by the wave.
-__device__ __forceinline__ uint64_t c4_wedge_bound(const DTrie &S, uint32_t b0, uint32_t b1) {
+// The relation linked to itself (LINK): for the element at position e of
+// vals, lnk[e] = its child list {off, length, ...} and hnt[e] = how many values
+// of that list lie below the head value of the set holding e.  For b at
+// position e of a's set, N(b)'s endpoints below a are then vals[lnk[e].x, +
+// hnt[e]): two loads at e instead of a value -> node lookup and a binary
+// search, each a dependent load.
+struct C4Link {
+  const uint4 *lnk;
+  const uint32_t *hnt;
+};
+
+// Wedge bound of a over its lower neighbors b = vals[b0..b1), summed by the
+// wave: the endpoints below a (LINK), or deg(b).
+template <bool LINK>
+__device__ __forceinline__ uint64_t c4_wedge_bound(const DTrie &S, const C4Link &L, uint32_t b0, uint32_t b1) {
   uint64_t t = 0;
   for (uint32_t bi = b0 + __lane_id(); bi < b1; bi += 64) {
-    const int32_t hb = node_of(S, S.vals[bi]);
-    if (hb >= 0) t += S.off[hb + 1] - S.off[hb];
+    if (LINK) {
+      t += L.hnt[bi];
+    } else {
+      const int32_t hb = node_of(S, S.vals[bi]);
+      if (hb >= 0) t += S.off[hb + 1] - S.off[hb];
+    }
   }
   return wave_sum(t);
 }
@@ -58,16 +75,22 @@ __device__ __forceinline__ uint64_t c4_wedge_bound(const DTrie &S, uint32_t b0, 
 // lists are streamed by groups of G lanes, 64/G lists at a time, G sized to
 // about four endpoints per lane per list (adjacent b's have similar degrees),
 // with four loads in flight per lane.
-template <class F>
-__device__ __forceinline__ void c4_wedges64(const DTrie &S, uint32_t a, uint32_t b0, uint32_t b1, F f) {
+template <bool LINK, class F>
+__device__ __forceinline__ void c4_wedges64(const DTrie &S, const C4Link &L, uint32_t a, uint32_t b0, uint32_t b1,
+                                            F f) {
   const uint32_t lane = __lane_id();
   const uint32_t bi = b0 + lane;
   uint32_t p0 = 0, len = 0;
   if (bi < b1) {
-    const int32_t hb = node_of(S, S.vals[bi]);
-    if (hb >= 0) {
-      p0 = S.off[hb];
-      len = lower_bound(S.vals, p0, S.off[hb + 1], a) - p0;
+    if (LINK) {
+      p0 = L.lnk[bi].x;
+      len = L.hnt[bi];
+    } else {
+      const int32_t hb = node_of(S, S.vals[bi]);
+      if (hb >= 0) {
+        p0 = S.off[hb];
+        len = lower_bound(S.vals, p0, S.off[hb + 1], a) - p0;
+      }
     }
   }
   const uint32_t nl = b1 - b0;
@@ -109,7 +132,8 @@ __device__ __forceinline__ bool c4_insert(uint32_t *K, uint32_t *Cn, uint32_t ma
 }
 
 // One wavefront per vertex a with a small wedge bound; larger ones are queued.
-__global__ __launch_bounds__(256) void k_c4_wave(DTrie S, uint64_t begin, uint64_t end, SegOut seg,
+template <bool LINK>
+__global__ __launch_bounds__(256) void k_c4_wave(DTrie S, C4Link L, uint64_t begin, uint64_t end, SegOut seg,
                                                  unsigned long long *__restrict__ acc, uint32_t *__restrict__ big,
                                                  uint32_t *__restrict__ ctl) {
   __shared__ uint32_t keys[4][C4_SLOTS];
@@ -124,7 +148,7 @@ __global__ __launch_bounds__(256) void k_c4_wave(DTrie S, uint64_t begin, uint64
     const uint32_t o0 = S.off[i];
     const uint32_t lb = lower_bound(S.vals, o0, S.off[i + 1], a);
     if (lb == o0) continue;
-    const uint64_t bound = c4_wedge_bound(S, o0, lb);
+    const uint64_t bound = c4_wedge_bound<LINK>(S, L, o0, lb);
     if (bound < 2) continue;
     if (bound > C4_WAVE_MAX) {
       if (lane == 0) big[atomicAdd(&ctl[C4_BIG_COUNT], 1u)] = (uint32_t)i;
@@ -138,8 +162,8 @@ __global__ __launch_bounds__(256) void k_c4_wave(DTrie S, uint64_t begin, uint64
     }
     wave_sync();
     for (uint32_t b0 = o0; b0 < lb; b0 += 64)
-      c4_wedges64(S, a, b0, min(b0 + 64, lb),
-                  [&](uint32_t c) { lost |= !c4_insert(K, Cn, ts - 1, c); });
+      c4_wedges64<LINK>(S, L, a, b0, min(b0 + 64, lb),
+                        [&](uint32_t c) { lost |= !c4_insert(K, Cn, ts - 1, c); });
     wave_sync();  // every lane's counts are in before they are summed
     uint64_t s = 0;
     for (uint32_t j = lane; j < ts; j += 64) s += pairs_of(Cn[j]);
@@ -172,7 +196,8 @@ __device__ __forceinline__ uint32_t block_min(uint32_t v, unsigned long long *re
 }
 
 // One workgroup per queued vertex.
-__global__ __launch_bounds__(256) void k_c4_block(DTrie S, const uint32_t *__restrict__ big, uint32_t *__restrict__ ctl,
+template <bool LINK>
+__global__ __launch_bounds__(256) void k_c4_block(DTrie S, C4Link L, const uint32_t *__restrict__ big, uint32_t *__restrict__ ctl,
                                                   uint32_t *__restrict__ gkeys, uint32_t *__restrict__ gcnts,
                                                   SegOut seg, unsigned long long *__restrict__ acc) {
   __shared__ uint32_t H[C4_DENSE];
@@ -196,10 +221,18 @@ __global__ __launch_bounds__(256) void k_c4_block(DTrie S, const uint32_t *__res
     uint64_t part = 0;
     uint32_t lowest = C4_EMPTY;
     for (uint32_t bi = o0 + threadIdx.x; bi < lb; bi += 256) {
-      const int32_t hb = node_of(S, S.vals[bi]);
-      if (hb >= 0 && S.off[hb + 1] > S.off[hb]) {
-        part += S.off[hb + 1] - S.off[hb];
-        lowest = min(lowest, S.vals[S.off[hb]]);
+      if (LINK) {
+        const uint32_t h = L.hnt[bi];
+        if (h) {
+          part += h;
+          lowest = min(lowest, S.vals[L.lnk[bi].x]);
+        }
+      } else {
+        const int32_t hb = node_of(S, S.vals[bi]);
+        if (hb >= 0 && S.off[hb + 1] > S.off[hb]) {
+          part += S.off[hb + 1] - S.off[hb];
+          lowest = min(lowest, S.vals[S.off[hb]]);
+        }
       }
     }
     const uint64_t bound = block_sum(part, red);
@@ -211,8 +244,8 @@ __global__ __launch_bounds__(256) void k_c4_block(DTrie S, const uint32_t *__res
       for (uint32_t j = threadIdx.x; j < R; j += 256) H[j] = 0;
       __syncthreads();
       for (uint32_t b0 = o0 + 64 * w; b0 < lb; b0 += 256)
-        c4_wedges64(S, a, b0, min(b0 + 64, lb),
-                    [&](uint32_t c) { atomicAdd(&H[c - lowest], 1u); });
+        c4_wedges64<LINK>(S, L, a, b0, min(b0 + 64, lb),
+                          [&](uint32_t c) { atomicAdd(&H[c - lowest], 1u); });
       __syncthreads();
       uint64_t t = 0;
       for (uint32_t j = threadIdx.x; j < R; j += 256) t += pairs_of(H[j]);
@@ -231,7 +264,7 @@ __global__ __launch_bounds__(256) void k_c4_block(DTrie S, const uint32_t *__res
         __threadfence();
         __syncthreads();
         for (uint32_t b0 = o0 + 64 * w; b0 < lb; b0 += 256)
-          c4_wedges64(S, a, b0, min(b0 + 64, lb), [&](uint32_t c) {
+          c4_wedges64<LINK>(S, L, a, b0, min(b0 + 64, lb), [&](uint32_t c) {
             if (P == 1 || c4_hash(c ^ 0x5BD1E995u) % P == p) lost |= !c4_insert(K, Cn, ts - 1, c);
           });
         __threadfence();
@@ -271,11 +304,22 @@ int cycle4_count(ehx_ctx *ctx, const ehx_trie *t, uint64_t begin, uint64_t end, 
   if ((rc = fill_u64(ctx, ac.as<uint64_t>(), 0, 8)) || (rc = fill_u32(ctx, cb.as<uint32_t>(), 0, C4_CTL_WORDS))) return rc;
   const DTrie D = view(t);
   const SegOut so = seg ? *seg : SegOut{};
+  // the relation's child pointers into itself with their hints (built once and
+  // cached on the trie, a derived index like its child lookup); without hints
+  // (none kept, or no room) the kernels look children up and search instead
+  C4Link L = {nullptr, nullptr};
+  if ((rc = trie_link(ctx, const_cast<ehx_trie *>(t), t, &L.lnk, &L.hnt))) return rc;
+  const bool link = L.lnk && L.hnt;
   EHX_HIP(hipEventRecord(ctx->ev[0], st));
   EHX_HIP(hipEventRecord(ctx->ev[EHX_EV_K0], st));
   if (n) {
-    k_c4_wave<<<grid_for(n * 64, 256, (unsigned)ctx->num_cus * 8), 256, 0, st>>>(
-        D, begin, end, so, ac.as<unsigned long long>(), ob.as<uint32_t>(), cb.as<uint32_t>());
+    const unsigned g = grid_for(n * 64, 256, (unsigned)ctx->num_cus * 8);
+    if (link)
+      k_c4_wave<true><<<g, 256, 0, st>>>(D, L, begin, end, so, ac.as<unsigned long long>(), ob.as<uint32_t>(),
+                                         cb.as<uint32_t>());
+    else
+      k_c4_wave<false><<<g, 256, 0, st>>>(D, L, begin, end, so, ac.as<unsigned long long>(), ob.as<uint32_t>(),
+                                          cb.as<uint32_t>());
     EHX_HIP(hipGetLastError());
   }
   uint32_t ctl[C4_CTL_WORDS];
@@ -287,8 +331,12 @@ int cycle4_count(ehx_ctx *ctx, const ehx_trie *t, uint64_t begin, uint64_t end, 
     const unsigned grid = ctl[C4_BIG_COUNT] < gmax ? ctl[C4_BIG_COUNT] : gmax;
     DevBuf &kb = ctx->scratch[S_SORT_A], &nb = ctx->scratch[S_SORT_B];
     if ((rc = kb.ensure((uint64_t)grid * C4_GSLOTS * 4)) || (rc = nb.ensure((uint64_t)grid * C4_GSLOTS * 4))) return rc;
-    k_c4_block<<<grid, 256, 0, st>>>(D, ob.as<uint32_t>(), cb.as<uint32_t>(), kb.as<uint32_t>(), nb.as<uint32_t>(), so,
-                                     ac.as<unsigned long long>());
+    if (link)
+      k_c4_block<true><<<grid, 256, 0, st>>>(D, L, ob.as<uint32_t>(), cb.as<uint32_t>(), kb.as<uint32_t>(),
+                                             nb.as<uint32_t>(), so, ac.as<unsigned long long>());
+    else
+      k_c4_block<false><<<grid, 256, 0, st>>>(D, L, ob.as<uint32_t>(), cb.as<uint32_t>(), kb.as<uint32_t>(),
+                                              nb.as<uint32_t>(), so, ac.as<unsigned long long>());
     EHX_HIP(hipGetLastError());
     if ((rc = copy_d2h(ctx, ctl, cb.p, sizeof(ctl)))) return rc;
     if (ctl[C4_ERR]) return fail(EHX_ERR_DEVICE, "4-cycle count: endpoint table overflow");

@@ -207,12 +207,14 @@ def test_cycle4_matches_oracle(ctx, oracle, scale):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("spread", [1, 12])
+@pytest.mark.parametrize("spread", [1, 12, 2000])
 def test_cycle4_hub_paths(ctx, oracle, spread):
     """Vertices past the wavefront bin: a dense core plus a wide star.  With
     spread 1 the hub's endpoint window fits the dense LDS counters; with ids
     spread out 12x it does not, and the hub goes through the global table in
-    two hash-partitioned passes."""
+    two hash-partitioned passes.  Spread 2000 leaves the ids too sparse for a
+    child lookup by value, so the kernels search each neighbor's list instead
+    of reading the relation's child pointers into itself."""
     rng = np.random.default_rng(11)
     core = 2000
     edges = set()
