@@ -377,10 +377,14 @@ struct XList {
   uint32_t o, d, ko, kn;
 };
 
-// A wave takes XW elements of E(p) per step: 64 on its own; 16 inside a
-// workgroup, so the four waves share even a small E(p) evenly.
+// A wave takes xw elements of E(p) per step: 64 on its own; inside a
+// workgroup 16, so the four waves share even a small E(p) evenly, or 64 when
+// E(p) holds at least XW_WIDE_MIN elements (a step's fixed costs -- the
+// ballots, the LDS rank tables, the short lists' window -- then serve four
+// times as many lists).
+constexpr uint32_t XW_WIDE_MIN = 256;
 template <bool BLOCK>
-__device__ __forceinline__ uint32_t x_width() { return BLOCK ? 16u : 64u; }
+__device__ __forceinline__ uint32_t x_width(uint32_t xn) { return (BLOCK && xn < XW_WIDE_MIN) ? 16u : 64u; }
 
 // The lane's x at position base + lane of E(p), and its child list; xl: E(p)'s
 // child pointers (A.xlink at E(p)'s offset), or null; xh: their hints
@@ -389,11 +393,11 @@ __device__ __forceinline__ uint32_t x_width() { return BLOCK ? 16u : 64u; }
 // parent value (at that value's chunk when streamed gap-coded).
 template <bool BLOCK, int MAXS>
 __device__ __forceinline__ XList x_list(const TunedPlan &P, const TunedArgs &A, const DSet *S, const uint32_t *xsrc,
-                                        uint32_t xn, uint32_t base, bool filter_s, int mi, int big_deg, int gap,
-                                        const uint4 *xl, const uint32_t *xh) {
+                                        uint32_t xn, uint32_t base, uint32_t xw, bool filter_s, int mi, int big_deg,
+                                        int gap, const uint4 *xl, const uint32_t *xh) {
   const DTrie &tx = A.tx;
   const uint32_t lane = __lane_id(), idx = base + lane;
-  bool ok = lane < x_width<BLOCK>() && idx < xn;
+  bool ok = lane < xw && idx < xn;
   XList r = {0u, 0u, 0u, 0u};
   if (xl && !filter_s && A.nh == 0) {  // one 16-byte load, the value itself not needed
     if (ok) {
@@ -461,13 +465,13 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
   const uint32_t wave = BLOCK ? (threadIdx.x >> 6) : 0u;
   const uint32_t nwaves = BLOCK ? (blockDim.x >> 6) : 1u;
   uint64_t hits = 0;
-  constexpr uint32_t XW = BLOCK ? 16u : 64u;
+  const uint32_t XW = x_width<BLOCK>(xn);
   // several loads in flight per lane for the LDS stagings; sets probed in HBM
   // (rare) stay one quad (chunk) per step, which keeps the kernel's code small
   constexpr bool UNROLL = MODE != 2;
   for (uint32_t base = wave * XW; base < xn; base += nwaves * XW) {
     XList xi = first;
-    if (base != wave * XW) xi = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, filter_s, mi, big_deg, gap, xl, xh);
+    if (base != wave * XW) xi = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, base, XW, filter_s, mi, big_deg, gap, xl, xh);
     const uint32_t o = xi.o, d = xi.d, ko = xi.ko, kn = xi.kn;
     const uint32_t G = 1u << slg, qt = lane >> slg, ql = lane & (G - 1u), lists = 64u >> slg;
     // long child sets held as bitset words, against a bitmap staging: 64 / G
@@ -700,7 +704,6 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   // one S set: E(p) is known now, so the first step's child lookups are issued
   // ahead of the staging
   XList first = {0u, 0u, 0u, 0u};
-  const uint32_t wave_x0 = wave * x_width<BLOCK>();
   const bool x_in_d = P.nd || use_desc;  // E(p) is Dd
   const uint4 *xl = (use_desc && A.xlink) ? A.xlink + dsc0.z : nullptr;
   const uint32_t *xh = nullptr;
@@ -713,7 +716,8 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   if (MAXS == 1) {
     const uint32_t *xs = x_in_d ? Dd.v : src;
     const uint32_t xn1 = x_in_d ? Dd.n : sn;
-    first = x_list<BLOCK, MAXS>(P, A, S, xs, xn1, wave_x0, false, 0, big_deg, gap, xl, xh);
+    const uint32_t xw = x_width<BLOCK>(xn1);
+    first = x_list<BLOCK, MAXS>(P, A, S, xs, xn1, wave * xw, xw, false, 0, big_deg, gap, xl, xh);
   }
   // staging decision (group-uniform)
   Stage st;
@@ -783,7 +787,10 @@ __device__ __forceinline__ uint64_t tuned_prefix(const TunedPlan &P, const Tuned
   const uint32_t *xsrc = x_in_d ? Dd.v : (src ? src : Smi.v);
   const uint32_t xn = x_in_d ? Dd.n : (src ? sn : Smi.n);
   const bool filter_s = !P.nd && !src;  // x runs over one S set: test it against the others
-  if (MAXS > 1) first = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, wave_x0, filter_s, mi, big_deg, gap, nullptr, nullptr);
+  if (MAXS > 1) {
+    const uint32_t xw = x_width<BLOCK>(xn);
+    first = x_list<BLOCK, MAXS>(P, A, S, xsrc, xn, wave * xw, xw, filter_s, mi, big_deg, gap, nullptr, nullptr);
+  }
   uint64_t hits;
   // one streaming loop per staging mode: no per-probe branch on the mode
   if (st.mode == 0)
