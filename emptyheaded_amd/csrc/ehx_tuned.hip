@@ -478,8 +478,8 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
     // at a time, one per group of G lanes, each lane ANDing 64-bit words of the
     // list with the staged bitmap's words over their common window (64 values
     // per word, popcount of the AND)
-    const bool wl = MODE == 0 && MAXS == 1 && d >= (uint32_t)big_deg && d && (kn & NAV_WORDS);
-    if (MODE == 0 && MAXS == 1) {
+    const bool wl = MODE == 0 && d >= (uint32_t)big_deg && d && (kn & NAV_WORDS);
+    if (MODE == 0) {
       const uint64_t wm = __ballot(wl);
       if (wm) {
         const uint32_t nwl = (uint32_t)__popcll(wm);
