@@ -109,6 +109,8 @@ SIGNATURES = {
     "ehx_refdb_close": ([_vp], C.c_int),
     "ehx_trie_orient": ([_vp, _vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), _u64p], C.c_int),
     "ehx_trie_link": ([_vp, _vp, _vp, C.POINTER(C.c_int)], C.c_int),
+    "ehx_trie_link_hints": ([_vp, _vp, _vp, C.c_int32, C.POINTER(C.c_uint32), C.c_uint64, C.POINTER(C.c_int)],
+                            C.c_int),
     "ehx_trie_load_reference": ([_vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
     "ehx_encoded_open": ([C.c_char_p, C.POINTER(_vp)], C.c_int),
     "ehx_trie_build_encoded": ([_vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)], C.c_int),
@@ -231,6 +233,22 @@ class Trie:
         ok = C.c_int(0)
         check(lib().ehx_trie_link(self.ctx.h, self.h, target.h, C.byref(ok)))
         return bool(ok.value)
+
+    def link_hints(self, target, route="auto"):
+        """(hints, state): the link's trim hints (uint32 per row: how many values
+        of the row's child set in `target` lie below the row's first column)
+        and how they were found (0 no link, 1 none kept, 2 searched, 3 from the
+        transposition sort).  route ("auto", "search" or "sort") applies when
+        the link is built by this call."""
+        routes = {"auto": 0, "search": 1, "sort": 2}
+        if route not in routes:
+            raise ValueError("route must be 'auto', 'search' or 'sort', not %r" % (route,))
+        n = int(self.num_rows)
+        out = np.zeros(n, dtype=np.uint32)
+        st = C.c_int(0)
+        check(lib().ehx_trie_link_hints(self.ctx.h, self.h, target.h, routes[route],
+                                        out.ctypes.data_as(C.POINTER(C.c_uint32)), n, C.byref(st)))
+        return out, st.value
 
     def orient(self, policy=POLICY_RULE, build=True):
         """(lower, reverse, counts): the rows x < y as a device trie and its

@@ -283,6 +283,15 @@ int ehx_trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int *linked)
   return rc;
 }
 
+int ehx_trie_link_hints(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int32_t route, uint32_t *hints_out,
+                        uint64_t n, int *state) {
+  if (!ctx || !src || !dst || (n && !hints_out) || !state) return fail(EHX_ERR_INVALID, "null argument");
+  if (src->device != ctx->device || dst->device != ctx->device)
+    return fail(EHX_ERR_INVALID, "trie lives on another device");
+  EHX_HIP(hipSetDevice(ctx->device));
+  return trie_link_hints(ctx, src, dst, route, hints_out, n, state);
+}
+
 int ehx_sssp_unweighted_dir(ehx_ctx *ctx, const ehx_trie *t, const ehx_trie *rev, uint32_t start, int32_t iterations,
                             uint32_t *dist_out, uint64_t nd, uint32_t *levels, int32_t bottom_up) {
   if (!ctx || !t || !rev || !dist_out) return fail(EHX_ERR_INVALID, "null argument");

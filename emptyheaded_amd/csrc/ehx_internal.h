@@ -98,11 +98,14 @@ struct ehx_trie {
   // device had no room
   // work[h]: the sum over set h of this trie of its elements' child-set
   // lengths: the tuned level's binning reads it instead of sampling
+  // transposed: the hints started from candidates of one sort (the target
+  // looked like this trie's reverse ordering) rather than a search each
   struct Link {
     uint64_t target;
     uint4 *arr;
     uint32_t *hint;
     uint64_t *work;
+    int transposed;
   };
   std::vector<Link> links;
   uint64_t uid = 0;  // unique per process, never reused
@@ -248,10 +251,20 @@ int scan_u32(ehx_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
 int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits);
 int reduce_max_u32(ehx_ctx *ctx, const uint32_t *in, uint64_t n, uint32_t *host_out);
 int fill_u32(ehx_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
-// src's child pointers into dst (built and cached on src on first use); null in
-// *out when dst has no child lookup by value
+// How a link finds its hints: chosen from a sample (LINK_AUTO), one search per
+// element, or one sort of src by value whenever every sampled element finds its
+// parent value in dst (hints checked against dst and searched where that fails).
+enum { LINK_AUTO = 0, LINK_SEARCH = 1, LINK_SORT = 2 };
+// src's child pointers into dst (built and cached on src on first use; the
+// route applies to that first build); null in *out when dst has no child
+// lookup by value
 int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out,
-              const uint32_t **hint_out = nullptr, const uint64_t **work_out = nullptr);
+              const uint32_t **hint_out = nullptr, const uint64_t **work_out = nullptr, int route = LINK_AUTO);
+// Builds (once) the link and copies its hints for src's n = m rows to host
+// (zeros without hints); *state: 0 no link, 1 no hints (all zero), 2 searched
+// hints, 3 hints from the sort.
+int trie_link_hints(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int route, uint32_t *host, uint64_t n,
+                    int *state);
 // The loaded per-pair table on the current device (null: none).
 const uint8_t *pair_table_current();
 int pair_table_load(int device, const uint8_t *table);

@@ -172,51 +172,124 @@ __global__ void k_nav(DTrie t, uint4 *__restrict__ nav) {
   }
 }
 
-__global__ void k_link(const uint32_t *__restrict__ vals, uint64_t m, DTrie dst, uint4 *__restrict__ arr) {
+// Child pointers: arr[e] = dst's nav entry of element e's value.  With keys
+// (the transposed hint build), also keys[e] = (e << 32) | value, and mx[0]
+// takes the largest value (a stable sort on the low bits then keeps e
+// ascending per value).
+template <bool KEYS>
+__global__ void k_link(const uint32_t *__restrict__ vals, uint64_t m, DTrie dst, uint4 *__restrict__ arr,
+                       uint64_t *__restrict__ keys, unsigned *__restrict__ mx) {
+  uint32_t v = 0;
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t x = vals[e];
     arr[e] = (x < dst.nav_len) ? dst.nav[x] : make_uint4(0, 0, 0, 0);
+    if (KEYS) {
+      keys[e] = (e << 32) | x;
+      v = max(v, x);
+    }
+  }
+  if (KEYS) {
+    for (int d = 32; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    if (__lane_id() == 0) atomicMax(mx, v);
   }
 }
 
-// hint[e] = number of values of element e's child set in dst (arr[e]) below
-// e's parent value in src: a binary search per element, a wavefront per src set.
-// any[0] = 1 when some hint is nonzero.
+// Sampled test of whether src is dst's reverse ordering, and how far apart the
+// child sets that one set's search reads lie.  Sample s is element e = s * m / ns
+// = (y, x): found[0] += 1 when y is in x's child set in dst; when e + 1 is in
+// the same set, found[1] += 1 and gap[0] += the distance (in rows of dst,
+// capped at LINK_GAP_CAP) between the child sets of e and e + 1.
+constexpr uint32_t LINK_GAP_CAP = 1u << 24;
+__global__ void k_link_probe(DTrie src, uint64_t m, DTrie dst, uint32_t ns, unsigned *__restrict__ found,
+                             unsigned long long *__restrict__ gap) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const uint32_t e = (uint32_t)((uint64_t)s * m / ns);
+  // the set holding e: the last h with off[h] <= e
+  uint32_t lo = 0, hi = src.nh;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (src.off[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t y = src.head[lo], x = src.vals[e];
+  const uint4 c = x < dst.nav_len ? dst.nav[x] : make_uint4(0, 0, 0, 0);
+  const uint32_t i = lower_bound(dst.vals, c.x, c.x + c.y, y);
+  if (i < c.x + c.y && dst.vals[i] == y) atomicAdd(&found[0], 1u);
+  if (e + 1 < src.off[lo + 1]) {
+    const uint32_t x2 = src.vals[e + 1];
+    const uint32_t o2 = x2 < dst.nav_len ? dst.nav[x2].x : 0u;
+    atomicAdd(&found[1], 1u);
+    atomicAdd(gap, (unsigned long long)min(o2 > c.x ? o2 - c.x : c.x - o2, LINK_GAP_CAP));
+  }
+}
+
+// Hints from the keys sorted by value.  Sorted position k holds src's element
+// e = (y, x); when dst's element k lies in x's child set and its value v has e
+// in its own set of src (so y = v), dst holds y at rank k - (x's child-set
+// offset) and that rank is the hint.  Every other element gets LINK_UNSET,
+// which k_link_hint replaces by a search.
+constexpr uint32_t LINK_UNSET = 0xFFFFFFFFu;
+__global__ void k_link_cand(const uint64_t *__restrict__ keys, uint64_t m, DTrie src, DTrie dst,
+                            uint32_t *__restrict__ hint) {
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = keys[k];
+    const uint32_t x = (uint32_t)key, e = (uint32_t)(key >> 32);
+    uint32_t r = LINK_UNSET;
+    if (x < dst.nav_len) {
+      const uint4 nx = dst.nav[x];
+      const uint32_t rr = (uint32_t)k - nx.x;
+      if (rr < nx.y) {
+        const uint32_t v = dst.vals[k];
+        if (v < src.nav_len) {
+          const uint4 nv = src.nav[v];
+          if (e - nv.x < nv.y) r = rr;
+        }
+      }
+    }
+    hint[e] = r;
+  }
+}
+
+// One wavefront per set h of src.  hint (when not null): hint[e] = number of
+// values of element e's child set in dst (arr[e]) below the set's value y,
+// kept where the transposed build already found it (cand) and binary-searched
+// elsewhere; any[0] = 1 when some hint is nonzero.  work (when not null):
+// work[h] = the sum of the set's child-set lengths (whole lengths, not less
+// the hints: binned by the trimmed lengths, the R-MAT-22 middle-vertex query
+// ran 5.45 ms at its best knobs against 5.11 ms).
 __global__ void k_link_hint(DTrie src, const uint32_t *__restrict__ dvals, const uint4 *__restrict__ arr,
-                            uint32_t *__restrict__ hint, unsigned *__restrict__ any) {
+                            uint32_t *__restrict__ hint, unsigned *__restrict__ any, int cand,
+                            uint64_t *__restrict__ work) {
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
   bool some = false;
   for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < src.nh; h += nwaves) {
     const uint32_t y = src.head[h], a1 = src.off[h + 1];
+    unsigned long long w = 0;
     for (uint32_t e = src.off[h] + __lane_id(); e < a1; e += 64) {
       const uint4 c = arr[e];
-      uint32_t lo = 0, hi = c.y;
-      if (hi && dvals[c.x] >= y) hi = 0;  // the whole set lies at or above y (the common case of
-                                          // a relation linked to itself in ascending orientation)
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (dvals[c.x + mid] < y) lo = mid + 1;
-        else hi = mid;
+      w += c.y;
+      if (!hint) continue;
+      uint32_t r = cand ? hint[e] : LINK_UNSET;
+      if (r == LINK_UNSET) {
+        uint32_t lo = 0, hi = c.y;
+        if (hi && dvals[c.x] >= y) hi = 0;  // the whole set lies at or above y (the common case of
+                                            // a relation linked to itself in ascending orientation)
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (dvals[c.x + mid] < y) lo = mid + 1;
+          else hi = mid;
+        }
+        hint[e] = r = lo;
       }
-      hint[e] = lo;
-      some |= lo != 0;
+      some |= r != 0;
+    }
+    if (work) {
+      w = wave_sum(w);
+      if (__lane_id() == 0) work[h] = w;
     }
   }
-  if (__any(some) && __lane_id() == 0) atomicOr(any, 1u);
-}
-
-// work[h] = sum over set h of src of its elements' child-set lengths in dst:
-// one wavefront per set.  (Whole lengths, not less the hints: binned by the
-// trimmed lengths, the R-MAT-22 middle-vertex query ran 5.45 ms at its best
-// knobs against 5.11 ms.)
-__global__ void k_link_work(DTrie src, const uint4 *__restrict__ arr, uint64_t *__restrict__ work) {
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t h = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; h < src.nh; h += nwaves) {
-    unsigned long long w = 0;
-    for (uint32_t e = src.off[h] + __lane_id(); e < src.off[h + 1]; e += 64) w += arr[e].y;
-    w = wave_sum(w);
-    if (__lane_id() == 0) work[h] = w;
-  }
+  if (hint && __any(some) && __lane_id() == 0) atomicOr(any, 1u);
 }
 
 __global__ void k_dense(const uint32_t *__restrict__ head, uint64_t nh, int32_t *__restrict__ dense) {
@@ -445,8 +518,41 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   return EHX_OK;
 }
 
+// Transposed hint build: taken (automatic route) for relations of at least
+// LINK_SORT_MIN rows whose LINK_SAMPLES sampled elements all find their parent
+// value in their child set (dst looks like src's reverse ordering), when the
+// child sets that consecutive elements of one set read lie at least
+// LINK_SORT_GAP rows of dst apart on average.  Searches closer together stay in
+// cache and beat the sort: the ego-network batch cut from R-MAT-22, renumbered
+// net by net, averages 393 rows (R->T) and 1,548 (its symmetric relation
+// linked to itself), R-MAT-22 linked to its reverse 172,897, R-MAT-18 17,118.
+static constexpr uint64_t LINK_SORT_MIN = 1u << 16;
+static constexpr uint32_t LINK_SAMPLES = 1024;
+static constexpr uint64_t LINK_SORT_GAP = 1u << 14;
+
+// The link's own flag words (a caller may hold live data in the other scratch
+// slots): [0] samples found, [1] samples with a successor, [2, 3] their gap
+// sum, [4] largest value of src, [5] some hint nonzero.
+enum { LF_FOUND = 0, LF_PAIRS = 1, LF_GAP = 2, LF_MAX = 4, LF_ANY = 5, LF_WORDS = 6 };
+
+static int link_sort_route(ehx_ctx *ctx, const ehx_trie *src, const ehx_trie *dst, unsigned *fl, int route,
+                           bool *sort) {
+  *sort = false;
+  if (route == LINK_SEARCH || !src->nav || !dst->nav || (route == LINK_AUTO && src->m < LINK_SORT_MIN)) return EHX_OK;
+  k_link_probe<<<(LINK_SAMPLES + 255) / 256, 256, 0, ctx->stream>>>(view(src), src->m, view(dst), LINK_SAMPLES, fl,
+                                                                  reinterpret_cast<unsigned long long *>(fl + LF_GAP));
+  EHX_HIP(hipGetLastError());
+  unsigned f[4] = {0, 0, 0, 0};
+  int rc = copy_d2h(ctx, f, fl, sizeof(f));
+  if (rc) return rc;
+  const uint64_t gap = (uint64_t)f[LF_GAP] | ((uint64_t)f[LF_GAP + 1] << 32);
+  if (f[LF_FOUND] != LINK_SAMPLES) return EHX_OK;
+  *sort = route == LINK_SORT || (f[LF_PAIRS] && gap >= LINK_SORT_GAP * f[LF_PAIRS]);
+  return EHX_OK;
+}
+
 int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **out, const uint32_t **hint_out,
-              const uint64_t **work_out) {
+              const uint64_t **work_out, int route) {
   *out = nullptr;
   if (hint_out) *hint_out = nullptr;
   if (work_out) *work_out = nullptr;
@@ -459,59 +565,97 @@ int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **ou
       if (work_out) *work_out = l.work;
       return EHX_OK;
     }
+  DevBuf &fb = ctx->scratch[S_FLAGS];
+  int rc;
+  if ((rc = fb.ensure(64)) || (rc = fill_u32(ctx, fb.as<uint32_t>(), 0u, LF_WORDS))) return rc;
+  unsigned *fl = fb.as<unsigned>();
+  bool sort = false;
+  if ((rc = link_sort_route(ctx, src, dst, fl, route, &sort))) return rc;
   uint4 *arr = nullptr;
-  hipError_t e = hipMalloc((void **)&arr, src->m * 16);
-  if (e != hipSuccess) {
+  uint32_t *hint = nullptr;
+  uint64_t *work = nullptr, *keys = nullptr;
+  // everything allocated before the first launch that writes it; a failed
+  // allocation drops what it would hold (no hints: child sets are streamed
+  // whole; no work sums: the binning samples the lists; no keys: every hint
+  // is searched)
+  if (hipMalloc((void **)&arr, src->m * 16) != hipSuccess) {
     (void)hipGetLastError();
     return EHX_OK;  // no room: the level looks children up by value instead
   }
-  k_link<<<grid_for(src->m, 256, 16384), 256, 0, ctx->stream>>>(src->vals, src->m, view(dst), arr);
-  EHX_HIP(hipGetLastError());
-  // hints, kept only when some child set starts below its element's parent
-  // value (for a relation linked to itself in ascending orientation none does)
-  uint32_t *hint = nullptr;
-  DevBuf &fb = ctx->scratch[S_FLAGS];
-  int rc = fb.ensure(64);
-  if (rc) {
-    (void)hipFree(arr);
-    return rc;
+  if (hipMalloc((void **)&hint, src->m * 4) != hipSuccess) {
+    (void)hipGetLastError();
+    hint = nullptr;
   }
-  if (hipMalloc((void **)&hint, src->m * 4) == hipSuccess) {
-    if ((rc = fill_u32(ctx, fb.as<uint32_t>(), 0u, 1))) {
-      (void)hipFree(arr);
-      (void)hipFree(hint);
-      return rc;
+  if (hipMalloc((void **)&work, src->nh * 8 + 8) != hipSuccess) {
+    (void)hipGetLastError();
+    work = nullptr;
+  }
+  if (sort && hint && hipMalloc((void **)&keys, src->m * 16) != hipSuccess) {
+    (void)hipGetLastError();
+    keys = nullptr;
+  }
+  hipError_t e = hipSuccess;
+  const unsigned g = grid_for(src->m, 256, 16384);
+  if (keys) k_link<true><<<g, 256, 0, ctx->stream>>>(src->vals, src->m, view(dst), arr, keys, fl + LF_MAX);
+  else k_link<false><<<g, 256, 0, ctx->stream>>>(src->vals, src->m, view(dst), arr, nullptr, nullptr);
+  e = hipGetLastError();
+  int cand = 0;
+  if (e == hipSuccess && keys) {
+    unsigned mx = 0;
+    if (!(rc = copy_d2h(ctx, &mx, fl + LF_MAX, 4)) &&
+        !(rc = radix_sort_u64(ctx, keys, keys + src->m, src->m, mx ? 32 - __builtin_clz(mx) : 1))) {
+      k_link_cand<<<g, 256, 0, ctx->stream>>>(keys, src->m, view(src), view(dst), hint);
+      e = hipGetLastError();
+      cand = 1;
     }
+  }
+  if (e == hipSuccess && !rc && (hint || work)) {
     k_link_hint<<<grid_for(src->nh * 64, 256, 16384), 256, 0, ctx->stream>>>(view(src), dst->vals, arr, hint,
-                                                                            fb.as<unsigned>());
-    EHX_HIP(hipGetLastError());
-    unsigned any = 0;
-    if ((rc = copy_d2h(ctx, &any, fb.p, 4))) {
-      (void)hipFree(arr);
-      (void)hipFree(hint);
-      return rc;
-    }
-    if (!any) {
-      EHX_HIP(hipFree(hint));
-      hint = nullptr;
-    }
-  } else {
-    (void)hipGetLastError();
-    hint = nullptr;  // no room: child sets are streamed whole
+                                                                            fl + LF_ANY, cand, work);
+    e = hipGetLastError();
   }
-  uint64_t *work = nullptr;
-  if (hipMalloc((void **)&work, src->nh * 8 + 8) == hipSuccess) {
-    k_link_work<<<grid_for(src->nh * 64, 256, 16384), 256, 0, ctx->stream>>>(view(src), arr, work);
-    EHX_HIP(hipGetLastError());
-  } else {
-    (void)hipGetLastError();
-    work = nullptr;  // no room: the binning samples the lists instead
+  unsigned any = 0;
+  if (e == hipSuccess && !rc && hint) rc = copy_d2h(ctx, &any, fl + LF_ANY, 4);
+  if (e == hipSuccess && !rc) e = hipStreamSynchronize(ctx->stream);  // other contexts may use it next
+  if (keys) (void)hipFree(keys);
+  if (e != hipSuccess || rc) {
+    (void)hipFree(arr);
+    if (hint) (void)hipFree(hint);
+    if (work) (void)hipFree(work);
+    return rc ? rc : fail(EHX_ERR_DEVICE, "trie link: %s", hipGetErrorString(e));
   }
-  EHX_HIP(hipStreamSynchronize(ctx->stream));  // other contexts may use it next
-  src->links.push_back({dst->uid, arr, hint, work});
+  // hints are kept only when some child set starts below its element's parent
+  // value (for a relation linked to itself in ascending orientation none does)
+  if (hint && !any) {
+    (void)hipFree(hint);
+    hint = nullptr;
+  }
+  src->links.push_back({dst->uid, arr, hint, work, hint ? cand : 0});
   *out = arr;
   if (hint_out) *hint_out = hint;
   if (work_out) *work_out = work;
+  return EHX_OK;
+}
+
+int trie_link_hints(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int route, uint32_t *host, uint64_t n,
+                    int *state) {
+  if (n != src->m) return fail(EHX_ERR_INVALID, "hint buffer holds %llu values, the trie %llu rows",
+                               (unsigned long long)n, (unsigned long long)src->m);
+  if (route != LINK_AUTO && route != LINK_SEARCH && route != LINK_SORT)
+    return fail(EHX_ERR_INVALID, "link route %d", route);
+  const uint4 *arr = nullptr;
+  const uint32_t *hint = nullptr;
+  int rc = trie_link(ctx, src, dst, &arr, &hint, nullptr, route);
+  if (rc) return rc;
+  int transposed = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_link_mu);
+    for (const auto &l : src->links)
+      if (l.target == dst->uid) transposed = l.transposed;
+  }
+  *state = !arr ? 0 : !hint ? 1 : transposed ? 3 : 2;
+  if (hint) return copy_d2h(ctx, host, hint, n * 4);
+  memset(host, 0, n * 4);
   return EHX_OK;
 }
 

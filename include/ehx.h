@@ -146,6 +146,18 @@ int ehx_trie_node_info(ehx_ctx *ctx, const ehx_trie *t, uint8_t *types, uint32_t
  * link exists, 0 when dst has no value-indexed child table or the device had
  * no room (queries then look children up by value). */
 int ehx_trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int *linked);
+/* Diagnostic, no counterpart in the reference (whose child pointers need no
+ * hints): builds the link as ehx_trie_link does and copies its trim hints to
+ * hints_out[n], n = src's row count: hint[e] = how many values of row e's child
+ * set in dst lie below row e's first-column value (zeros when the link keeps
+ * none).  route, for a link not built yet: 0 as ehx_trie_link chooses, 1 one
+ * search per row, 2 one sort of src by value whenever every sampled row finds
+ * its first-column value in its child set (dst looks like src's reverse
+ * ordering; each hint is checked against dst and searched where that fails).
+ * *state: 0 no link, 1 no hints kept (all zero), 2 searched hints, 3 hints
+ * from the sort. */
+int ehx_trie_link_hints(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, int32_t route, uint32_t *hints_out,
+                        uint64_t n, int *state);
 /* Child lookup for a batch of keys (TrieBlock::get_next_block(data),
  * storage_engine/include/trie/TrieBlock.hpp:86-111): node index or -1. */
 int ehx_trie_lookup(ehx_ctx *ctx, const ehx_trie *t, const uint32_t *keys, uint64_t n,
