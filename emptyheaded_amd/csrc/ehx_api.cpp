@@ -118,6 +118,7 @@ int ehx_ctx_destroy(ehx_ctx *ctx) {
   tuned_forget_graphs();
   query_graph_forget();
   for (auto &b : ctx->scratch) b.release();
+  upload_release(ctx);
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(ctx->stream);

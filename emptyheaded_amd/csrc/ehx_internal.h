@@ -55,12 +55,20 @@ uint64_t devbuf_generation();
 #define EHX_EV_K0 (2 * EHX_MAX_ATTRS + 2)
 #define EHX_EV_K1 (2 * EHX_MAX_ATTRS + 3)
 
+// Pinned host staging for large uploads (upload_h2d): UPLOAD_THREADS host
+// threads, each with two UPLOAD_CHUNK-byte pinned buffers and an event per
+// buffer; allocated on the first large upload, freed with the context.
+constexpr int UPLOAD_THREADS = 8;
+constexpr size_t UPLOAD_CHUNK = 8u << 20;
+
 struct ehx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[2 * EHX_MAX_ATTRS + 4] = {};
   ehx::DevBuf scratch[28];
   int num_cus = 256;
+  void *pin[UPLOAD_THREADS][2] = {};
+  hipEvent_t pin_ev[UPLOAD_THREADS][2] = {};
 };
 
 struct ehx_trie {
@@ -271,6 +279,12 @@ int pair_table_load(int device, const uint8_t *table);
 int fill_u64(ehx_ctx *ctx, uint64_t *p, uint64_t v, uint64_t n);
 int copy_d2h(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
 int copy_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
+// Enqueues the copy of `bytes` of pageable host memory to dst on the context's
+// stream; returns once src may be reused (the copy may still be in flight).
+// Large copies go through the context's pinned staging, several host threads
+// filling it while earlier chunks are on the bus.
+int upload_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes);
+void upload_release(ehx_ctx *ctx);
 
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 20) {
   uint64_t g = (n + block - 1) / block;

@@ -9,7 +9,11 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <string.h>
+
 #include <atomic>
+#include <thread>
+#include <vector>
 
 #include "ehx_internal.h"
 
@@ -288,6 +292,65 @@ int copy_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes) {
   EHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   EHX_HIP(hipStreamSynchronize(ctx->stream));
   return EHX_OK;
+}
+
+int upload_h2d(ehx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (bytes < 2 * UPLOAD_CHUNK) {
+    if (bytes) EHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return EHX_OK;
+  }
+  if (!ctx->pin[0][0]) {
+    for (int t = 0; t < UPLOAD_THREADS; t++)
+      for (int b = 0; b < 2; b++) {
+        hipError_t e = hipHostMalloc(&ctx->pin[t][b], UPLOAD_CHUNK, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->pin_ev[t][b], hipEventDisableTiming);
+        if (e != hipSuccess) {
+          (void)hipGetLastError();
+          upload_release(ctx);  // no pinned memory: the runtime stages the copy itself
+          EHX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+          return EHX_OK;
+        }
+      }
+  }
+  // thread t copies bytes [lo_t, hi_t) through its two buffers in turn, waiting
+  // for a buffer's previous copy (this call's or an earlier one's) before refilling it
+  const size_t per = (bytes + UPLOAD_THREADS - 1) / UPLOAD_THREADS;
+  std::vector<hipError_t> err(UPLOAD_THREADS, hipSuccess);
+  auto work = [&](int t) {
+    const size_t lo = std::min(bytes, per * t), hi = std::min(bytes, lo + per);
+    err[t] = hipSetDevice(ctx->device);  // a new thread starts on device 0
+    int b = 0;
+    for (size_t o = lo; o < hi && err[t] == hipSuccess; o += UPLOAD_CHUNK, b ^= 1) {
+      const size_t n = std::min(UPLOAD_CHUNK, hi - o);
+      hipError_t e = hipEventSynchronize(ctx->pin_ev[t][b]);
+      if (e == hipSuccess) {
+        memcpy(ctx->pin[t][b], (const char *)src + o, n);
+        e = hipMemcpyAsync((char *)dst + o, ctx->pin[t][b], n, hipMemcpyHostToDevice, ctx->stream);
+      }
+      if (e == hipSuccess) e = hipEventRecord(ctx->pin_ev[t][b], ctx->stream);
+      err[t] = e;
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < UPLOAD_THREADS; t++) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+  for (hipError_t e : err)
+    if (e != hipSuccess) return fail(EHX_ERR_DEVICE, "upload: %s", hipGetErrorString(e));
+  return EHX_OK;
+}
+
+void upload_release(ehx_ctx *ctx) {
+  for (int t = 0; t < UPLOAD_THREADS; t++)
+    for (int b = 0; b < 2; b++) {
+      if (ctx->pin_ev[t][b]) {
+        (void)hipEventSynchronize(ctx->pin_ev[t][b]);
+        (void)hipEventDestroy(ctx->pin_ev[t][b]);
+      }
+      if (ctx->pin[t][b]) (void)hipHostFree(ctx->pin[t][b]);
+      ctx->pin_ev[t][b] = nullptr;
+      ctx->pin[t][b] = nullptr;
+    }
 }
 
 // Bumped whenever a context buffer is freed or moves: a captured graph that

@@ -350,8 +350,7 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   if (!on_device) {
     DevBuf &i0 = ctx->scratch[S_IN0], &i1 = ctx->scratch[S_IN1];
     if ((rc = i0.ensure(rows * 4)) || (rc = i1.ensure(rows * 4))) return rc;
-    EHX_HIP(hipMemcpyAsync(i0.p, c0, rows * 4, hipMemcpyHostToDevice, st));
-    EHX_HIP(hipMemcpyAsync(i1.p, c1, rows * 4, hipMemcpyHostToDevice, st));
+    if ((rc = upload_h2d(ctx, i0.p, c0, rows * 4)) || (rc = upload_h2d(ctx, i1.p, c1, rows * 4))) return rc;
     d0 = i0.as<uint32_t>();
     d1 = i1.as<uint32_t>();
   }
