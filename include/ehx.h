@@ -109,9 +109,12 @@ int ehx_version(void);
 const char *ehx_last_error(void);
 /* Number of visible GPUs (0 on a machine without one). */
 int ehx_device_count(int *count);
-/* A context owns a HIP stream and grow-only scratch on one GPU.  Replaces the
- * reference's thread_pool::initializeThreadPool (src/utils/thread_pool.cpp:116-130)
- * as the per-process execution resource. */
+/* A context owns a HIP stream and grow-only scratch on one GPU (and, once it
+ * has uploaded a large relation, pinned host staging).  Calls on one context
+ * come from one host thread at a time; threads that work concurrently each
+ * take their own context.  Replaces the reference's
+ * thread_pool::initializeThreadPool (src/utils/thread_pool.cpp:116-130) as the
+ * per-process execution resource. */
 int ehx_ctx_create(int device, ehx_ctx **out);
 int ehx_ctx_destroy(ehx_ctx *ctx);
 int ehx_ctx_synchronize(ehx_ctx *ctx);
