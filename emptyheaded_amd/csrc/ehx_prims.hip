@@ -190,8 +190,8 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t *__restrict__
   }
 }
 
-int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits) {
-  if (n <= 1 || bits <= 0) return EHX_OK;
+int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits, int from) {
+  if (n <= 1 || bits <= from) return EHX_OK;
   if (n >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "radix sort: more than 2^32 keys");
   uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
   uint64_t hn = (uint64_t)ntiles * 256;
@@ -200,9 +200,9 @@ int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int 
   if (rc) return rc;
   uint32_t *hist = hb.as<uint32_t>(), *ofs = hist + hn + 1;
   uint64_t *src = keys, *dst = tmp;
-  int passes = (bits + 7) / 8;
+  const int passes = (bits - from + 7) / 8;
   for (int p = 0; p < passes; p++) {
-    int shift = 8 * p;
+    const int shift = from + 8 * p;
     k_rs_hist<<<ntiles, 256, 0, ctx->stream>>>(src, n, shift, hist, ntiles);
     EHX_HIP(hipGetLastError());
     rc = scan_u32(ctx, hist, ofs, hn);

@@ -24,10 +24,25 @@
 
 namespace ehx {
 
+// keys[i] = (c0[i] << b1) | c1[i]; order[0] = 1 when some row's key exceeds
+// the next row's, order[1] = 1 when some row's c1 exceeds the next row's.
 __global__ void k_pack(const uint32_t *__restrict__ c0, const uint32_t *__restrict__ c1, uint64_t n, int b1,
-                       uint64_t *__restrict__ keys) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-    keys[i] = ((uint64_t)c0[i] << b1) | (uint64_t)c1[i];
+                       uint64_t *__restrict__ keys, unsigned *__restrict__ order) {
+  bool down = false, down1 = false;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = ((uint64_t)c0[i] << b1) | (uint64_t)c1[i];
+    keys[i] = k;
+    if (i + 1 < n) {
+      const uint32_t x1 = c1[i + 1];
+      down |= k > (((uint64_t)c0[i + 1] << b1) | (uint64_t)x1);
+      down1 |= c1[i] > x1;
+    }
+  }
+  // a flag already set is not set again: unordered rows would otherwise send
+  // one atomic per wavefront to the same word
+  const volatile unsigned *seen = order;
+  if (__any(down) && __lane_id() == 0 && !seen[0]) atomicOr(&order[0], 1u);
+  if (__any(down1) && __lane_id() == 0 && !seen[1]) atomicOr(&order[1], 1u);
 }
 
 // flag[i] = 1 when keys[i] starts a new run of (keys >> shift).
@@ -366,9 +381,16 @@ int trie_build(ehx_ctx *ctx, const uint32_t *c0, const uint32_t *c1, uint64_t ro
   uint32_t *flags = fl.as<uint32_t>();
   uint64_t *pos = ps.as<uint64_t>();
   const unsigned G = grid_for(rows, 256, 16384);
-  k_pack<<<G, 256, 0, st>>>(d0, d1, rows, b1, keys);
+  // rows already in order need no sort; rows whose second column never
+  // decreases need only the passes over the first column's bits (a stable sort
+  // on them keeps the second column ascending within each first-column value)
+  unsigned *order = reinterpret_cast<unsigned *>(pos);
+  if ((rc = fill_u32(ctx, order, 0u, 2))) return rc;
+  k_pack<<<G, 256, 0, st>>>(d0, d1, rows, b1, keys, order);
   EHX_HIP(hipGetLastError());
-  if ((rc = radix_sort_u64(ctx, keys, tmp, rows, b0 + b1))) return rc;
+  unsigned ord[2] = {1, 1};
+  if ((rc = copy_d2h(ctx, ord, order, 8))) return rc;
+  if (ord[0] && (rc = radix_sort_u64(ctx, keys, tmp, rows, b0 + b1, ord[1] ? 0 : b1))) return rc;
   // unique rows
   k_run_flags<<<G, 256, 0, st>>>(keys, rows, 0, flags);
   EHX_HIP(hipGetLastError());

@@ -49,6 +49,39 @@ def test_trie_build_matches_oracle(ctx, oracle, policy):
     assert np.array_equal(nodes[: len(heads)], np.arange(len(heads))) and (nodes[len(heads):] == -1).all()
 
 
+@pytest.mark.parametrize("order", ["sorted", "by_second", "shuffled", "one_swap_first", "one_swap_last"])
+def test_trie_build_row_orders(ctx, oracle, order):
+    """The build skips its sort for rows already in order and sorts only the
+    first column's bits for rows whose second column never decreases; every
+    row order gives the oracle's trie.  The swapped cases are sorted rows with
+    one adjacent pair exchanged at either end."""
+    rng = np.random.default_rng(17)
+    src, dst = random_relation(rng, 3000, 200000, 70000)
+    dst[:40000] = rng.integers(0, 5000, size=40000)  # dense sets -> bitsets
+    src = np.concatenate([src, src[:1000]]).astype(np.uint32)  # repeated rows
+    dst = np.concatenate([dst, dst[:1000]]).astype(np.uint32)
+    if order == "by_second":
+        o = np.lexsort((src, dst))  # second column ascending, first column not
+    elif order == "shuffled":
+        o = rng.permutation(len(src))
+    else:
+        o = np.lexsort((dst, src))
+        if order == "one_swap_first":
+            o[[0, 1]] = o[[1, 0]]
+        elif order == "one_swap_last":
+            o[[-2, -1]] = o[[-1, -2]]
+    src, dst = src[o], dst[o]
+    say("trie build row order", order)
+    T = E.Trie(ctx, src, dst)
+    O = oracle.Trie(src, dst, E.POLICY_RULE)
+    a, b = T.tuples()
+    oa, ob = O.tuples()
+    assert np.array_equal(a, oa) and np.array_equal(b, ob)
+    types, cards = T.node_info()
+    otypes, ocards = O.node_info(len(types))
+    assert np.array_equal(cards, ocards) and np.array_equal(types, otypes)
+
+
 def test_trie_build_large_keys(ctx, oracle):
     rng = np.random.default_rng(3)
     src = rng.integers(0, 2**32, size=30000, dtype=np.uint64).astype(np.uint32)
