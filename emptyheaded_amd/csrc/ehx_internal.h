@@ -257,8 +257,10 @@ int scan_u32(ehx_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n);
 // Stable LSD radix sort of n 64-bit keys on their low `bits` bits; keys[] holds
 // the result, tmp[] has n entries.
 // Keys already sorted on their low `from` bits need only the passes over bits
-// [from, bits).
-int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits, int from = 0);
+// [from, bits).  With `result`, the sorted keys are left in whichever of keys
+// and tmp the last pass wrote, and *result points there.
+int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits, int from = 0,
+                   uint64_t **result = nullptr);
 int reduce_max_u32(ehx_ctx *ctx, const uint32_t *in, uint64_t n, uint32_t *host_out);
 int fill_u32(ehx_ctx *ctx, uint32_t *p, uint32_t v, uint64_t n);
 // How a link finds its hints: chosen from a sample (LINK_AUTO), one search per

@@ -190,7 +190,8 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint64_t *__restrict__
   }
 }
 
-int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits, int from) {
+int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int bits, int from, uint64_t **result) {
+  if (result) *result = keys;
   if (n <= 1 || bits <= from) return EHX_OK;
   if (n >= (1ull << 32)) return fail(EHX_ERR_UNSUPPORTED, "radix sort: more than 2^32 keys");
   uint32_t ntiles = (uint32_t)((n + RS_TILE - 1) / RS_TILE);
@@ -211,7 +212,8 @@ int radix_sort_u64(ehx_ctx *ctx, uint64_t *keys, uint64_t *tmp, uint64_t n, int 
     EHX_HIP(hipGetLastError());
     uint64_t *t = src; src = dst; dst = t;
   }
-  if (src != keys) EHX_HIP(hipMemcpyAsync(keys, src, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  if (result) *result = src;
+  else if (src != keys) EHX_HIP(hipMemcpyAsync(keys, src, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
   return EHX_OK;
 }
 

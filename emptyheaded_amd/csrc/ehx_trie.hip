@@ -623,9 +623,10 @@ int trie_link(ehx_ctx *ctx, ehx_trie *src, const ehx_trie *dst, const uint4 **ou
   int cand = 0;
   if (e == hipSuccess && keys) {
     unsigned mx = 0;
+    uint64_t *sorted = keys;
     if (!(rc = copy_d2h(ctx, &mx, fl + LF_MAX, 4)) &&
-        !(rc = radix_sort_u64(ctx, keys, keys + src->m, src->m, mx ? 32 - __builtin_clz(mx) : 1))) {
-      k_link_cand<<<g, 256, 0, ctx->stream>>>(keys, src->m, view(src), view(dst), hint);
+        !(rc = radix_sort_u64(ctx, keys, keys + src->m, src->m, mx ? 32 - __builtin_clz(mx) : 1, 0, &sorted))) {
+      k_link_cand<<<g, 256, 0, ctx->stream>>>(sorted, src->m, view(src), view(dst), hint);
       e = hipGetLastError();
       cand = 1;
     }
