@@ -549,11 +549,23 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
           const uint4 *q = tx.pk + 2ull * sko;
           const uint32_t nk = (bd + PK_VALS - 1) / PK_VALS, klast = nk - 1u, pad = nk * PK_VALS - bd;
           uint32_t h32 = 0, corr = 0, c = ql;
-          for (; UNROLL && c + G < nk; c += 2 * G) {  // two chunks (30 values) in flight
-            const uint4 a0 = q[2 * c], b0 = q[2 * c + 1], a1 = q[2 * (c + G)], b1 = q[2 * (c + G) + 1];
-            uint32_t l0, l1;
-            h32 += probe_chunk<MODE, MAXS>(st, S, P.ns, a0, b0, l0) + probe_chunk<MODE, MAXS>(st, S, P.ns, a1, b1, l1);
-            if (c + G == klast) corr = l1;
+          // two chunks (30 values) in flight per step while some lane of the wave
+          // has two left; a lane with one left takes it in the same step (its
+          // second chunk a masked repeat), so a one-chunk step runs only when no
+          // lane has two left
+          for (; UNROLL; c += 2 * G) {
+            const bool two = c + G < nk;
+            if (!__ballot(two)) break;
+            if (c < nk) {
+              const uint32_t c2 = two ? c + G : c;
+              const uint4 a0 = q[2 * c], b0 = q[2 * c + 1], a1 = q[2 * c2], b1 = q[2 * c2 + 1];
+              uint32_t l0, l1;
+              const uint32_t p0 = probe_chunk<MODE, MAXS>(st, S, P.ns, a0, b0, l0);
+              const uint32_t p1 = probe_chunk<MODE, MAXS>(st, S, P.ns, a1, b1, l1);
+              h32 += p0 + (two ? p1 : 0u);
+              if (c == klast) corr = l0;
+              if (two && c2 == klast) corr = l1;
+            }
           }
           for (; c < nk; c += G) {
             uint32_t l0;
@@ -594,12 +606,19 @@ __device__ __forceinline__ uint64_t stream_children(const TunedPlan &P, const Tu
                probe_m<MODE, MAXS>(st, S, P.ns, v3.x) + probe_m<MODE, MAXS>(st, S, P.ns, v3.y) +
                probe_m<MODE, MAXS>(st, S, P.ns, v3.z) + probe_m<MODE, MAXS>(st, S, P.ns, v3.w);
       }
-      for (; UNROLL && i + G < n4; i += 2 * G) {
-        const uint4 v0 = q[i], v1 = q[i + G];
-        h32 += probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
-               probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w) +
-               probe_m<MODE, MAXS>(st, S, P.ns, v1.x) + probe_m<MODE, MAXS>(st, S, P.ns, v1.y) +
-               probe_m<MODE, MAXS>(st, S, P.ns, v1.z) + probe_m<MODE, MAXS>(st, S, P.ns, v1.w);
+      // two quads per step while some lane has two left; a lane with one left
+      // takes it in the same step (its second quad a masked repeat)
+      for (; UNROLL; i += 2 * G) {
+        const bool two = i + G < n4;
+        if (!__ballot(two)) break;
+        if (i < n4) {
+          const uint4 v0 = q[i], v1 = q[two ? i + G : i];
+          const uint32_t p0 = probe_m<MODE, MAXS>(st, S, P.ns, v0.x) + probe_m<MODE, MAXS>(st, S, P.ns, v0.y) +
+                              probe_m<MODE, MAXS>(st, S, P.ns, v0.z) + probe_m<MODE, MAXS>(st, S, P.ns, v0.w);
+          const uint32_t p1 = probe_m<MODE, MAXS>(st, S, P.ns, v1.x) + probe_m<MODE, MAXS>(st, S, P.ns, v1.y) +
+                              probe_m<MODE, MAXS>(st, S, P.ns, v1.z) + probe_m<MODE, MAXS>(st, S, P.ns, v1.w);
+          h32 += p0 + (two ? p1 : 0u);
+        }
       }
       for (; i < n4; i += G) {
         const uint4 v0 = q[i];
